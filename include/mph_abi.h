@@ -1,0 +1,170 @@
+/*
+ * mph_abi.h -- C ABI of the MI355X MPH-I time stepper (libmph_hip.so).
+ *
+ * The reference (Masahiro-Kondo-AIST/MphImplicit) has no library interface: its hot
+ * path is a set of file-static functions over global arrays called in a fixed order
+ * from main() (source/main.cpp:174-211 prototypes, 386-474 time loop).  This header
+ * is the drop-in boundary that replaces that path: one call per reference entry
+ * point or phase, plain pointers and sizes, status codes instead of exit()/assert.
+ *
+ * Conventions
+ *  - Every function returns MPH_OK (0) or a negative MPH_E_* code; the message of
+ *    the last failure on the calling thread is available from mph_last_error().
+ *  - A scene owns all of its device state; host arrays passed in are copied.
+ *  - A scene handle is not thread-safe.  Distinct scenes may be driven from
+ *    distinct host threads.  Each scene runs on its own HIP stream.
+ *  - Arrays read back are in the current sorted particle order, exactly as the
+ *    reference keeps them after calculateCellParticle (source/main.cpp:1315-1378).
+ */
+#ifndef MPH_ABI_H
+#define MPH_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPH_TYPE_COUNT 4 /* types 0,1 fluid; 2,3 wall (source/main.cpp:52-56) */
+
+#define MPH_OK 0
+#define MPH_E_ARG (-1)     /* invalid argument or size                         */
+#define MPH_E_IO (-2)      /* file could not be opened or parsed               */
+#define MPH_E_DEVICE (-3)  /* HIP runtime error                                */
+#define MPH_E_NOMEM (-4)   /* device or host allocation failed                 */
+#define MPH_E_STATE (-5)   /* scene state the reference cannot handle either   */
+#define MPH_E_FIELD (-6)   /* unknown field name                               */
+#define MPH_E_NOTIMPL (-7) /* mode not available in this build                 */
+
+/* Dot-product summation order used by the conjugate-residual solver. */
+#define MPH_DOT_TREE 1 /* fixed two-level tree over 256-particle chunks */
+
+/* Precision of the solver's inner products with the particle matrix. */
+#define MPH_PRECISION_FP64 0  /* plain FP64, bit-compatible with the reference   */
+#define MPH_PRECISION_MIXED 1 /* FP32 SpMV + multigrid inside FP64 refinement    */
+
+/*
+ * Parameters of a run: the keywords of the reference's *.data file
+ * (readDataFile, source/main.cpp:523-563) plus the compile-time switches of the
+ * reference turned into run-time fields (TWO_DIMENSIONAL, MULTIGRID_SOLVER,
+ * the CR stop rule rr/rr0 < 1e-12 at source/main.cpp:4346).
+ */
+typedef struct mph_params {
+    int dim; /* 2 (TWO_DIMENSIONAL) or 3 */
+    double dt;
+    double output_interval;
+    double vtk_output_interval;
+    double end_time;
+    double radius_ratio_a;
+    double radius_ratio_p;
+    double radius_ratio_v;
+    double density[MPH_TYPE_COUNT];
+    double bulk_modulus[MPH_TYPE_COUNT];
+    double bulk_viscosity[MPH_TYPE_COUNT];
+    double shear_viscosity[MPH_TYPE_COUNT];
+    double surface_tension[MPH_TYPE_COUNT];
+    double interaction_ratio[MPH_TYPE_COUNT][MPH_TYPE_COUNT];
+    double gravity[3];
+    double wall_center[MPH_TYPE_COUNT][3];
+    double wall_velocity[MPH_TYPE_COUNT][3];
+    double wall_omega[MPH_TYPE_COUNT][3];
+    /* solver controls; mph_params_default() reproduces the reference build */
+    double cr_tolerance; /* stop when rr/rr0 < cr_tolerance (reference: 1e-12)  */
+    int multigrid;       /* 1: MG-preconditioned CR (MULTIGRID_SOLVER), 0: CR    */
+    int precision;       /* MPH_PRECISION_*                                      */
+    int max_iterations;  /* <=0: 3*fluid count, as the reference                 */
+} mph_params;
+
+/* Particles and domain, the content of a *.grid / *.prof file
+ * (readGridFile, source/main.cpp:565-694). */
+typedef struct mph_grid {
+    double time;
+    int np;
+    double spacing;
+    double domain_min[3];
+    double domain_max[3];
+    const int* type;   /* [np]     */
+    const double* pos; /* [np][3]  */
+    const double* vel; /* [np][3]  */
+} mph_grid;
+
+/* Per-call statistics of mph_scene_step. */
+typedef struct mph_step_stats {
+    int steps_done;
+    int cr_iterations; /* of the last step (source/main.cpp:4381 "iter")          */
+    int cr_iterations_total;
+    double rr0, rr, rs0, rs; /* CR summary of the last step (main.cpp:4381)         */
+    int vtk_steps;           /* steps on which the VTK diagnostics were computed    */
+    double ms_phase[12];     /* summed device time per MPH_PHASE_* (if timing on)   */
+    double ms_total;
+} mph_step_stats;
+
+/* Phases of one time step, in the order of source/main.cpp:386-474. */
+#define MPH_PHASE_KINEMATICS 1  /* calculateConvection/Wall/PeriodicBoundary, resetForce */
+#define MPH_PHASE_CELLS 2       /* calculateCellParticle                                  */
+#define MPH_PHASE_NEIGHBOR 3    /* calculateNeighbor                                      */
+#define MPH_PHASE_EXPLICIT 4    /* DensityA ... calculateAcceleration                     */
+#define MPH_PHASE_MATRIX_A 5    /* calculateMatrixA                                       */
+#define MPH_PHASE_MATRIX_C 6    /* calculateMatrixC                                       */
+#define MPH_PHASE_MULTIPLY_C 7  /* multiplyMatrixC                                        */
+#define MPH_PHASE_MULTIGRID 8   /* calculateMultiGridMatrix                               */
+#define MPH_PHASE_SOLVE 9       /* solveWithConjugatedGradient                            */
+#define MPH_PHASE_DIAGNOSTICS 10 /* DivergenceP, PressureP, ViscosityV, VirialStress      */
+#define MPH_PHASE_ADVANCE 11    /* Time += Dt and output bookkeeping                      */
+
+/* Parameters equal to the reference build's defaults: 3D, all .data fields zero
+ * except Dt=1e100 (source/main.cpp:75), tolerance 1e-12, multigrid on, FP64. */
+void mph_params_default(mph_params* p);
+
+/* readDataFile (source/main.cpp:523-563): keyword parser, unknown lines ignored. */
+int mph_read_data_file(const char* path, mph_params* p);
+
+/* readGridFile (source/main.cpp:565-694), in two calls: the header gives np, then
+ * the caller supplies arrays of np entries. */
+int mph_read_grid_header(const char* path, int* np);
+int mph_read_grid_file(const char* path, int np, double* time, double* spacing,
+                       double domain_min[3], double domain_max[3], int* type,
+                       double* pos, double* vel);
+
+typedef struct mph_scene mph_scene;
+
+/* Create a scene on HIP device `device` and run the reference's initialisation
+ * (initializeWeight/Fluid/Wall/Domain and the first cell sort, neighbour search and
+ * densities, source/main.cpp:353-375). */
+int mph_scene_create(const mph_params* p, const mph_grid* g, int device, mph_scene** out);
+
+/* Advance nsteps iterations of the reference's time loop (source/main.cpp:386-474),
+ * including the output-step diagnostics on VTK steps.  `st` may be NULL. */
+int mph_scene_step(mph_scene* s, int nsteps, mph_step_stats* st);
+
+/* Run one MPH_PHASE_* of the time step on the current state (parity tests). */
+int mph_scene_run_phase(mph_scene* s, int phase);
+
+/* Copy a named field to or from the host.  Sizes are in bytes and must equal the
+ * field's current size (mph_scene_field_size).  Field names follow the
+ * reference's globals: "Position", "Velocity", "Property", "Force", "NeighborInd",
+ * "CsrCofA", "VectorB", ... (listed in mphimplicit_amd/csrc/mph_fields.inc). */
+int mph_scene_field_size(mph_scene* s, const char* field, size_t* bytes, int* elem_bytes);
+int mph_scene_read(mph_scene* s, const char* field, void* host, size_t bytes);
+int mph_scene_write(mph_scene* s, const char* field, const void* host, size_t bytes);
+
+/* Scalars of the scene: "Time", "Dt", "ParticleCount", "FluidCount", ... */
+int mph_scene_scalar(mph_scene* s, const char* name, double* value);
+
+/* writeProfFile (source/main.cpp:696-720) and writeVtkFile (722-883). */
+int mph_scene_write_prof(mph_scene* s, const char* path);
+int mph_scene_write_vtk(mph_scene* s, const char* path);
+
+/* Per-phase device timing with HIP events (adds one sync per phase when on). */
+int mph_scene_set_timing(mph_scene* s, int on);
+
+int mph_scene_destroy(mph_scene* s);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* mph_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
