@@ -152,6 +152,9 @@ int mph_scene_write(mph_scene* s, const char* field, const void* host, size_t by
 /* Scalars of the scene: "Time", "Dt", "ParticleCount", "FluidCount", ... */
 int mph_scene_scalar(mph_scene* s, const char* name, double* value);
 
+/* Particle spacing and domain box of the scene (the *.grid header). */
+int mph_scene_domain(mph_scene* s, double* spacing, double domain_min[3], double domain_max[3]);
+
 /* writeProfFile (source/main.cpp:696-720) and writeVtkFile (722-883). */
 int mph_scene_write_prof(mph_scene* s, const char* path);
 int mph_scene_write_vtk(mph_scene* s, const char* path);

@@ -1,0 +1,130 @@
+// mph_kernels.h -- launchers of the HIP kernels (one scene, one stream).
+//
+// Layouts in HBM (all device pointers):
+//   particles    AoS double[np][3] for Position/Velocity/Force (as the reference),
+//                double[np] scalars, int Property[np], int perm[np]
+//   cell keys    uint64 keys[p2] = (CellIndex << 32) | CellParticle while sorting;
+//                int cellIndex[np] (sorted keys), int cellStart[2T+1]
+//   neighbours   int64 ptr[np+1], int ind[E]; same for the RadiusP lists
+//   A            sliced 3x3-block ELL: slice = 64 consecutive fluid particles,
+//                width = max fluid-neighbour count in the slice, slot
+//                (slice, k, lane) = (sliceStart[slice] + k)*64 + lane;
+//                acol[slot] = neighbour particle; aval[(slot/64*9 + e)*64 + lane]
+//                holds entry e = 3r+s of the block (coalesced per entry)
+//   C            cval[3*(ptrP[i] + jn) + r] for fluid i, P-neighbour jn
+//   multigrid    per level: A stored [shift][row], vectors [row]
+//   CR vectors   double[3*nf], row 3i+r as the reference
+#pragma once
+
+#include "mph_common.h"
+
+namespace mph {
+
+struct WallKin {  // per wall type: rotation, angular and linear velocity, centre
+    double R[4][3][3];
+    double omega[4][3];
+    double vel[4][3];
+    double center[4][3];
+};
+
+struct Level {  // geometry + storage of one multigrid level
+    int mn[3], mx[3], cnt[3], cs[3];
+    int cells, nrow, nshift;
+    double* A;     // [nshift][nrow]
+    double* invD;  // [nrow]
+    double* S[2];  // ping-pong smoother iterate
+    double* R;
+    double* Q;
+    int cur;       // which S buffer holds the current iterate
+};
+
+struct ASlices {
+    int nslices;
+    const int* width;          // [nslices]
+    const long long* start;    // [nslices+1]
+    int* col;                  // [slots]
+    double* val;               // [slots*9]
+    float* val32;              // [slots*9] (mixed precision copy) or null
+};
+
+constexpr int kSlice = 64;    // particles per A slice (one wavefront)
+constexpr int kChunk = 256;   // particles per dot-product chunk / CR workgroup
+
+// ---- cells (calculateCellParticle) ----
+void launch_hash(hipStream_t st, const DevParams& P, const double* pos, const int* type, uint64_t* keys);
+void launch_bitonic(hipStream_t st, const DevParams& P, uint64_t* keys);
+void launch_cell_start(hipStream_t st, const DevParams& P, const uint64_t* keys, int* cellStart);
+void launch_reorder(hipStream_t st, const DevParams& P, const uint64_t* keys, const int* type, const int* perm,
+                    const double* pos, const double* vel, int* type2, int* perm2, double* pos2, double* vel2,
+                    int* cellIndex);
+// ---- neighbours (calculateNeighbor) ----
+void launch_neighbor_count(hipStream_t st, const DevParams& P, const double* pos, const int* cellIndex,
+                           const int* cellStart, int* nCount, int* nFluid, int* nCountP);
+void launch_scan_i64(hipStream_t st, const int* in, long long* out, int n, long long* scratch, int scratch_len);
+int scan_scratch_len(int n);
+void launch_neighbor_fill(hipStream_t st, const DevParams& P, const double* pos, const int* cellIndex,
+                          const int* cellStart, const int* nCount, const long long* nPtr, const long long* nPtrP,
+                          int* nInd, int* nIndP);
+void launch_slice_width(hipStream_t st, int nf, int nslices, const int* nFluid, int* width);
+// ---- kinematics and explicit forces ----
+void launch_kinematics(hipStream_t st, const DevParams& P, const WallKin& wk, const int* type, double* pos,
+                       double* vel, double* force);
+void launch_densities(hipStream_t st, const DevParams& P, const int* type, const double* pos, const int* nCount,
+                      const long long* nPtr, const int* nInd, double* densA, double* gcen, double* volStrainP);
+void launch_coefficients(hipStream_t st, const DevParams& P, const int* type, const double* densA,
+                         const double* volStrainP, double* mass, double* kappa, double* lambda, double* mu,
+                         double* pressA);
+void launch_forces(hipStream_t st, const DevParams& P, const int* type, const double* pos, const int* nCount,
+                   const long long* nPtr, const int* nInd, const double* pressA, const double* gcen,
+                   const double* mass, double* force, double* vel);
+// ---- implicit assembly ----
+void launch_matrix_a(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
+                     const double* mass, const double* mu, const int* nCount, const int* nFluid,
+                     const long long* nPtr, const int* nInd, ASlices A, int* selfIdx, double* b);
+void launch_matrix_c(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP,
+                     const long long* nPtrP, const int* nIndP, double* cval);
+void launch_vector_p(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
+                     const double* kappa, const double* lambda, const double* volStrainP, const int* nCountP,
+                     const long long* nPtrP, const int* nIndP, double* vecP);
+void launch_b_minus_cp(hipStream_t st, const DevParams& P, const int* nCountP, const long long* nPtrP,
+                       const int* nIndP, const double* cval, const double* vecP, double* b);
+void launch_cac(hipStream_t st, const DevParams& P, const int* nFluid, const int* nCountP, const long long* nPtrP,
+                const int* nIndP, const double* cval, const double* lambda, ASlices A);
+// ---- multigrid ----
+void launch_fluid_bbox(hipStream_t st, const DevParams& P, const int* cellStart, int* bbox);
+void launch_inv_diag(hipStream_t st, const DevParams& P, const int* selfIdx, ASlices A, double* invDiagA);
+void launch_mg_galerkin0(hipStream_t st, const DevParams& P, const Level& L, const int* cellStart,
+                         const int* nFluid, const long long* nPtr, const int* nInd, ASlices A);
+void launch_mg_galerkinL(hipStream_t st, const Level& S, const Level& L);
+void launch_mg_invd(hipStream_t st, const Level& L);
+void launch_fill(hipStream_t st, double* x, long long n, double v);
+void launch_mg_p2g(hipStream_t st, const DevParams& P, Level& L, const int* cellStart, const double* q);
+void launch_mg_begin(hipStream_t st, Level& L);        // S *= 0; R = Q
+void launch_mg_update(hipStream_t st, Level& L);       // S += invD R
+void launch_mg_residual(hipStream_t st, Level& L);     // R = Q - A S
+void launch_mg_residual_update(hipStream_t st, Level& L);  // R = Q - A S; S += invD R
+void launch_mg_restrict(hipStream_t st, const Level& S, Level& L);
+void launch_mg_prolong(hipStream_t st, const Level& L, Level& S);
+void launch_mg_g2p(hipStream_t st, const DevParams& P, const Level& L0, const int* cellIndex,
+                   const double* invDiagA, const double* q, double* u, double* part);
+// ---- conjugate residual ----
+void launch_cr_init(hipStream_t st, const DevParams& P, const double* vel, const double* force,
+                    const double* mass, double* x);
+void launch_spmv(hipStream_t st, const DevParams& P, ASlices A, const int* nFluid, double alpha, const double* x,
+                 double beta, const double* yin, double* y, double* y2, double* p_copy, const double* dotx,
+                 double* part);
+void launch_dot_partial(hipStream_t st, const DevParams& P, const double* x, const double* y, double* part);
+void launch_dot_final(hipStream_t st, const DevParams& P, const double* part, double* scal, int slot, int shift_slot);
+void launch_pq_update(hipStream_t st, const DevParams& P, const double* scal, const double* s, const double* y,
+                      double* p, double* q);
+void launch_xrs_update(hipStream_t st, const DevParams& P, const double* scal, const double* p, const double* q,
+                       const double* u, double* x, double* r, double* s, double* part);
+void launch_store_velocity(hipStream_t st, const DevParams& P, const double* x, double* vel);
+// ---- output-step diagnostics ----
+void launch_diagnostics(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
+                        const double* mu, const double* lambda, const double* kappa, const double* volStrainP,
+                        const double* pressA, const double* gcen, const int* nCount, const long long* nPtr,
+                        const int* nInd, const int* nCountP, const long long* nPtrP, const int* nIndP, double* divP,
+                        double* pressP, double* force, double* virialStress, double* virialP);
+
+}  // namespace mph

@@ -1,0 +1,447 @@
+// mph_physics.hip -- kinematics, explicit particle interactions and implicit assembly.
+//
+// Replaces calculateConvection/Wall/PeriodicBoundary/resetForce, the explicit force
+// loops (source/main.cpp:1678-2128, 4741-4752), calculateMatrixA/C and
+// multiplyMatrixC (2137-2643), and the output-step diagnostics (1932-2063,
+// 4494-4738).  One thread owns one particle; each of its sums runs over the
+// neighbour list in list order, so every value is bit-identical to the reference.
+#include "mph_kernels.h"
+
+namespace mph {
+
+namespace {
+
+inline int nblk(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+struct Rel {
+    double x[3];
+    double r2;
+};
+
+__device__ inline Rel rel(const DevParams& P, const double* __restrict__ pos, int i, int j) {
+    Rel o;
+    for (int d = 0; d < 3; ++d) o.x[d] = min_image(pos[3 * j + d], pos[3 * i + d], P.W[d], P.hW[d]);
+    o.r2 = (o.x[0] * o.x[0] + o.x[1] * o.x[1] + o.x[2] * o.x[2]);
+    return o;
+}
+
+// value (r,s) of block k of fluid particle i in the sliced A layout
+__device__ inline long long aval_index(const long long* __restrict__ start, int i, int k, int e) {
+    const int s = i / kSlice, lane = i % kSlice;
+    return ((start[s] + k) * 9 + e) * kSlice + lane;
+}
+__device__ inline long long acol_index(const long long* __restrict__ start, int i, int k) {
+    const int s = i / kSlice, lane = i % kSlice;
+    return (start[s] + k) * kSlice + lane;
+}
+
+// calculateConvection, calculateWall, calculatePeriodicBoundary, resetForce
+__global__ void k_kinematics(DevParams P, WallKin wk, const int* __restrict__ type, double* __restrict__ pos,
+                             double* __restrict__ vel, double* __restrict__ force) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.np) return;
+    double x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    if (i < P.nf) {
+        for (int d = 0; d < 3; ++d) x[d] += vel[3 * i + d] * P.dt;
+    } else {
+        const int t = type[i];
+        const double(&R)[3][3] = wk.R[t];
+        const double* w = wk.omega[t];
+        const double* V = wk.vel[t];
+        const double* c = wk.center[t];
+        double r[3] = {x[0] - c[0], x[1] - c[1], x[2] - c[2]};
+        r[0] = R[0][0] * r[0] + R[0][1] * r[1] + R[0][2] * r[2];  // in place, main.cpp:2108-2110
+        r[1] = R[1][0] * r[0] + R[1][1] * r[1] + R[1][2] * r[2];
+        r[2] = R[2][0] * r[0] + R[2][1] * r[1] + R[2][2] * r[2];
+        vel[3 * i + 0] = w[1] * r[2] - w[2] * r[1] + V[0];
+        vel[3 * i + 1] = w[2] * r[0] - w[0] * r[2] + V[1];
+        vel[3 * i + 2] = w[0] * r[1] - w[1] * r[0] + V[2];
+        x[0] = r[0] + c[0] + V[0] * P.dt;
+        x[1] = r[1] + c[1] + V[1] * P.dt;
+        x[2] = r[2] + c[2] + V[2] * P.dt;
+    }
+    for (int d = 0; d < 3; ++d) {
+        const double t = x[d] - P.dmin[d];
+        pos[3 * i + d] = (t - P.W[d] * floor(t / P.W[d])) + P.dmin[d];
+        force[3 * i + d] = 0.0;
+    }
+}
+
+// calculateDensityA, calculateGravityCenter, calculateDensityP (three sums, one list walk)
+__global__ void k_densities(DevParams P, const int* __restrict__ type, const double* __restrict__ pos,
+                            const int* __restrict__ nCount, const long long* __restrict__ nPtr,
+                            const int* __restrict__ nInd, double* __restrict__ densA, double* __restrict__ gcen,
+                            double* __restrict__ volStrainP) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.np) return;
+    const int ti = type[i];
+    double sa = 0.0, sg0 = 0.0, sg1 = 0.0, sg2 = 0.0, sp = 0.0;
+    const long long b = nPtr[i], e = b + nCount[i];
+    for (long long k = b; k < e; ++k) {
+        const int j = nInd[k];
+        if (i == j) continue;
+        const double ratio = P.ratio[ti][type[j]];
+        const Rel q = rel(P, pos, i, j);
+        if (P.radA2 - q.r2 >= 0) {
+            const double r = sqrt(q.r2);
+            const double weight = ratio * wa_k(P.Ka, r, P.radA);
+            sa += weight;
+        }
+        if (P.radG2 - q.r2 >= 0) {
+            const double r = sqrt(q.r2);
+            const double weight = ratio * wq_k(P.Kg, r, P.radG);
+            sg0 += q.x[0] * weight / P.R2g * P.radG;
+            sg1 += q.x[1] * weight / P.R2g * P.radG;
+            sg2 += q.x[2] * weight / P.R2g * P.radG;
+        }
+        if (P.radP2 - q.r2 >= 0) {
+            const double r = sqrt(q.r2);
+            const double weight = wq_k(P.Kp, r, P.radP);
+            sp += weight;
+        }
+    }
+    densA[i] = sa;
+    gcen[3 * i + 0] = sg0;
+    gcen[3 * i + 1] = sg1;
+    gcen[3 * i + 2] = sg2;
+    volStrainP[i] = (sp - P.N0p);
+}
+
+// calculatePhysicalCoefficients (1704-1735) and the PressureA values (1807-1812)
+__global__ void k_coefficients(DevParams P, const int* __restrict__ type, const double* __restrict__ densA,
+                               const double* __restrict__ volStrainP, double* __restrict__ mass,
+                               double* __restrict__ kappa, double* __restrict__ lambda, double* __restrict__ mu,
+                               double* __restrict__ pressA) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.np) return;
+    const int t = type[i];
+    mass[i] = P.density[t] * P.V;
+    kappa[i] = volStrainP[i] < 0.0 ? 0.0 : P.bulk[t];
+    lambda[i] = P.bulkVisc[t];
+    mu[i] = P.shear[t];
+    double pa = P.cofA[t] * (densA[i] - P.N0a) / P.l;
+    if (P.N0a <= densA[i]) pa = 0.0;
+    pressA[i] = pa;
+}
+
+// calculatePressureA forces (1814-1848), calculateDiffuseInterface (1851-1901),
+// calculateGravity (2065-2074), calculateAcceleration (2076-2086)
+__global__ void k_forces(DevParams P, const int* __restrict__ type, const double* __restrict__ pos,
+                         const int* __restrict__ nCount, const long long* __restrict__ nPtr,
+                         const int* __restrict__ nInd, const double* __restrict__ pressA,
+                         const double* __restrict__ gcen, const double* __restrict__ mass, double* __restrict__ force,
+                         double* __restrict__ vel) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.np) return;
+    const int ti = type[i];
+    const double ai = P.ai[ti];
+    const double aj = P.ai[ti];  // the reference uses i's type for aj (main.cpp:1863)
+    const double VoL = P.V / P.l;
+    const double pai = pressA[i];
+    const double gi0 = gcen[3 * i], gi1 = gcen[3 * i + 1], gi2 = gcen[3 * i + 2];
+    double fa[3] = {0.0, 0.0, 0.0}, fd[3] = {0.0, 0.0, 0.0};
+    const long long b = nPtr[i], e = b + nCount[i];
+    for (long long k = b; k < e; ++k) {
+        const int j = nInd[k];
+        if (i == j) continue;
+        const int tj = type[j];
+        const double rij_ = P.ratio[ti][tj], rji_ = P.ratio[tj][ti];
+        const Rel q = rel(P, pos, i, j);
+        if (q.r2 == 0.0) continue;
+        if (P.radA2 - q.r2 > 0) {
+            const double r = sqrt(q.r2);
+            const double dwij = rij_ * dwadr_k(P.Ka, P.invA, r, P.radA);
+            const double dwji = rji_ * dwadr_k(P.Ka, P.invA, r, P.radA);
+            const double paj = pressA[j];
+            for (int d = 0; d < 3; ++d) fa[d] += (pai * dwij + paj * dwji) * (q.x[d] / r) * P.V;
+        }
+        if (P.radG2 - q.r2 > 0) {
+            const double r = sqrt(q.r2);
+            const double wij = rij_ * wq_k(P.Kg, r, P.radG);
+            const double wji = rji_ * wq_k(P.Kg, r, P.radG);
+            const double gj[3] = {gcen[3 * j], gcen[3 * j + 1], gcen[3 * j + 2]};
+            const double gi[3] = {gi0, gi1, gi2};
+            for (int d = 0; d < 3; ++d) fd[d] -= (aj * gj[d] * wji - ai * gi[d] * wij) / P.R2g * P.radG * VoL;
+            const double dwij = rij_ * dwq_k(P.Kg, P.m2G, r, P.radG);
+            const double dwji = rji_ * dwq_k(P.Kg, P.m2G, r, P.radG);
+            double gr = 0.0;
+            for (int d = 0; d < 3; ++d) gr += (aj * gj[d] * dwji - ai * gi[d] * dwij) * q.x[d];
+            for (int d = 0; d < 3; ++d) fd[d] -= (gr) * (q.x[d] / r) / P.R2g * P.radG * VoL;
+        }
+    }
+    double F[3];
+    for (int d = 0; d < 3; ++d) {
+        F[d] = force[3 * i + d];
+        F[d] += fa[d];
+        F[d] += fd[d];
+    }
+    if (i < P.nf) {
+        const double m = mass[i];
+        for (int d = 0; d < 3; ++d) {
+            F[d] += m * P.gravity[d];
+            vel[3 * i + d] += F[d] / m * P.dt;
+        }
+    }
+    for (int d = 0; d < 3; ++d) force[3 * i + d] = F[d];
+}
+
+// calculateMatrixA values (2251-2335): viscous blocks, inertia, wall terms of b
+__global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const double* __restrict__ pos,
+                           const double* __restrict__ vel, const double* __restrict__ mass,
+                           const double* __restrict__ mu, const int* __restrict__ nCount,
+                           const int* __restrict__ nFluid, const long long* __restrict__ nPtr,
+                           const int* __restrict__ nInd, const long long* __restrict__ start, int* __restrict__ acol,
+                           double* __restrict__ aval, int* __restrict__ selfIdx, double* __restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nf) return;
+    int iN = 0;
+    double self[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double sv0 = 0.0, sv1 = 0.0, sv2 = 0.0;
+    const double mui = mu[i];
+    const long long b0 = nPtr[i];
+    const int nfl = nFluid[i], nc = nCount[i];
+    for (int jn = 0; jn < nc; ++jn) {
+        const int j = nInd[b0 + jn];
+        const bool fluidj = jn < nfl;
+        if (fluidj) acol[acol_index(start, i, jn)] = j;
+        if (i == j) {
+            iN = jn;
+            continue;
+        }
+        double c9[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const Rel q = rel(P, pos, i, j);
+        const bool inside = q.r2 != 0.0 && P.radV2 - q.r2 > 0;
+        if (inside) {
+            const double dij = sqrt(q.r2);
+            const double wdrij = -dwq_k(P.Kv, P.m2V, dij, P.radV);
+            const double e[3] = {q.x[0] / dij, q.x[1] / dij, q.x[2] / dij};
+            const double muj = mu[j];
+            const double muij = 2.0 * (mui * muj) / (mui + muj);
+            for (int r = 0; r < 3; ++r)
+                for (int s = 0; s < 3; ++s) {
+                    const double coef = +P.visc * muij * e[s] * e[r] * wdrij / dij;
+                    self[3 * r + s] += coef;
+                    c9[3 * r + s] = coef;
+                }
+            if (is_wall(type[j])) {
+                const double v0 = vel[3 * j], v1 = vel[3 * j + 1], v2 = vel[3 * j + 2];
+                sv0 += c9[0] * v0; sv0 += c9[1] * v1; sv0 += c9[2] * v2;
+                sv1 += c9[3] * v0; sv1 += c9[4] * v1; sv1 += c9[5] * v2;
+                sv2 += c9[6] * v0; sv2 += c9[7] * v1; sv2 += c9[8] * v2;
+            }
+        }
+        if (fluidj)
+            for (int e9 = 0; e9 < 9; ++e9) aval[aval_index(start, i, jn, e9)] = inside ? -c9[e9] : 0.0;
+    }
+    const double coef = mass[i] / P.V / P.dt;
+    self[0] += coef;
+    self[4] += coef;
+    self[8] += coef;
+    for (int e9 = 0; e9 < 9; ++e9) aval[aval_index(start, i, iN, e9)] = self[e9];
+    selfIdx[i] = iN;
+    const double bv[3] = {0.0 + sv0, 0.0 + sv1, 0.0 + sv2};
+    for (int d = 0; d < 3; ++d) b[3 * i + d] = bv[d] + coef * vel[3 * i + d];
+}
+
+// calculateMatrixC values (2463-2507): C[3i+r][k] = e_r * (-w'_p)
+__global__ void k_matrix_c(DevParams P, const double* __restrict__ pos, const int* __restrict__ nCountP,
+                           const long long* __restrict__ nPtrP, const int* __restrict__ nIndP,
+                           double* __restrict__ cval) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nf) return;
+    int iN = 0;
+    double self[3] = {0.0, 0.0, 0.0};
+    const long long b0 = nPtrP[i];
+    const int n = nCountP[i];
+    for (int jn = 0; jn < n; ++jn) {
+        const int j = nIndP[b0 + jn];
+        double c3[3] = {0.0, 0.0, 0.0};
+        if (i == j) {
+            iN = jn;
+        } else {
+            const Rel q = rel(P, pos, i, j);
+            if (q.r2 != 0.0 && P.radP2 - q.r2 > 0) {
+                const double dij = sqrt(q.r2);
+                const double e[3] = {q.x[0] / dij, q.x[1] / dij, q.x[2] / dij};
+                const double wpdrij = -dwq_k(P.Kp, P.m2P, dij, P.radP);
+                for (int r = 0; r < 3; ++r) {
+                    c3[r] = e[r] * wpdrij;
+                    self[r] += c3[r];
+                }
+            }
+        }
+        for (int r = 0; r < 3; ++r) cval[3 * (b0 + jn) + r] = c3[r];
+    }
+    for (int r = 0; r < 3; ++r) cval[3 * (b0 + iN) + r] = self[r];
+}
+
+// VectorP, main.cpp:2510-2549
+__global__ void k_vector_p(DevParams P, const int* __restrict__ type, const double* __restrict__ pos,
+                           const double* __restrict__ vel, const double* __restrict__ kappa,
+                           const double* __restrict__ lambda, const double* __restrict__ volStrainP,
+                           const int* __restrict__ nCountP, const long long* __restrict__ nPtrP,
+                           const int* __restrict__ nIndP, double* __restrict__ vecP) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.np) return;
+    double pv = 0.0;
+    if (volStrainP[i] > 0.0) pv = kappa[i] * volStrainP[i];
+    const bool walli = is_wall(type[i]);
+    const double li = lambda[i];
+    double sum = 0.0;
+    const long long b0 = nPtrP[i];
+    const int n = nCountP[i];
+    for (int jn = 0; jn < n; ++jn) {
+        const int j = nIndP[b0 + jn];
+        if (i == j) continue;
+        const Rel q = rel(P, pos, i, j);
+        if (q.r2 == 0.0) continue;
+        if (P.radP2 - q.r2 > 0) {
+            const double dij = sqrt(q.r2);
+            const double e[3] = {q.x[0] / dij, q.x[1] / dij, q.x[2] / dij};
+            const double wpdrij = -dwq_k(P.Kp, P.m2P, dij, P.radP);
+            const bool wallj = is_wall(type[j]);
+            for (int s = 0; s < 3; ++s) {
+                if (walli) {
+                    const double coef = +e[s] * wpdrij;
+                    sum += li * coef * vel[3 * i + s];
+                }
+                if (wallj) {
+                    const double coef = -e[s] * wpdrij;
+                    sum += li * coef * vel[3 * j + s];
+                }
+            }
+        }
+    }
+    vecP[i] = pv + sum;
+}
+
+// b -= C P (multiplyMatrixC, 2575-2584), accumulated directly into b
+__global__ void k_b_minus_cp(DevParams P, const int* __restrict__ nCountP, const long long* __restrict__ nPtrP,
+                             const int* __restrict__ nIndP, const double* __restrict__ cval,
+                             const double* __restrict__ vecP, double* __restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nf) return;
+    double b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
+    const long long p0 = nPtrP[i];
+    const int n = nCountP[i];
+    for (int jn = 0; jn < n; ++jn) {
+        const double pk = vecP[nIndP[p0 + jn]];
+        const double* c = &cval[3 * (p0 + jn)];
+        b0 -= c[0] * pk;
+        b1 -= c[1] * pk;
+        b2 -= c[2] * pk;
+    }
+    b[3 * i] = b0;
+    b[3 * i + 1] = b1;
+    b[3 * i + 2] = b2;
+}
+
+// A += C Lambda C^T (multiplyMatrixC, 2587-2642).  One workgroup per A slice:
+// wave w handles blocks k = w, w+4, ... for the slice's 64 particles (lanes), so
+// the read-modify-write of every block entry is coalesced.  The two P-neighbour
+// lists are merged in ascending order, as in the reference.
+__global__ void __launch_bounds__(256) k_cac(DevParams P, const int* __restrict__ nFluid,
+                                             const int* __restrict__ nCountP, const long long* __restrict__ nPtrP,
+                                             const int* __restrict__ nIndP, const double* __restrict__ cval,
+                                             const double* __restrict__ lambda, const int* __restrict__ width,
+                                             const long long* __restrict__ start, const int* __restrict__ acol,
+                                             double* __restrict__ aval) {
+    const int slice = blockIdx.x;
+    const int lane = threadIdx.x % kSlice, wave = threadIdx.x / kSlice;
+    const int i = slice * kSlice + lane;
+    if (i >= P.nf) return;
+    const int nfl = nFluid[i];
+    const long long pi = nPtrP[i];
+    const int ni = nCountP[i];
+    const int w = width[slice];
+    for (int k = wave; k < w; k += 4) {
+        if (k >= nfl) break;
+        const long long slot = (start[slice] + k) * kSlice + lane;
+        const int j = acol[slot];
+        const long long pj = nPtrP[j];
+        const int nj = nCountP[j];
+        double sum[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        int a = 0, c = 0;
+        while (a < ni && c < nj) {
+            const int ka = nIndP[pi + a], kc = nIndP[pj + c];
+            if (ka == kc) {
+                const double lk = lambda[ka];
+                const double* ci = &cval[3 * (pi + a)];
+                const double* cj = &cval[3 * (pj + c)];
+                for (int r = 0; r < 3; ++r)
+                    for (int s = 0; s < 3; ++s) sum[3 * r + s] += ci[r] * lk * cj[s];
+                a++;
+                c++;
+            } else if (ka < kc) {
+                a++;
+            } else {
+                c++;
+            }
+        }
+        for (int e9 = 0; e9 < 9; ++e9) {
+            const long long idx = ((start[slice] + k) * 9 + e9) * kSlice + lane;
+            aval[idx] += sum[e9];
+        }
+    }
+}
+
+}  // namespace
+
+void launch_kinematics(hipStream_t st, const DevParams& P, const WallKin& wk, const int* type, double* pos,
+                       double* vel, double* force) {
+    hipLaunchKernelGGL(k_kinematics, dim3(nblk(P.np, 256)), dim3(256), 0, st, P, wk, type, pos, vel, force);
+}
+
+void launch_densities(hipStream_t st, const DevParams& P, const int* type, const double* pos, const int* nCount,
+                      const long long* nPtr, const int* nInd, double* densA, double* gcen, double* volStrainP) {
+    hipLaunchKernelGGL(k_densities, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, type, pos, nCount, nPtr, nInd, densA,
+                       gcen, volStrainP);
+}
+
+void launch_coefficients(hipStream_t st, const DevParams& P, const int* type, const double* densA,
+                         const double* volStrainP, double* mass, double* kappa, double* lambda, double* mu,
+                         double* pressA) {
+    hipLaunchKernelGGL(k_coefficients, dim3(nblk(P.np, 256)), dim3(256), 0, st, P, type, densA, volStrainP, mass,
+                       kappa, lambda, mu, pressA);
+}
+
+void launch_forces(hipStream_t st, const DevParams& P, const int* type, const double* pos, const int* nCount,
+                   const long long* nPtr, const int* nInd, const double* pressA, const double* gcen,
+                   const double* mass, double* force, double* vel) {
+    hipLaunchKernelGGL(k_forces, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, type, pos, nCount, nPtr, nInd, pressA,
+                       gcen, mass, force, vel);
+}
+
+void launch_matrix_a(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
+                     const double* mass, const double* mu, const int* nCount, const int* nFluid,
+                     const long long* nPtr, const int* nInd, ASlices A, int* selfIdx, double* b) {
+    hipLaunchKernelGGL(k_matrix_a, dim3(nblk(P.nf, kSlice)), dim3(kSlice), 0, st, P, type, pos, vel, mass, mu,
+                       nCount, nFluid, nPtr, nInd, A.start, A.col, A.val, selfIdx, b);
+}
+
+void launch_matrix_c(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP,
+                     const long long* nPtrP, const int* nIndP, double* cval) {
+    hipLaunchKernelGGL(k_matrix_c, dim3(nblk(P.nf, 128)), dim3(128), 0, st, P, pos, nCountP, nPtrP, nIndP, cval);
+}
+
+void launch_vector_p(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
+                     const double* kappa, const double* lambda, const double* volStrainP, const int* nCountP,
+                     const long long* nPtrP, const int* nIndP, double* vecP) {
+    hipLaunchKernelGGL(k_vector_p, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, type, pos, vel, kappa, lambda,
+                       volStrainP, nCountP, nPtrP, nIndP, vecP);
+}
+
+void launch_b_minus_cp(hipStream_t st, const DevParams& P, const int* nCountP, const long long* nPtrP,
+                       const int* nIndP, const double* cval, const double* vecP, double* b) {
+    hipLaunchKernelGGL(k_b_minus_cp, dim3(nblk(P.nf, 128)), dim3(128), 0, st, P, nCountP, nPtrP, nIndP, cval, vecP,
+                       b);
+}
+
+void launch_cac(hipStream_t st, const DevParams& P, const int* nFluid, const int* nCountP, const long long* nPtrP,
+                const int* nIndP, const double* cval, const double* lambda, ASlices A) {
+    if (A.nslices == 0) return;
+    hipLaunchKernelGGL(k_cac, dim3(A.nslices), dim3(256), 0, st, P, nFluid, nCountP, nPtrP, nIndP, cval, lambda,
+                       A.width, A.start, A.col, A.val);
+}
+
+}  // namespace mph

@@ -1,0 +1,1018 @@
+// mph_scene.cpp -- host side of the HIP time stepper: scene state, phase drivers and
+// the C ABI of include/mph_abi.h.
+//
+// One scene = one HIP stream and persistent device buffers that only grow (the
+// reference mallocs and frees every CSR, multigrid and neighbour array each step).
+// Per step the host reads back one small record (neighbour totals, A slots, fluid
+// bounding box) and, inside the solve, one scalar per CR iteration (rr, for the
+// stop rule rr/rr0 < tol of source/main.cpp:4346).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mph_abi.h"
+#include "mph_kernels.h"
+
+using namespace mph;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(MPH_E_DEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_));         \
+    } while (0)
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;  // elements
+    int ensure(size_t n) {
+        if (n <= cap && p) return MPH_OK;
+        size_t want = n + n / 4 + 64;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipMalloc((void**)&p, want * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return fail(MPH_E_NOMEM, "device allocation of " + std::to_string(want * sizeof(T)) + " bytes failed");
+        }
+        cap = want;
+        return MPH_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct HostLevel {
+    Level L{};
+    DBuf<double> A, invD, S0, S1, R, Q;
+};
+
+enum { kRR0 = 0, kRS0 = 1, kRR = 2, kRHO = 3, kRHOP = 4, kTMP = 5, kRS = 6 };
+
+}  // namespace
+
+struct mph_scene {
+    mph_params prm{};
+    DevParams P{};
+    double dmax[3] = {0, 0, 0};
+    int device = 0;
+    hipStream_t st = nullptr;
+    // host-side state of the reference's globals
+    double time = 0, outputNext = 0, vtkOutputNext = 0;
+    int iStep = 0;
+    bool vtkStep = false;
+    WallKin wk{};
+    double Swa = 1, Swg = 1, Swp = 1, Swv = 1;
+    // particles
+    DBuf<int> type, type2, perm, perm2;
+    DBuf<double> pos, pos2, vel, vel2, force, mass, kappa, lambda, mu, densA, gcen, pressA, volStrainP, divP,
+        pressP, virialP, virialStress;
+    // cells
+    DBuf<uint64_t> keys;
+    DBuf<int> cellIndex, cellStart, bbox;
+    // neighbours
+    DBuf<int> nCount, nFluid, nCountP, nInd, nIndP;
+    DBuf<long long> nPtr, nPtrP, scanScratch;
+    long long E = 0, EP = 0;
+    // A and C
+    int nslices = 0;
+    DBuf<int> width, selfIdx, acol;
+    DBuf<long long> sliceStart;
+    DBuf<double> aval, b, cval, vecP;
+    long long slots = 0;
+    // multigrid
+    int hbbox[6] = {0, 0, 0, 0, 0, 0};
+    int depth = 0;
+    std::vector<HostLevel> lv;
+    DBuf<double> invDiagA;
+    // CR
+    DBuf<double> x, r, s, y, p, q, u, part, scal;
+    double* hpin = nullptr;  // pinned host scalars
+    int crIter = 0;
+    double rr0 = 0, rr = 0, rs0 = 0, rs = 0;
+    // timing
+    bool timing = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double msPhase[12] = {0};
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// constants, as initializeWeight/Fluid/Wall/Domain (source/main.cpp:886-1162)
+
+double weight_prefactor(double Sw, double h, int dim) { return dim == 2 ? 1.0 / Sw * 1.0 / (h * h) : 1.0 / Sw * 1.0 / (h * h * h); }
+
+double lattice_sum(double l, double radius, int dim, double K, bool kernelA) {
+    const int range = (int)std::ceil(radius / l);
+    const int rz = dim == 2 ? 0 : range;
+    double sum = 0.0;
+    for (int ix = -range; ix <= range; ++ix)
+        for (int iy = -range; iy <= range; ++iy)
+            for (int iz = -rz; iz <= rz; ++iz) {
+                if (ix == 0 && iy == 0 && iz == 0) continue;
+                const double x = l * ((double)ix), y = l * ((double)iy), z = l * ((double)iz);
+                const double r2 = x * x + y * y + z * z;
+                if (r2 <= radius * radius) {
+                    const double r = std::sqrt(r2);
+                    sum += kernelA ? wa_k(K, r, radius) : wq_k(K, r, radius);
+                }
+            }
+    return sum;
+}
+
+int init_constants(mph_scene& S, const mph_grid& g) {
+    const mph_params& p = S.prm;
+    DevParams& P = S.P;
+    const int dim = p.dim;
+    const double l = g.spacing;
+    P.dim = dim;
+    P.np = g.np;
+    P.l = l;
+    P.V = dim == 2 ? l * l : l * l * l;
+    P.dt = p.dt;
+    P.radA = p.radius_ratio_a * l;
+    P.radG = p.radius_ratio_a * l;  // RadiusRatioG = RadiusRatioA (main.cpp:888)
+    P.radP = p.radius_ratio_p * l;
+    P.radV = p.radius_ratio_v * l;
+    if (dim == 2) {
+        S.Swa = 1.0 / 2.0 * 2.0 / 15.0 * M_PI / l / l;
+        S.Swg = 1.0 / 2.0 * 1.0 / 3.0 * M_PI / l / l;
+        S.Swp = 1.0 / 2.0 * 1.0 / 3.0 * M_PI / l / l;
+        S.Swv = 1.0 / 2.0 * 1.0 / 3.0 * M_PI / l / l;
+        P.R2g = 1.0 / 2.0 * 1.0 / 30.0 * M_PI * P.radG * P.radG / l / l / S.Swg;
+    } else {
+        S.Swa = 1.0 / 3.0 * 1.0 / 5.0 * M_PI / l / l / l;
+        S.Swg = 1.0 / 3.0 * 2.0 / 5.0 * M_PI / l / l / l;
+        S.Swp = 1.0 / 3.0 * 2.0 / 5.0 * M_PI / l / l / l;
+        S.Swv = 1.0 / 3.0 * 2.0 / 5.0 * M_PI / l / l / l;
+        P.R2g = 1.0 / 3.0 * 4.0 / 105.0 * M_PI * P.radG * P.radG / l / l / l / S.Swg;
+    }
+    P.Ka = weight_prefactor(S.Swa, P.radA, dim);
+    P.Kg = weight_prefactor(S.Swg, P.radG, dim);
+    P.Kp = weight_prefactor(S.Swp, P.radP, dim);
+    P.Kv = weight_prefactor(S.Swv, P.radV, dim);
+    P.invA = 1.0 / P.radA;
+    P.m2G = -2.0 / P.radG;
+    P.m2P = -2.0 / P.radP;
+    P.m2V = -2.0 / P.radV;
+    P.N0a = lattice_sum(l, P.radA, dim, P.Ka, true);
+    P.N0p = lattice_sum(l, P.radP, dim, P.Kp, false);
+    P.radA2 = P.radA * P.radA;
+    P.radG2 = P.radG * P.radG;
+    P.radP2 = P.radP * P.radP;
+    P.radV2 = P.radV * P.radV;
+    // initializeFluid (988-1021)
+    double integN, integX;
+    if (dim == 2) { P.cofK = 0.350778153; integN = 0.024679383; integX = 0.226126699; }
+    else          { P.cofK = 0.326976006; integN = 0.021425779; integX = 0.233977488; }
+    for (int t = 0; t < 4; ++t) {
+        P.cofA[t] = p.surface_tension[t] / ((P.radG / l) * (integN + P.cofK * P.cofK * integX));
+        P.ai[t] = P.cofA[t] * (P.cofK) * (P.cofK);
+        P.density[t] = p.density[t];
+        P.bulk[t] = p.bulk_modulus[t];
+        P.bulkVisc[t] = p.bulk_viscosity[t];
+        P.shear[t] = p.shear_viscosity[t];
+        for (int u = 0; u < 4; ++u) P.ratio[t][u] = p.interaction_ratio[t][u];
+    }
+    P.visc = dim == 2 ? 8.0 : 10.0;
+    for (int d = 0; d < 3; ++d) P.gravity[d] = p.gravity[d];
+    // initializeWall (1023-1062)
+    for (int w = 2; w < 4; ++w) {
+        const double* om = p.wall_omega[w];
+        double n[3] = {0.0, 0.0, 0.0};
+        const double theta = std::fabs(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]);
+        if (theta != 0.0)
+            for (int d = 0; d < 3; ++d) n[d] = om[d] / theta;
+        const double q0 = n[0] * std::sin(theta * p.dt / 2.0), q1 = n[1] * std::sin(theta * p.dt / 2.0),
+                     q2 = n[2] * std::sin(theta * p.dt / 2.0), q3 = std::cos(theta * p.dt / 2.0);
+        double(&R)[3][3] = S.wk.R[w];
+        R[0][0] = q0 * q0 - q1 * q1 - q2 * q2 + q3 * q3;
+        R[0][1] = 2.0 * (q0 * q1 - q2 * q3);
+        R[0][2] = 2.0 * (q0 * q2 + q1 * q3);
+        R[1][0] = 2.0 * (q0 * q1 + q2 * q3);
+        R[1][1] = -q0 * q0 + q1 * q1 - q2 * q2 + q3 * q3;
+        R[1][2] = 2.0 * (q1 * q2 - q0 * q3);
+        R[2][0] = 2.0 * (q0 * q2 - q1 * q3);
+        R[2][1] = 2.0 * (q1 * q2 + q0 * q3);
+        R[2][2] = -q0 * q0 - q1 * q1 + q2 * q2 + q3 * q3;
+        for (int d = 0; d < 3; ++d) {
+            S.wk.omega[w][d] = om[d];
+            S.wk.vel[w][d] = p.wall_velocity[w][d];
+            S.wk.center[w][d] = p.wall_center[w][d];
+        }
+    }
+    // initializeDomain (1064-1111)
+    double m = 0.0;
+    m = (P.radA > m) ? P.radA : m;
+    m = (2.0 * P.radP > m) ? 2.0 * P.radP : m;
+    m = (P.radV > m) ? P.radV : m;
+    P.maxR = m;
+    P.maxR2 = m * m;
+    for (int d = 0; d < 3; ++d) {
+        P.dmin[d] = g.domain_min[d];
+        P.W[d] = g.domain_max[d] - g.domain_min[d];
+        P.hW[d] = 0.5 * P.W[d];
+    }
+    double cc[3];
+    cc[0] = std::floor((g.domain_max[0] - g.domain_min[0]) / m);
+    cc[1] = std::floor((g.domain_max[1] - g.domain_min[1]) / m);
+    cc[2] = dim == 2 ? 1 : std::floor((g.domain_max[2] - g.domain_min[2]) / m);
+    for (int d = 0; d < 3; ++d) P.C[d] = (int)cc[d];
+    for (int d = 0; d < (dim == 2 ? 2 : 3); ++d)
+        if (P.C[d] < 3)
+            return fail(MPH_E_STATE, "domain narrower than 3 neighbour cells along axis " + std::to_string(d) +
+                                         " (the reference's 27-cell search would visit cells twice)");
+    P.T = cc[0] * cc[1] * cc[2];
+    for (int d = 0; d < 3; ++d) P.cw[d] = P.W[d] / P.C[d];
+    P.range[0] = (int)std::ceil(P.maxR / P.cw[0]);
+    P.range[1] = (int)std::ceil(P.maxR / P.cw[1]);
+    P.range[2] = dim == 2 ? 0 : (int)std::ceil(P.maxR / P.cw[2]);
+    for (int d = 0; d < 3; ++d)
+        if (2 * P.range[d] + 1 > 3)
+            return fail(MPH_E_STATE, "neighbour range exceeds one cell (assert of main.cpp:1402)");
+    // PowerParticleCount (640-645)
+    P.p2pow = 0;
+    while ((P.np >> P.p2pow) != 0) ++P.p2pow;
+    P.p2 = 1 << P.p2pow;
+    return MPH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// phases
+
+ASlices aslices(mph_scene& S) {
+    ASlices A;
+    A.nslices = S.nslices;
+    A.width = S.width.p;
+    A.start = S.sliceStart.p;
+    A.col = S.acol.p;
+    A.val = S.aval.p;
+    A.val32 = nullptr;
+    return A;
+}
+
+int phase_kinematics(mph_scene& S) {
+    launch_kinematics(S.st, S.P, S.wk, S.type.p, S.pos.p, S.vel.p, S.force.p);
+    for (int w = 2; w < 4; ++w)  // WallCenter += WallVelocity*Dt (main.cpp:2123-2127)
+        for (int d = 0; d < 3; ++d) S.wk.center[w][d] += S.wk.vel[w][d] * S.P.dt;
+    return MPH_OK;
+}
+
+int phase_cells(mph_scene& S) {
+    const DevParams& P = S.P;
+    launch_hash(S.st, P, S.pos.p, S.type.p, S.keys.p);
+    launch_bitonic(S.st, P, S.keys.p);
+    launch_cell_start(S.st, P, S.keys.p, S.cellStart.p);
+    launch_reorder(S.st, P, S.keys.p, S.type.p, S.perm.p, S.pos.p, S.vel.p, S.type2.p, S.perm2.p, S.pos2.p, S.vel2.p,
+                   S.cellIndex.p);
+    std::swap(S.type, S.type2);
+    std::swap(S.perm, S.perm2);
+    std::swap(S.pos, S.pos2);
+    std::swap(S.vel, S.vel2);
+    // fluid bounding box for the multigrid depth (main.cpp:2779-2802)
+    const int init[6] = {0, 0, 0, P.C[0], P.C[1], P.C[2]};
+    HIPCHK(hipMemcpyAsync(S.bbox.p, init, sizeof init, hipMemcpyHostToDevice, S.st));
+    launch_fluid_bbox(S.st, P, S.cellStart.p, S.bbox.p);
+    return MPH_OK;
+}
+
+int phase_neighbor(mph_scene& S) {
+    const DevParams& P = S.P;
+    launch_neighbor_count(S.st, P, S.pos.p, S.cellIndex.p, S.cellStart.p, S.nCount.p, S.nFluid.p, S.nCountP.p);
+    launch_scan_i64(S.st, S.nCount.p, S.nPtr.p, P.np, S.scanScratch.p, (int)S.scanScratch.cap);
+    launch_scan_i64(S.st, S.nCountP.p, S.nPtrP.p, P.np, S.scanScratch.p, (int)S.scanScratch.cap);
+    launch_slice_width(S.st, P.nf, S.nslices, S.nFluid.p, S.width.p);
+    launch_scan_i64(S.st, S.width.p, S.sliceStart.p, S.nslices, S.scanScratch.p, (int)S.scanScratch.cap);
+    long long* h = (long long*)S.hpin;
+    HIPCHK(hipMemcpyAsync(&h[0], S.nPtr.p + P.np, 8, hipMemcpyDeviceToHost, S.st));
+    HIPCHK(hipMemcpyAsync(&h[1], S.nPtrP.p + P.np, 8, hipMemcpyDeviceToHost, S.st));
+    HIPCHK(hipMemcpyAsync(&h[2], S.sliceStart.p + S.nslices, 8, hipMemcpyDeviceToHost, S.st));
+    HIPCHK(hipMemcpyAsync(&h[3], S.bbox.p, 6 * sizeof(int), hipMemcpyDeviceToHost, S.st));
+    HIPCHK(hipStreamSynchronize(S.st));
+    S.E = h[0];
+    S.EP = h[1];
+    S.slots = h[2] * kSlice;
+    std::memcpy(S.hbbox, &h[3], 6 * sizeof(int));
+    int rc;
+    if ((rc = S.nInd.ensure(S.E > 0 ? S.E : 1)) || (rc = S.nIndP.ensure(S.EP > 0 ? S.EP : 1))) return rc;
+    launch_neighbor_fill(S.st, P, S.pos.p, S.cellIndex.p, S.cellStart.p, S.nCount.p, S.nPtr.p, S.nPtrP.p, S.nInd.p,
+                         S.nIndP.p);
+    return MPH_OK;
+}
+
+int phase_explicit(mph_scene& S) {
+    const DevParams& P = S.P;
+    launch_densities(S.st, P, S.type.p, S.pos.p, S.nCount.p, S.nPtr.p, S.nInd.p, S.densA.p, S.gcen.p, S.volStrainP.p);
+    launch_coefficients(S.st, P, S.type.p, S.densA.p, S.volStrainP.p, S.mass.p, S.kappa.p, S.lambda.p, S.mu.p,
+                        S.pressA.p);
+    launch_forces(S.st, P, S.type.p, S.pos.p, S.nCount.p, S.nPtr.p, S.nInd.p, S.pressA.p, S.gcen.p, S.mass.p,
+                  S.force.p, S.vel.p);
+    return MPH_OK;
+}
+
+int phase_matrix_a(mph_scene& S) {
+    int rc;
+    if ((rc = S.acol.ensure(S.slots > 0 ? S.slots : 1)) || (rc = S.aval.ensure(S.slots > 0 ? 9 * S.slots : 1)))
+        return rc;
+    launch_matrix_a(S.st, S.P, S.type.p, S.pos.p, S.vel.p, S.mass.p, S.mu.p, S.nCount.p, S.nFluid.p, S.nPtr.p,
+                    S.nInd.p, aslices(S), S.selfIdx.p, S.b.p);
+    return MPH_OK;
+}
+
+int phase_matrix_c(mph_scene& S) {
+    int rc;
+    if ((rc = S.cval.ensure(3 * (S.EP > 0 ? S.EP : 1)))) return rc;
+    launch_matrix_c(S.st, S.P, S.pos.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p);
+    launch_vector_p(S.st, S.P, S.type.p, S.pos.p, S.vel.p, S.kappa.p, S.lambda.p, S.volStrainP.p, S.nCountP.p,
+                    S.nPtrP.p, S.nIndP.p, S.vecP.p);
+    return MPH_OK;
+}
+
+int phase_multiply_c(mph_scene& S) {
+    launch_b_minus_cp(S.st, S.P, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.vecP.p, S.b.p);
+    launch_cac(S.st, S.P, S.nFluid.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.lambda.p, aslices(S));
+    return MPH_OK;
+}
+
+// calculateMultiGridDepth (2804-2850) on the bbox read back with the neighbour totals
+int phase_multigrid(mph_scene& S) {
+    if (!S.prm.multigrid) return MPH_OK;
+    const int* bb = S.hbbox;  // max x,y,z then min x,y,z
+    double mgc = 0.0;
+    if ((bb[0] - bb[3]) > mgc) mgc = (bb[0] - bb[3] + 1);
+    if ((bb[1] - bb[4]) > mgc) mgc = (bb[1] - bb[4] + 1);
+    if ((bb[2] - bb[5]) > mgc) mgc = (bb[2] - bb[5] + 1);
+    int depth = 0;
+    while ((1 << depth) <= mgc) depth++;
+    if (depth < 2)
+        return fail(MPH_E_STATE, "multigrid depth < 2: all fluid in one cell (the reference V-cycle reads level -1)");
+    S.depth = depth;
+    if ((int)S.lv.size() < depth) S.lv.resize(depth);
+    int rc;
+    for (int l = 0; l < depth; ++l) {
+        HostLevel& H = S.lv[l];
+        Level& L = H.L;
+        for (int d = 0; d < 3; ++d) {
+            L.mn[d] = bb[3 + d] >> l;
+            L.mx[d] = (bb[d] >> l) + 1;
+            L.cnt[d] = L.mx[d] - L.mn[d];
+            L.cs[d] = 3 < L.cnt[d] ? 3 : L.cnt[d];
+        }
+        L.cells = L.cnt[0] * L.cnt[1] * L.cnt[2];
+        L.nrow = 3 * L.cells;
+        L.nshift = 3 * L.cs[0] * L.cs[1] * L.cs[2];
+        if ((rc = H.A.ensure((size_t)L.nrow * L.nshift)) || (rc = H.invD.ensure(L.nrow)) ||
+            (rc = H.S0.ensure(L.nrow)) || (rc = H.S1.ensure(L.nrow)) || (rc = H.R.ensure(L.nrow)) ||
+            (rc = H.Q.ensure(L.nrow)))
+            return rc;
+        L.A = H.A.p;
+        L.invD = H.invD.p;
+        L.S[0] = H.S0.p;
+        L.S[1] = H.S1.p;
+        L.R = H.R.p;
+        L.Q = H.Q.p;
+        L.cur = 0;
+    }
+    launch_inv_diag(S.st, S.P, S.selfIdx.p, aslices(S), S.invDiagA.p);
+    launch_mg_galerkin0(S.st, S.P, S.lv[0].L, S.cellStart.p, S.nFluid.p, S.nPtr.p, S.nInd.p, aslices(S));
+    for (int l = 1; l < depth; ++l) launch_mg_galerkinL(S.st, S.lv[l - 1].L, S.lv[l].L);
+    for (int l = 0; l < depth; ++l) launch_mg_invd(S.st, S.lv[l].L);
+    return MPH_OK;
+}
+
+// preconditionWithMultiGrid (4122-4266): out = D^-1 in + G2P V(P2G in); `part`
+// receives the chunk partials of in.out when non-null
+void precondition(mph_scene& S, const double* in, double* out, double* part) {
+    hipStream_t st = S.st;
+    const int D = S.depth;
+    Level& L0 = S.lv[0].L;
+    launch_mg_p2g(st, S.P, L0, S.cellStart.p, in);
+    launch_mg_begin(st, L0);
+    launch_mg_update(st, L0);  // pre-smooth x2: {S += invD R; R = Q - A S}
+    launch_mg_residual_update(st, L0);
+    launch_mg_residual(st, L0);
+    for (int l = 1; l < D - 1; ++l) {
+        Level& L = S.lv[l].L;
+        launch_mg_restrict(st, S.lv[l - 1].L, L);
+        launch_mg_begin(st, L);
+        launch_mg_update(st, L);  // pre-smooth x3
+        launch_mg_residual_update(st, L);
+        launch_mg_residual_update(st, L);
+        launch_mg_residual(st, L);
+    }
+    {
+        Level& L = S.lv[D - 1].L;
+        launch_mg_restrict(st, S.lv[D - 2].L, L);
+        launch_mg_begin(st, L);
+        launch_mg_update(st, L);  // x4, then one more S += invD R
+        for (int k = 0; k < 4; ++k) launch_mg_residual_update(st, L);
+        launch_mg_prolong(st, L, S.lv[D - 2].L);
+    }
+    for (int l = D - 2; l > 0; --l) {
+        Level& L = S.lv[l].L;
+        for (int k = 0; k < 3; ++k) launch_mg_residual_update(st, L);  // post-smooth x3
+        launch_mg_prolong(st, L, S.lv[l - 1].L);
+    }
+    for (int k = 0; k < 2; ++k) launch_mg_residual_update(st, L0);  // post-smooth x2
+    launch_mg_g2p(st, S.P, L0, S.cellIndex.p, S.invDiagA.p, in, out, part);
+}
+
+// without MULTIGRID_SOLVER the preconditioner is a copy (main.cpp:4331, 4341, 4369)
+int apply_m(mph_scene& S, const double* in, double* out, double* part) {
+    if (S.prm.multigrid) {
+        precondition(S, in, out, part);
+    } else {
+        HIPCHK(hipMemcpyAsync(out, in, sizeof(double) * 3 * (size_t)S.P.nf, hipMemcpyDeviceToDevice, S.st));
+        if (part) launch_dot_partial(S.st, S.P, in, out, part);
+    }
+    return MPH_OK;
+}
+
+int read_scalar(mph_scene& S, int slot, double* v) {
+    HIPCHK(hipMemcpyAsync(S.hpin + 8, S.scal.p + slot, 8, hipMemcpyDeviceToHost, S.st));
+    HIPCHK(hipStreamSynchronize(S.st));
+    *v = S.hpin[8];
+    return MPH_OK;
+}
+
+// solveWithConjugatedGradient (4270-4421)
+int phase_solve(mph_scene& S) {
+    const DevParams& P = S.P;
+    hipStream_t st = S.st;
+    const long long n = 3LL * P.nf;
+    const ASlices A = aslices(S);
+    launch_cr_init(st, P, S.vel.p, S.force.p, S.mass.p, S.x.p);
+    for (double* v : {S.r.p, S.s.p, S.y.p, S.p.p, S.q.p, S.u.p}) launch_fill(st, v, n, -1.0);
+    if (S.prm.multigrid)
+        for (int l = 0; l < S.depth; ++l) {
+            Level& L = S.lv[l].L;
+            for (double* v : {L.S[0], L.S[1], L.R, L.Q}) launch_fill(st, v, L.nrow, -1.0);
+            L.cur = 0;
+        }
+    int rc;
+    // s = M^-1 b; rs0 = b.s; rr0 = b.b
+    if ((rc = apply_m(S, S.b.p, S.s.p, S.part.p))) return rc;
+    launch_dot_final(st, P, S.part.p, S.scal.p, kRS0, -1);
+    launch_dot_partial(st, P, S.b.p, S.b.p, S.part.p);
+    launch_dot_final(st, P, S.part.p, S.scal.p, kRR0, -1);
+    // r = b - A x; s = M^-1 r
+    launch_spmv(st, P, A, S.nFluid.p, -1.0, S.x.p, 1.0, S.b.p, S.r.p, nullptr, nullptr, nullptr, nullptr);
+    if ((rc = apply_m(S, S.r.p, S.s.p, nullptr))) return rc;
+    launch_dot_partial(st, P, S.r.p, S.r.p, S.part.p);
+    launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1);
+    if ((rc = read_scalar(S, kRR0, &S.rr0))) return rc;
+    const long long maxit = S.prm.max_iterations > 0 ? S.prm.max_iterations : n;
+    long long iter;
+    for (iter = 0; iter < maxit; ++iter) {
+        if ((rc = read_scalar(S, kRR, &S.rr))) return rc;
+        if (S.rr / S.rr0 < S.prm.cr_tolerance) break;
+        if (iter == 0) {  // y = A s; p = s; q = A p (same sum); rho = s.y
+            launch_spmv(st, P, A, S.nFluid.p, 1.0, S.s.p, 0.0, S.y.p, S.y.p, S.q.p, S.p.p, S.s.p, S.part.p);
+            launch_dot_final(st, P, S.part.p, S.scal.p, kRHO, kRHOP);
+        } else {
+            launch_spmv(st, P, A, S.nFluid.p, 1.0, S.s.p, 0.0, S.y.p, S.y.p, nullptr, nullptr, S.s.p, S.part.p);
+            launch_dot_final(st, P, S.part.p, S.scal.p, kRHO, kRHOP);
+            launch_pq_update(st, P, S.scal.p, S.s.p, S.y.p, S.p.p, S.q.p);
+        }
+        if ((rc = apply_m(S, S.q.p, S.u.p, S.part.p))) return rc;  // u = M^-1 q; tmp = q.u
+        launch_dot_final(st, P, S.part.p, S.scal.p, kTMP, -1);
+        launch_xrs_update(st, P, S.scal.p, S.p.p, S.q.p, S.u.p, S.x.p, S.r.p, S.s.p, S.part.p);
+        launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1);
+    }
+    launch_dot_partial(st, P, S.r.p, S.s.p, S.part.p);
+    launch_dot_final(st, P, S.part.p, S.scal.p, kRS, -1);
+    launch_store_velocity(st, P, S.x.p, S.vel.p);
+    HIPCHK(hipMemcpyAsync(S.hpin + 16, S.scal.p, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    S.crIter = (int)iter;
+    S.rr0 = S.hpin[16 + kRR0];
+    S.rs0 = S.hpin[16 + kRS0];
+    S.rr = S.hpin[16 + kRR];
+    S.rs = S.hpin[16 + kRS];
+    return MPH_OK;
+}
+
+int phase_diagnostics(mph_scene& S) {
+    launch_diagnostics(S.st, S.P, S.type.p, S.pos.p, S.vel.p, S.mu.p, S.lambda.p, S.kappa.p, S.volStrainP.p,
+                       S.pressA.p, S.gcen.p, S.nCount.p, S.nPtr.p, S.nInd.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p,
+                       S.divP.p, S.pressP.p, S.force.p, S.virialStress.p, S.virialP.p);
+    return MPH_OK;
+}
+
+int run_phase(mph_scene& S, int ph) {
+    if (S.timing) (void)hipEventRecord(S.ev[0], S.st);
+    int rc;
+    switch (ph) {
+        case MPH_PHASE_KINEMATICS: rc = phase_kinematics(S); break;
+        case MPH_PHASE_CELLS: rc = phase_cells(S); break;
+        case MPH_PHASE_NEIGHBOR: rc = phase_neighbor(S); break;
+        case MPH_PHASE_EXPLICIT: rc = phase_explicit(S); break;
+        case MPH_PHASE_MATRIX_A: rc = phase_matrix_a(S); break;
+        case MPH_PHASE_MATRIX_C: rc = phase_matrix_c(S); break;
+        case MPH_PHASE_MULTIPLY_C: rc = phase_multiply_c(S); break;
+        case MPH_PHASE_MULTIGRID: rc = phase_multigrid(S); break;
+        case MPH_PHASE_SOLVE: rc = phase_solve(S); break;
+        case MPH_PHASE_DIAGNOSTICS: rc = phase_diagnostics(S); break;
+        case MPH_PHASE_ADVANCE:
+            S.time += S.P.dt;
+            S.iStep++;
+            rc = MPH_OK;
+            break;
+        default: return fail(MPH_E_ARG, "unknown phase " + std::to_string(ph));
+    }
+    if (rc) return rc;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(MPH_E_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+    if (S.timing) {
+        (void)hipEventRecord(S.ev[1], S.st);
+        HIPCHK(hipEventSynchronize(S.ev[1]));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, S.ev[0], S.ev[1]);
+        if (ph >= 0 && ph < 12) S.msPhase[ph] += ms;
+    }
+    return MPH_OK;
+}
+
+int allocate(mph_scene& S) {
+    const DevParams& P = S.P;
+    const size_t np = P.np, nf = P.nf;
+    int rc;
+#define ENS(buf, n) \
+    if ((rc = (buf).ensure((n) > 0 ? (n) : 1))) return rc
+    ENS(S.type, np); ENS(S.type2, np); ENS(S.perm, np); ENS(S.perm2, np);
+    ENS(S.pos, 3 * np); ENS(S.pos2, 3 * np); ENS(S.vel, 3 * np); ENS(S.vel2, 3 * np); ENS(S.force, 3 * np);
+    ENS(S.mass, np); ENS(S.kappa, np); ENS(S.lambda, np); ENS(S.mu, np); ENS(S.densA, np); ENS(S.gcen, 3 * np);
+    ENS(S.pressA, np); ENS(S.volStrainP, np); ENS(S.divP, np); ENS(S.pressP, np); ENS(S.virialP, np);
+    ENS(S.virialStress, 9 * np);
+    ENS(S.keys, (size_t)P.p2); ENS(S.cellIndex, np); ENS(S.cellStart, 2 * (size_t)P.T + 2); ENS(S.bbox, 8);
+    ENS(S.nCount, np); ENS(S.nFluid, np); ENS(S.nCountP, np); ENS(S.nPtr, np + 1); ENS(S.nPtrP, np + 1);
+    ENS(S.scanScratch, (size_t)scan_scratch_len((int)np + 1) + 64);
+    S.nslices = (int)((nf + kSlice - 1) / kSlice);
+    ENS(S.width, (size_t)S.nslices); ENS(S.sliceStart, (size_t)S.nslices + 1); ENS(S.selfIdx, nf);
+    ENS(S.b, 3 * nf); ENS(S.vecP, np); ENS(S.invDiagA, 3 * nf);
+    ENS(S.x, 3 * nf); ENS(S.r, 3 * nf); ENS(S.s, 3 * nf); ENS(S.y, 3 * nf); ENS(S.p, 3 * nf); ENS(S.q, 3 * nf);
+    ENS(S.u, 3 * nf); ENS(S.part, nf / kChunk + 2); ENS(S.scal, 16);
+#undef ENS
+    HIPCHK(hipHostMalloc((void**)&S.hpin, 64 * sizeof(double), hipHostMallocDefault));
+    // zero-fill the per-particle state the reference allocates uninitialised
+    for (DBuf<double>* d : {&S.force, &S.mass, &S.kappa, &S.lambda, &S.mu, &S.densA, &S.gcen, &S.pressA,
+                            &S.volStrainP, &S.divP, &S.pressP, &S.virialP, &S.virialStress, &S.b, &S.vecP})
+        HIPCHK(hipMemsetAsync(d->p, 0, d->cap * sizeof(double), S.st));
+    return MPH_OK;
+}
+
+void destroy(mph_scene* S) {
+    if (!S) return;
+    if (S->st) (void)hipStreamSynchronize(S->st);
+    for (DBuf<int>* b : {&S->type, &S->type2, &S->perm, &S->perm2, &S->cellIndex, &S->cellStart, &S->bbox,
+                         &S->nCount, &S->nFluid, &S->nCountP, &S->nInd, &S->nIndP, &S->width, &S->selfIdx, &S->acol})
+        b->release();
+    for (DBuf<double>* b : {&S->pos, &S->pos2, &S->vel, &S->vel2, &S->force, &S->mass, &S->kappa, &S->lambda,
+                            &S->mu, &S->densA, &S->gcen, &S->pressA, &S->volStrainP, &S->divP, &S->pressP,
+                            &S->virialP, &S->virialStress, &S->aval, &S->b, &S->cval, &S->vecP, &S->invDiagA, &S->x,
+                            &S->r, &S->s, &S->y, &S->p, &S->q, &S->u, &S->part, &S->scal})
+        b->release();
+    for (DBuf<long long>* b : {&S->nPtr, &S->nPtrP, &S->scanScratch, &S->sliceStart}) b->release();
+    S->keys.release();
+    for (HostLevel& H : S->lv)
+        for (DBuf<double>* b : {&H.A, &H.invD, &H.S0, &H.S1, &H.R, &H.Q}) b->release();
+    if (S->hpin) (void)hipHostFree(S->hpin);
+    for (hipEvent_t& e : S->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (S->st) (void)hipStreamDestroy(S->st);
+    delete S;
+}
+
+// ---------------------------------------------------------------------------
+// field access (arrays in the reference's layouts, current sorted order)
+
+template <class T>
+int d2h(mph_scene& S, std::vector<T>& out, const T* src, size_t n) {
+    out.resize(n);
+    if (n) HIPCHK(hipMemcpyAsync(out.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, S.st));
+    HIPCHK(hipStreamSynchronize(S.st));
+    return MPH_OK;
+}
+
+struct Blob {
+    std::vector<char> bytes;
+    int esize = 0;
+};
+
+template <class T>
+void put(Blob& b, const std::vector<T>& v) {
+    b.esize = sizeof(T);
+    b.bytes.resize(v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(b.bytes.data(), v.data(), b.bytes.size());
+}
+
+struct DevField {
+    void* p;
+    size_t count;
+    int esize;
+};
+
+DevField device_field(mph_scene& S, const std::string& n) {
+    const size_t np = S.P.np, nf = S.P.nf;
+    auto F = [](auto* p, size_t c) { return DevField{(void*)p, c, (int)sizeof(*p)}; };
+    if (n == "Position") return F(S.pos.p, 3 * np);
+    if (n == "Velocity") return F(S.vel.p, 3 * np);
+    if (n == "Force") return F(S.force.p, 3 * np);
+    if (n == "Property") return F(S.type.p, np);
+    if (n == "ParticleIndex") return F(S.perm.p, np);
+    if (n == "Mass") return F(S.mass.p, np);
+    if (n == "Kappa") return F(S.kappa.p, np);
+    if (n == "Lambda") return F(S.lambda.p, np);
+    if (n == "Mu") return F(S.mu.p, np);
+    if (n == "DensityA") return F(S.densA.p, np);
+    if (n == "GravityCenter") return F(S.gcen.p, 3 * np);
+    if (n == "PressureA") return F(S.pressA.p, np);
+    if (n == "VolStrainP") return F(S.volStrainP.p, np);
+    if (n == "DivergenceP") return F(S.divP.p, np);
+    if (n == "PressureP") return F(S.pressP.p, np);
+    if (n == "VirialPressureAtParticle") return F(S.virialP.p, np);
+    if (n == "VirialStressAtParticle") return F(S.virialStress.p, 9 * np);
+    if (n == "NeighborCount") return F(S.nCount.p, np);
+    if (n == "NeighborFluidCount") return F(S.nFluid.p, np);
+    if (n == "NeighborCountP") return F(S.nCountP.p, np);
+    if (n == "NeighborPtr") return F(S.nPtr.p, np + 1);
+    if (n == "NeighborPtrP") return F(S.nPtrP.p, np + 1);
+    if (n == "NeighborInd") return F(S.nInd.p, (size_t)S.E);
+    if (n == "NeighborIndP") return F(S.nIndP.p, (size_t)S.EP);
+    if (n == "VectorB") return F(S.b.p, 3 * nf);
+    if (n == "VectorP") return F(S.vecP.p, np);
+    if (n == "InvDiagA") return F(S.invDiagA.p, 3 * nf);
+    return DevField{nullptr, 0, 0};
+}
+
+// derived fields assembled on the host from the device layouts
+int host_field(mph_scene& S, const std::string& n, Blob& out, bool& found) {
+    found = true;
+    const int np = S.P.np, nf = S.P.nf, T = S.P.T;
+    int rc;
+    if (n == "CellIndex" || n == "CellParticle") {
+        std::vector<uint64_t> k;
+        if ((rc = d2h(S, k, S.keys.p, (size_t)S.P.p2))) return rc;
+        std::vector<int> v(k.size());
+        for (size_t i = 0; i < k.size(); ++i) v[i] = n == "CellIndex" ? (int)(k[i] >> 32) : (int)(uint32_t)k[i];
+        put(out, v);
+        return MPH_OK;
+    }
+    if (n.rfind("Cell", 0) == 0 &&
+        (n.find("ParticleBegin") != std::string::npos || n.find("ParticleEnd") != std::string::npos)) {
+        std::vector<int> cs;
+        if ((rc = d2h(S, cs, S.cellStart.p, 2 * (size_t)T + 1))) return rc;
+        std::vector<int> v(T);
+        const bool wall = n.find("Wall") != std::string::npos, end = n.find("End") != std::string::npos;
+        for (int c = 0; c < T; ++c) v[c] = cs[(wall ? T : 0) + c + (end ? 1 : 0)];
+        put(out, v);
+        return MPH_OK;
+    }
+    if (n == "CsrPtrA" || n == "CsrIndA" || n == "CsrCofA" || n == "AVal" || n == "ACol") {
+        std::vector<int> nfl, col;
+        std::vector<long long> start;
+        std::vector<double> val;
+        if ((rc = d2h(S, nfl, S.nFluid.p, nf)) || (rc = d2h(S, start, S.sliceStart.p, (size_t)S.nslices + 1)))
+            return rc;
+        if ((rc = d2h(S, col, S.acol.p, (size_t)S.slots)) || (rc = d2h(S, val, S.aval.p, 9 * (size_t)S.slots)))
+            return rc;
+        if (n == "AVal") { put(out, val); return MPH_OK; }
+        if (n == "ACol") { put(out, col); return MPH_OK; }
+        std::vector<long long> ptr(3 * (size_t)nf + 1, 0);
+        for (int i = 0; i < nf; ++i)
+            for (int r = 0; r < 3; ++r) ptr[3 * i + r + 1] = ptr[3 * i + r] + 3LL * nfl[i];
+        if (n == "CsrPtrA") { put(out, ptr); return MPH_OK; }
+        const size_t nnz = ptr.back();
+        std::vector<int> ind(nnz);
+        std::vector<double> cof(nnz);
+        for (int i = 0; i < nf; ++i) {
+            const int sl = i / kSlice, lane = i % kSlice;
+            for (int k = 0; k < nfl[i]; ++k) {
+                const long long slot = (start[sl] + k) * kSlice + lane;
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) {
+                        const long long at = ptr[3 * i + r] + 3LL * k + c;
+                        ind[at] = 3 * col[slot] + c;
+                        cof[at] = val[((start[sl] + k) * 9 + 3 * r + c) * kSlice + lane];
+                    }
+            }
+        }
+        if (n == "CsrIndA") put(out, ind);
+        else put(out, cof);
+        return MPH_OK;
+    }
+    if (n == "CsrPtrC" || n == "CsrIndC" || n == "CsrCofC") {
+        std::vector<int> cnt, ind;
+        std::vector<long long> ptrP;
+        std::vector<double> cv;
+        if ((rc = d2h(S, cnt, S.nCountP.p, np)) || (rc = d2h(S, ptrP, S.nPtrP.p, (size_t)np + 1)) ||
+            (rc = d2h(S, ind, S.nIndP.p, (size_t)S.EP)) || (rc = d2h(S, cv, S.cval.p, 3 * (size_t)S.EP)))
+            return rc;
+        std::vector<long long> ptr(3 * (size_t)nf + 1, 0);
+        for (int i = 0; i < nf; ++i)
+            for (int r = 0; r < 3; ++r) ptr[3 * i + r + 1] = ptr[3 * i + r] + cnt[i];
+        if (n == "CsrPtrC") { put(out, ptr); return MPH_OK; }
+        std::vector<int> ci(ptr.back());
+        std::vector<double> cc(ptr.back());
+        for (int i = 0; i < nf; ++i)
+            for (int r = 0; r < 3; ++r)
+                for (int k = 0; k < cnt[i]; ++k) {
+                    ci[ptr[3 * i + r] + k] = ind[ptrP[i] + k];
+                    cc[ptr[3 * i + r] + k] = cv[3 * (ptrP[i] + k) + r];
+                }
+        if (n == "CsrIndC") put(out, ci);
+        else put(out, cc);
+        return MPH_OK;
+    }
+    if (n.rfind("MGCofA_L", 0) == 0 || n.rfind("MGInvD_L", 0) == 0) {
+        const int l = atoi(n.c_str() + 8);
+        if (l < 0 || l >= S.depth) return fail(MPH_E_FIELD, "no multigrid level " + std::to_string(l));
+        const Level& L = S.lv[l].L;
+        if (n[2] == 'I') {
+            std::vector<double> v;
+            if ((rc = d2h(S, v, L.invD, (size_t)L.nrow))) return rc;
+            put(out, v);
+            return MPH_OK;
+        }
+        std::vector<double> t;
+        if ((rc = d2h(S, t, L.A, (size_t)L.nrow * L.nshift))) return rc;
+        std::vector<double> v(t.size());
+        for (int row = 0; row < L.nrow; ++row)
+            for (int sh = 0; sh < L.nshift; ++sh) v[(size_t)row * L.nshift + sh] = t[(size_t)sh * L.nrow + row];
+        put(out, v);
+        return MPH_OK;
+    }
+    found = false;
+    return MPH_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+const char* mph_last_error(void) { return g_err.c_str(); }
+
+void mph_params_default(mph_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->dim = 3;
+    p->dt = 1.0e100;
+    p->cr_tolerance = 1.0e-12;
+    p->multigrid = 1;
+    p->precision = MPH_PRECISION_FP64;
+    p->max_iterations = 0;
+}
+
+int mph_scene_create(const mph_params* p, const mph_grid* g, int device, mph_scene** out) {
+    if (!p || !g || !out) return fail(MPH_E_ARG, "null argument");
+    *out = nullptr;
+    if (p->dim != 2 && p->dim != 3) return fail(MPH_E_ARG, "dim must be 2 or 3");
+    if (g->np <= 0 || !g->type || !g->pos || !g->vel) return fail(MPH_E_ARG, "empty particle set");
+    if (p->precision != MPH_PRECISION_FP64) return fail(MPH_E_NOTIMPL, "only FP64 precision is built");
+    int nf = 0;
+    for (int i = 0; i < g->np; ++i) {
+        if (g->type[i] < 0 || g->type[i] > 3) return fail(MPH_E_ARG, "particle type outside 0..3");
+        nf += g->type[i] < 2;
+    }
+    if (nf == 0) return fail(MPH_E_STATE, "no fluid particle");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(MPH_E_DEVICE, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(MPH_E_ARG, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    mph_scene* S = new mph_scene();
+    S->prm = *p;
+    S->device = device;
+    int rc = init_constants(*S, *g);
+    if (rc) { delete S; return rc; }
+    S->P.nf = nf;
+    if (hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking) != hipSuccess) {
+        delete S;
+        return fail(MPH_E_DEVICE, "stream creation failed");
+    }
+    (void)hipEventCreate(&S->ev[0]);
+    (void)hipEventCreate(&S->ev[1]);
+    if ((rc = allocate(*S))) { destroy(S); return rc; }
+    const size_t np = g->np;
+    std::vector<int> perm(np);
+    for (size_t i = 0; i < np; ++i) perm[i] = (int)i;
+    if (hipMemcpyAsync(S->type.p, g->type, np * sizeof(int), hipMemcpyHostToDevice, S->st) != hipSuccess ||
+        hipMemcpyAsync(S->perm.p, perm.data(), np * sizeof(int), hipMemcpyHostToDevice, S->st) != hipSuccess ||
+        hipMemcpyAsync(S->pos.p, g->pos, 3 * np * sizeof(double), hipMemcpyHostToDevice, S->st) != hipSuccess ||
+        hipMemcpyAsync(S->vel.p, g->vel, 3 * np * sizeof(double), hipMemcpyHostToDevice, S->st) != hipSuccess ||
+        hipStreamSynchronize(S->st) != hipSuccess) {
+        destroy(S);
+        return fail(MPH_E_DEVICE, "initial copy failed");
+    }
+    S->time = g->time;
+    for (int d = 0; d < 3; ++d) S->dmax[d] = g->domain_max[d];
+    // init sequence, main.cpp:369-375 (the VTK of output.vtk is the caller's)
+    if ((rc = run_phase(*S, MPH_PHASE_CELLS)) || (rc = run_phase(*S, MPH_PHASE_NEIGHBOR))) {
+        destroy(S);
+        return rc;
+    }
+    launch_densities(S->st, S->P, S->type.p, S->pos.p, S->nCount.p, S->nPtr.p, S->nInd.p, S->densA.p, S->gcen.p,
+                     S->volStrainP.p);
+    if (hipStreamSynchronize(S->st) != hipSuccess) {
+        destroy(S);
+        return fail(MPH_E_DEVICE, "init sequence failed");
+    }
+    S->iStep = (int)(S->time / S->P.dt);
+    S->outputNext = S->time;
+    S->vtkOutputNext = S->time;
+    *out = S;
+    return MPH_OK;
+}
+
+int mph_scene_run_phase(mph_scene* S, int phase) {
+    if (!S) return fail(MPH_E_ARG, "null scene");
+    HIPCHK(hipSetDevice(S->device));
+    int rc = run_phase(*S, phase);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(S->st));
+    if (phase == MPH_PHASE_DIAGNOSTICS) S->vtkStep = true;
+    return MPH_OK;
+}
+
+int mph_scene_step(mph_scene* S, int nsteps, mph_step_stats* st) {
+    if (!S) return fail(MPH_E_ARG, "null scene");
+    HIPCHK(hipSetDevice(S->device));
+    mph_step_stats local{};
+    for (int k = 0; k < 12; ++k) S->msPhase[k] = 0;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    (void)hipEventRecord(t0, S->st);
+    int rc = MPH_OK;
+    for (int k = 0; k < nsteps && rc == MPH_OK; ++k) {
+        const double dt = S->P.dt;
+        if (!(S->time < S->prm.end_time + 1.0e-5 * dt)) break;
+        if (S->time + 1.0e-5 * dt >= S->outputNext) S->outputNext += S->prm.output_interval;
+        for (int ph = MPH_PHASE_KINEMATICS; ph <= MPH_PHASE_SOLVE && rc == MPH_OK; ++ph) rc = run_phase(*S, ph);
+        if (rc) break;
+        S->vtkStep = false;
+        if (S->time + 1.0e-5 * dt >= S->vtkOutputNext) {
+            if ((rc = run_phase(*S, MPH_PHASE_DIAGNOSTICS))) break;
+            S->vtkOutputNext += S->prm.vtk_output_interval;
+            S->vtkStep = true;
+            local.vtk_steps++;
+        }
+        run_phase(*S, MPH_PHASE_ADVANCE);
+        local.steps_done++;
+        local.cr_iterations = S->crIter;
+        local.cr_iterations_total += S->crIter;
+    }
+    (void)hipEventRecord(t1, S->st);
+    (void)hipEventSynchronize(t1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    local.ms_total = ms;
+    local.rr0 = S->rr0; local.rr = S->rr; local.rs0 = S->rs0; local.rs = S->rs;
+    for (int k = 0; k < 12; ++k) local.ms_phase[k] = S->msPhase[k];
+    if (st) *st = local;
+    return rc;
+}
+
+int mph_scene_field_size(mph_scene* S, const char* field, size_t* bytes, int* esize) {
+    if (!S || !field) return fail(MPH_E_ARG, "null argument");
+    DevField f = device_field(*S, field);
+    if (f.esize) {
+        if (bytes) *bytes = f.count * f.esize;
+        if (esize) *esize = f.esize;
+        return MPH_OK;
+    }
+    Blob b;
+    bool found;
+    int rc = host_field(*S, field, b, found);
+    if (rc) return rc;
+    if (!found) return fail(MPH_E_FIELD, std::string("unknown field ") + field);
+    if (bytes) *bytes = b.bytes.size();
+    if (esize) *esize = b.esize;
+    return MPH_OK;
+}
+
+int mph_scene_read(mph_scene* S, const char* field, void* host, size_t bytes) {
+    if (!S || !field || (!host && bytes)) return fail(MPH_E_ARG, "null argument");
+    HIPCHK(hipSetDevice(S->device));
+    DevField f = device_field(*S, field);
+    if (f.esize) {
+        if (bytes != f.count * f.esize) return fail(MPH_E_ARG, std::string("size mismatch for ") + field);
+        if (bytes) HIPCHK(hipMemcpyAsync(host, f.p, bytes, hipMemcpyDeviceToHost, S->st));
+        HIPCHK(hipStreamSynchronize(S->st));
+        return MPH_OK;
+    }
+    Blob b;
+    bool found;
+    int rc = host_field(*S, field, b, found);
+    if (rc) return rc;
+    if (!found) return fail(MPH_E_FIELD, std::string("unknown field ") + field);
+    if (bytes != b.bytes.size()) return fail(MPH_E_ARG, std::string("size mismatch for ") + field);
+    if (bytes) std::memcpy(host, b.bytes.data(), bytes);
+    return MPH_OK;
+}
+
+int mph_scene_write(mph_scene* S, const char* field, const void* host, size_t bytes) {
+    if (!S || !field || (!host && bytes)) return fail(MPH_E_ARG, "null argument");
+    HIPCHK(hipSetDevice(S->device));
+    const std::string n = field;
+    DevField f = device_field(*S, n);
+    if (!f.esize) return fail(MPH_E_FIELD, std::string("field is not writable: ") + field);
+    if (n == "Property") {  // types decide the fluid count; they must not change it
+        const int* t = (const int*)host;
+        int nf = 0;
+        for (size_t i = 0; i < bytes / sizeof(int); ++i) {
+            if (t[i] < 0 || t[i] > 3) return fail(MPH_E_ARG, "particle type outside 0..3");
+            nf += t[i] < 2;
+        }
+        if (nf != S->P.nf) return fail(MPH_E_ARG, "writing Property may not change the fluid count");
+    }
+    if (bytes != f.count * f.esize) return fail(MPH_E_ARG, std::string("size mismatch for ") + field);
+    if (bytes) HIPCHK(hipMemcpyAsync(f.p, host, bytes, hipMemcpyHostToDevice, S->st));
+    HIPCHK(hipStreamSynchronize(S->st));
+    return MPH_OK;
+}
+
+int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
+    if (!S || !name || !v) return fail(MPH_E_ARG, "null argument");
+    const std::string n = name;
+    const DevParams& P = S->P;
+    if (n == "Time") *v = S->time;
+    else if (n == "Dt") *v = P.dt;
+    else if (n == "iStep") *v = S->iStep;
+    else if (n == "OutputNext") *v = S->outputNext;
+    else if (n == "VtkOutputNext") *v = S->vtkOutputNext;
+    else if (n == "VtkStep") *v = S->vtkStep;
+    else if (n == "ParticleCount") *v = P.np;
+    else if (n == "FluidCount") *v = P.nf;
+    else if (n == "PowerParticleCount") *v = P.p2;
+    else if (n == "TotalCellCount") *v = P.T;
+    else if (n == "CellCount0") *v = P.C[0];
+    else if (n == "CellCount1") *v = P.C[1];
+    else if (n == "CellCount2") *v = P.C[2];
+    else if (n == "N0a") *v = P.N0a;
+    else if (n == "N0p") *v = P.N0p;
+    else if (n == "Swa") *v = S->Swa;
+    else if (n == "Swg") *v = S->Swg;
+    else if (n == "Swp") *v = S->Swp;
+    else if (n == "Swv") *v = S->Swv;
+    else if (n == "R2g") *v = P.R2g;
+    else if (n == "MaxRadius") *v = P.maxR;
+    else if (n == "NeighborIndCount") *v = (double)S->E;
+    else if (n == "NeighborIndCountP") *v = (double)S->EP;
+    else if (n == "ASlots") *v = (double)S->slots;
+    else if (n == "MultiGridDepth") *v = S->depth;
+    else if (n == "CRIterations") *v = S->crIter;
+    else if (n == "rr0") *v = S->rr0;
+    else if (n == "rr") *v = S->rr;
+    else if (n == "rs0") *v = S->rs0;
+    else if (n == "rs") *v = S->rs;
+    else return fail(MPH_E_FIELD, "unknown scalar " + n);
+    return MPH_OK;
+}
+
+int mph_scene_domain(mph_scene* S, double* spacing, double dmin[3], double dmax[3]) {
+    if (!S || !spacing || !dmin || !dmax) return fail(MPH_E_ARG, "null argument");
+    *spacing = S->P.l;
+    for (int d = 0; d < 3; ++d) {
+        dmin[d] = S->P.dmin[d];
+        dmax[d] = S->dmax[d];
+    }
+    return MPH_OK;
+}
+
+int mph_scene_set_timing(mph_scene* S, int on) {
+    if (!S) return fail(MPH_E_ARG, "null scene");
+    S->timing = on != 0;
+    return MPH_OK;
+}
+
+int mph_scene_destroy(mph_scene* S) {
+    if (!S) return MPH_OK;
+    (void)hipSetDevice(S->device);
+    destroy(S);
+    return MPH_OK;
+}
+
+}  // extern "C"

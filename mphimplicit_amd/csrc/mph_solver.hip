@@ -1,0 +1,755 @@
+// mph_solver.hip -- multigrid hierarchy, V-cycle and conjugate-residual kernels.
+//
+// Replaces calculateMultiGridMatrix and its builders (source/main.cpp:2761-3546),
+// preconditionWithMultiGrid (4122-4266) and solveWithConjugatedGradient (4270-4421).
+// The reference materialises P2G, G2P, R, P and the level matrices as CSR; their
+// patterns are fixed stencils on the fluid bounding box, so here they are applied
+// implicitly (no index arrays), with the reference's per-row summation order.
+// Dot products use one fixed tree (chunks of 256 particles, then 256 lanes over the
+// chunk sums), independent of launch configuration; the CPU oracle implements the
+// same tree so the solve can be compared bit for bit.
+#include "mph_kernels.h"
+
+namespace mph {
+
+namespace {
+
+inline int nblk(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+__device__ inline int cell_id(const DevParams& P, int cx, int cy, int cz) {
+    const int* C = P.C;
+    if (P.dim == 2) return ((cx) % C[0] + C[0]) % C[0] * C[1] + ((cy) % C[1] + C[1]) % C[1];
+    return ((cx) % C[0] + C[0]) % C[0] * C[1] * C[2] + ((cy) % C[1] + C[1]) % C[1] * C[2] +
+           ((cz) % C[2] + C[2]) % C[2];
+}
+
+__device__ inline void grid_xyz(const Level& L, int g, int& gx, int& gy, int& gz) {
+    gx = (g / (L.cnt[1] * L.cnt[2])) % L.cnt[0];
+    gy = (g / L.cnt[2]) % L.cnt[1];
+    gz = g % L.cnt[2];
+}
+__device__ inline int grid_id(const Level& L, int gx, int gy, int gz) {
+    return (L.cnt[1] * L.cnt[2]) * gx + L.cnt[2] * gy + gz;
+}
+
+// pairwise tree over the 256 values of a workgroup (t[l] += t[l+h], h = 128..1);
+// the result is valid in thread 0
+__device__ inline double block_tree_sum(double t, double* sh) {
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    if (threadIdx.x < 128) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + 128];
+    __syncthreads();
+    double v = 0.0;
+    if (threadIdx.x < 64) {
+        v = sh[threadIdx.x] + sh[threadIdx.x + 64];
+        for (int h = 32; h >= 1; h >>= 1) v = v + __shfl_down(v, h, 64);
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// hierarchy build
+
+// bounding box of fluid-occupied cells (calculateMultiGridDepth, 2786-2802)
+__global__ void k_fluid_bbox(DevParams P, const int* __restrict__ cellStart, int* __restrict__ bbox) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= P.T) return;
+    if (cellStart[c + 1] - cellStart[c] > 0) {
+        const int cx = (c / (P.C[1] * P.C[2])) % P.C[0], cy = (c / P.C[2]) % P.C[1], cz = c % P.C[2];
+        atomicMax(&bbox[0], cx);
+        atomicMax(&bbox[1], cy);
+        atomicMax(&bbox[2], cz);
+        atomicMin(&bbox[3], cx);
+        atomicMin(&bbox[4], cy);
+        atomicMin(&bbox[5], cz);
+    }
+}
+
+// InvDiagA = 1/a_ii (calculateInvDiagA, 3045-3069)
+__global__ void k_inv_diag(DevParams P, const int* __restrict__ selfIdx, const long long* __restrict__ start,
+                           const double* __restrict__ aval, double* __restrict__ invDiagA) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nf) return;
+    const int s = i / kSlice, lane = i % kSlice, k = selfIdx[i];
+    for (int r = 0; r < 3; ++r) {
+        const double v = aval[((start[s] + k) * 9 + 4 * r) * kSlice + lane];
+        invDiagA[3 * i + r] = v != 0.0 ? 1.0 / v : 0.0;
+    }
+}
+
+// level-0 matrix = sum of the particle blocks of every cell pair, in the order of
+// calculateMultiGridCsrA (3351-3410): particles of the row cell ascending, then the
+// blocks of each particle ascending.  Thread = (cell g, stencil cell c); the blocks
+// of a particle that fall in one neighbour cell are a contiguous run of its sorted
+// fluid-neighbour list, located by binary search.
+__global__ void k_mg_galerkin0(DevParams P, Level L, const int* __restrict__ cellStart,
+                               const int* __restrict__ nFluid, const long long* __restrict__ nPtr,
+                               const int* __restrict__ nInd, const long long* __restrict__ start,
+                               const double* __restrict__ aval) {
+    const int ncs = L.cs[0] * L.cs[1] * L.cs[2];
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)L.cells * ncs) return;
+    const int g = (int)(t / ncs), c = (int)(t % ncs);
+    int gx, gy, gz;
+    grid_xyz(L, g, gx, gy, gz);
+    const int cx = gx + L.mn[0], cy = gy + L.mn[1], cz = gz + L.mn[2];
+    double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int ic = cell_id(P, cx, cy, cz);
+    const int p0 = cellStart[ic], p1 = cellStart[ic + 1];
+    // neighbour cells of the row cell in ascending CellId (as the neighbour search)
+    int nb[27];
+    int nnb = 0;
+    {
+        const int* C = P.C;
+        const int rx = P.range[0], ry = P.range[1], rz = P.range[2];
+        int jx[3], jy[3], jz[3];
+        for (int k = 0; k < 2 * rx + 1; ++k) jx[k] = ((cx - rx + k) % C[0] + C[0]) % C[0];
+        for (int k = 0; k < 2 * ry + 1; ++k) jy[k] = ((cy - ry + k) % C[1] + C[1]) % C[1];
+        for (int k = 0; k < 2 * rz + 1; ++k) jz[k] = ((cz - rz + k) % C[2] + C[2]) % C[2];
+        const int bx = (2 * rx) - (cx + rx) % C[0], by = (2 * ry) - (cy + ry) % C[1], bz = (2 * rz) - (cz + rz) % C[2];
+        const int x0 = bx > 0 ? bx : 0, y0 = by > 0 ? by : 0, z0 = bz > 0 ? bz : 0;
+        for (int a = x0; a < x0 + (2 * rx + 1); ++a)
+            for (int b = y0; b < y0 + (2 * ry + 1); ++b)
+                for (int cc = z0; cc < z0 + (2 * rz + 1); ++cc) {
+                    const int ax = jx[a % (2 * rx + 1)], ay = jy[b % (2 * ry + 1)], az = jz[cc % (2 * rz + 1)];
+                    const int jc = cell_id(P, ax, ay, az);
+                    if (cellStart[jc] == cellStart[jc + 1]) continue;  // no fluid there
+                    const int dx = (ax - L.mn[0]) - gx, dy = (ay - L.mn[1]) - gy, dz = (az - L.mn[2]) - gz;
+                    const int ix = (dx + 1 + L.cnt[0]) % L.cnt[0];
+                    const int iy = (dy + 1 + L.cnt[1]) % L.cnt[1];
+                    const int iz = (dz + 1 + L.cnt[2]) % L.cnt[2];
+                    if (L.cs[2] * L.cs[1] * ix + L.cs[2] * iy + iz == c) nb[nnb++] = jc;
+                }
+    }
+    for (int ip = p0; ip < p1 && nnb > 0; ++ip) {
+        const long long base = nPtr[ip];
+        const int nfl = nFluid[ip];
+        const int s = ip / kSlice, lane = ip % kSlice;
+        const long long sb = start[s];
+        for (int q = 0; q < nnb; ++q) {
+            const int jlo = cellStart[nb[q]], jhi = cellStart[nb[q] + 1];
+            int lo = 0, hi = nfl;  // first list position with j >= jlo
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (nInd[base + mid] < jlo) lo = mid + 1;
+                else hi = mid;
+            }
+            for (int k = lo; k < nfl && nInd[base + k] < jhi; ++k) {
+                const double* v = aval + ((sb + k) * 9) * kSlice + lane;
+                for (int e = 0; e < 9; ++e) acc[e] += v[e * kSlice];
+            }
+        }
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int sd = 0; sd < 3; ++sd) L.A[(long long)(3 * c + sd) * L.nrow + 3 * g + r] = acc[3 * r + sd];
+}
+
+// coarse level = R A_fine P by summation (3414-3507): thread = (coarse cell, stencil
+// cell); children in R order, then fine stencil entries in shift order.
+__global__ void k_mg_galerkinL(Level S, Level L) {
+    const int ncs = L.cs[0] * L.cs[1] * L.cs[2];
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)L.cells * ncs) return;
+    const int g = (int)(t / ncs), c = (int)(t % ncs);
+    int lx, ly, lz;
+    grid_xyz(L, g, lx, ly, lz);
+    const int lcx = lx + L.mn[0], lcy = ly + L.mn[1], lcz = lz + L.mn[2];
+    const int fmx = S.mn[0] > 2 * lcx, fmy = S.mn[1] > 2 * lcy, fmz = S.mn[2] > 2 * lcz;
+    const int fMx = S.mx[0] < 2 * (lcx + 1), fMy = S.mx[1] < 2 * (lcy + 1), fMz = S.mx[2] < 2 * (lcz + 1);
+    double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int ax = 0; ax < 2 - fmx - fMx; ++ax)
+        for (int ay = 0; ay < 2 - fmy - fMy; ++ay)
+            for (int az = 0; az < 2 - fmz - fMz; ++az) {
+                const int sgx = (2 * lcx + ax + fmx) - S.mn[0];
+                const int sgy = (2 * lcy + ay + fmy) - S.mn[1];
+                const int sgz = (2 * lcz + az + fmz) - S.mn[2];
+                const int srow0 = 3 * grid_id(S, sgx, sgy, sgz);
+                for (int ex = 0; ex < S.cs[0]; ++ex)
+                    for (int ey = 0; ey < S.cs[1]; ++ey)
+                        for (int ez = 0; ez < S.cs[2]; ++ez) {
+                            const int jsx = (sgx + ex - 1 + S.cnt[0]) % S.cnt[0];
+                            const int jsy = (sgy + ey - 1 + S.cnt[1]) % S.cnt[1];
+                            const int jsz = (sgz + ez - 1 + S.cnt[2]) % S.cnt[2];
+                            const int jlx = ((jsx + S.mn[0]) / 2) - L.mn[0];
+                            const int jly = ((jsy + S.mn[1]) / 2) - L.mn[1];
+                            const int jlz = ((jsz + S.mn[2]) / 2) - L.mn[2];
+                            const int ix = ((jlx - lx) + 1 + L.cnt[0]) % L.cnt[0];
+                            const int iy = ((jly - ly) + 1 + L.cnt[1]) % L.cnt[1];
+                            const int iz = ((jlz - lz) + 1 + L.cnt[2]) % L.cnt[2];
+                            if (L.cs[2] * L.cs[1] * ix + L.cs[2] * iy + iz != c) continue;
+                            const int cS = S.cs[2] * S.cs[1] * ex + S.cs[2] * ey + ez;
+                            for (int r = 0; r < 3; ++r)
+                                for (int sd = 0; sd < 3; ++sd)
+                                    acc[3 * r + sd] += S.A[(long long)(3 * cS + sd) * S.nrow + srow0 + r];
+                        }
+            }
+    for (int r = 0; r < 3; ++r)
+        for (int sd = 0; sd < 3; ++sd) L.A[(long long)(3 * c + sd) * L.nrow + 3 * g + r] = acc[3 * r + sd];
+}
+
+// invD = 2 / sum |a| per row (calculateMultiGridInvDiagA, 3510-3546)
+__global__ void k_mg_invd(Level L) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= L.nrow) return;
+    double sum = 0.0;
+    for (int sh = 0; sh < L.nshift; ++sh) sum += fabs(L.A[(long long)sh * L.nrow + row]);
+    L.invD[row] = sum != 0.0 ? 2.0 / sum : 0.0;
+}
+
+__global__ void k_fill(double* x, long long n, double v) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] = v;
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle (preconditionWithMultiGrid).  Every step is myDcsrmv / myDdmv /
+// myDcopy / myDscal of the reference: y = beta*y + sum_k (alpha*a_k)*x_k with the
+// sum formed first, y = beta*y + (alpha*d)*x.
+
+// P2G: Q0 = 0*Q0 + sum over the cell's fluid particles of q (2964-2997)
+__global__ void k_mg_p2g(DevParams P, Level L, const int* __restrict__ cellStart, const double* __restrict__ q) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= L.nrow) return;
+    const int g = row / 3, rd = row % 3;
+    int gx, gy, gz;
+    grid_xyz(L, g, gx, gy, gz);
+    const int ic = cell_id(P, gx + L.mn[0], gy + L.mn[1], gz + L.mn[2]);
+    double sum = 0.0;
+    for (int ip = cellStart[ic]; ip < cellStart[ic + 1]; ++ip) sum += 1.0 * 1.0 * q[3 * ip + rd];
+    double y = L.Q[row];
+    y *= 0.0;
+    y += sum;
+    L.Q[row] = y;
+}
+
+__global__ void k_mg_begin(Level L) {  // myDscal(0.0, S); myDcopy(Q, R)
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= L.nrow) return;
+    double* S = L.S[L.cur];
+    S[row] *= 0.0;
+    L.R[row] = L.Q[row];
+}
+
+__global__ void k_mg_update(Level L) {  // myDdmv(1.0, invD, R, 1.0, S)
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= L.nrow) return;
+    double* S = L.S[L.cur];
+    double y = S[row];
+    y *= 1.0;
+    y += 1.0 * L.invD[row] * L.R[row];
+    S[row] = y;
+}
+
+__device__ inline double mg_row_sum(const Level& L, const double* __restrict__ x, int row) {
+    const int g = row / 3;
+    int gx, gy, gz;
+    grid_xyz(L, g, gx, gy, gz);
+    double sum = 0.0;
+    int sh = 0;
+    for (int ex = 0; ex < L.cs[0]; ++ex) {
+        const int jx = (gx + ex - 1 + L.cnt[0]) % L.cnt[0];
+        for (int ey = 0; ey < L.cs[1]; ++ey) {
+            const int jy = (gy + ey - 1 + L.cnt[1]) % L.cnt[1];
+            for (int ez = 0; ez < L.cs[2]; ++ez) {
+                const int jz = (gz + ez - 1 + L.cnt[2]) % L.cnt[2];
+                const int col = 3 * grid_id(L, jx, jy, jz);
+                for (int sd = 0; sd < 3; ++sd, ++sh)
+                    sum += -1.0 * L.A[(long long)sh * L.nrow + row] * x[col + sd];
+            }
+        }
+    }
+    return sum;
+}
+
+__global__ void k_mg_residual(Level L) {  // myDcopy(Q, R); myDcsrmv(-1.0, A, S, 1.0, R)
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= L.nrow) return;
+    const double sum = mg_row_sum(L, L.S[L.cur], row);
+    double y = L.Q[row];
+    y *= 1.0;
+    y += sum;
+    L.R[row] = y;
+}
+
+// residual then smoother update of the same row; the new iterate goes to the other
+// S buffer so every thread still reads the old one
+__global__ void k_mg_residual_update(Level L) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= L.nrow) return;
+    const double* So = L.S[L.cur];
+    const double sum = mg_row_sum(L, So, row);
+    double y = L.Q[row];
+    y *= 1.0;
+    y += sum;
+    L.R[row] = y;
+    double s = So[row];
+    s *= 1.0;
+    s += 1.0 * L.invD[row] * y;
+    L.S[L.cur ^ 1][row] = s;
+}
+
+// restriction (R of 3115-3175): Q_L = 0*Q_L + sum of the clipped 2x2x2 children of R_S
+__global__ void k_mg_restrict(Level S, Level L) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= L.nrow) return;
+    const int g = row / 3, rd = row % 3;
+    int lx, ly, lz;
+    grid_xyz(L, g, lx, ly, lz);
+    const int lcx = lx + L.mn[0], lcy = ly + L.mn[1], lcz = lz + L.mn[2];
+    const int fmx = S.mn[0] > 2 * lcx, fmy = S.mn[1] > 2 * lcy, fmz = S.mn[2] > 2 * lcz;
+    const int fMx = S.mx[0] < 2 * (lcx + 1), fMy = S.mx[1] < 2 * (lcy + 1), fMz = S.mx[2] < 2 * (lcz + 1);
+    double sum = 0.0;
+    for (int ax = 0; ax < 2 - fmx - fMx; ++ax)
+        for (int ay = 0; ay < 2 - fmy - fMy; ++ay)
+            for (int az = 0; az < 2 - fmz - fMz; ++az) {
+                const int sg = grid_id(S, (2 * lcx + ax + fmx) - S.mn[0], (2 * lcy + ay + fmy) - S.mn[1],
+                                       (2 * lcz + az + fmz) - S.mn[2]);
+                sum += 1.0 * 1.0 * S.R[3 * sg + rd];
+            }
+    double y = L.Q[row];
+    y *= 0.0;
+    y += sum;
+    L.Q[row] = y;
+}
+
+// prolongation (P of 3216-3241): S_S += injection of the parent's S_L
+__global__ void k_mg_prolong(Level L, Level S) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= S.nrow) return;
+    const int g = row / 3, rd = row % 3;
+    int sx, sy, sz;
+    grid_xyz(S, g, sx, sy, sz);
+    const int lg = grid_id(L, ((sx + S.mn[0]) / 2) - L.mn[0], ((sy + S.mn[1]) / 2) - L.mn[1],
+                           ((sz + S.mn[2]) / 2) - L.mn[2]);
+    double sum = 0.0;
+    sum += 1.0 * 1.0 * L.S[L.cur][3 * lg + rd];
+    double* Sv = S.S[S.cur];
+    double y = Sv[row];
+    y *= 1.0;
+    y += sum;
+    Sv[row] = y;
+}
+
+// u = D^-1 q (4144-4148) + G2P S0 (4260), and the chunk partial of q.u
+__global__ void __launch_bounds__(256) k_mg_g2p(DevParams P, Level L, const int* __restrict__ cellIndex,
+                                                const double* __restrict__ invDiagA, const double* __restrict__ q,
+                                                double* __restrict__ u, double* __restrict__ part) {
+    __shared__ double sh[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double t = 0.0;
+    if (i < P.nf) {
+        const int key = cellIndex[i];
+        const int cx = key / (P.C[1] * P.C[2]) % P.C[0], cy = key / (P.C[2]) % P.C[1], cz = key % P.C[2];
+        const int g = grid_id(L, cx - L.mn[0], cy - L.mn[1], cz - L.mn[2]);
+        const double* S0 = L.S[L.cur];
+        double uu[3], qq[3];
+        for (int r = 0; r < 3; ++r) {
+            qq[r] = q[3 * i + r];
+            double y = u[3 * i + r];
+            y *= 0.0;
+            y += 1.0 * invDiagA[3 * i + r] * qq[r];
+            double sum = 0.0;
+            sum += 1.0 * 1.0 * S0[3 * g + r];
+            y *= 1.0;
+            y += sum;
+            uu[r] = y;
+            u[3 * i + r] = y;
+        }
+        t = (qq[0] * uu[0] + qq[1] * uu[1]) + qq[2] * uu[2];
+    }
+    if (part) {
+        const double v = block_tree_sum(t, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// conjugate residual
+
+__global__ void k_cr_init(DevParams P, const double* __restrict__ vel, const double* __restrict__ force,
+                          const double* __restrict__ mass, double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nf) return;
+    for (int d = 0; d < 3; ++d) x[3 * i + d] = vel[3 * i + d] - force[3 * i + d] / mass[i] * P.dt;
+}
+
+// y = beta*yin + sum_k (alpha*a_k)*x_k over the sliced A (myDcsrmvForA, 4033-4050);
+// optional second output y2 = beta*y2 + sum (same sum), optional copy p = x, and the
+// chunk partial of dotx.y.  Workgroup = 4 slices = one 256-particle dot chunk.
+__global__ void __launch_bounds__(256) k_spmv(DevParams P, const int* __restrict__ nFluid,
+                                              const long long* __restrict__ start, const int* __restrict__ col,
+                                              const double* __restrict__ val, double alpha,
+                                              const double* __restrict__ x, double beta, const double* yin,
+                                              double* y, double* y2, double* p_copy,
+                                              const double* __restrict__ dotx, double* __restrict__ part) {
+    __shared__ double sh[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double t = 0.0;
+    if (i < P.nf) {
+        const int w = nFluid[i];
+        const int lane = i % kSlice;
+        const long long sb = start[i / kSlice];
+        const int* cp = col + sb * kSlice + lane;
+        const double* vp = val + sb * 9 * kSlice + lane;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int k = 0; k < w; ++k) {
+            const int j = cp[(long long)k * kSlice];
+            const double* v = vp + (long long)k * 9 * kSlice;
+            const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
+            s0 += alpha * v[0 * kSlice] * x0;
+            s0 += alpha * v[1 * kSlice] * x1;
+            s0 += alpha * v[2 * kSlice] * x2;
+            s1 += alpha * v[3 * kSlice] * x0;
+            s1 += alpha * v[4 * kSlice] * x1;
+            s1 += alpha * v[5 * kSlice] * x2;
+            s2 += alpha * v[6 * kSlice] * x0;
+            s2 += alpha * v[7 * kSlice] * x1;
+            s2 += alpha * v[8 * kSlice] * x2;
+        }
+        const double sm[3] = {s0, s1, s2};
+        double out[3];
+        for (int r = 0; r < 3; ++r) {
+            double yy = yin[3 * i + r];
+            yy *= beta;
+            yy += sm[r];
+            out[r] = yy;
+        }
+        if (y2)
+            for (int r = 0; r < 3; ++r) {
+                double yy = y2[3 * i + r];
+                yy *= beta;
+                yy += sm[r];
+                y2[3 * i + r] = yy;
+            }
+        for (int r = 0; r < 3; ++r) y[3 * i + r] = out[r];
+        if (p_copy)
+            for (int r = 0; r < 3; ++r) p_copy[3 * i + r] = x[3 * i + r];
+        if (part) {
+            const double d0 = dotx[3 * i], d1 = dotx[3 * i + 1], d2 = dotx[3 * i + 2];
+            t = (d0 * out[0] + d1 * out[1]) + d2 * out[2];
+        }
+    }
+    if (part) {
+        const double v = block_tree_sum(t, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_dot_partial(int nf, const double* __restrict__ x,
+                                                     const double* __restrict__ y, double* __restrict__ part) {
+    __shared__ double sh[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double t = 0.0;
+    if (i < nf) t = (x[3 * i] * y[3 * i] + x[3 * i + 1] * y[3 * i + 1]) + x[3 * i + 2] * y[3 * i + 2];
+    const double v = block_tree_sum(t, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// fold the chunk partials: lane l sums chunks l, l+256, ... from 0.0, then the tree.
+// scal[shift_slot] receives the previous scal[slot] first (rho -> rho_prev).
+__global__ void __launch_bounds__(256) k_dot_final(const double* __restrict__ part, int nchunks, double* scal,
+                                                   int slot, int shift_slot) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int c = threadIdx.x; c < nchunks; c += 256) acc = acc + part[c];
+    const double v = block_tree_sum(acc, sh);
+    if (threadIdx.x == 0) {
+        if (shift_slot >= 0) scal[shift_slot] = scal[slot];
+        scal[slot] = v;
+    }
+}
+
+enum { kRR0 = 0, kRS0 = 1, kRR = 2, kRHO = 3, kRHOP = 4, kTMP = 5, kRS = 6 };
+
+// p = beta*p + 1.0*s, q = beta*q + 1.0*y with beta = rho/rho_prev (4360-4364)
+__global__ void k_pq_update(int nf, const double* __restrict__ scal, const double* __restrict__ s,
+                            const double* __restrict__ y, double* __restrict__ p, double* __restrict__ q) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 3 * nf) return;
+    const double beta = scal[kRHO] / scal[kRHOP];
+    double pp = p[k];
+    pp *= beta;
+    pp += 1.0 * s[k];
+    p[k] = pp;
+    double qq = q[k];
+    qq *= beta;
+    qq += 1.0 * y[k];
+    q[k] = qq;
+}
+
+// alpha = rho/tmp; x += alpha p; r -= alpha q; s -= alpha u (4372-4375) + r.r partial
+__global__ void __launch_bounds__(256) k_xrs_update(int nf, const double* __restrict__ scal,
+                                                    const double* __restrict__ p, const double* __restrict__ q,
+                                                    const double* __restrict__ u, double* __restrict__ x,
+                                                    double* __restrict__ r, double* __restrict__ s,
+                                                    double* __restrict__ part) {
+    __shared__ double sh[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double t = 0.0;
+    if (i < nf) {
+        const double alpha = scal[kRHO] / scal[kTMP];
+        const double ma = -alpha;
+        double rr[3];
+        for (int d = 0; d < 3; ++d) {
+            const int k = 3 * i + d;
+            x[k] += alpha * p[k];
+            rr[d] = r[k] + ma * q[k];
+            r[k] = rr[d];
+            s[k] += ma * u[k];
+        }
+        t = (rr[0] * rr[0] + rr[1] * rr[1]) + rr[2] * rr[2];
+    }
+    const double v = block_tree_sum(t, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+__global__ void k_store_velocity(int nf, const double* __restrict__ x, double* __restrict__ vel) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < 3 * nf) vel[k] = x[k];
+}
+
+// ---------------------------------------------------------------------------
+// output-step diagnostics: calculateDivergenceP (1932-1969), calculatePressureP
+// (1971-2015), calculateViscosityV (2017-2063), calculateVirialStressAtParticle
+// (4494-4738)
+
+struct Rel {
+    double x[3];
+    double r2;
+};
+__device__ inline Rel rel(const DevParams& P, const double* __restrict__ pos, int i, int j) {
+    Rel o;
+    for (int d = 0; d < 3; ++d) o.x[d] = min_image(pos[3 * j + d], pos[3 * i + d], P.W[d], P.hW[d]);
+    o.r2 = (o.x[0] * o.x[0] + o.x[1] * o.x[1] + o.x[2] * o.x[2]);
+    return o;
+}
+
+__global__ void k_divergence(DevParams P, const double* __restrict__ pos, const double* __restrict__ vel,
+                             const double* __restrict__ lambda, const double* __restrict__ kappa,
+                             const double* __restrict__ volStrainP, const int* __restrict__ nCount,
+                             const long long* __restrict__ nPtr, const int* __restrict__ nInd,
+                             double* __restrict__ divP, double* __restrict__ pressP) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.np) return;
+    double sum = 0.0;
+    const long long b = nPtr[i], e = b + nCount[i];
+    for (long long k = b; k < e; ++k) {
+        const int j = nInd[k];
+        if (i == j) continue;
+        const Rel q = rel(P, pos, i, j);
+        if (q.r2 == 0.0) continue;
+        if (P.radP2 - q.r2 >= 0) {
+            const double r = sqrt(q.r2);
+            const double dw = dwq_k(P.Kp, P.m2P, r, P.radP);
+            const double ev[3] = {q.x[0] / r, q.x[1] / r, q.x[2] / r};
+            for (int d = 0; d < 3; ++d) {
+                const double u = vel[3 * j + d] - vel[3 * i + d];
+                sum -= u * ev[d] * dw;
+            }
+        }
+    }
+    divP[i] = sum;
+    double pp = -lambda[i] * sum;
+    if (volStrainP[i] > 0.0) pp += kappa[i] * volStrainP[i];
+    pressP[i] = pp;
+}
+
+__global__ void k_diag_forces(DevParams P, const int* __restrict__ type, const double* __restrict__ pos,
+                              const double* __restrict__ vel, const double* __restrict__ mu,
+                              const double* __restrict__ pressA, const double* __restrict__ gcen,
+                              const double* __restrict__ pressP, const int* __restrict__ nCount,
+                              const long long* __restrict__ nPtr, const int* __restrict__ nInd,
+                              const int* __restrict__ nCountP, const long long* __restrict__ nPtrP,
+                              const int* __restrict__ nIndP, double* __restrict__ force,
+                              double* __restrict__ virialStress, double* __restrict__ virialP) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.np) return;
+    const int ti = type[i];
+    const double V = P.V, VoL = P.V / P.l;
+    const double ppi = pressP[i], pai = pressA[i], mui = mu[i];
+    const double gi[3] = {gcen[3 * i], gcen[3 * i + 1], gcen[3 * i + 2]};
+    const double vi[3] = {vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]};
+    const double a_ = P.ai[ti];
+    double st1[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    {
+        const long long b = nPtrP[i], e = b + nCountP[i];
+        for (long long k = b; k < e; ++k) {  // pressureP virial (4512-4555)
+            const int j = nIndP[k];
+            if (i == j) continue;
+            const Rel q = rel(P, pos, i, j);
+            if (q.r2 == 0.0) continue;
+            if (P.radP2 - q.r2 > 0) {
+                const double r = sqrt(q.r2);
+                const double dw = dwq_k(P.Kp, P.m2P, r, P.radP);
+                const double gw[3] = {dw * q.x[0] / r, dw * q.x[1] / r, dw * q.x[2] / r};
+                double fij[3];
+                for (int d = 0; d < 3; ++d) fij[d] = (ppi) * gw[d] * V;
+                for (int a = 0; a < 3; ++a)
+                    for (int c = 0; c < 3; ++c) st1[3 * a + c] += 1.0 * fij[a] * q.x[c] / V;
+            }
+        }
+    }
+    double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0};
+    double sa[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, sv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, sg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const long long b = nPtr[i], e = b + nCount[i];
+    for (long long k = b; k < e; ++k) {
+        const int j = nInd[k];
+        if (i == j) continue;
+        const Rel q = rel(P, pos, i, j);
+        if (q.r2 == 0.0) continue;
+        const double r = sqrt(q.r2);
+        if (P.radP2 - q.r2 > 0) {  // pressure force (1983-2014)
+            const double dw = dwq_k(P.Kp, P.m2P, r, P.radP);
+            const double gw[3] = {dw * q.x[0] / r, dw * q.x[1] / r, dw * q.x[2] / r};
+            const double ppj = pressP[j];
+            for (int d = 0; d < 3; ++d) fp[d] += (ppi + ppj) * gw[d] * V;
+        }
+        if (P.radV2 - q.r2 > 0) {  // viscosity force (2036-2056) and virial (4620-4645)
+            const double dwij = -dwq_k(P.Kv, P.m2V, r, P.radV);
+            const double ev[3] = {q.x[0] / r, q.x[1] / r, q.x[2] / r};
+            const double u[3] = {vel[3 * j] - vi[0], vel[3 * j + 1] - vi[1], vel[3 * j + 2] - vi[2]};
+            const double muj = mu[j];
+            const double muij = 2.0 * (mui * muj) / (mui + muj);
+            double fij[3];
+            for (int d = 0; d < 3; ++d) {
+                fij[d] = P.visc * muij * (u[0] * ev[0] + u[1] * ev[1] + u[2] * ev[2]) * ev[d] * dwij / r * V;
+                fv[d] += fij[d];
+            }
+            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c) sv[3 * a + c] += 0.5 * fij[a] * q.x[c] / V;
+        }
+        if (P.radA2 - q.r2 > 0) {  // pressureA virial (4575-4592)
+            const double ratio = P.ratio[ti][type[j]];
+            const double dw = ratio * dwadr_k(P.Ka, P.invA, r, P.radA);
+            const double gw[3] = {dw * q.x[0] / r, dw * q.x[1] / r, dw * q.x[2] / r};
+            double fij[3];
+            for (int d = 0; d < 3; ++d) fij[d] = (pai) * gw[d] * V;
+            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c) sa[3 * a + c] += 1.0 * fij[a] * q.x[c] / V;
+        }
+        if (P.radG2 - q.r2 > 0) {  // diffuse-interface virial, both terms (4672-4716)
+            const double ratio = P.ratio[ti][type[j]];
+            const double weight = ratio * wq_k(P.Kg, r, P.radG);
+            double fij[3];
+            for (int d = 0; d < 3; ++d) fij[d] = -a_ * (-gi[d]) * weight / P.R2g * P.radG * VoL;
+            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c) sg[3 * a + c] += 1.0 * fij[a] * q.x[c] / V;
+            const double dw = ratio * dwq_k(P.Kg, P.m2G, r, P.radG);
+            const double gw[3] = {dw * q.x[0] / r, dw * q.x[1] / r, dw * q.x[2] / r};
+            double gr = 0.0;
+            for (int d = 0; d < 3; ++d) gr += (-gi[d]) * q.x[d];
+            for (int d = 0; d < 3; ++d) fij[d] = -a_ * (gr) * gw[d] / P.R2g * P.radG * VoL;
+            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c) sg[3 * a + c] += 1.0 * fij[a] * q.x[c] / V;
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        double F = force[3 * i + d];
+        F += fp[d];
+        F += fv[d];
+        force[3 * i + d] = F;
+    }
+    double vs[9];
+    for (int k = 0; k < 9; ++k) {
+        double v = 0.0;
+        v += st1[k];
+        v += sa[k];
+        v += sv[k];
+        v += sg[k];
+        vs[k] = v;
+        virialStress[9 * (long long)i + k] = v;
+    }
+    virialP[i] = P.dim == 2 ? -1.0 / 2.0 * (vs[0] + vs[4]) : -1.0 / 3.0 * (vs[0] + vs[4] + vs[8]);
+}
+
+}  // namespace
+
+void launch_fluid_bbox(hipStream_t st, const DevParams& P, const int* cellStart, int* bbox) {
+    hipLaunchKernelGGL(k_fluid_bbox, dim3(nblk(P.T, 256)), dim3(256), 0, st, P, cellStart, bbox);
+}
+void launch_inv_diag(hipStream_t st, const DevParams& P, const int* selfIdx, ASlices A, double* invDiagA) {
+    hipLaunchKernelGGL(k_inv_diag, dim3(nblk(P.nf, 256)), dim3(256), 0, st, P, selfIdx, A.start, A.val, invDiagA);
+}
+void launch_mg_galerkin0(hipStream_t st, const DevParams& P, const Level& L, const int* cellStart,
+                         const int* nFluid, const long long* nPtr, const int* nInd, ASlices A) {
+    const long long n = (long long)L.cells * L.cs[0] * L.cs[1] * L.cs[2];
+    hipLaunchKernelGGL(k_mg_galerkin0, dim3(nblk(n, 128)), dim3(128), 0, st, P, L, cellStart, nFluid, nPtr, nInd,
+                       A.start, A.val);
+}
+void launch_mg_galerkinL(hipStream_t st, const Level& S, const Level& L) {
+    const long long n = (long long)L.cells * L.cs[0] * L.cs[1] * L.cs[2];
+    hipLaunchKernelGGL(k_mg_galerkinL, dim3(nblk(n, 128)), dim3(128), 0, st, S, L);
+}
+void launch_mg_invd(hipStream_t st, const Level& L) {
+    hipLaunchKernelGGL(k_mg_invd, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, L);
+}
+void launch_fill(hipStream_t st, double* x, long long n, double v) {
+    if (n > 0) hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, st, x, n, v);
+}
+void launch_mg_p2g(hipStream_t st, const DevParams& P, Level& L, const int* cellStart, const double* q) {
+    hipLaunchKernelGGL(k_mg_p2g, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, P, L, cellStart, q);
+}
+void launch_mg_begin(hipStream_t st, Level& L) {
+    hipLaunchKernelGGL(k_mg_begin, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, L);
+}
+void launch_mg_update(hipStream_t st, Level& L) {
+    hipLaunchKernelGGL(k_mg_update, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, L);
+}
+void launch_mg_residual(hipStream_t st, Level& L) {
+    hipLaunchKernelGGL(k_mg_residual, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, L);
+}
+void launch_mg_residual_update(hipStream_t st, Level& L) {
+    hipLaunchKernelGGL(k_mg_residual_update, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, L);
+    L.cur ^= 1;
+}
+void launch_mg_restrict(hipStream_t st, const Level& S, Level& L) {
+    hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, S, L);
+}
+void launch_mg_prolong(hipStream_t st, const Level& L, Level& S) {
+    hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(S.nrow, 256)), dim3(256), 0, st, L, S);
+}
+void launch_mg_g2p(hipStream_t st, const DevParams& P, const Level& L0, const int* cellIndex,
+                   const double* invDiagA, const double* q, double* u, double* part) {
+    hipLaunchKernelGGL(k_mg_g2p, dim3(nblk(P.nf, 256)), dim3(256), 0, st, P, L0, cellIndex, invDiagA, q, u, part);
+}
+void launch_cr_init(hipStream_t st, const DevParams& P, const double* vel, const double* force,
+                    const double* mass, double* x) {
+    hipLaunchKernelGGL(k_cr_init, dim3(nblk(P.nf, 256)), dim3(256), 0, st, P, vel, force, mass, x);
+}
+void launch_spmv(hipStream_t st, const DevParams& P, ASlices A, const int* nFluid, double alpha, const double* x,
+                 double beta, const double* yin, double* y, double* y2, double* p_copy, const double* dotx,
+                 double* part) {
+    hipLaunchKernelGGL(k_spmv, dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P, nFluid, A.start, A.col, A.val,
+                       alpha, x, beta, yin, y, y2, p_copy, dotx, part);
+}
+void launch_dot_partial(hipStream_t st, const DevParams& P, const double* x, const double* y, double* part) {
+    hipLaunchKernelGGL(k_dot_partial, dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P.nf, x, y, part);
+}
+void launch_dot_final(hipStream_t st, const DevParams& P, const double* part, double* scal, int slot,
+                      int shift_slot) {
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, st, part, nblk(P.nf, kChunk), scal, slot, shift_slot);
+}
+void launch_pq_update(hipStream_t st, const DevParams& P, const double* scal, const double* s, const double* y,
+                      double* p, double* q) {
+    hipLaunchKernelGGL(k_pq_update, dim3(nblk(3LL * P.nf, 256)), dim3(256), 0, st, P.nf, scal, s, y, p, q);
+}
+void launch_xrs_update(hipStream_t st, const DevParams& P, const double* scal, const double* p, const double* q,
+                       const double* u, double* x, double* r, double* s, double* part) {
+    hipLaunchKernelGGL(k_xrs_update, dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P.nf, scal, p, q, u, x, r, s,
+                       part);
+}
+void launch_store_velocity(hipStream_t st, const DevParams& P, const double* x, double* vel) {
+    hipLaunchKernelGGL(k_store_velocity, dim3(nblk(3LL * P.nf, 256)), dim3(256), 0, st, P.nf, x, vel);
+}
+void launch_diagnostics(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
+                        const double* mu, const double* lambda, const double* kappa, const double* volStrainP,
+                        const double* pressA, const double* gcen, const int* nCount, const long long* nPtr,
+                        const int* nInd, const int* nCountP, const long long* nPtrP, const int* nIndP, double* divP,
+                        double* pressP, double* force, double* virialStress, double* virialP) {
+    hipLaunchKernelGGL(k_divergence, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, pos, vel, lambda, kappa,
+                       volStrainP, nCount, nPtr, nInd, divP, pressP);
+    hipLaunchKernelGGL(k_diag_forces, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, type, pos, vel, mu, pressA, gcen,
+                       pressP, nCount, nPtr, nInd, nCountP, nPtrP, nIndP, force, virialStress, virialP);
+}
+
+}  // namespace mph
