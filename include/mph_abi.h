@@ -159,7 +159,18 @@ int mph_scene_domain(mph_scene* s, double* spacing, double domain_min[3], double
 int mph_scene_write_prof(mph_scene* s, const char* path);
 int mph_scene_write_vtk(mph_scene* s, const char* path);
 
-/* Per-phase device timing with HIP events (adds one sync per phase when on). */
+/* The same two writers over caller arrays (host only, no device involved):
+ * byte-identical output for identical values. */
+int mph_write_prof_arrays(const char* path, double time, int n, double spacing, const double domain_min[3],
+                          const double domain_max[3], const int* type, const double* pos, const double* vel);
+int mph_write_vtk_arrays(const char* path, int n, const int* type, const double* pos, const double* vel,
+                         const double* force, const double* vol_strain_p, const double* divergence_p,
+                         const double* pressure_p, const double* virial_pressure, const int* neighbor_count);
+
+/* Timing with HIP events on the scene stream.  Bit 0: per-phase device time
+ * (adds one sync per phase, summed into mph_step_stats.ms_phase).  Bit 1: every
+ * launch of the particle-matrix SpMV is bracketed by events; the summed time and
+ * the launch count are the scalars "SpmvMs" and "SpmvLaunches". */
 int mph_scene_set_timing(mph_scene* s, int on);
 
 int mph_scene_destroy(mph_scene* s);

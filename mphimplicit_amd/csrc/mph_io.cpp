@@ -135,20 +135,13 @@ int fetch(mph_scene* s, const char* name, std::vector<T>& v) {
 
 extern "C" {
 
-// writeProfFile, main.cpp:696-720
-int mph_scene_write_prof(mph_scene* s, const char* path) {
-    if (!s || !path) return MPH_E_ARG;
-    double t = 0, spacing = 0, dmin[3], dmax[3];
-    int rc = mph_scene_scalar(s, "Time", &t);
-    if (rc || (rc = mph_scene_domain(s, &spacing, dmin, dmax))) return rc;
-    std::vector<int> type;
-    std::vector<double> pos, vel;
-    if ((rc = fetch(s, "Property", type)) || (rc = fetch(s, "Position", pos)) || (rc = fetch(s, "Velocity", vel)))
-        return rc;
+// writeProfFile, main.cpp:696-720, over host arrays
+int mph_write_prof_arrays(const char* path, double time, int n, double spacing, const double dmin[3],
+                          const double dmax[3], const int* type, const double* pos, const double* vel) {
+    if (!path || n < 0 || (n > 0 && (!type || !pos || !vel))) return MPH_E_ARG;
     FILE* fp = std::fopen(path, "w");
     if (!fp) return MPH_E_IO;
-    const int n = (int)type.size();
-    std::fprintf(fp, "%e\n", t);
+    std::fprintf(fp, "%e\n", time);
     std::fprintf(fp, "%d %e %e %e %e %e %e %e\n", n, spacing, dmin[0], dmax[0], dmin[1], dmax[1], dmin[2], dmax[2]);
     for (int i = 0; i < n; ++i)
         std::fprintf(fp, "%d %e %e %e  %e %e %e\n", type[i], pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], vel[3 * i],
@@ -158,20 +151,16 @@ int mph_scene_write_prof(mph_scene* s, const char* path) {
     return MPH_OK;
 }
 
-// writeVtkFile, main.cpp:722-883
-int mph_scene_write_vtk(mph_scene* s, const char* path) {
-    if (!s || !path) return MPH_E_ARG;
-    std::vector<int> type, ncount;
-    std::vector<double> pos, vel, force, vs, dv, pp, vp;
-    int rc;
-    if ((rc = fetch(s, "Property", type)) || (rc = fetch(s, "Position", pos)) || (rc = fetch(s, "Velocity", vel)) ||
-        (rc = fetch(s, "Force", force)) || (rc = fetch(s, "VolStrainP", vs)) || (rc = fetch(s, "DivergenceP", dv)) ||
-        (rc = fetch(s, "PressureP", pp)) || (rc = fetch(s, "VirialPressureAtParticle", vp)) ||
-        (rc = fetch(s, "NeighborCount", ncount)))
-        return rc;
+// writeVtkFile, main.cpp:722-883, over host arrays
+int mph_write_vtk_arrays(const char* path, int n, const int* type, const double* pos, const double* vel,
+                         const double* force, const double* volStrainP, const double* divergenceP,
+                         const double* pressureP, const double* virialPressure, const int* neighborCount) {
+    if (!path || n < 0) return MPH_E_ARG;
+    if (n > 0 && (!type || !pos || !vel || !force || !volStrainP || !divergenceP || !pressureP || !virialPressure ||
+                  !neighborCount))
+        return MPH_E_ARG;
     FILE* fp = std::fopen(path, "w");
     if (!fp) return MPH_E_IO;
-    const int n = (int)type.size();
     std::fprintf(fp, "# vtk DataFile Version 2.0\n");
     std::fprintf(fp, "Unstructured Grid Example\n");
     std::fprintf(fp, "ASCII\n");
@@ -192,16 +181,16 @@ int mph_scene_write_vtk(mph_scene* s, const char* path) {
     for (int i = 0; i < n; ++i) std::fprintf(fp, "%d\n", type[i]);
     std::fprintf(fp, "\n");
     const char* names[4] = {"VolStrainP", "DivergenceP", "PressureP", "VirialPressureAtParticle"};
-    const std::vector<double>* vals[4] = {&vs, &dv, &pp, &vp};
+    const double* vals[4] = {volStrainP, divergenceP, pressureP, virialPressure};
     for (int f = 0; f < 4; ++f) {
         std::fprintf(fp, "SCALARS %s float 1\n", names[f]);
         std::fprintf(fp, "LOOKUP_TABLE default\n");
-        for (int i = 0; i < n; ++i) std::fprintf(fp, "%e\n", (float)(*vals[f])[i]);
+        for (int i = 0; i < n; ++i) std::fprintf(fp, "%e\n", (float)vals[f][i]);
         std::fprintf(fp, "\n");
     }
     std::fprintf(fp, "SCALARS neighbor float 1\n");
     std::fprintf(fp, "LOOKUP_TABLE default\n");
-    for (int i = 0; i < n; ++i) std::fprintf(fp, "%d\n", ncount[i]);
+    for (int i = 0; i < n; ++i) std::fprintf(fp, "%d\n", neighborCount[i]);
     std::fprintf(fp, "\n");
     std::fprintf(fp, "VECTORS velocity float\n");
     for (int i = 0; i < n; ++i)
@@ -214,6 +203,32 @@ int mph_scene_write_vtk(mph_scene* s, const char* path) {
     std::fflush(fp);
     std::fclose(fp);
     return MPH_OK;
+}
+
+int mph_scene_write_prof(mph_scene* s, const char* path) {
+    if (!s || !path) return MPH_E_ARG;
+    double t = 0, spacing = 0, dmin[3], dmax[3];
+    int rc = mph_scene_scalar(s, "Time", &t);
+    if (rc || (rc = mph_scene_domain(s, &spacing, dmin, dmax))) return rc;
+    std::vector<int> type;
+    std::vector<double> pos, vel;
+    if ((rc = fetch(s, "Property", type)) || (rc = fetch(s, "Position", pos)) || (rc = fetch(s, "Velocity", vel)))
+        return rc;
+    return mph_write_prof_arrays(path, t, (int)type.size(), spacing, dmin, dmax, type.data(), pos.data(), vel.data());
+}
+
+int mph_scene_write_vtk(mph_scene* s, const char* path) {
+    if (!s || !path) return MPH_E_ARG;
+    std::vector<int> type, ncount;
+    std::vector<double> pos, vel, force, vs, dv, pp, vp;
+    int rc;
+    if ((rc = fetch(s, "Property", type)) || (rc = fetch(s, "Position", pos)) || (rc = fetch(s, "Velocity", vel)) ||
+        (rc = fetch(s, "Force", force)) || (rc = fetch(s, "VolStrainP", vs)) || (rc = fetch(s, "DivergenceP", dv)) ||
+        (rc = fetch(s, "PressureP", pp)) || (rc = fetch(s, "VirialPressureAtParticle", vp)) ||
+        (rc = fetch(s, "NeighborCount", ncount)))
+        return rc;
+    return mph_write_vtk_arrays(path, (int)type.size(), type.data(), pos.data(), vel.data(), force.data(), vs.data(),
+                                dv.data(), pp.data(), vp.data(), ncount.data());
 }
 
 }  // extern "C"

@@ -111,6 +111,12 @@ struct mph_scene {
     bool timing = false;
     hipEvent_t ev[2] = {nullptr, nullptr};
     double msPhase[12] = {0};
+    // per-launch timing of the particle-matrix SpMV (events on the scene stream)
+    bool spmvTiming = false;
+    std::vector<hipEvent_t> spmvEv;
+    int spmvUsed = 0;
+    double spmvMs = 0;
+    long long spmvLaunches = 0;
 };
 
 namespace {
@@ -453,12 +459,43 @@ int read_scalar(mph_scene& S, int slot, double* v) {
     return MPH_OK;
 }
 
+// the particle-matrix SpMV, bracketed by events when per-launch timing is on
+void spmv(mph_scene& S, double alpha, const double* x, double beta, const double* yin, double* y, double* y2,
+          double* p_copy, const double* dotx, double* part) {
+    if (S.spmvTiming) {
+        if ((int)S.spmvEv.size() < S.spmvUsed + 2) {
+            hipEvent_t e0, e1;
+            (void)hipEventCreate(&e0);
+            (void)hipEventCreate(&e1);
+            S.spmvEv.push_back(e0);
+            S.spmvEv.push_back(e1);
+        }
+        (void)hipEventRecord(S.spmvEv[S.spmvUsed], S.st);
+    }
+    launch_spmv(S.st, S.P, aslices(S), S.nFluid.p, alpha, x, beta, yin, y, y2, p_copy, dotx, part);
+    if (S.spmvTiming) {
+        (void)hipEventRecord(S.spmvEv[S.spmvUsed + 1], S.st);
+        S.spmvUsed += 2;
+    }
+}
+
+// collect the SpMV event pairs recorded since the last call (stream must be idle)
+void collect_spmv_timing(mph_scene& S) {
+    for (int k = 0; k + 1 < S.spmvUsed; k += 2) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, S.spmvEv[k], S.spmvEv[k + 1]) == hipSuccess) {
+            S.spmvMs += ms;
+            S.spmvLaunches++;
+        }
+    }
+    S.spmvUsed = 0;
+}
+
 // solveWithConjugatedGradient (4270-4421)
 int phase_solve(mph_scene& S) {
     const DevParams& P = S.P;
     hipStream_t st = S.st;
     const long long n = 3LL * P.nf;
-    const ASlices A = aslices(S);
     launch_cr_init(st, P, S.vel.p, S.force.p, S.mass.p, S.x.p);
     for (double* v : {S.r.p, S.s.p, S.y.p, S.p.p, S.q.p, S.u.p}) launch_fill(st, v, n, -1.0);
     if (S.prm.multigrid)
@@ -474,7 +511,7 @@ int phase_solve(mph_scene& S) {
     launch_dot_partial(st, P, S.b.p, S.b.p, S.part.p);
     launch_dot_final(st, P, S.part.p, S.scal.p, kRR0, -1);
     // r = b - A x; s = M^-1 r
-    launch_spmv(st, P, A, S.nFluid.p, -1.0, S.x.p, 1.0, S.b.p, S.r.p, nullptr, nullptr, nullptr, nullptr);
+    spmv(S, -1.0, S.x.p, 1.0, S.b.p, S.r.p, nullptr, nullptr, nullptr, nullptr);
     if ((rc = apply_m(S, S.r.p, S.s.p, nullptr))) return rc;
     launch_dot_partial(st, P, S.r.p, S.r.p, S.part.p);
     launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1);
@@ -483,12 +520,13 @@ int phase_solve(mph_scene& S) {
     long long iter;
     for (iter = 0; iter < maxit; ++iter) {
         if ((rc = read_scalar(S, kRR, &S.rr))) return rc;
+        if (S.spmvTiming) collect_spmv_timing(S);
         if (S.rr / S.rr0 < S.prm.cr_tolerance) break;
         if (iter == 0) {  // y = A s; p = s; q = A p (same sum); rho = s.y
-            launch_spmv(st, P, A, S.nFluid.p, 1.0, S.s.p, 0.0, S.y.p, S.y.p, S.q.p, S.p.p, S.s.p, S.part.p);
+            spmv(S, 1.0, S.s.p, 0.0, S.y.p, S.y.p, S.q.p, S.p.p, S.s.p, S.part.p);
             launch_dot_final(st, P, S.part.p, S.scal.p, kRHO, kRHOP);
         } else {
-            launch_spmv(st, P, A, S.nFluid.p, 1.0, S.s.p, 0.0, S.y.p, S.y.p, nullptr, nullptr, S.s.p, S.part.p);
+            spmv(S, 1.0, S.s.p, 0.0, S.y.p, S.y.p, nullptr, nullptr, S.s.p, S.part.p);
             launch_dot_final(st, P, S.part.p, S.scal.p, kRHO, kRHOP);
             launch_pq_update(st, P, S.scal.p, S.s.p, S.y.p, S.p.p, S.q.p);
         }
@@ -502,6 +540,7 @@ int phase_solve(mph_scene& S) {
     launch_store_velocity(st, P, S.x.p, S.vel.p);
     HIPCHK(hipMemcpyAsync(S.hpin + 16, S.scal.p, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    if (S.spmvTiming) collect_spmv_timing(S);
     S.crIter = (int)iter;
     S.rr0 = S.hpin[16 + kRR0];
     S.rs0 = S.hpin[16 + kRS0];
@@ -597,6 +636,7 @@ void destroy(mph_scene* S) {
     if (S->hpin) (void)hipHostFree(S->hpin);
     for (hipEvent_t& e : S->ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t& e : S->spmvEv) (void)hipEventDestroy(e);
     if (S->st) (void)hipStreamDestroy(S->st);
     delete S;
 }
@@ -983,6 +1023,16 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
     else if (n == "NeighborIndCountP") *v = (double)S->EP;
     else if (n == "ASlots") *v = (double)S->slots;
     else if (n == "MultiGridDepth") *v = S->depth;
+    else if (n == "SpmvMs") *v = S->spmvMs;
+    else if (n == "SpmvLaunches") *v = (double)S->spmvLaunches;
+    else if (n == "ABlocks" || n == "CBlocks") {
+        std::vector<int> c;
+        int rc = d2h(*S, c, n[0] == 'A' ? S->nFluid.p : S->nCountP.p, (size_t)P.nf);
+        if (rc) return rc;
+        double t = 0;
+        for (int k : c) t += k;
+        *v = t;
+    }
     else if (n == "CRIterations") *v = S->crIter;
     else if (n == "rr0") *v = S->rr0;
     else if (n == "rr") *v = S->rr;
@@ -1004,7 +1054,10 @@ int mph_scene_domain(mph_scene* S, double* spacing, double dmin[3], double dmax[
 
 int mph_scene_set_timing(mph_scene* S, int on) {
     if (!S) return fail(MPH_E_ARG, "null scene");
-    S->timing = on != 0;
+    S->timing = (on & 1) != 0;
+    S->spmvTiming = (on & 2) != 0;
+    S->spmvMs = 0;
+    S->spmvLaunches = 0;
     return MPH_OK;
 }
 

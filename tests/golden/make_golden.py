@@ -1,0 +1,126 @@
+#!/usr/bin/env python3
+"""Regenerate the golden fixtures under tests/golden/ from the reference itself.
+
+Needs the reference tree (default /root/reference) and the binaries that
+`make -C oracle ref` builds into oracle/_ref/ from it:
+  * GeneratorForMph  -- the reference pre-processor, expands the 2D samples' *.boid
+  * ref_probe_2d/3d  -- the unmodified reference solver driven phase by phase,
+                        dumping every array after each phase (oracle/ref_probe.cpp)
+
+For every case it stores, per time step, the SHA-256 digest of each dumped array
+(and the arrays themselves where they are small: CR scalars, velocities and
+positions at the end of the step, the multigrid geometry), plus the inputs.  The
+fixtures are data only; no reference source is copied.
+
+    python tests/golden/make_golden.py [--reference /root/reference]
+"""
+from __future__ import annotations
+
+import argparse
+import glob
+import hashlib
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from mphimplicit_amd import files, scenes  # noqa: E402
+
+DT = {"i4": np.int32, "i8": np.int64, "f8": np.float64}
+FULL = {"CR", "Velocity_end", "Position_end", "MultiGridDepth", "MultiGridCellMin", "MultiGridCellMax",
+        "MultiGridCount", "Ranges", "Time", "consts", "CellCount", "Power", "PrecondIn"}
+
+
+def digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def load(d):
+    out = {}
+    for line in open(os.path.join(d, "manifest.txt")):
+        n, t, c = line.split()
+        out[n] = np.fromfile(os.path.join(d, n + ".bin"), dtype=DT[t])
+    return out
+
+
+def run_case(name, data, grid, dim, steps, ref_bin, outdir, extra_files=()):
+    work = tempfile.mkdtemp(prefix="golden_")
+    try:
+        exe = os.path.join(ref_bin, "ref_probe_%dd" % dim)
+        env = dict(os.environ, REF_PROBE_QUIET="1")
+        subprocess.run([exe, data, grid, os.path.join(work, "dump"), str(steps)], check=True, env=env,
+                       stderr=subprocess.DEVNULL, stdout=subprocess.DEVNULL)
+        fx = {"dim": np.int32(dim), "steps": np.int32(steps), "data_text": np.array(open(data).read())}
+        g = files.read_grid_file(grid)
+        fx["in_type"] = g.type
+        fx["in_pos"] = g.pos
+        fx["in_vel"] = g.vel
+        fx["in_hdr"] = np.array([g.time, g.spacing, *g.domain_min, *g.domain_max])
+        for tag in ["init"] + ["step%03d" % s for s in range(steps)]:
+            d = os.path.join(work, "dump", tag)
+            if not os.path.isdir(d):
+                break
+            arrs = load(d)
+            for n, a in arrs.items():
+                fx["%s/%s/sha" % (tag, n)] = np.array(digest(a))
+                fx["%s/%s/len" % (tag, n)] = np.int64(a.size)
+                if n in FULL:
+                    fx["%s/%s" % (tag, n)] = a
+            for f in extra_files:
+                p = os.path.join(d, f)
+                if os.path.exists(p):
+                    fx["%s/file/%s" % (tag, f)] = np.frombuffer(open(p, "rb").read(), dtype=np.uint8)
+        path = os.path.join(outdir, name + ".npz")
+        np.savez_compressed(path, **fx)
+        print("%-12s %d steps -> %s (%.0f KB)" % (name, steps, path, os.path.getsize(path) / 1024))
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--reference", default="/root/reference")
+    ap.add_argument("--out", default=HERE)
+    args = ap.parse_args()
+    ref_bin = os.path.join(ROOT, "oracle", "_ref")
+    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "REFERENCE=" + args.reference], check=True,
+                   stdout=subprocess.DEVNULL)
+    tmp = tempfile.mkdtemp(prefix="golden_in_")
+    try:
+        # the reference's 3D sample ships its *.grid
+        cube = glob.glob(os.path.join(args.reference, "results3d", "Cube*"))[0]
+        run_case("cube3d", os.path.join(cube, "box.data"), os.path.join(cube, "box.grid"), 3, 2, ref_bin, args.out)
+        # 2D samples: expand *.boid with the reference generator
+        for name, case, steps in [("squaredrop2d", "SquareDropSkip_l00400_rv35_rp17", 3),
+                                  ("bridge2d", "Bridge_l00100_deg000_re35_mu01_g98", 2),
+                                  ("glued2d", "GluedLightSquare_l00400_rv35_rp17", 1)]:
+            d = os.path.join(tmp, case)
+            os.makedirs(d)
+            for f in ("box.boid", "box.data"):
+                shutil.copy(os.path.join(args.reference, "results", case, f), d)
+            subprocess.run([os.path.join(ref_bin, "GeneratorForMph"), "box"], cwd=d, check=True,
+                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            run_case(name, os.path.join(d, "box.data"), os.path.join(d, "box.grid"), 2, steps, ref_bin, args.out)
+        # seeded synthetic scenes (inputs written with %.17e, VTK/PROF files kept)
+        for name, fn, dim, steps in [("tiny2d", scenes.tiny_2d, 2, 3), ("tiny3d", scenes.tiny_3d, 3, 3)]:
+            p, g = fn(steps=steps)
+            p.vtk_output_interval = p.dt  # every step is a VTK step
+            p.output_interval = p.dt
+            d = os.path.join(tmp, name)
+            os.makedirs(d)
+            files.write_data_file(os.path.join(d, "box.data"), p)
+            files.write_grid_file(os.path.join(d, "box.grid"), g)
+            run_case(name, os.path.join(d, "box.data"), os.path.join(d, "box.grid"), dim, steps, ref_bin, args.out,
+                     extra_files=("ref.vtk", "ref.prof"))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    main()
