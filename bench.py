@@ -79,10 +79,13 @@ def run_gpu(args):
               flush=True)
     spmv_ms = scene.scalar("SpmvMs")
     launches = scene.scalar("SpmvLaunches")
+    print("[bench] SpMV timing: %.3f ms over %d launches (error code %d)"
+          % (spmv_ms, launches, scene.scalar("SpmvTimingError")), file=sys.stderr, flush=True)
     total = float(np.sum(lat))
     avg_launch_s = spmv_ms / 1e3 / max(launches, 1)
-    achieved = spmv_bytes_sum / max(launches, 1) / avg_launch_s / 1e9
-    achieved_ref = spmv_ref_sum / max(launches, 1) / avg_launch_s / 1e9
+    ok = launches > 0 and avg_launch_s > 0
+    achieved = spmv_bytes_sum / launches / avg_launch_s / 1e9 if ok else None
+    achieved_ref = spmv_ref_sum / launches / avg_launch_s / 1e9 if ok else None
     scene.close()
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
@@ -97,15 +100,15 @@ def run_gpu(args):
         "roofline": {
             "bound": "hbm",
             "kernel": "k_spmv (particle matrix, sliced 3x3 blocks, FP64)",
-            "achieved": round(achieved, 1),
+            "achieved": round(achieved, 1) if ok else None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if ok else None,
             "traffic": traffic,
             "avg_launch_ms": round(1e3 * avg_launch_s, 4),
             "launches": int(launches),
             "bytes_per_launch": round(spmv_bytes_sum / max(launches, 1)),
-            "achieved_reference_csr_equivalent": round(achieved_ref, 1),
+            "achieved_reference_csr_equivalent": round(achieved_ref, 1) if ok else None,
         },
         "setup_s": setup_s,
     }
@@ -154,7 +157,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (scenes shard by scene, "
-              "not by particle; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "not by particle). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     g = run_gpu(args)
     lat_ms = np.array(g["lat"]) * 1e3

@@ -117,6 +117,7 @@ struct mph_scene {
     int spmvUsed = 0;
     double spmvMs = 0;
     long long spmvLaunches = 0;
+    int spmvTimingErr = 0;
 };
 
 namespace {
@@ -464,17 +465,20 @@ void spmv(mph_scene& S, double alpha, const double* x, double beta, const double
           double* p_copy, const double* dotx, double* part) {
     if (S.spmvTiming) {
         if ((int)S.spmvEv.size() < S.spmvUsed + 2) {
-            hipEvent_t e0, e1;
-            (void)hipEventCreate(&e0);
-            (void)hipEventCreate(&e1);
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            hipError_t e = hipEventCreateWithFlags(&e0, hipEventDefault);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&e1, hipEventDefault);
+            if (e != hipSuccess) S.spmvTimingErr = (int)e;
             S.spmvEv.push_back(e0);
             S.spmvEv.push_back(e1);
         }
-        (void)hipEventRecord(S.spmvEv[S.spmvUsed], S.st);
+        hipError_t e = hipEventRecord(S.spmvEv[S.spmvUsed], S.st);
+        if (e != hipSuccess) S.spmvTimingErr = (int)e;
     }
     launch_spmv(S.st, S.P, aslices(S), S.nFluid.p, alpha, x, beta, yin, y, y2, p_copy, dotx, part);
     if (S.spmvTiming) {
-        (void)hipEventRecord(S.spmvEv[S.spmvUsed + 1], S.st);
+        hipError_t e = hipEventRecord(S.spmvEv[S.spmvUsed + 1], S.st);
+        if (e != hipSuccess) S.spmvTimingErr = (int)e;
         S.spmvUsed += 2;
     }
 }
@@ -483,9 +487,13 @@ void spmv(mph_scene& S, double alpha, const double* x, double beta, const double
 void collect_spmv_timing(mph_scene& S) {
     for (int k = 0; k + 1 < S.spmvUsed; k += 2) {
         float ms = 0;
-        if (hipEventElapsedTime(&ms, S.spmvEv[k], S.spmvEv[k + 1]) == hipSuccess) {
+        hipError_t e = hipEventSynchronize(S.spmvEv[k + 1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, S.spmvEv[k], S.spmvEv[k + 1]);
+        if (e == hipSuccess) {
             S.spmvMs += ms;
             S.spmvLaunches++;
+        } else {
+            S.spmvTimingErr = (int)e;
         }
     }
     S.spmvUsed = 0;
@@ -1025,6 +1033,7 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
     else if (n == "MultiGridDepth") *v = S->depth;
     else if (n == "SpmvMs") *v = S->spmvMs;
     else if (n == "SpmvLaunches") *v = (double)S->spmvLaunches;
+    else if (n == "SpmvTimingError") *v = S->spmvTimingErr;
     else if (n == "ABlocks" || n == "CBlocks") {
         std::vector<int> c;
         int rc = d2h(*S, c, n[0] == 'A' ? S->nFluid.p : S->nCountP.p, (size_t)P.nf);

@@ -144,8 +144,8 @@ class Scene:
     def run_phase(self, phase: int) -> None:
         _check(lib().mph_scene_run_phase(self.h, int(phase)))
 
-    def set_timing(self, on: bool) -> None:
-        _check(lib().mph_scene_set_timing(self.h, 1 if on else 0))
+    def set_timing(self, on: int) -> None:
+        _check(lib().mph_scene_set_timing(self.h, int(on)))
 
     def read(self, name: str) -> np.ndarray:
         nb = C.c_size_t()
