@@ -32,86 +32,124 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spmv_traffic_3d_1m.json")
 
 
-def spmv_bytes(blocks: float, nf: int) -> tuple[float, float]:
+def spmv_bytes(blocks: float, nf: int, value_bytes: int = 8) -> tuple[float, float]:
     """Algorithmic bytes of one y = A s launch: (this format, CSR reference format).
 
-    Sliced 3x3-block layout: 72 B of values + 4 B column per block, 4 B row length
-    per particle, and the vectors once each (s read, y read for beta*y, y written:
-    3 x 8 B per row, n = 3*nf rows).  Reference CSR (main.cpp:4033-4050): 8 B value +
-    4 B column per nonzero (9 per block), 8 B row pointer per row, s read and y
-    written once (16 B per row)."""
+    Sliced 3x3-block layout: 9 values (8 B in FP64, 4 B in FP32) + 4 B column per
+    block, 4 B row length per particle, and the vectors once each (s read, y read
+    for beta*y, y written: 3 x 8 B per row, n = 3*nf rows).  Reference CSR
+    (main.cpp:4033-4050): 8 B value + 4 B column per nonzero (9 per block), 8 B row
+    pointer per row, s read and y written once (16 B per row)."""
     n = 3 * nf
-    mine = 76.0 * blocks + 4.0 * nf + 24.0 * n
+    mine = (9.0 * value_bytes + 4.0) * blocks + 4.0 * nf + 24.0 * n
     ref = 12.0 * 9.0 * blocks + 8.0 * (n + 1) + 16.0 * n
     return mine, ref
 
 
-def run_gpu(args):
-    from mphimplicit_amd import scenes
+def roofline(kernel, ms, launches, bytes_sum, ref_sum, traffic):
+    avg_s = ms / 1e3 / max(launches, 1)
+    ok = launches > 0 and avg_s > 0
+    achieved = bytes_sum / launches / avg_s / 1e9 if ok else None
+    achieved_ref = ref_sum / launches / avg_s / 1e9 if ok else None
+    return {
+        "bound": "hbm",
+        "kernel": kernel,
+        "achieved": round(achieved, 1) if ok else None,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4) if ok else None,
+        "traffic": traffic,
+        "avg_launch_ms": round(1e3 * avg_s, 4),
+        "launches": int(launches),
+        "bytes_per_launch": round(bytes_sum / max(launches, 1)),
+        "achieved_reference_csr_equivalent": round(achieved_ref, 1) if ok else None,
+    }
+
+
+def run_gpu(args, precision="fp64"):
+    """Create the scene, run warmup + timed steps; return timings, the SpMV roofline
+    and the fields used for the precision comparison (Velocity and PressureP after
+    the first step, whose output-step diagnostics run inside it, and after the last)."""
+    from mphimplicit_amd import abi, scenes
     from mphimplicit_amd.scene import Scene
 
     params, grid = scenes.CONFIGS[args.config]()
     params.cr_tolerance = args.rel_residual ** 2
     params.end_time = 1.0e30
+    params.precision = abi.PRECISION_MIXED if precision == "mixed" else abi.PRECISION_FP64
     nf = int((grid.type < 2).sum())
     t = time.perf_counter()
     scene = Scene(params, grid, device=0)
     setup_s = time.perf_counter() - t
-    print("[bench] scene %s: %d particles (%d fluid), created in %.1f s" % (args.config, grid.np, nf, setup_s),
-          file=sys.stderr, flush=True)
+    print("[bench] %s scene %s: %d particles (%d fluid), created in %.1f s"
+          % (precision, args.config, grid.np, nf, setup_s), file=sys.stderr, flush=True)
+    first = None
     for k in range(args.warmup):
         st = scene.step(1)
+        if first is None:
+            first = {"Velocity": scene.read("Velocity"), "PressureP": scene.read("PressureP")}
         print("[bench] warmup step %d: %d CR iterations" % (k, st.cr_iterations), file=sys.stderr, flush=True)
     scene.set_timing(2)  # bracket every SpMV launch with HIP events
-    lat, iters, spmv_bytes_sum, spmv_ref_sum, launches_prev = [], [], 0.0, 0.0, 0.0
+    lat, iters, refine = [], [], []
+    b64 = r64 = b32 = r32 = 0.0
+    l64_prev = l32_prev = 0.0
     for k in range(args.steps):
         t0 = time.perf_counter()
         st = scene.step(1)
         lat.append(time.perf_counter() - t0)
         iters.append(st.cr_iterations)
-        launches = scene.scalar("SpmvLaunches")
+        refine.append(int(scene.scalar("RefinementSteps")))
+        l64, l32 = scene.scalar("SpmvLaunches"), scene.scalar("SpmvLaunchesF32")
         blocks = scene.scalar("ABlocks")
-        mine, ref = spmv_bytes(blocks, nf)
-        spmv_bytes_sum += mine * (launches - launches_prev)
-        spmv_ref_sum += ref * (launches - launches_prev)
-        launches_prev = launches
+        mine, ref = spmv_bytes(blocks, nf, 8)
+        b64 += mine * (l64 - l64_prev)
+        r64 += ref * (l64 - l64_prev)
+        mine, ref = spmv_bytes(blocks, nf, 4)
+        b32 += mine * (l32 - l32_prev)
+        r32 += ref * (l32 - l32_prev)
+        l64_prev, l32_prev = l64, l32
+        if first is None:
+            first = {"Velocity": scene.read("Velocity"), "PressureP": scene.read("PressureP")}
         print("[bench] step %d: %.1f ms, %d CR iterations" % (k, 1e3 * lat[-1], st.cr_iterations), file=sys.stderr,
               flush=True)
-    spmv_ms = scene.scalar("SpmvMs")
-    launches = scene.scalar("SpmvLaunches")
-    print("[bench] SpMV timing: %.3f ms over %d launches (error code %d)"
-          % (spmv_ms, launches, scene.scalar("SpmvTimingError")), file=sys.stderr, flush=True)
-    total = float(np.sum(lat))
-    avg_launch_s = spmv_ms / 1e3 / max(launches, 1)
-    ok = launches > 0 and avg_launch_s > 0
-    achieved = spmv_bytes_sum / launches / avg_launch_s / 1e9 if ok else None
-    achieved_ref = spmv_ref_sum / launches / avg_launch_s / 1e9 if ok else None
-    scene.close()
+    print("[bench] SpMV timing: FP64 %.3f ms over %d launches, FP32 %.3f ms over %d launches (error code %d)"
+          % (scene.scalar("SpmvMs"), l64_prev, scene.scalar("SpmvMsF32"), l32_prev,
+             scene.scalar("SpmvTimingError")), file=sys.stderr, flush=True)
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    rf64 = roofline("k_spmv (particle matrix, sliced 3x3 blocks, FP64 values)", scene.scalar("SpmvMs"), l64_prev,
+                    b64, r64, traffic if precision == "fp64" else None)
+    rf32 = roofline("k_spmv (particle matrix, sliced 3x3 blocks, FP32 values)", scene.scalar("SpmvMsF32"), l32_prev,
+                    b32, r32, None)
+    scene.run_phase(abi.PHASE_DIAGNOSTICS)
+    last = {"Velocity": scene.read("Velocity"), "PressureP": scene.read("PressureP")}
+    scene.close()
     return {
         "nf": nf,
         "np": grid.np,
         "lat": lat,
-        "total": total,
+        "total": float(np.sum(lat)),
         "iters": iters,
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_spmv (particle matrix, sliced 3x3 blocks, FP64)",
-            "achieved": round(achieved, 1) if ok else None,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4) if ok else None,
-            "traffic": traffic,
-            "avg_launch_ms": round(1e3 * avg_launch_s, 4),
-            "launches": int(launches),
-            "bytes_per_launch": round(spmv_bytes_sum / max(launches, 1)),
-            "achieved_reference_csr_equivalent": round(achieved_ref, 1) if ok else None,
-        },
+        "refine": refine,
+        "roofline": rf64,
+        "roofline_f32": rf32,
         "setup_s": setup_s,
+        "first": first,
+        "last": last,
     }
+
+
+def field_error(a, b):
+    """max |a - b| relative to max |b| for each field both carry."""
+    out = {}
+    for k in ("Velocity", "PressureP"):
+        if a is None or b is None or k not in a or k not in b:
+            continue
+        scale = float(np.abs(b[k]).max())
+        out[k] = float(np.abs(a[k] - b[k]).max() / scale) if scale > 0 else float(np.abs(a[k] - b[k]).max())
+    return out
 
 
 def run_cpu_baseline(args):
@@ -131,6 +169,7 @@ def run_cpu_baseline(args):
     t = time.perf_counter()
     st = orc.step(args.cpu_steps)
     dt = time.perf_counter() - t
+    fields = {"Velocity": orc.read("Velocity"), "PressureP": orc.read("PressureP")}
     orc.close()
     print("[bench] cpu baseline: %d step(s) in %.1f s (%d CR iterations, %d threads, init %.1f s)"
           % (args.cpu_steps, dt, st.cr_iterations, threads, init_s), file=sys.stderr, flush=True)
@@ -142,7 +181,20 @@ def run_cpu_baseline(args):
         "sample": "first %d time step(s) of the same %s scene after its initial sort and neighbour search, "
                   "oracle/mph_oracle.cpp built -O3 -fopenmp (no -march), OpenMP dot products, %d threads; "
                   "%d CR iterations, %.1f s wall" % (args.cpu_steps, args.config, threads, st.cr_iterations, dt),
-    }
+    }, fields
+
+
+def copy_probe(mib: int) -> None:
+    """dst.copy_(src) over mib MiB: each launch reads and writes exactly mib MiB."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}), flush=True)
 
 
 def main():
@@ -154,7 +206,15 @@ def main():
     ap.add_argument("--rel-residual", type=float, default=1.0e-8, help="CR stop: sqrt(rr/rr0) below this")
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--mixed", choices=["on", "off"], default="on",
+                    help="also run the mixed-precision form (FP32 matrices inside FP64 refinement)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (the known-size kernel that "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (scenes shard by scene, "
               "not by particle). Run with --gpus 1.", file=sys.stderr)
@@ -188,8 +248,34 @@ def main():
         "roofline": g["roofline"],
         "cpu_baseline": None,
     }
+    m = None
+    if args.mixed == "on":
+        m = run_gpu(args, "mixed")
+        mlat = np.array(m["lat"]) * 1e3
+        mv = m["nf"] * args.steps / m["total"]
+        out["mixed_precision"] = {
+            "value": round(mv, 1),
+            "unit": "fluid-particle-steps/s",
+            "ms_per_step": round(float(mlat.mean()), 3),
+            "p50_step_ms": round(float(np.percentile(mlat, 50)), 3),
+            "p99_step_ms": round(float(np.percentile(mlat, 99)), 3),
+            "cr_iterations_inner": m["iters"],
+            "refinement_steps": m["refine"],
+            "speedup_vs_plain_fp64": round(mv / out["value"], 3),
+            "arithmetic": "FP32 particle matrix and multigrid stencils (coarsest level FP64), FP64 vectors, dots and outer residual; "
+                          "same stop rule ||b - A x||/||b|| < %g with the FP64 matrix" % args.rel_residual,
+            "roofline_fp32_spmv": m["roofline_f32"],
+            "roofline_fp64_residual_spmv": m["roofline"],
+            "rel_error_vs_fp64_gpu_after_first_step": field_error(m["first"], g["first"]),
+            "rel_error_vs_fp64_gpu_after_last_step": field_error(m["last"], g["last"]),
+        }
     if args.cpu_baseline == "on":
-        out["cpu_baseline"] = run_cpu_baseline(args)
+        cb, cfields = run_cpu_baseline(args)
+        if args.cpu_steps == 1:
+            cb["rel_error_fp64_gpu_vs_cpu_after_first_step"] = field_error(g["first"], cfields)
+            if m is not None:
+                cb["rel_error_mixed_gpu_vs_cpu_after_first_step"] = field_error(m["first"], cfields)
+        out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
 

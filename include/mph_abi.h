@@ -170,7 +170,8 @@ int mph_write_vtk_arrays(const char* path, int n, const int* type, const double*
 /* Timing with HIP events on the scene stream.  Bit 0: per-phase device time
  * (adds one sync per phase, summed into mph_step_stats.ms_phase).  Bit 1: every
  * launch of the particle-matrix SpMV is bracketed by events; the summed time and
- * the launch count are the scalars "SpmvMs" and "SpmvLaunches". */
+ * the launch count are the scalars "SpmvMs" and "SpmvLaunches" (FP64 matrix) and
+ * "SpmvMsF32" and "SpmvLaunchesF32" (FP32 matrix, mixed precision only). */
 int mph_scene_set_timing(mph_scene* s, int on);
 
 int mph_scene_destroy(mph_scene* s);

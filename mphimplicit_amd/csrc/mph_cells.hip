@@ -171,10 +171,11 @@ __device__ inline double dist2(const DevParams& P, const double* __restrict__ po
     return q0 * q0 + q1 * q1 + q2 * q2;
 }
 
-// count pass, main.cpp:1406-1511
+// count pass, main.cpp:1406-1511; also the largest P-neighbour count of a fluid
+// particle (it sizes the LDS staging of the C Lambda C^T kernel)
 __global__ void k_neighbor_count(DevParams P, const double* __restrict__ pos, const int* __restrict__ cellIndex,
                                  const int* __restrict__ cellStart, int* __restrict__ nCount,
-                                 int* __restrict__ nFluid, int* __restrict__ nCountP) {
+                                 int* __restrict__ nFluid, int* __restrict__ nCountP, int* __restrict__ maxFluidP) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.np) return;
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
@@ -205,13 +206,18 @@ __global__ void k_neighbor_count(DevParams P, const double* __restrict__ pos, co
     nCount[i] = n;
     nFluid[i] = nfl;
     nCountP[i] = npp;
+    if (i < P.nf) atomicMax(maxFluidP, npp);
 }
 
-// fill pass, main.cpp:1567-1666
+// fill pass, main.cpp:1567-1666.  For fluid particles it also records where the
+// fluid neighbours of each visited cell start in the list (runStart[i][q], q in
+// visiting order, runStart[i][ncells] = fluid count): the multigrid level-0 build
+// reads these runs directly.
 __global__ void k_neighbor_fill(DevParams P, const double* __restrict__ pos, const int* __restrict__ cellIndex,
                                 const int* __restrict__ cellStart, const int* __restrict__ nCount,
                                 const long long* __restrict__ nPtr, const long long* __restrict__ nPtrP,
-                                int* __restrict__ nInd, int* __restrict__ nIndP) {
+                                int* __restrict__ nInd, int* __restrict__ nIndP,
+                                unsigned short* __restrict__ runStart) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.np) return;
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
@@ -226,7 +232,15 @@ __global__ void k_neighbor_fill(DevParams P, const double* __restrict__ pos, con
             }
         }
     };
-    for_cells<true>(P, cellStart, key, put);
+    unsigned short* rs = i < P.nf ? runStart + (long long)i * kRuns : nullptr;
+    int q = 0;
+    const long long k0 = k;
+    for_cells<true>(P, cellStart, key, [&](int lo, int hi) {
+        if (rs) rs[q] = (unsigned short)(k - k0);
+        q++;
+        put(lo, hi);
+    });
+    if (rs) rs[q] = (unsigned short)(k - k0);
     if (!(i >= P.nf && nCount[i] == 0)) for_cells<false>(P, cellStart, key, put);
 }
 
@@ -344,16 +358,16 @@ void launch_reorder(hipStream_t st, const DevParams& P, const uint64_t* keys, co
 }
 
 void launch_neighbor_count(hipStream_t st, const DevParams& P, const double* pos, const int* cellIndex,
-                           const int* cellStart, int* nCount, int* nFluid, int* nCountP) {
+                           const int* cellStart, int* nCount, int* nFluid, int* nCountP, int* maxFluidP) {
     hipLaunchKernelGGL(k_neighbor_count, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, pos, cellIndex, cellStart,
-                       nCount, nFluid, nCountP);
+                       nCount, nFluid, nCountP, maxFluidP);
 }
 
 void launch_neighbor_fill(hipStream_t st, const DevParams& P, const double* pos, const int* cellIndex,
                           const int* cellStart, const int* nCount, const long long* nPtr, const long long* nPtrP,
-                          int* nInd, int* nIndP) {
+                          int* nInd, int* nIndP, unsigned short* runStart) {
     hipLaunchKernelGGL(k_neighbor_fill, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, pos, cellIndex, cellStart,
-                       nCount, nPtr, nPtrP, nInd, nIndP);
+                       nCount, nPtr, nPtrP, nInd, nIndP, runStart);
 }
 
 int scan_scratch_len(int n) { return nblk(n > 0 ? n : 1, kScanTile); }

@@ -14,16 +14,21 @@
 
 #include "../../include/mph_abi.h"
 
+void mph_internal_set_error(const char* msg);  // mph_scene.cpp
+
 namespace {
-thread_local std::string g_io_err;
+int io_fail(int code, const std::string& what) {
+    mph_internal_set_error(what.c_str());
+    return code;
 }
+}  // namespace
 
 extern "C" {
 
 int mph_read_data_file(const char* path, mph_params* p) {
-    if (!path || !p) return MPH_E_ARG;
+    if (!path || !p) return io_fail(MPH_E_ARG, "null argument");
     FILE* fp = std::fopen(path, "r");
-    if (!fp) return MPH_E_IO;
+    if (!fp) return io_fail(MPH_E_IO, std::string("cannot open ") + path);
     char buf[1024];
     while (!std::feof(fp) && !std::ferror(fp)) {
         if (std::fgets(buf, sizeof buf, fp) == nullptr) continue;
@@ -74,22 +79,23 @@ int mph_read_data_file(const char* path, mph_params* p) {
 }
 
 int mph_read_grid_header(const char* path, int* np) {
-    if (!path || !np) return MPH_E_ARG;
+    if (!path || !np) return io_fail(MPH_E_ARG, "null argument");
     FILE* fp = std::fopen(path, "r");
-    if (!fp) return MPH_E_IO;
+    if (!fp) return io_fail(MPH_E_IO, std::string("cannot open ") + path);
     char buf[1024];
     int rc = MPH_E_IO;
     if (std::fgets(buf, sizeof buf, fp) && std::fgets(buf, sizeof buf, fp) && std::sscanf(buf, "%d", np) == 1)
         rc = *np >= 0 ? MPH_OK : MPH_E_IO;
     std::fclose(fp);
-    return rc;
+    return rc ? io_fail(rc, std::string("malformed grid header in ") + path) : rc;
 }
 
 int mph_read_grid_file(const char* path, int np, double* time, double* spacing, double dmin[3], double dmax[3],
                        int* type, double* pos, double* vel) {
-    if (!path || !time || !spacing || !dmin || !dmax || (np > 0 && (!type || !pos || !vel))) return MPH_E_ARG;
+    if (!path || !time || !spacing || !dmin || !dmax || (np > 0 && (!type || !pos || !vel)))
+        return io_fail(MPH_E_ARG, "null argument");
     FILE* fp = std::fopen(path, "r");
-    if (!fp) return MPH_E_IO;
+    if (!fp) return io_fail(MPH_E_IO, std::string("cannot open ") + path);
     char buf[1024];
     int count = 0;
     if (!std::fgets(buf, sizeof buf, fp) || std::sscanf(buf, "%lf", time) != 1 || !std::fgets(buf, sizeof buf, fp) ||
@@ -97,14 +103,14 @@ int mph_read_grid_file(const char* path, int np, double* time, double* spacing, 
                     &dmin[2], &dmax[2]) != 8 ||
         count != np) {
         std::fclose(fp);
-        return MPH_E_IO;
+        return io_fail(MPH_E_IO, std::string("malformed grid header in ") + path);
     }
     for (int i = 0; i < np; ++i) {
         if (!std::fgets(buf, sizeof buf, fp) ||
             std::sscanf(buf, "%d  %lf %lf %lf  %lf %lf %lf", &type[i], &pos[3 * i], &pos[3 * i + 1], &pos[3 * i + 2],
                         &vel[3 * i], &vel[3 * i + 1], &vel[3 * i + 2]) != 7) {
             std::fclose(fp);
-            return MPH_E_IO;
+            return io_fail(MPH_E_IO, std::string("malformed particle line ") + std::to_string(i) + " in " + path);
         }
     }
     std::fclose(fp);

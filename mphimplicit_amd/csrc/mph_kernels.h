@@ -36,6 +36,8 @@ struct Level {  // geometry + storage of one multigrid level
     double* R;
     double* Q;
     int cur;       // which S buffer holds the current iterate
+    float* A32;    // FP32 copy of A for the mixed-precision inner solve (or null)
+    int f32;       // smoothers read A32 instead of A
 };
 
 struct ASlices {
@@ -49,6 +51,8 @@ struct ASlices {
 
 constexpr int kSlice = 64;    // particles per A slice (one wavefront)
 constexpr int kChunk = 256;   // particles per dot-product chunk / CR workgroup
+constexpr int kRuns = 28;     // run starts per fluid particle: 27 neighbour cells + end
+constexpr int kMaxP = 40;     // P-neighbours staged in LDS per particle by the C Lambda C^T kernel
 
 // ---- cells (calculateCellParticle) ----
 void launch_hash(hipStream_t st, const DevParams& P, const double* pos, const int* type, uint64_t* keys);
@@ -59,12 +63,12 @@ void launch_reorder(hipStream_t st, const DevParams& P, const uint64_t* keys, co
                     int* cellIndex);
 // ---- neighbours (calculateNeighbor) ----
 void launch_neighbor_count(hipStream_t st, const DevParams& P, const double* pos, const int* cellIndex,
-                           const int* cellStart, int* nCount, int* nFluid, int* nCountP);
+                           const int* cellStart, int* nCount, int* nFluid, int* nCountP, int* maxFluidP);
 void launch_scan_i64(hipStream_t st, const int* in, long long* out, int n, long long* scratch, int scratch_len);
 int scan_scratch_len(int n);
 void launch_neighbor_fill(hipStream_t st, const DevParams& P, const double* pos, const int* cellIndex,
                           const int* cellStart, const int* nCount, const long long* nPtr, const long long* nPtrP,
-                          int* nInd, int* nIndP);
+                          int* nInd, int* nIndP, unsigned short* runStart);
 void launch_slice_width(hipStream_t st, int nf, int nslices, const int* nFluid, int* width);
 // ---- kinematics and explicit forces ----
 void launch_kinematics(hipStream_t st, const DevParams& P, const WallKin& wk, const int* type, double* pos,
@@ -82,19 +86,20 @@ void launch_matrix_a(hipStream_t st, const DevParams& P, const int* type, const 
                      const double* mass, const double* mu, const int* nCount, const int* nFluid,
                      const long long* nPtr, const int* nInd, ASlices A, int* selfIdx, double* b);
 void launch_matrix_c(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP,
-                     const long long* nPtrP, const int* nIndP, double* cval);
+                     const long long* nPtrP, const int* nIndP, const double* lambda, double* cval, double* dval);
 void launch_vector_p(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
                      const double* kappa, const double* lambda, const double* volStrainP, const int* nCountP,
                      const long long* nPtrP, const int* nIndP, double* vecP);
 void launch_b_minus_cp(hipStream_t st, const DevParams& P, const int* nCountP, const long long* nPtrP,
                        const int* nIndP, const double* cval, const double* vecP, double* b);
 void launch_cac(hipStream_t st, const DevParams& P, const int* nFluid, const int* nCountP, const long long* nPtrP,
-                const int* nIndP, const double* cval, const double* lambda, ASlices A);
+                const int* nIndP, const double* cval, const double* dval, const double* lambda, ASlices A,
+                bool staged);  // also writes A.val32 when it is non-null
 // ---- multigrid ----
 void launch_fluid_bbox(hipStream_t st, const DevParams& P, const int* cellStart, int* bbox);
 void launch_inv_diag(hipStream_t st, const DevParams& P, const int* selfIdx, ASlices A, double* invDiagA);
 void launch_mg_galerkin0(hipStream_t st, const DevParams& P, const Level& L, const int* cellStart,
-                         const int* nFluid, const long long* nPtr, const int* nInd, ASlices A);
+                         const unsigned short* runStart, ASlices A);
 void launch_mg_galerkinL(hipStream_t st, const Level& S, const Level& L);
 void launch_mg_invd(hipStream_t st, const Level& L);
 void launch_fill(hipStream_t st, double* x, long long n, double v);
@@ -112,7 +117,9 @@ void launch_cr_init(hipStream_t st, const DevParams& P, const double* vel, const
                     const double* mass, double* x);
 void launch_spmv(hipStream_t st, const DevParams& P, ASlices A, const int* nFluid, double alpha, const double* x,
                  double beta, const double* yin, double* y, double* y2, double* p_copy, const double* dotx,
-                 double* part);
+                 double* part, bool f32 = false);
+void launch_convert_f32(hipStream_t st, const double* in, float* out, long long n);
+void launch_axpy(hipStream_t st, long long n, const double* x, double* y);  // y += 1.0*x
 void launch_dot_partial(hipStream_t st, const DevParams& P, const double* x, const double* y, double* part);
 void launch_dot_final(hipStream_t st, const DevParams& P, const double* part, double* scal, int slot, int shift_slot);
 void launch_pq_update(hipStream_t st, const DevParams& P, const double* scal, const double* s, const double* y,

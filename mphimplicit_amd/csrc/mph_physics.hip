@@ -243,10 +243,13 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
     for (int d = 0; d < 3; ++d) b[3 * i + d] = bv[d] + coef * vel[3 * i + d];
 }
 
-// calculateMatrixC values (2463-2507): C[3i+r][k] = e_r * (-w'_p)
+// calculateMatrixC values (2463-2507): C[3i+r][k] = e_r * (-w'_p).  Also
+// D[3i+r][k] = C[3i+r][k] * Lambda[k], the left factor of every product of the
+// C Lambda C^T sum (main.cpp:2614 evaluates (C_ik * Lambda_k) * C_jk).
 __global__ void k_matrix_c(DevParams P, const double* __restrict__ pos, const int* __restrict__ nCountP,
                            const long long* __restrict__ nPtrP, const int* __restrict__ nIndP,
-                           double* __restrict__ cval) {
+                           const double* __restrict__ lambda, double* __restrict__ cval,
+                           double* __restrict__ dval) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.nf) return;
     int iN = 0;
@@ -273,6 +276,10 @@ __global__ void k_matrix_c(DevParams P, const double* __restrict__ pos, const in
         for (int r = 0; r < 3; ++r) cval[3 * (b0 + jn) + r] = c3[r];
     }
     for (int r = 0; r < 3; ++r) cval[3 * (b0 + iN) + r] = self[r];
+    for (int jn = 0; jn < n; ++jn) {
+        const double lk = lambda[nIndP[b0 + jn]];
+        for (int r = 0; r < 3; ++r) dval[3 * (b0 + jn) + r] = cval[3 * (b0 + jn) + r] * lk;
+    }
 }
 
 // VectorP, main.cpp:2510-2549
@@ -385,6 +392,76 @@ __global__ void __launch_bounds__(256) k_cac(DevParams P, const int* __restrict_
     }
 }
 
+// the same sums with the slice's own P-neighbour lists (indices and D = C*Lambda)
+// staged in LDS; the neighbour j's list is read from global memory
+__global__ void __launch_bounds__(256) k_cac_lds(DevParams P, const int* __restrict__ nFluid,
+                                                 const int* __restrict__ nCountP,
+                                                 const long long* __restrict__ nPtrP, const int* __restrict__ nIndP,
+                                                 const double* __restrict__ cval, const double* __restrict__ dval,
+                                                 const int* __restrict__ width, const long long* __restrict__ start,
+                                                 const int* __restrict__ acol, double* __restrict__ aval,
+                                                 float* __restrict__ aval32) {
+    __shared__ int sk[kSlice * kMaxP];
+    __shared__ double sd[kSlice * kMaxP * 3];
+    const int slice = blockIdx.x;
+    const int base = slice * kSlice;
+    for (int t = threadIdx.x; t < kSlice * kMaxP; t += blockDim.x) {
+        const int l = t / kMaxP, a = t % kMaxP, i = base + l;
+        if (i < P.nf && a < nCountP[i]) {
+            const long long e = nPtrP[i] + a;
+            sk[t] = nIndP[e];
+            sd[3 * t] = dval[3 * e];
+            sd[3 * t + 1] = dval[3 * e + 1];
+            sd[3 * t + 2] = dval[3 * e + 2];
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x % kSlice, wave = threadIdx.x / kSlice;
+    const int i = base + lane;
+    if (i >= P.nf) return;
+    const int nfl = nFluid[i];
+    const int ni = nCountP[i];
+    const int* ki = sk + lane * kMaxP;
+    const double* di = sd + 3 * lane * kMaxP;
+    const long long sb = start[slice];
+    for (int k = wave; k < nfl; k += 4) {
+        const long long slot = (sb + k) * kSlice + lane;
+        const int j = acol[slot];
+        const long long pj = nPtrP[j];
+        const int nj = nCountP[j];
+        double sum[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        int a = 0, c = 0;
+        int ka = ni > 0 ? ki[0] : 0;
+        int kc = nj > 0 ? nIndP[pj] : 0;
+        while (a < ni && c < nj) {
+            if (ka == kc) {
+                const double* cj = &cval[3 * (pj + c)];
+                const double c0 = cj[0], c1 = cj[1], c2 = cj[2];
+                const double d0 = di[3 * a], d1 = di[3 * a + 1], d2 = di[3 * a + 2];
+                sum[0] += d0 * c0; sum[1] += d0 * c1; sum[2] += d0 * c2;
+                sum[3] += d1 * c0; sum[4] += d1 * c1; sum[5] += d1 * c2;
+                sum[6] += d2 * c0; sum[7] += d2 * c1; sum[8] += d2 * c2;
+                a++;
+                c++;
+                if (a < ni) ka = ki[a];
+                if (c < nj) kc = nIndP[pj + c];
+            } else if (ka < kc) {
+                a++;
+                if (a < ni) ka = ki[a];
+            } else {
+                c++;
+                if (c < nj) kc = nIndP[pj + c];
+            }
+        }
+        for (int e9 = 0; e9 < 9; ++e9) {
+            const long long at = ((sb + k) * 9 + e9) * kSlice + lane;
+            const double v = aval[at] + sum[e9];
+            aval[at] = v;
+            if (aval32) aval32[at] = (float)v;
+        }
+    }
+}
+
 }  // namespace
 
 void launch_kinematics(hipStream_t st, const DevParams& P, const WallKin& wk, const int* type, double* pos,
@@ -420,8 +497,9 @@ void launch_matrix_a(hipStream_t st, const DevParams& P, const int* type, const 
 }
 
 void launch_matrix_c(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP,
-                     const long long* nPtrP, const int* nIndP, double* cval) {
-    hipLaunchKernelGGL(k_matrix_c, dim3(nblk(P.nf, 128)), dim3(128), 0, st, P, pos, nCountP, nPtrP, nIndP, cval);
+                     const long long* nPtrP, const int* nIndP, const double* lambda, double* cval, double* dval) {
+    hipLaunchKernelGGL(k_matrix_c, dim3(nblk(P.nf, 128)), dim3(128), 0, st, P, pos, nCountP, nPtrP, nIndP, lambda,
+                       cval, dval);
 }
 
 void launch_vector_p(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
@@ -438,10 +516,15 @@ void launch_b_minus_cp(hipStream_t st, const DevParams& P, const int* nCountP, c
 }
 
 void launch_cac(hipStream_t st, const DevParams& P, const int* nFluid, const int* nCountP, const long long* nPtrP,
-                const int* nIndP, const double* cval, const double* lambda, ASlices A) {
+                const int* nIndP, const double* cval, const double* dval, const double* lambda, ASlices A,
+                bool staged) {
     if (A.nslices == 0) return;
-    hipLaunchKernelGGL(k_cac, dim3(A.nslices), dim3(256), 0, st, P, nFluid, nCountP, nPtrP, nIndP, cval, lambda,
-                       A.width, A.start, A.col, A.val);
+    if (staged)
+        hipLaunchKernelGGL(k_cac_lds, dim3(A.nslices), dim3(256), 0, st, P, nFluid, nCountP, nPtrP, nIndP, cval,
+                           dval, A.width, A.start, A.col, A.val, A.val32);
+    else
+        hipLaunchKernelGGL(k_cac, dim3(A.nslices), dim3(256), 0, st, P, nFluid, nCountP, nPtrP, nIndP, cval, lambda,
+                           A.width, A.start, A.col, A.val);
 }
 
 }  // namespace mph

@@ -62,6 +62,7 @@ struct DBuf {
 struct HostLevel {
     Level L{};
     DBuf<double> A, invD, S0, S1, R, Q;
+    DBuf<float> A32;
 };
 
 enum { kRR0 = 0, kRS0 = 1, kRR = 2, kRHO = 3, kRHOP = 4, kTMP = 5, kRS = 6 };
@@ -88,14 +89,17 @@ struct mph_scene {
     DBuf<uint64_t> keys;
     DBuf<int> cellIndex, cellStart, bbox;
     // neighbours
-    DBuf<int> nCount, nFluid, nCountP, nInd, nIndP;
+    DBuf<int> nCount, nFluid, nCountP, nInd, nIndP, maxP;
     DBuf<long long> nPtr, nPtrP, scanScratch;
+    DBuf<unsigned short> runStart;
     long long E = 0, EP = 0;
+    int hmaxP = 0;
     // A and C
     int nslices = 0;
     DBuf<int> width, selfIdx, acol;
     DBuf<long long> sliceStart;
-    DBuf<double> aval, b, cval, vecP;
+    DBuf<double> aval, b, cval, dval, vecP;
+    DBuf<float> aval32;  // FP32 copy of A (mixed precision only)
     long long slots = 0;
     // multigrid
     int hbbox[6] = {0, 0, 0, 0, 0, 0};
@@ -103,7 +107,8 @@ struct mph_scene {
     std::vector<HostLevel> lv;
     DBuf<double> invDiagA;
     // CR
-    DBuf<double> x, r, s, y, p, q, u, part, scal;
+    DBuf<double> x, r, s, y, p, q, u, part, scal, rOut, dvec;
+    int irOuter = 0;
     double* hpin = nullptr;  // pinned host scalars
     int crIter = 0;
     double rr0 = 0, rr = 0, rs0 = 0, rs = 0;
@@ -114,9 +119,10 @@ struct mph_scene {
     // per-launch timing of the particle-matrix SpMV (events on the scene stream)
     bool spmvTiming = false;
     std::vector<hipEvent_t> spmvEv;
+    std::vector<char> spmvEvF32;  // per event pair: FP32-matrix launch
     int spmvUsed = 0;
-    double spmvMs = 0;
-    long long spmvLaunches = 0;
+    double spmvMs = 0, spmvMs32 = 0;
+    long long spmvLaunches = 0, spmvLaunches32 = 0;
     int spmvTimingErr = 0;
 };
 
@@ -265,6 +271,8 @@ int init_constants(mph_scene& S, const mph_grid& g) {
 // ---------------------------------------------------------------------------
 // phases
 
+bool mixed(const mph_scene& S) { return S.prm.precision == MPH_PRECISION_MIXED; }
+
 ASlices aslices(mph_scene& S) {
     ASlices A;
     A.nslices = S.nslices;
@@ -272,7 +280,7 @@ ASlices aslices(mph_scene& S) {
     A.start = S.sliceStart.p;
     A.col = S.acol.p;
     A.val = S.aval.p;
-    A.val32 = nullptr;
+    A.val32 = mixed(S) ? S.aval32.p : nullptr;
     return A;
 }
 
@@ -303,7 +311,9 @@ int phase_cells(mph_scene& S) {
 
 int phase_neighbor(mph_scene& S) {
     const DevParams& P = S.P;
-    launch_neighbor_count(S.st, P, S.pos.p, S.cellIndex.p, S.cellStart.p, S.nCount.p, S.nFluid.p, S.nCountP.p);
+    HIPCHK(hipMemsetAsync(S.maxP.p, 0, sizeof(int), S.st));
+    launch_neighbor_count(S.st, P, S.pos.p, S.cellIndex.p, S.cellStart.p, S.nCount.p, S.nFluid.p, S.nCountP.p,
+                          S.maxP.p);
     launch_scan_i64(S.st, S.nCount.p, S.nPtr.p, P.np, S.scanScratch.p, (int)S.scanScratch.cap);
     launch_scan_i64(S.st, S.nCountP.p, S.nPtrP.p, P.np, S.scanScratch.p, (int)S.scanScratch.cap);
     launch_slice_width(S.st, P.nf, S.nslices, S.nFluid.p, S.width.p);
@@ -313,7 +323,9 @@ int phase_neighbor(mph_scene& S) {
     HIPCHK(hipMemcpyAsync(&h[1], S.nPtrP.p + P.np, 8, hipMemcpyDeviceToHost, S.st));
     HIPCHK(hipMemcpyAsync(&h[2], S.sliceStart.p + S.nslices, 8, hipMemcpyDeviceToHost, S.st));
     HIPCHK(hipMemcpyAsync(&h[3], S.bbox.p, 6 * sizeof(int), hipMemcpyDeviceToHost, S.st));
+    HIPCHK(hipMemcpyAsync(&h[6], S.maxP.p, sizeof(int), hipMemcpyDeviceToHost, S.st));
     HIPCHK(hipStreamSynchronize(S.st));
+    S.hmaxP = *(int*)&h[6];
     S.E = h[0];
     S.EP = h[1];
     S.slots = h[2] * kSlice;
@@ -321,7 +333,7 @@ int phase_neighbor(mph_scene& S) {
     int rc;
     if ((rc = S.nInd.ensure(S.E > 0 ? S.E : 1)) || (rc = S.nIndP.ensure(S.EP > 0 ? S.EP : 1))) return rc;
     launch_neighbor_fill(S.st, P, S.pos.p, S.cellIndex.p, S.cellStart.p, S.nCount.p, S.nPtr.p, S.nPtrP.p, S.nInd.p,
-                         S.nIndP.p);
+                         S.nIndP.p, S.runStart.p);
     return MPH_OK;
 }
 
@@ -346,16 +358,22 @@ int phase_matrix_a(mph_scene& S) {
 
 int phase_matrix_c(mph_scene& S) {
     int rc;
-    if ((rc = S.cval.ensure(3 * (S.EP > 0 ? S.EP : 1)))) return rc;
-    launch_matrix_c(S.st, S.P, S.pos.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p);
+    if ((rc = S.cval.ensure(3 * (S.EP > 0 ? S.EP : 1))) || (rc = S.dval.ensure(3 * (S.EP > 0 ? S.EP : 1)))) return rc;
+    launch_matrix_c(S.st, S.P, S.pos.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.lambda.p, S.cval.p, S.dval.p);
     launch_vector_p(S.st, S.P, S.type.p, S.pos.p, S.vel.p, S.kappa.p, S.lambda.p, S.volStrainP.p, S.nCountP.p,
                     S.nPtrP.p, S.nIndP.p, S.vecP.p);
     return MPH_OK;
 }
 
 int phase_multiply_c(mph_scene& S) {
+    int rc;
+    if (mixed(S) && (rc = S.aval32.ensure(S.slots > 0 ? 9 * S.slots : 1))) return rc;
     launch_b_minus_cp(S.st, S.P, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.vecP.p, S.b.p);
-    launch_cac(S.st, S.P, S.nFluid.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.lambda.p, aslices(S));
+    const bool staged = S.hmaxP <= kMaxP;
+    launch_cac(S.st, S.P, S.nFluid.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.dval.p, S.lambda.p, aslices(S),
+               staged);
+    // the staged kernel writes the FP32 copy while it rewrites A; otherwise convert
+    if (mixed(S) && !staged) launch_convert_f32(S.st, S.aval.p, S.aval32.p, 9 * S.slots);
     return MPH_OK;
 }
 
@@ -397,11 +415,20 @@ int phase_multigrid(mph_scene& S) {
         L.R = H.R.p;
         L.Q = H.Q.p;
         L.cur = 0;
+        L.f32 = 0;
+        L.A32 = nullptr;
+        if (mixed(S) && l + 1 < depth) {  // the coarsest level stays FP64
+            if ((rc = H.A32.ensure((size_t)L.nrow * L.nshift))) return rc;
+            L.A32 = H.A32.p;
+        }
     }
     launch_inv_diag(S.st, S.P, S.selfIdx.p, aslices(S), S.invDiagA.p);
-    launch_mg_galerkin0(S.st, S.P, S.lv[0].L, S.cellStart.p, S.nFluid.p, S.nPtr.p, S.nInd.p, aslices(S));
+    launch_mg_galerkin0(S.st, S.P, S.lv[0].L, S.cellStart.p, S.runStart.p, aslices(S));
     for (int l = 1; l < depth; ++l) launch_mg_galerkinL(S.st, S.lv[l - 1].L, S.lv[l].L);
     for (int l = 0; l < depth; ++l) launch_mg_invd(S.st, S.lv[l].L);
+    if (mixed(S))
+        for (int l = 0; l + 1 < depth; ++l)
+            launch_convert_f32(S.st, S.lv[l].L.A, S.lv[l].L.A32, (long long)S.lv[l].L.nrow * S.lv[l].L.nshift);
     return MPH_OK;
 }
 
@@ -462,7 +489,7 @@ int read_scalar(mph_scene& S, int slot, double* v) {
 
 // the particle-matrix SpMV, bracketed by events when per-launch timing is on
 void spmv(mph_scene& S, double alpha, const double* x, double beta, const double* yin, double* y, double* y2,
-          double* p_copy, const double* dotx, double* part) {
+          double* p_copy, const double* dotx, double* part, bool f32 = false) {
     if (S.spmvTiming) {
         if ((int)S.spmvEv.size() < S.spmvUsed + 2) {
             hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -471,11 +498,13 @@ void spmv(mph_scene& S, double alpha, const double* x, double beta, const double
             if (e != hipSuccess) S.spmvTimingErr = (int)e;
             S.spmvEv.push_back(e0);
             S.spmvEv.push_back(e1);
+            S.spmvEvF32.push_back(0);
         }
+        S.spmvEvF32[S.spmvUsed / 2] = f32 ? 1 : 0;
         hipError_t e = hipEventRecord(S.spmvEv[S.spmvUsed], S.st);
         if (e != hipSuccess) S.spmvTimingErr = (int)e;
     }
-    launch_spmv(S.st, S.P, aslices(S), S.nFluid.p, alpha, x, beta, yin, y, y2, p_copy, dotx, part);
+    launch_spmv(S.st, S.P, aslices(S), S.nFluid.p, alpha, x, beta, yin, y, y2, p_copy, dotx, part, f32);
     if (S.spmvTiming) {
         hipError_t e = hipEventRecord(S.spmvEv[S.spmvUsed + 1], S.st);
         if (e != hipSuccess) S.spmvTimingErr = (int)e;
@@ -490,8 +519,13 @@ void collect_spmv_timing(mph_scene& S) {
         hipError_t e = hipEventSynchronize(S.spmvEv[k + 1]);
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, S.spmvEv[k], S.spmvEv[k + 1]);
         if (e == hipSuccess) {
-            S.spmvMs += ms;
-            S.spmvLaunches++;
+            if (S.spmvEvF32[k / 2]) {
+                S.spmvMs32 += ms;
+                S.spmvLaunches32++;
+            } else {
+                S.spmvMs += ms;
+                S.spmvLaunches++;
+            }
         } else {
             S.spmvTimingErr = (int)e;
         }
@@ -557,6 +591,89 @@ int phase_solve(mph_scene& S) {
     return MPH_OK;
 }
 
+// FP32 stencils on every level but the coarsest, whose smoothing stays FP64
+void set_levels_f32(mph_scene& S, int on) {
+    for (int l = 0; l + 1 < S.depth; ++l) S.lv[l].L.f32 = on;
+}
+
+// Mixed precision: FP64 iterative refinement around the same preconditioned CR run
+// with FP32-stored particle and multigrid matrices (the coarsest multigrid level,
+// vectors, accumulation and dot products stay FP64).  The outer residual r = b - A x is always formed with the
+// FP64 matrix and decides convergence: ||r||/||b|| below sqrt(cr_tolerance).
+int phase_solve_mixed(mph_scene& S) {
+    const DevParams& P = S.P;
+    hipStream_t st = S.st;
+    const long long n = 3LL * P.nf;
+    int rc;
+    if ((rc = S.rOut.ensure(n > 0 ? n : 1)) || (rc = S.dvec.ensure(n > 0 ? n : 1))) return rc;
+    launch_cr_init(st, P, S.vel.p, S.force.p, S.mass.p, S.x.p);
+    for (double* v : {S.r.p, S.s.p, S.y.p, S.p.p, S.q.p, S.u.p}) launch_fill(st, v, n, -1.0);
+    if (S.prm.multigrid)
+        for (int l = 0; l < S.depth; ++l) {
+            Level& L = S.lv[l].L;
+            for (double* v : {L.S[0], L.S[1], L.R, L.Q}) launch_fill(st, v, L.nrow, -1.0);
+            L.cur = 0;
+        }
+    launch_dot_partial(st, P, S.b.p, S.b.p, S.part.p);
+    launch_dot_final(st, P, S.part.p, S.scal.p, kRR0, -1);
+    spmv(S, -1.0, S.x.p, 1.0, S.b.p, S.rOut.p, nullptr, nullptr, nullptr, nullptr);
+    launch_dot_partial(st, P, S.rOut.p, S.rOut.p, S.part.p);
+    launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1);
+    if ((rc = read_scalar(S, kRR0, &S.rr0))) return rc;
+    const long long maxit = S.prm.max_iterations > 0 ? S.prm.max_iterations : n;
+    long long total = 0;
+    int outer = 0;
+    double rrOut = 0;
+    for (; outer < 50; ++outer) {
+        if ((rc = read_scalar(S, kRR, &rrOut))) return rc;
+        if (S.spmvTiming) collect_spmv_timing(S);
+        if (rrOut / S.rr0 < S.prm.cr_tolerance || total >= maxit) break;
+        // inner solve A32 d = rOut from d = 0 down to the remaining reduction
+        double target = S.prm.cr_tolerance * S.rr0 / rrOut;
+        if (target < 1.0e-10) target = 1.0e-10;
+        set_levels_f32(S, 1);
+        launch_fill(st, S.dvec.p, n, 0.0);
+        HIPCHK(hipMemcpyAsync(S.r.p, S.rOut.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if ((rc = apply_m(S, S.r.p, S.s.p, nullptr))) return rc;
+        const double rri0 = rrOut;
+        double rri = rrOut;
+        for (long long it = 0; total < maxit; ++it, ++total) {
+            if (it > 0) {
+                if ((rc = read_scalar(S, kRR, &rri))) return rc;
+                if (S.spmvTiming) collect_spmv_timing(S);
+            }
+            if (rri / rri0 < target) break;
+            if (it == 0) {
+                spmv(S, 1.0, S.s.p, 0.0, S.y.p, S.y.p, S.q.p, S.p.p, S.s.p, S.part.p, true);
+                launch_dot_final(st, P, S.part.p, S.scal.p, kRHO, kRHOP);
+            } else {
+                spmv(S, 1.0, S.s.p, 0.0, S.y.p, S.y.p, nullptr, nullptr, S.s.p, S.part.p, true);
+                launch_dot_final(st, P, S.part.p, S.scal.p, kRHO, kRHOP);
+                launch_pq_update(st, P, S.scal.p, S.s.p, S.y.p, S.p.p, S.q.p);
+            }
+            if ((rc = apply_m(S, S.q.p, S.u.p, S.part.p))) return rc;
+            launch_dot_final(st, P, S.part.p, S.scal.p, kTMP, -1);
+            launch_xrs_update(st, P, S.scal.p, S.p.p, S.q.p, S.u.p, S.dvec.p, S.r.p, S.s.p, S.part.p);
+            launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1);
+        }
+        set_levels_f32(S, 0);
+        // x += d; rOut = b - A64 x
+        launch_axpy(st, n, S.dvec.p, S.x.p);
+        spmv(S, -1.0, S.x.p, 1.0, S.b.p, S.rOut.p, nullptr, nullptr, nullptr, nullptr);
+        launch_dot_partial(st, P, S.rOut.p, S.rOut.p, S.part.p);
+        launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1);
+    }
+    launch_store_velocity(st, P, S.x.p, S.vel.p);
+    HIPCHK(hipStreamSynchronize(st));
+    if (S.spmvTiming) collect_spmv_timing(S);
+    S.crIter = (int)total;
+    S.irOuter = outer;
+    S.rr = rrOut;
+    S.rs0 = 0;
+    S.rs = 0;
+    return MPH_OK;
+}
+
 int phase_diagnostics(mph_scene& S) {
     launch_diagnostics(S.st, S.P, S.type.p, S.pos.p, S.vel.p, S.mu.p, S.lambda.p, S.kappa.p, S.volStrainP.p,
                        S.pressA.p, S.gcen.p, S.nCount.p, S.nPtr.p, S.nInd.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p,
@@ -576,7 +693,7 @@ int run_phase(mph_scene& S, int ph) {
         case MPH_PHASE_MATRIX_C: rc = phase_matrix_c(S); break;
         case MPH_PHASE_MULTIPLY_C: rc = phase_multiply_c(S); break;
         case MPH_PHASE_MULTIGRID: rc = phase_multigrid(S); break;
-        case MPH_PHASE_SOLVE: rc = phase_solve(S); break;
+        case MPH_PHASE_SOLVE: rc = mixed(S) ? phase_solve_mixed(S) : phase_solve(S); break;
         case MPH_PHASE_DIAGNOSTICS: rc = phase_diagnostics(S); break;
         case MPH_PHASE_ADVANCE:
             S.time += S.P.dt;
@@ -611,6 +728,7 @@ int allocate(mph_scene& S) {
     ENS(S.virialStress, 9 * np);
     ENS(S.keys, (size_t)P.p2); ENS(S.cellIndex, np); ENS(S.cellStart, 2 * (size_t)P.T + 2); ENS(S.bbox, 8);
     ENS(S.nCount, np); ENS(S.nFluid, np); ENS(S.nCountP, np); ENS(S.nPtr, np + 1); ENS(S.nPtrP, np + 1);
+    ENS(S.maxP, 4); ENS(S.runStart, nf * kRuns);
     ENS(S.scanScratch, (size_t)scan_scratch_len((int)np + 1) + 64);
     S.nslices = (int)((nf + kSlice - 1) / kSlice);
     ENS(S.width, (size_t)S.nslices); ENS(S.sliceStart, (size_t)S.nslices + 1); ENS(S.selfIdx, nf);
@@ -630,17 +748,23 @@ void destroy(mph_scene* S) {
     if (!S) return;
     if (S->st) (void)hipStreamSynchronize(S->st);
     for (DBuf<int>* b : {&S->type, &S->type2, &S->perm, &S->perm2, &S->cellIndex, &S->cellStart, &S->bbox,
-                         &S->nCount, &S->nFluid, &S->nCountP, &S->nInd, &S->nIndP, &S->width, &S->selfIdx, &S->acol})
+                         &S->nCount, &S->nFluid, &S->nCountP, &S->nInd, &S->nIndP, &S->width, &S->selfIdx, &S->acol,
+                         &S->maxP})
         b->release();
+    S->runStart.release();
     for (DBuf<double>* b : {&S->pos, &S->pos2, &S->vel, &S->vel2, &S->force, &S->mass, &S->kappa, &S->lambda,
                             &S->mu, &S->densA, &S->gcen, &S->pressA, &S->volStrainP, &S->divP, &S->pressP,
-                            &S->virialP, &S->virialStress, &S->aval, &S->b, &S->cval, &S->vecP, &S->invDiagA, &S->x,
+                            &S->virialP, &S->virialStress, &S->aval, &S->b, &S->cval, &S->dval, &S->vecP,
+                            &S->invDiagA, &S->rOut, &S->dvec, &S->x,
                             &S->r, &S->s, &S->y, &S->p, &S->q, &S->u, &S->part, &S->scal})
         b->release();
     for (DBuf<long long>* b : {&S->nPtr, &S->nPtrP, &S->scanScratch, &S->sliceStart}) b->release();
     S->keys.release();
-    for (HostLevel& H : S->lv)
+    for (HostLevel& H : S->lv) {
         for (DBuf<double>* b : {&H.A, &H.invD, &H.S0, &H.S1, &H.R, &H.Q}) b->release();
+        H.A32.release();
+    }
+    S->aval32.release();
     if (S->hpin) (void)hipHostFree(S->hpin);
     for (hipEvent_t& e : S->ev)
         if (e) (void)hipEventDestroy(e);
@@ -817,6 +941,9 @@ int host_field(mph_scene& S, const std::string& n, Blob& out, bool& found) {
 // ===========================================================================
 // C ABI
 
+// lets the host-only I/O translation unit report its failures through mph_last_error
+void mph_internal_set_error(const char* msg) { g_err = msg ? msg : ""; }
+
 extern "C" {
 
 const char* mph_last_error(void) { return g_err.c_str(); }
@@ -837,7 +964,8 @@ int mph_scene_create(const mph_params* p, const mph_grid* g, int device, mph_sce
     *out = nullptr;
     if (p->dim != 2 && p->dim != 3) return fail(MPH_E_ARG, "dim must be 2 or 3");
     if (g->np <= 0 || !g->type || !g->pos || !g->vel) return fail(MPH_E_ARG, "empty particle set");
-    if (p->precision != MPH_PRECISION_FP64) return fail(MPH_E_NOTIMPL, "only FP64 precision is built");
+    if (p->precision != MPH_PRECISION_FP64 && p->precision != MPH_PRECISION_MIXED)
+        return fail(MPH_E_ARG, "unknown precision mode");
     int nf = 0;
     for (int i = 0; i < g->np; ++i) {
         if (g->type[i] < 0 || g->type[i] > 3) return fail(MPH_E_ARG, "particle type outside 0..3");
@@ -1031,8 +1159,11 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
     else if (n == "NeighborIndCountP") *v = (double)S->EP;
     else if (n == "ASlots") *v = (double)S->slots;
     else if (n == "MultiGridDepth") *v = S->depth;
+    else if (n == "RefinementSteps") *v = S->irOuter;
     else if (n == "SpmvMs") *v = S->spmvMs;
     else if (n == "SpmvLaunches") *v = (double)S->spmvLaunches;
+    else if (n == "SpmvMsF32") *v = S->spmvMs32;
+    else if (n == "SpmvLaunchesF32") *v = (double)S->spmvLaunches32;
     else if (n == "SpmvTimingError") *v = S->spmvTimingErr;
     else if (n == "ABlocks" || n == "CBlocks") {
         std::vector<int> c;

@@ -79,12 +79,11 @@ __global__ void k_inv_diag(DevParams P, const int* __restrict__ selfIdx, const l
 
 // level-0 matrix = sum of the particle blocks of every cell pair, in the order of
 // calculateMultiGridCsrA (3351-3410): particles of the row cell ascending, then the
-// blocks of each particle ascending.  Thread = (cell g, stencil cell c); the blocks
-// of a particle that fall in one neighbour cell are a contiguous run of its sorted
-// fluid-neighbour list, located by binary search.
+// blocks of each particle ascending.  Thread = (cell g, stencil cell c).  The blocks
+// of a particle that fall in one neighbour cell form a contiguous run of its sorted
+// fluid-neighbour list; the neighbour search recorded where each run starts.
 __global__ void k_mg_galerkin0(DevParams P, Level L, const int* __restrict__ cellStart,
-                               const int* __restrict__ nFluid, const long long* __restrict__ nPtr,
-                               const int* __restrict__ nInd, const long long* __restrict__ start,
+                               const unsigned short* __restrict__ runStart, const long long* __restrict__ start,
                                const double* __restrict__ aval) {
     const int ncs = L.cs[0] * L.cs[1] * L.cs[2];
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -96,9 +95,8 @@ __global__ void k_mg_galerkin0(DevParams P, Level L, const int* __restrict__ cel
     double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int ic = cell_id(P, cx, cy, cz);
     const int p0 = cellStart[ic], p1 = cellStart[ic + 1];
-    // neighbour cells of the row cell in ascending CellId (as the neighbour search)
-    int nb[27];
-    int nnb = 0;
+    // visiting order of the neighbour search (ascending CellId); which runs map to c
+    unsigned mask = 0;
     {
         const int* C = P.C;
         const int rx = P.range[0], ry = P.range[1], rz = P.range[2];
@@ -108,9 +106,10 @@ __global__ void k_mg_galerkin0(DevParams P, Level L, const int* __restrict__ cel
         for (int k = 0; k < 2 * rz + 1; ++k) jz[k] = ((cz - rz + k) % C[2] + C[2]) % C[2];
         const int bx = (2 * rx) - (cx + rx) % C[0], by = (2 * ry) - (cy + ry) % C[1], bz = (2 * rz) - (cz + rz) % C[2];
         const int x0 = bx > 0 ? bx : 0, y0 = by > 0 ? by : 0, z0 = bz > 0 ? bz : 0;
+        int q = 0;
         for (int a = x0; a < x0 + (2 * rx + 1); ++a)
             for (int b = y0; b < y0 + (2 * ry + 1); ++b)
-                for (int cc = z0; cc < z0 + (2 * rz + 1); ++cc) {
+                for (int cc = z0; cc < z0 + (2 * rz + 1); ++cc, ++q) {
                     const int ax = jx[a % (2 * rx + 1)], ay = jy[b % (2 * ry + 1)], az = jz[cc % (2 * rz + 1)];
                     const int jc = cell_id(P, ax, ay, az);
                     if (cellStart[jc] == cellStart[jc + 1]) continue;  // no fluid there
@@ -118,28 +117,23 @@ __global__ void k_mg_galerkin0(DevParams P, Level L, const int* __restrict__ cel
                     const int ix = (dx + 1 + L.cnt[0]) % L.cnt[0];
                     const int iy = (dy + 1 + L.cnt[1]) % L.cnt[1];
                     const int iz = (dz + 1 + L.cnt[2]) % L.cnt[2];
-                    if (L.cs[2] * L.cs[1] * ix + L.cs[2] * iy + iz == c) nb[nnb++] = jc;
+                    if (L.cs[2] * L.cs[1] * ix + L.cs[2] * iy + iz == c) mask |= 1u << q;
                 }
     }
-    for (int ip = p0; ip < p1 && nnb > 0; ++ip) {
-        const long long base = nPtr[ip];
-        const int nfl = nFluid[ip];
-        const int s = ip / kSlice, lane = ip % kSlice;
-        const long long sb = start[s];
-        for (int q = 0; q < nnb; ++q) {
-            const int jlo = cellStart[nb[q]], jhi = cellStart[nb[q] + 1];
-            int lo = 0, hi = nfl;  // first list position with j >= jlo
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (nInd[base + mid] < jlo) lo = mid + 1;
-                else hi = mid;
-            }
-            for (int k = lo; k < nfl && nInd[base + k] < jhi; ++k) {
-                const double* v = aval + ((sb + k) * 9) * kSlice + lane;
-                for (int e = 0; e < 9; ++e) acc[e] += v[e * kSlice];
+    if (mask)
+        for (int ip = p0; ip < p1; ++ip) {
+            const unsigned short* rs = runStart + (long long)ip * kRuns;
+            const int lane = ip % kSlice;
+            const double* vb = aval + (start[ip / kSlice] * 9) * kSlice + lane;
+            for (unsigned m = mask; m; m &= m - 1) {
+                const int q = __builtin_ctz(m);
+                const int k1 = rs[q + 1];
+                for (int k = rs[q]; k < k1; ++k) {
+                    const double* v = vb + (long long)k * 9 * kSlice;
+                    for (int e = 0; e < 9; ++e) acc[e] += v[e * kSlice];
+                }
             }
         }
-    }
     for (int r = 0; r < 3; ++r)
         for (int sd = 0; sd < 3; ++sd) L.A[(long long)(3 * c + sd) * L.nrow + 3 * g + r] = acc[3 * r + sd];
 }
@@ -240,7 +234,8 @@ __global__ void k_mg_update(Level L) {  // myDdmv(1.0, invD, R, 1.0, S)
     S[row] = y;
 }
 
-__device__ inline double mg_row_sum(const Level& L, const double* __restrict__ x, int row) {
+template <class VT>
+__device__ inline double mg_row_sum(const Level& L, const VT* __restrict__ A, const double* __restrict__ x, int row) {
     const int g = row / 3;
     int gx, gy, gz;
     grid_xyz(L, g, gx, gy, gz);
@@ -254,17 +249,19 @@ __device__ inline double mg_row_sum(const Level& L, const double* __restrict__ x
                 const int jz = (gz + ez - 1 + L.cnt[2]) % L.cnt[2];
                 const int col = 3 * grid_id(L, jx, jy, jz);
                 for (int sd = 0; sd < 3; ++sd, ++sh)
-                    sum += -1.0 * L.A[(long long)sh * L.nrow + row] * x[col + sd];
+                    sum += -1.0 * (double)A[(long long)sh * L.nrow + row] * x[col + sd];
             }
         }
     }
     return sum;
 }
 
+template <class VT>
 __global__ void k_mg_residual(Level L) {  // myDcopy(Q, R); myDcsrmv(-1.0, A, S, 1.0, R)
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= L.nrow) return;
-    const double sum = mg_row_sum(L, L.S[L.cur], row);
+    const VT* A = sizeof(VT) == 8 ? (const VT*)L.A : (const VT*)L.A32;
+    const double sum = mg_row_sum(L, A, L.S[L.cur], row);
     double y = L.Q[row];
     y *= 1.0;
     y += sum;
@@ -273,11 +270,13 @@ __global__ void k_mg_residual(Level L) {  // myDcopy(Q, R); myDcsrmv(-1.0, A, S,
 
 // residual then smoother update of the same row; the new iterate goes to the other
 // S buffer so every thread still reads the old one
+template <class VT>
 __global__ void k_mg_residual_update(Level L) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= L.nrow) return;
     const double* So = L.S[L.cur];
-    const double sum = mg_row_sum(L, So, row);
+    const VT* A = sizeof(VT) == 8 ? (const VT*)L.A : (const VT*)L.A32;
+    const double sum = mg_row_sum(L, A, So, row);
     double y = L.Q[row];
     y *= 1.0;
     y += sum;
@@ -373,39 +372,62 @@ __global__ void k_cr_init(DevParams P, const double* __restrict__ vel, const dou
     for (int d = 0; d < 3; ++d) x[3 * i + d] = vel[3 * i + d] - force[3 * i + d] / mass[i] * P.dt;
 }
 
+// Workgroup -> 256-particle chunk, arranged so that each XCD (hardware assigns
+// workgroup b to XCD b % 8) walks a contiguous range of chunks: neighbouring
+// particles' vector entries are then gathered through the same L2.
+__device__ inline int xcd_chunk(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // y = beta*yin + sum_k (alpha*a_k)*x_k over the sliced A (myDcsrmvForA, 4033-4050);
-// optional second output y2 = beta*y2 + sum (same sum), optional copy p = x, and the
-// chunk partial of dotx.y.  Workgroup = 4 slices = one 256-particle dot chunk.
+// alpha is +1 or -1, so alpha*a is exact and is applied as a sign.  Optional second
+// output y2 = beta*y2 + sum (same sum), optional copy p = x, and the chunk partial
+// of dotx.y.  Workgroup = 4 slices = one 256-particle dot chunk.  The row sums run
+// over the blocks in list order, entries s = 0,1,2 within a block, as the CSR.
+template <bool kNeg, class VT>
 __global__ void __launch_bounds__(256) k_spmv(DevParams P, const int* __restrict__ nFluid,
                                               const long long* __restrict__ start, const int* __restrict__ col,
-                                              const double* __restrict__ val, double alpha,
-                                              const double* __restrict__ x, double beta, const double* yin,
-                                              double* y, double* y2, double* p_copy,
-                                              const double* __restrict__ dotx, double* __restrict__ part) {
+                                              const VT* __restrict__ val, const double* __restrict__ x,
+                                              double beta, const double* yin, double* y, double* y2,
+                                              double* p_copy, const double* __restrict__ dotx,
+                                              double* __restrict__ part) {
     __shared__ double sh[256];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    const int i = chunk * 256 + threadIdx.x;
     double t = 0.0;
     if (i < P.nf) {
         const int w = nFluid[i];
         const int lane = i % kSlice;
         const long long sb = start[i / kSlice];
         const int* cp = col + sb * kSlice + lane;
-        const double* vp = val + sb * 9 * kSlice + lane;
+        const VT* vp = val + sb * 9 * kSlice + lane;
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        for (int k = 0; k < w; ++k) {
-            const int j = cp[(long long)k * kSlice];
-            const double* v = vp + (long long)k * 9 * kSlice;
-            const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
-            s0 += alpha * v[0 * kSlice] * x0;
-            s0 += alpha * v[1 * kSlice] * x1;
-            s0 += alpha * v[2 * kSlice] * x2;
-            s1 += alpha * v[3 * kSlice] * x0;
-            s1 += alpha * v[4 * kSlice] * x1;
-            s1 += alpha * v[5 * kSlice] * x2;
-            s2 += alpha * v[6 * kSlice] * x0;
-            s2 += alpha * v[7 * kSlice] * x1;
-            s2 += alpha * v[8 * kSlice] * x2;
+#define MPH_LD(k, e) (kNeg ? -(double)__builtin_nontemporal_load(vp + ((long long)(k) * 9 + (e)) * kSlice) \
+                           : (double)__builtin_nontemporal_load(vp + ((long long)(k) * 9 + (e)) * kSlice))
+#define MPH_BLOCK(k, j)                                                                  \
+    {                                                                                    \
+        const double x0 = x[3 * (j)], x1 = x[3 * (j) + 1], x2 = x[3 * (j) + 2];          \
+        const double a0 = MPH_LD(k, 0), a1 = MPH_LD(k, 1), a2 = MPH_LD(k, 2);            \
+        const double a3 = MPH_LD(k, 3), a4 = MPH_LD(k, 4), a5 = MPH_LD(k, 5);            \
+        const double a6 = MPH_LD(k, 6), a7 = MPH_LD(k, 7), a8 = MPH_LD(k, 8);            \
+        s0 += a0 * x0; s0 += a1 * x1; s0 += a2 * x2;                                     \
+        s1 += a3 * x0; s1 += a4 * x1; s1 += a5 * x2;                                     \
+        s2 += a6 * x0; s2 += a7 * x1; s2 += a8 * x2;                                     \
+    }
+        int k = 0;
+        for (; k + 2 <= w; k += 2) {
+            const int j0 = __builtin_nontemporal_load(cp + (long long)k * kSlice);
+            const int j1 = __builtin_nontemporal_load(cp + (long long)(k + 1) * kSlice);
+            MPH_BLOCK(k, j0)
+            MPH_BLOCK(k + 1, j1)
         }
+        if (k < w) {
+            const int j0 = __builtin_nontemporal_load(cp + (long long)k * kSlice);
+            MPH_BLOCK(k, j0)
+        }
+#undef MPH_BLOCK
+#undef MPH_LD
         const double sm[3] = {s0, s1, s2};
         double out[3];
         for (int r = 0; r < 3; ++r) {
@@ -431,7 +453,7 @@ __global__ void __launch_bounds__(256) k_spmv(DevParams P, const int* __restrict
     }
     if (part) {
         const double v = block_tree_sum(t, sh);
-        if (threadIdx.x == 0) part[blockIdx.x] = v;
+        if (threadIdx.x == 0) part[chunk] = v;
     }
 }
 
@@ -671,10 +693,10 @@ void launch_inv_diag(hipStream_t st, const DevParams& P, const int* selfIdx, ASl
     hipLaunchKernelGGL(k_inv_diag, dim3(nblk(P.nf, 256)), dim3(256), 0, st, P, selfIdx, A.start, A.val, invDiagA);
 }
 void launch_mg_galerkin0(hipStream_t st, const DevParams& P, const Level& L, const int* cellStart,
-                         const int* nFluid, const long long* nPtr, const int* nInd, ASlices A) {
+                         const unsigned short* runStart, ASlices A) {
     const long long n = (long long)L.cells * L.cs[0] * L.cs[1] * L.cs[2];
-    hipLaunchKernelGGL(k_mg_galerkin0, dim3(nblk(n, 128)), dim3(128), 0, st, P, L, cellStart, nFluid, nPtr, nInd,
-                       A.start, A.val);
+    hipLaunchKernelGGL(k_mg_galerkin0, dim3(nblk(n, 128)), dim3(128), 0, st, P, L, cellStart, runStart, A.start,
+                       A.val);
 }
 void launch_mg_galerkinL(hipStream_t st, const Level& S, const Level& L) {
     const long long n = (long long)L.cells * L.cs[0] * L.cs[1] * L.cs[2];
@@ -696,10 +718,12 @@ void launch_mg_update(hipStream_t st, Level& L) {
     hipLaunchKernelGGL(k_mg_update, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, L);
 }
 void launch_mg_residual(hipStream_t st, Level& L) {
-    hipLaunchKernelGGL(k_mg_residual, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, L);
+    if (L.f32) hipLaunchKernelGGL(k_mg_residual<float>, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, L);
+    else hipLaunchKernelGGL(k_mg_residual<double>, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, L);
 }
 void launch_mg_residual_update(hipStream_t st, Level& L) {
-    hipLaunchKernelGGL(k_mg_residual_update, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, L);
+    if (L.f32) hipLaunchKernelGGL(k_mg_residual_update<float>, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, L);
+    else hipLaunchKernelGGL(k_mg_residual_update<double>, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, L);
     L.cur ^= 1;
 }
 void launch_mg_restrict(hipStream_t st, const Level& S, Level& L) {
@@ -718,9 +742,40 @@ void launch_cr_init(hipStream_t st, const DevParams& P, const double* vel, const
 }
 void launch_spmv(hipStream_t st, const DevParams& P, ASlices A, const int* nFluid, double alpha, const double* x,
                  double beta, const double* yin, double* y, double* y2, double* p_copy, const double* dotx,
-                 double* part) {
-    hipLaunchKernelGGL(k_spmv, dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P, nFluid, A.start, A.col, A.val,
-                       alpha, x, beta, yin, y, y2, p_copy, dotx, part);
+                 double* part, bool f32) {
+    // alpha is +1 or -1 at every call site (main.cpp:4337, 4353, 4357)
+    const dim3 g(nblk(P.nf, kChunk)), b(kChunk);
+    if (f32) {
+        if (alpha < 0)
+            hipLaunchKernelGGL((k_spmv<true, float>), g, b, 0, st, P, nFluid, A.start, A.col, A.val32, x, beta, yin,
+                               y, y2, p_copy, dotx, part);
+        else
+            hipLaunchKernelGGL((k_spmv<false, float>), g, b, 0, st, P, nFluid, A.start, A.col, A.val32, x, beta, yin,
+                               y, y2, p_copy, dotx, part);
+    } else {
+        if (alpha < 0)
+            hipLaunchKernelGGL((k_spmv<true, double>), g, b, 0, st, P, nFluid, A.start, A.col, A.val, x, beta, yin,
+                               y, y2, p_copy, dotx, part);
+        else
+            hipLaunchKernelGGL((k_spmv<false, double>), g, b, 0, st, P, nFluid, A.start, A.col, A.val, x, beta, yin,
+                               y, y2, p_copy, dotx, part);
+    }
+}
+
+__global__ void k_convert_f32_(const double* __restrict__ in, float* __restrict__ out, long long n) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = (float)in[k];
+}
+void launch_convert_f32(hipStream_t st, const double* in, float* out, long long n) {
+    if (n > 0) hipLaunchKernelGGL(k_convert_f32_, dim3(nblk(n, 256)), dim3(256), 0, st, in, out, n);
+}
+
+__global__ void k_axpy1_(long long n, const double* __restrict__ x, double* __restrict__ y) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) y[k] += 1.0 * x[k];
+}
+void launch_axpy(hipStream_t st, long long n, const double* x, double* y) {
+    if (n > 0) hipLaunchKernelGGL(k_axpy1_, dim3(nblk(n, 256)), dim3(256), 0, st, n, x, y);
 }
 void launch_dot_partial(hipStream_t st, const DevParams& P, const double* x, const double* y, double* part) {
     hipLaunchKernelGGL(k_dot_partial, dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P.nf, x, y, part);
