@@ -184,6 +184,91 @@ def run_cpu_baseline(args):
     }, fields
 
 
+def run_batch(args):
+    """Batched sweep: `count` seeded scenes (2D/3D mix, log-uniform sizes and
+    viscosities, random gravity directions) advanced `steps` time steps, (a) one
+    scene at a time in plain FP64 with mph_scene_step, (b) all together with
+    mph_batch_step in mixed precision, (c) all together in plain FP64.  Scene
+    creation (upload, initial sort) is outside the timed regions."""
+    from mphimplicit_amd import abi, scenes
+    from mphimplicit_amd.scene import Batch, Scene
+
+    sweep = scenes.batch_sweep(args.batch_count, steps=args.batch_steps)
+    nf = [int((g.type < 2).sum()) for _, g in sweep]
+    npart = [int(g.np) for _, g in sweep]
+
+    def mk(precision):
+        out = []
+        for p, g in sweep:
+            q = p.copy()
+            q.precision = precision
+            q.cr_tolerance = args.rel_residual ** 2
+            out.append(Scene(q, g))
+        return out
+
+    # (a) one scene at a time, plain FP64
+    t_seq = 0.0
+    for k, (p, g) in enumerate(sweep):
+        q = p.copy()
+        q.cr_tolerance = args.rel_residual ** 2
+        s = Scene(q, g)
+        t = time.perf_counter()
+        for _ in range(args.batch_steps):
+            s.step(1)
+        t_seq += time.perf_counter() - t
+        s.close()
+    print("[bench] batch sweep: sequential FP64 %.2f s" % t_seq, file=sys.stderr, flush=True)
+    res = {}
+    for name, prec in (("batched_mixed", abi.PRECISION_MIXED), ("batched_fp64", abi.PRECISION_FP64)):
+        members = mk(prec)
+        with Batch() as b:
+            for s in members:
+                b.add(s)
+            lat = []
+            for _ in range(args.batch_steps):
+                t = time.perf_counter()
+                b.step()
+                lat.append(time.perf_counter() - t)
+        for s in members:
+            s.close()
+        res[name] = lat
+        print("[bench] batch sweep: %s %.2f s" % (name, sum(lat)), file=sys.stderr, flush=True)
+    work = sum(nf) * args.batch_steps
+    v_seq = work / t_seq
+    out = {
+        "workload": "%d seeded scenes (2D/3D, %d..%d fluid particles, %d fluid / %d total), %d steps each"
+                    % (args.batch_count, min(nf), max(nf), sum(nf), sum(npart), args.batch_steps),
+        "unit": "fluid-particle-steps/s",
+        "sequential_fp64": round(v_seq, 1),
+    }
+    for name, lat in res.items():
+        v = work / sum(lat)
+        out[name] = round(v, 1)
+        out[name + "_speedup_vs_sequential_fp64"] = round(v / v_seq, 3)
+        out[name + "_p50_step_ms"] = round(1e3 * float(np.percentile(lat, 50)), 2)
+        out[name + "_p99_step_ms"] = round(1e3 * float(np.percentile(lat, 99)), 2)
+    return out
+
+
+def run_serve(args):
+    """Serving stream: Poisson arrivals of scene-step requests against the wall
+    clock for a bounded time, continuous batching over one resident batch."""
+    from mphimplicit_amd import scenes
+    from mphimplicit_amd.serve import Server, make_requests
+
+    stream = scenes.serve_stream(args.serve_requests, rate=args.serve_rate)
+    reqs = make_requests(stream, tol=args.rel_residual ** 2)
+    srv = Server()
+    try:
+        r = srv.run(reqs, duration_s=args.serve_seconds,
+                    log=lambda m: print(m, file=sys.stderr, flush=True))
+    finally:
+        srv.close()
+    r["workload"] = ("Poisson arrivals at %g req/s, 1-20 steps, lognormal fluid counts (median 50k, sigma 1, "
+                     "clipped 5k-500k), 2D/3D, mixed precision, %g s run" % (args.serve_rate, args.serve_seconds))
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+
+
 def copy_probe(mib: int) -> None:
     """dst.copy_(src) over mib MiB: each launch reads and writes exactly mib MiB."""
     import torch
@@ -208,6 +293,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--mixed", choices=["on", "off"], default="on",
                     help="also run the mixed-precision form (FP32 matrices inside FP64 refinement)")
+    ap.add_argument("--batch", choices=["on", "off"], default="on", help="batched-sweep leg")
+    ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--serve", choices=["on", "off"], default="on", help="serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the known-size kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -269,6 +361,10 @@ def main():
             "rel_error_vs_fp64_gpu_after_first_step": field_error(m["first"], g["first"]),
             "rel_error_vs_fp64_gpu_after_last_step": field_error(m["last"], g["last"]),
         }
+    if args.batch == "on":
+        out["batched"] = run_batch(args)
+    if args.serve == "on":
+        out["serving"] = run_serve(args)
     if args.cpu_baseline == "on":
         cb, cfields = run_cpu_baseline(args)
         if args.cpu_steps == 1:

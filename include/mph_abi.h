@@ -176,6 +176,30 @@ int mph_scene_set_timing(mph_scene* s, int on);
 
 int mph_scene_destroy(mph_scene* s);
 
+/* ---- Batched executor -------------------------------------------------------
+ * No counterpart in the reference, which advances one scene per process
+ * (main.cpp:386-474); a batch replaces running many such processes side by side.
+ * A batch holds up to 256 caller-owned scenes of one device (they stay owned by the
+ * caller; destroy the batch or remove a scene before destroying that scene).
+ * mph_batch_step advances every member whose time is below its end_time by one
+ * time step: the phases before the solve on each scene's stream, then the CR
+ * solves of all members in lockstep (one launch per kernel for the whole batch,
+ * one host synchronisation per iteration), then diagnostics and the advance.
+ * Each scene's results are bit-identical to mph_scene_step on that scene.
+ * `stats` (may be NULL) receives one record per member, in mph_batch_add order;
+ * ms_total is the wall time of the whole batch step.  A failing step leaves the
+ * members in an unspecified state. */
+typedef struct mph_batch mph_batch;
+int mph_batch_create(int device, mph_batch** out);
+int mph_batch_add(mph_batch* b, mph_scene* s);
+int mph_batch_remove(mph_batch* b, mph_scene* s);
+int mph_batch_size(const mph_batch* b);
+int mph_batch_step(mph_batch* b, mph_step_stats* stats);
+int mph_batch_destroy(mph_batch* b);
+
+/* Device bytes held by the scene (all persistent buffers, as now grown). */
+int mph_scene_device_bytes(mph_scene* s, unsigned long long* bytes);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char* mph_last_error(void);
 

@@ -1,0 +1,528 @@
+// mph_batch.cpp -- batched executor: many independent scenes advance one time step
+// together on one device.
+//
+// The reference advances one scene per process.  Here a batch holds caller-owned
+// scenes (mph_scene_create) and each mph_batch_step runs, for every member:
+//   * the phases before the solve on the scene's own stream, interleaved across
+//     scenes so small scenes overlap on the GPU, with ONE host synchronisation for
+//     all neighbour-list totals of the batch;
+//   * the CR solve of all members in lockstep: one launch per kernel covers every
+//     scene (scene-table kernels of mph_batch.hip) and one host synchronisation
+//     per iteration reads the stop-rule scalars of the whole batch.  A scene that
+//     has converged drops out of the launch sets; multigrid levels of scenes with
+//     different depths are scheduled level by level so every scene sees its own
+//     V-cycle (preconditionWithMultiGrid, main.cpp:4122-4266) in its own order;
+//   * diagnostics and the time advance on the scene's stream.
+// Every scene runs the same arithmetic as mph_scene_step, so its results are
+// bit-identical alone and in any batch.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mph_batch.h"
+#include "mph_scene_impl.h"
+
+using namespace mph;
+using namespace mph::host;
+using namespace mph::batch;
+
+struct mph_batch {
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::vector<mph_scene*> scenes;
+    DBuf<BScene> tab;
+    DBuf<int> off;
+    DBuf<double> scal;        // [ns][8] CR scalars of the solve
+    double* hscal = nullptr;  // pinned copy
+    size_t hscalCap = 0;
+    std::vector<BScene> htab;
+    std::vector<int> hoff;
+};
+
+namespace {
+
+struct Set {
+    SceneSet m{};
+    void add(int s) { m.w[s >> 6] |= 1ull << (s & 63); }
+    bool has(int s) const { return (m.w[s >> 6] >> (s & 63)) & 1ull; }
+    bool empty() const {
+        for (unsigned long long w : m.w)
+            if (w) return false;
+        return true;
+    }
+};
+
+int ensure_pinned(mph_batch& B, size_t n) {
+    if (B.hscal && B.hscalCap >= n) return MPH_OK;
+    if (B.hscal) (void)hipHostFree(B.hscal);
+    B.hscal = nullptr;
+    B.hscalCap = 0;
+    HIPCHK(hipHostMalloc((void**)&B.hscal, n * sizeof(double), hipHostMallocDefault));
+    B.hscalCap = n;
+    return MPH_OK;
+}
+
+// scene tables and block offsets of one solve over `sc` (after the multigrid build)
+int build_tables(mph_batch& B, const std::vector<mph_scene*>& sc, Tables& T) {
+    const int ns = (int)sc.size();
+    int rc;
+    if ((rc = B.tab.ensure(ns)) || (rc = B.off.ensure((size_t)kOffRows * (ns + 1))) ||
+        (rc = B.scal.ensure((size_t)ns * 8)) || (rc = ensure_pinned(B, (size_t)ns * 8)))
+        return rc;
+    B.htab.assign(ns, BScene{});
+    B.hoff.assign((size_t)kOffRows * (ns + 1), 0);
+    for (int s = 0; s < ns; ++s) {
+        mph_scene& S = *sc[s];
+        if (S.depth > kMaxLevels) return fail(MPH_E_STATE, "multigrid depth above the batch limit");
+        if (mixed(S)) {
+            const long long n = 3LL * S.P.nf;
+            if ((rc = S.rOut.ensure(n > 0 ? n : 1)) || (rc = S.dvec.ensure(n > 0 ? n : 1))) return rc;
+        }
+        BScene& b = B.htab[s];
+        b.P = S.P;
+        const ASlices A = aslices(S);
+        b.nFluid = S.nFluid.p;
+        b.start = A.start;
+        b.col = A.col;
+        b.val = A.val;
+        b.val32 = A.val32;
+        b.cellStart = S.cellStart.p;
+        b.cellIndex = S.cellIndex.p;
+        b.invDiagA = S.invDiagA.p;
+        b.force = S.force.p;
+        b.mass = S.mass.p;
+        b.vel = S.vel.p;
+        double* v[kNVec] = {S.x.p, S.r.p, S.s.p, S.y.p, S.p.p, S.q.p, S.u.p, S.b.p, S.rOut.p, S.dvec.p};
+        for (int k = 0; k < kNVec; ++k) b.vec[k] = v[k];
+        b.part = S.part.p;
+        b.scal = B.scal.p + 8 * (size_t)s;
+        b.mg = S.prm.multigrid ? 1 : 0;
+        b.depth = b.mg ? S.depth : 0;
+        for (int l = 0; l < b.depth; ++l) b.lv[l] = S.lv[l].L;
+        auto put = [&](int row, int blocks) { B.hoff[(size_t)row * (ns + 1) + s + 1] = blocks; };
+        put(kOffChunk, (S.P.nf + 255) / 256);
+        put(kOffElem, (int)((3LL * S.P.nf + 255) / 256));
+        for (int l = 0; l < b.depth; ++l) put(kOffLevel + l, (b.lv[l].nrow + 255) / 256);
+    }
+    for (int row = 0; row < kOffRows; ++row)
+        for (int s = 0; s < ns; ++s) B.hoff[(size_t)row * (ns + 1) + s + 1] += B.hoff[(size_t)row * (ns + 1) + s];
+    HIPCHK(hipMemcpyAsync(B.tab.p, B.htab.data(), ns * sizeof(BScene), hipMemcpyHostToDevice, B.st));
+    HIPCHK(hipMemcpyAsync(B.off.p, B.hoff.data(), B.hoff.size() * sizeof(int), hipMemcpyHostToDevice, B.st));
+    T.st = B.st;
+    T.tab = B.tab.p;
+    T.off = B.off.p;
+    T.ns = ns;
+    T.hoff = B.hoff.data();
+    return MPH_OK;
+}
+
+int read_scalars(mph_batch& B, int ns) {
+    HIPCHK(hipMemcpyAsync(B.hscal, B.scal.p, (size_t)ns * 8 * sizeof(double), hipMemcpyDeviceToHost, B.st));
+    HIPCHK(hipStreamSynchronize(B.st));
+    return MPH_OK;
+}
+
+// the V-cycle of every scene in `on`, level by level; cur[s][l] is the ping-pong
+// state of scene s at level l and advances with every smoothing sweep
+struct VCycle {
+    const Tables& T;
+    const std::vector<mph_scene*>& sc;
+    std::vector<std::vector<int>> cur;
+    VCycle(const Tables& t, const std::vector<mph_scene*>& s) : T(t), sc(s), cur(s.size()) {
+        for (size_t k = 0; k < s.size(); ++k) cur[k].assign(kMaxLevels, 0);
+    }
+    SceneSet curset(const Set& on, int l) const {
+        Set c;
+        for (int s = 0; s < (int)sc.size(); ++s)
+            if (on.has(s) && cur[s][l]) c.add(s);
+        return c.m;
+    }
+    void ru(const Set& on, int l, bool f32) {
+        if (on.empty()) return;
+        mg_residual_update(T, on.m, l, curset(on, l), f32);
+        for (int s = 0; s < (int)sc.size(); ++s)
+            if (on.has(s)) cur[s][l] ^= 1;
+    }
+    // precondition (main.cpp:4122-4266) of in -> out for the multigrid scenes in `on`
+    void apply(const Set& on, int in, int out, bool part, bool f32) {
+        if (on.empty()) return;
+        int maxD = 0;
+        for (int s = 0; s < (int)sc.size(); ++s)
+            if (on.has(s)) maxD = std::max(maxD, sc[s]->depth);
+        mg_p2g(T, on.m, in);
+        mg_begin(T, on.m, 0, curset(on, 0));
+        mg_update(T, on.m, 0, curset(on, 0));  // pre-smooth x2
+        ru(on, 0, f32);
+        mg_residual(T, on.m, 0, curset(on, 0), f32);
+        for (int l = 1; l <= maxD - 1; ++l) {
+            Set inner, coarsest, both;
+            for (int s = 0; s < (int)sc.size(); ++s) {
+                if (!on.has(s)) continue;
+                const int D = sc[s]->depth;
+                if (D - 1 > l) inner.add(s), both.add(s);
+                else if (D - 1 == l) coarsest.add(s), both.add(s);
+            }
+            if (both.empty()) continue;
+            mg_restrict(T, both.m, l);
+            mg_begin(T, both.m, l, curset(both, l));
+            mg_update(T, both.m, l, curset(both, l));
+            ru(both, l, f32);
+            ru(both, l, f32);
+            if (!inner.empty()) mg_residual(T, inner.m, l, curset(inner, l), f32);  // pre-smooth x3
+            ru(coarsest, l, f32);                                                  // coarsest: x4, then
+            ru(coarsest, l, f32);                                                  // the prolongation
+            if (!coarsest.empty()) mg_prolong(T, coarsest.m, l, curset(coarsest, l), curset(coarsest, l - 1));
+        }
+        for (int l = maxD - 2; l >= 1; --l) {
+            Set up;
+            for (int s = 0; s < (int)sc.size(); ++s)
+                if (on.has(s) && sc[s]->depth - 2 >= l) up.add(s);
+            if (up.empty()) continue;
+            for (int k = 0; k < 3; ++k) ru(up, l, f32);  // post-smooth x3
+            mg_prolong(T, up.m, l, curset(up, l), curset(up, l - 1));
+        }
+        ru(on, 0, f32);  // post-smooth x2
+        ru(on, 0, f32);
+        mg_g2p(T, on.m, curset(on, 0), in, out, part);
+    }
+};
+
+// apply_m of mph_scene.cpp for a set of scenes: the V-cycle, or a copy without it
+void apply_m(VCycle& V, const Set& on, int in, int out, bool part, bool f32) {
+    Set mg, plain;
+    for (int s = 0; s < (int)V.sc.size(); ++s)
+        if (on.has(s)) (V.sc[s]->prm.multigrid ? mg : plain).add(s);
+    if (!plain.empty()) {
+        copy(V.T, plain.m, in, out);
+        if (part) dot_partial(V.T, plain.m, in, out);
+    }
+    V.apply(mg, in, out, part, f32);
+}
+
+void solve_prologue(const Tables& T, const std::vector<mph_scene*>& sc, const Set& all) {
+    cr_init(T, all.m);
+    for (int role : {kR, kS, kY, kP, kQ, kU}) fill(T, all.m, role, -1.0);
+    int maxD = 0;
+    for (mph_scene* S : sc) maxD = std::max(maxD, S->prm.multigrid ? S->depth : 0);
+    for (int l = 0; l < maxD; ++l) {
+        Set at;
+        for (int s = 0; s < (int)sc.size(); ++s)
+            if (all.has(s) && sc[s]->prm.multigrid && sc[s]->depth > l) at.add(s);
+        if (!at.empty()) fill_level(T, at.m, l, -1.0);
+    }
+}
+
+void finish_levels(const VCycle& V) {
+    for (int s = 0; s < (int)V.sc.size(); ++s)
+        for (int l = 0; l < V.sc[s]->depth && l < kMaxLevels; ++l) V.sc[s]->lv[l].L.cur = V.cur[s][l];
+}
+
+// solveWithConjugatedGradient (4270-4421) for every scene of `sc` in lockstep
+int solve_fp64(mph_batch& B, const std::vector<mph_scene*>& sc) {
+    const int ns = (int)sc.size();
+    Tables T{};
+    int rc;
+    if ((rc = build_tables(B, sc, T))) return rc;
+    Set all;
+    for (int s = 0; s < ns; ++s) all.add(s);
+    VCycle V(T, sc);
+    solve_prologue(T, sc, all);
+    apply_m(V, all, kB, kS, true, false);  // s = M^-1 b; rs0 = b.s; rr0 = b.b
+    dot_final(T, all.m, kRS0, -1);
+    dot_partial(T, all.m, kB, kB);
+    dot_final(T, all.m, kRR0, -1);
+    spmv(T, all.m, SceneSet{}, false, -1.0, kX, 1.0, kB, kR, -1, -1, -1);  // r = b - A x; s = M^-1 r
+    apply_m(V, all, kR, kS, false, false);
+    dot_partial(T, all.m, kR, kR);
+    dot_final(T, all.m, kRR, -1);
+    if ((rc = read_scalars(B, ns))) return rc;
+    std::vector<double> rr0(ns);
+    std::vector<long long> iter(ns, 0), maxit(ns);
+    for (int s = 0; s < ns; ++s) {
+        rr0[s] = B.hscal[8 * s + kRR0];
+        const long long n = 3LL * sc[s]->P.nf;
+        maxit[s] = sc[s]->prm.max_iterations > 0 ? sc[s]->prm.max_iterations : n;
+    }
+    Set active = all;
+    for (;;) {
+        Set now, first, rest;
+        for (int s = 0; s < ns; ++s) {
+            if (!active.has(s)) continue;
+            if (iter[s] >= maxit[s]) continue;
+            if (B.hscal[8 * s + kRR] / rr0[s] < sc[s]->prm.cr_tolerance) continue;
+            now.add(s);
+            (iter[s] == 0 ? first : rest).add(s);
+        }
+        active = now;
+        if (active.empty()) break;
+        // y = A s (first iteration also q = y, p = s); rho = s.y
+        spmv(T, active.m, first.m, false, 1.0, kS, 0.0, kY, kY, kQ, kP, kS);
+        dot_final(T, active.m, kRHO, kRHOP);
+        pq_update(T, rest.m);
+        apply_m(V, active, kQ, kU, true, false);  // u = M^-1 q; tmp = q.u
+        dot_final(T, active.m, kTMP, -1);
+        xrs_update(T, active.m, kX);
+        dot_final(T, active.m, kRR, -1);
+        for (int s = 0; s < ns; ++s)
+            if (active.has(s)) iter[s]++;
+        if ((rc = read_scalars(B, ns))) return rc;
+    }
+    dot_partial(T, all.m, kR, kS);
+    dot_final(T, all.m, kRS, -1);
+    store_velocity(T, all.m);
+    if ((rc = read_scalars(B, ns))) return rc;
+    for (int s = 0; s < ns; ++s) {
+        mph_scene& S = *sc[s];
+        S.crIter = (int)iter[s];
+        S.rr0 = B.hscal[8 * s + kRR0];
+        S.rs0 = B.hscal[8 * s + kRS0];
+        S.rr = B.hscal[8 * s + kRR];
+        S.rs = B.hscal[8 * s + kRS];
+    }
+    finish_levels(V);
+    return MPH_OK;
+}
+
+// the mixed-precision solve of mph_scene.cpp (FP64 iterative refinement around the
+// CR with FP32 matrices) for every scene of `sc`, each at its own point of the
+// refinement: per tick a scene runs one inner iteration, or its outer update
+int solve_mixed(mph_batch& B, const std::vector<mph_scene*>& sc) {
+    const int ns = (int)sc.size();
+    Tables T{};
+    int rc;
+    if ((rc = build_tables(B, sc, T))) return rc;
+    Set all;
+    for (int s = 0; s < ns; ++s) all.add(s);
+    VCycle V(T, sc);
+    solve_prologue(T, sc, all);
+    dot_partial(T, all.m, kB, kB);
+    dot_final(T, all.m, kRR0, -1);
+    spmv(T, all.m, SceneSet{}, false, -1.0, kX, 1.0, kB, kROut, -1, -1, -1);
+    dot_partial(T, all.m, kROut, kROut);
+    dot_final(T, all.m, kRR, -1);
+    if ((rc = read_scalars(B, ns))) return rc;
+    enum { kCheck, kInner, kUpdate, kDone };
+    struct St {
+        int state = kCheck;
+        long long total = 0, it = 0, maxit = 0;
+        int outer = 0;
+        double rr0 = 0, rrOut = 0, rri0 = 0, target = 0;
+    };
+    std::vector<St> z(ns);
+    for (int s = 0; s < ns; ++s) {
+        z[s].rr0 = B.hscal[8 * s + kRR0];
+        const long long n = 3LL * sc[s]->P.nf;
+        z[s].maxit = sc[s]->prm.max_iterations > 0 ? sc[s]->prm.max_iterations : n;
+    }
+    for (;;) {
+        Set update, restart, inner, rest;
+        for (int s = 0; s < ns; ++s) {
+            St& q = z[s];
+            const double tol = sc[s]->prm.cr_tolerance;
+            if (q.state == kUpdate) {  // the outer update of the last tick has been read
+                q.outer++;
+                q.state = kCheck;
+            }
+            if (q.state == kCheck) {
+                if (q.outer >= 50) {
+                    q.state = kDone;
+                    continue;
+                }
+                q.rrOut = B.hscal[8 * s + kRR];
+                if (q.rrOut / q.rr0 < tol || q.total >= q.maxit) {
+                    q.state = kDone;
+                    continue;
+                }
+                q.target = tol * q.rr0 / q.rrOut;
+                if (q.target < 1.0e-10) q.target = 1.0e-10;
+                q.rri0 = q.rrOut;
+                q.it = 0;
+                q.state = kInner;
+                restart.add(s);
+                inner.add(s);
+                continue;
+            }
+            if (q.state == kInner) {
+                const double rri = B.hscal[8 * s + kRR];
+                if (rri / q.rri0 < q.target || q.total >= q.maxit) {
+                    q.state = kUpdate;
+                    update.add(s);
+                } else {
+                    inner.add(s);
+                    rest.add(s);
+                }
+            }
+        }
+        if (update.empty() && inner.empty()) break;
+        if (!update.empty()) {  // x += d; rOut = b - A64 x; rr = rOut.rOut
+            axpy(T, update.m, kD, kX);
+            spmv(T, update.m, SceneSet{}, false, -1.0, kX, 1.0, kB, kROut, -1, -1, -1);
+            dot_partial(T, update.m, kROut, kROut);
+            dot_final(T, update.m, kRR, -1);
+        }
+        if (!restart.empty()) {  // d = 0; r = rOut; s = M32^-1 r
+            fill(T, restart.m, kD, 0.0);
+            copy(T, restart.m, kROut, kR);
+            apply_m(V, restart, kR, kS, false, true);
+        }
+        if (!inner.empty()) {
+            spmv(T, inner.m, restart.m, true, 1.0, kS, 0.0, kY, kY, kQ, kP, kS);
+            dot_final(T, inner.m, kRHO, kRHOP);
+            pq_update(T, rest.m);
+            apply_m(V, inner, kQ, kU, true, true);
+            dot_final(T, inner.m, kTMP, -1);
+            xrs_update(T, inner.m, kD);
+            dot_final(T, inner.m, kRR, -1);
+            for (int s = 0; s < ns; ++s)
+                if (inner.has(s)) z[s].it++, z[s].total++;
+        }
+        if ((rc = read_scalars(B, ns))) return rc;
+    }
+    store_velocity(T, all.m);
+    HIPCHK(hipStreamSynchronize(B.st));
+    for (int s = 0; s < ns; ++s) {
+        mph_scene& S = *sc[s];
+        S.crIter = (int)z[s].total;
+        S.irOuter = z[s].outer;
+        S.rr0 = z[s].rr0;
+        S.rr = z[s].rrOut;
+        S.rs0 = 0;
+        S.rs = 0;
+    }
+    finish_levels(V);
+    return MPH_OK;
+}
+
+int sync_all(const std::vector<mph_scene*>& sc) {
+    for (mph_scene* S : sc) HIPCHK(hipStreamSynchronize(S->st));
+    return MPH_OK;
+}
+
+int check_launch() {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(MPH_E_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+    return MPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mph_batch_create(int device, mph_batch** out) {
+    if (!out) return fail(MPH_E_ARG, "null output pointer");
+    *out = nullptr;
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(MPH_E_ARG, "no HIP device " + std::to_string(device));
+    HIPCHK(hipSetDevice(device));
+    mph_batch* B = new mph_batch;
+    B->device = device;
+    if (hipStreamCreateWithFlags(&B->st, hipStreamNonBlocking) != hipSuccess) {
+        delete B;
+        return fail(MPH_E_DEVICE, "stream creation failed");
+    }
+    *out = B;
+    return MPH_OK;
+}
+
+int mph_batch_add(mph_batch* B, mph_scene* S) {
+    if (!B || !S) return fail(MPH_E_ARG, "null argument");
+    if (S->device != B->device) return fail(MPH_E_ARG, "scene lives on another device than the batch");
+    if (std::find(B->scenes.begin(), B->scenes.end(), S) != B->scenes.end())
+        return fail(MPH_E_ARG, "scene is already in the batch");
+    if ((int)B->scenes.size() >= kMaxScenes)
+        return fail(MPH_E_ARG, "batch is full (" + std::to_string(kMaxScenes) + " scenes)");
+    B->scenes.push_back(S);
+    return MPH_OK;
+}
+
+int mph_batch_remove(mph_batch* B, mph_scene* S) {
+    if (!B || !S) return fail(MPH_E_ARG, "null argument");
+    auto it = std::find(B->scenes.begin(), B->scenes.end(), S);
+    if (it == B->scenes.end()) return fail(MPH_E_ARG, "scene is not in the batch");
+    B->scenes.erase(it);
+    return MPH_OK;
+}
+
+int mph_batch_size(const mph_batch* B) { return B ? (int)B->scenes.size() : 0; }
+
+int mph_batch_step(mph_batch* B, mph_step_stats* stats) {
+    if (!B) return fail(MPH_E_ARG, "null batch");
+    HIPCHK(hipSetDevice(B->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    const int n = (int)B->scenes.size();
+    std::vector<mph_scene*> act;
+    std::vector<int> idx;
+    for (int k = 0; k < n; ++k)
+        if (step_wanted(*B->scenes[k])) {
+            act.push_back(B->scenes[k]);
+            idx.push_back(k);
+        }
+    std::vector<int> vtk(act.size(), 0);
+    int rc = MPH_OK;
+    for (mph_scene* S : act) step_begin(*S);
+    for (int ph : {MPH_PHASE_KINEMATICS, MPH_PHASE_CELLS})
+        for (mph_scene* S : act)
+            if ((rc = run_phase(*S, ph))) return rc;
+    for (mph_scene* S : act)
+        if ((rc = phase_neighbor_launch(*S))) return rc;
+    if ((rc = sync_all(act))) return rc;
+    for (mph_scene* S : act)
+        if ((rc = phase_neighbor_finish(*S))) return rc;
+    if ((rc = check_launch())) return rc;
+    for (int ph = MPH_PHASE_EXPLICIT; ph <= MPH_PHASE_MULTIGRID; ++ph)
+        for (mph_scene* S : act)
+            if ((rc = run_phase(*S, ph))) return rc;
+    if ((rc = sync_all(act))) return rc;
+    std::vector<mph_scene*> plain, mix;
+    for (mph_scene* S : act) (mixed(*S) ? mix : plain).push_back(S);
+    if (!plain.empty() && (rc = solve_fp64(*B, plain))) return rc;
+    if (!mix.empty() && (rc = solve_mixed(*B, mix))) return rc;
+    if ((rc = check_launch())) return rc;
+    HIPCHK(hipStreamSynchronize(B->st));
+    for (size_t k = 0; k < act.size(); ++k) {
+        mph_scene& S = *act[k];
+        if (step_vtk_due(S)) {
+            if ((rc = run_phase(S, MPH_PHASE_DIAGNOSTICS))) return rc;
+            step_vtk_done(S);
+            vtk[k] = 1;
+        }
+        if ((rc = run_phase(S, MPH_PHASE_ADVANCE))) return rc;
+    }
+    if ((rc = sync_all(act))) return rc;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) {
+        for (int k = 0; k < n; ++k) stats[k] = mph_step_stats{};
+        for (size_t k = 0; k < act.size(); ++k) {
+            mph_step_stats& st = stats[idx[k]];
+            const mph_scene& S = *act[k];
+            st.steps_done = 1;
+            st.cr_iterations = S.crIter;
+            st.cr_iterations_total = S.crIter;
+            st.vtk_steps = vtk[k];
+            st.rr0 = S.rr0;
+            st.rr = S.rr;
+            st.rs0 = S.rs0;
+            st.rs = S.rs;
+            st.ms_total = ms;
+        }
+    }
+    return MPH_OK;
+}
+
+int mph_batch_destroy(mph_batch* B) {
+    if (!B) return MPH_OK;
+    (void)hipSetDevice(B->device);
+    if (B->st) (void)hipStreamSynchronize(B->st);
+    B->tab.release();
+    B->off.release();
+    B->scal.release();
+    if (B->hscal) (void)hipHostFree(B->hscal);
+    if (B->st) (void)hipStreamDestroy(B->st);
+    delete B;
+    return MPH_OK;
+}
+
+}  // extern "C"

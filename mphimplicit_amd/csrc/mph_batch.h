@@ -1,0 +1,84 @@
+// mph_batch.h -- internal: scene-table launches of the CR loop and the V-cycle for
+// the batched executor.  One launch covers every scene of a batch: workgroup b
+// belongs to the scene s with off[row][s] <= b < off[row][s+1] and runs that
+// scene's rows with that scene's pointers, through the same bodies as the
+// single-scene kernels (mph_cr_body.h).  Scenes outside the launch's set return
+// at once, so per-scene control flow (converged scenes, different multigrid
+// depths, ping-pong buffers) travels as bit sets in the kernel arguments.
+#pragma once
+
+#include "mph_kernels.h"
+
+namespace mph {
+namespace batch {
+
+constexpr int kMaxScenes = 256;  // bits of a SceneSet
+constexpr int kMaxLevels = 20;   // multigrid depth = floor(log2(cells per axis)) + 1
+
+// CR vectors by role
+enum { kX = 0, kR, kS, kY, kP, kQ, kU, kB, kROut, kD, kNVec };
+
+// rows of the block-offset table: 256-particle chunks, 256-element blocks of the
+// 3*nf vectors, then 256-row blocks of each multigrid level
+enum { kOffChunk = 0, kOffElem = 1, kOffLevel = 2, kOffRows = kOffLevel + kMaxLevels };
+
+struct SceneSet {
+    unsigned long long w[kMaxScenes / 64];
+};
+
+struct BScene {
+    DevParams P;
+    const int* nFluid;
+    const long long* start;
+    const int* col;
+    const double* val;
+    const float* val32;
+    const int* cellStart;
+    const int* cellIndex;
+    const double* invDiagA;
+    const double* force;
+    const double* mass;
+    double* vel;
+    double* vec[kNVec];
+    double* part;
+    double* scal;
+    int depth;
+    int mg;
+    Level lv[kMaxLevels];  // .cur is ignored: it comes with each launch
+};
+
+// device tables of one batch solve; hoff is the host copy of the offsets
+struct Tables {
+    hipStream_t st;
+    const BScene* tab;
+    const int* off;
+    int ns;
+    const int* hoff;
+    int blocks(int row) const { return hoff[row * (ns + 1) + ns]; }
+};
+
+void fill(const Tables& T, const SceneSet& on, int role, double v);
+void fill_level(const Tables& T, const SceneSet& on, int l, double v);
+void cr_init(const Tables& T, const SceneSet& on);
+void copy(const Tables& T, const SceneSet& on, int src, int dst);
+void axpy(const Tables& T, const SceneSet& on, int src, int dst);  // dst += 1.0*src
+// y = beta*yin + sum (alpha*a) x with alpha = +-1; y2 and p_copy only for scenes in
+// `first`; dot partial of dot.y into the scene's part when dot >= 0
+void spmv(const Tables& T, const SceneSet& on, const SceneSet& first, bool f32, double alpha, int x, double beta,
+          int yin, int y, int y2, int p_copy, int dot);
+void dot_partial(const Tables& T, const SceneSet& on, int a, int b);
+void dot_final(const Tables& T, const SceneSet& on, int slot, int shift_slot);
+void pq_update(const Tables& T, const SceneSet& on);
+void xrs_update(const Tables& T, const SceneSet& on, int x);
+void store_velocity(const Tables& T, const SceneSet& on);
+void mg_p2g(const Tables& T, const SceneSet& on, int in);
+void mg_begin(const Tables& T, const SceneSet& on, int l, const SceneSet& cur);
+void mg_update(const Tables& T, const SceneSet& on, int l, const SceneSet& cur);
+void mg_residual(const Tables& T, const SceneSet& on, int l, const SceneSet& cur, bool f32);
+void mg_residual_update(const Tables& T, const SceneSet& on, int l, const SceneSet& cur, bool f32);
+void mg_restrict(const Tables& T, const SceneSet& on, int l);  // level l-1 -> l
+void mg_prolong(const Tables& T, const SceneSet& on, int l, const SceneSet& curL, const SceneSet& curS);  // l -> l-1
+void mg_g2p(const Tables& T, const SceneSet& on, const SceneSet& cur0, int in, int out, bool part);
+
+}  // namespace batch
+}  // namespace mph

@@ -1,0 +1,364 @@
+// mph_cr_body.h -- per-row / per-chunk bodies of the multigrid V-cycle and the
+// conjugate-residual loop (source/main.cpp:4122-4421).
+//
+// The single-scene kernels (mph_solver.hip) and the scene-table kernels of the
+// batched executor (mph_batch.hip) both call these functions, so a scene's
+// arithmetic is the same instruction sequence whether it runs alone or in a batch.
+// Every body keeps the reference's operand order (y = beta*y + sum, with the sum
+// formed first, as myDcsrmv / myDdmv / myDcopy / myDscal of main.cpp:4015-4119).
+#pragma once
+
+#include "mph_kernels.h"
+
+namespace mph {
+namespace body {
+
+__host__ __device__ inline int nblk(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// CR scalars: rr0, rs0, rr, rho, rho_prev, tmp (= q.u), rs
+enum { kRR0 = 0, kRS0 = 1, kRR = 2, kRHO = 3, kRHOP = 4, kTMP = 5, kRS = 6, kNScal = 8 };
+
+__device__ inline int cell_id(const DevParams& P, int cx, int cy, int cz) {
+    const int* C = P.C;
+    if (P.dim == 2) return ((cx) % C[0] + C[0]) % C[0] * C[1] + ((cy) % C[1] + C[1]) % C[1];
+    return ((cx) % C[0] + C[0]) % C[0] * C[1] * C[2] + ((cy) % C[1] + C[1]) % C[1] * C[2] +
+           ((cz) % C[2] + C[2]) % C[2];
+}
+
+__device__ inline void grid_xyz(const Level& L, int g, int& gx, int& gy, int& gz) {
+    gx = (g / (L.cnt[1] * L.cnt[2])) % L.cnt[0];
+    gy = (g / L.cnt[2]) % L.cnt[1];
+    gz = g % L.cnt[2];
+}
+__device__ inline int grid_id(const Level& L, int gx, int gy, int gz) {
+    return (L.cnt[1] * L.cnt[2]) * gx + L.cnt[2] * gy + gz;
+}
+
+// pairwise tree over the 256 values of a workgroup (t[l] += t[l+h], h = 128..1);
+// the result is valid in thread 0.  Every thread of the workgroup must call it.
+__device__ inline double block_tree_sum(double t, double* sh) {
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    if (threadIdx.x < 128) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + 128];
+    __syncthreads();
+    double v = 0.0;
+    if (threadIdx.x < 64) {
+        v = sh[threadIdx.x] + sh[threadIdx.x + 64];
+        for (int h = 32; h >= 1; h >>= 1) v = v + __shfl_down(v, h, 64);
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// multigrid rows (preconditionWithMultiGrid)
+
+// P2G: Q0 = 0*Q0 + sum over the cell's fluid particles of q (2964-2997)
+__device__ inline void mg_p2g_row(const DevParams& P, const Level& L, const int* __restrict__ cellStart,
+                                  const double* __restrict__ q, int row) {
+    const int g = row / 3, rd = row % 3;
+    int gx, gy, gz;
+    grid_xyz(L, g, gx, gy, gz);
+    const int ic = cell_id(P, gx + L.mn[0], gy + L.mn[1], gz + L.mn[2]);
+    double sum = 0.0;
+    for (int ip = cellStart[ic]; ip < cellStart[ic + 1]; ++ip) sum += 1.0 * 1.0 * q[3 * ip + rd];
+    double y = L.Q[row];
+    y *= 0.0;
+    y += sum;
+    L.Q[row] = y;
+}
+
+__device__ inline void mg_begin_row(const Level& L, int row) {  // myDscal(0.0, S); myDcopy(Q, R)
+    double* S = L.S[L.cur];
+    S[row] *= 0.0;
+    L.R[row] = L.Q[row];
+}
+
+__device__ inline void mg_update_row(const Level& L, int row) {  // myDdmv(1.0, invD, R, 1.0, S)
+    double* S = L.S[L.cur];
+    double y = S[row];
+    y *= 1.0;
+    y += 1.0 * L.invD[row] * L.R[row];
+    S[row] = y;
+}
+
+template <class VT>
+__device__ inline double mg_row_sum(const Level& L, const VT* __restrict__ A, const double* __restrict__ x, int row) {
+    const int g = row / 3;
+    int gx, gy, gz;
+    grid_xyz(L, g, gx, gy, gz);
+    double sum = 0.0;
+    int sh = 0;
+    for (int ex = 0; ex < L.cs[0]; ++ex) {
+        const int jx = (gx + ex - 1 + L.cnt[0]) % L.cnt[0];
+        for (int ey = 0; ey < L.cs[1]; ++ey) {
+            const int jy = (gy + ey - 1 + L.cnt[1]) % L.cnt[1];
+            for (int ez = 0; ez < L.cs[2]; ++ez) {
+                const int jz = (gz + ez - 1 + L.cnt[2]) % L.cnt[2];
+                const int col = 3 * grid_id(L, jx, jy, jz);
+                for (int sd = 0; sd < 3; ++sd, ++sh)
+                    sum += -1.0 * (double)A[(long long)sh * L.nrow + row] * x[col + sd];
+            }
+        }
+    }
+    return sum;
+}
+
+// myDcopy(Q, R); myDcsrmv(-1.0, A, S, 1.0, R)
+template <class VT>
+__device__ inline void mg_residual_row(const Level& L, int row) {
+    const VT* A = sizeof(VT) == 8 ? (const VT*)L.A : (const VT*)L.A32;
+    const double sum = mg_row_sum(L, A, L.S[L.cur], row);
+    double y = L.Q[row];
+    y *= 1.0;
+    y += sum;
+    L.R[row] = y;
+}
+
+// residual then smoother update of the same row; the new iterate goes to the other
+// S buffer so every thread still reads the old one
+template <class VT>
+__device__ inline void mg_residual_update_row(const Level& L, int row) {
+    const double* So = L.S[L.cur];
+    const VT* A = sizeof(VT) == 8 ? (const VT*)L.A : (const VT*)L.A32;
+    const double sum = mg_row_sum(L, A, So, row);
+    double y = L.Q[row];
+    y *= 1.0;
+    y += sum;
+    L.R[row] = y;
+    double s = So[row];
+    s *= 1.0;
+    s += 1.0 * L.invD[row] * y;
+    L.S[L.cur ^ 1][row] = s;
+}
+
+// restriction (R of 3115-3175): Q_L = 0*Q_L + sum of the clipped 2x2x2 children of R_S
+__device__ inline void mg_restrict_row(const Level& S, const Level& L, int row) {
+    const int g = row / 3, rd = row % 3;
+    int lx, ly, lz;
+    grid_xyz(L, g, lx, ly, lz);
+    const int lcx = lx + L.mn[0], lcy = ly + L.mn[1], lcz = lz + L.mn[2];
+    const int fmx = S.mn[0] > 2 * lcx, fmy = S.mn[1] > 2 * lcy, fmz = S.mn[2] > 2 * lcz;
+    const int fMx = S.mx[0] < 2 * (lcx + 1), fMy = S.mx[1] < 2 * (lcy + 1), fMz = S.mx[2] < 2 * (lcz + 1);
+    double sum = 0.0;
+    for (int ax = 0; ax < 2 - fmx - fMx; ++ax)
+        for (int ay = 0; ay < 2 - fmy - fMy; ++ay)
+            for (int az = 0; az < 2 - fmz - fMz; ++az) {
+                const int sg = grid_id(S, (2 * lcx + ax + fmx) - S.mn[0], (2 * lcy + ay + fmy) - S.mn[1],
+                                       (2 * lcz + az + fmz) - S.mn[2]);
+                sum += 1.0 * 1.0 * S.R[3 * sg + rd];
+            }
+    double y = L.Q[row];
+    y *= 0.0;
+    y += sum;
+    L.Q[row] = y;
+}
+
+// prolongation (P of 3216-3241): S_S += injection of the parent's S_L
+__device__ inline void mg_prolong_row(const Level& L, const Level& S, int row) {
+    const int g = row / 3, rd = row % 3;
+    int sx, sy, sz;
+    grid_xyz(S, g, sx, sy, sz);
+    const int lg = grid_id(L, ((sx + S.mn[0]) / 2) - L.mn[0], ((sy + S.mn[1]) / 2) - L.mn[1],
+                           ((sz + S.mn[2]) / 2) - L.mn[2]);
+    double sum = 0.0;
+    sum += 1.0 * 1.0 * L.S[L.cur][3 * lg + rd];
+    double* Sv = S.S[S.cur];
+    double y = Sv[row];
+    y *= 1.0;
+    y += sum;
+    Sv[row] = y;
+}
+
+// u = D^-1 q (4144-4148) + G2P S0 (4260), and the chunk partial of q.u (when part)
+__device__ inline void mg_g2p_chunk(const DevParams& P, const Level& L, const int* __restrict__ cellIndex,
+                                    const double* __restrict__ invDiagA, const double* __restrict__ q,
+                                    double* __restrict__ u, double* __restrict__ part, int chunk, double* sh) {
+    const int i = chunk * 256 + threadIdx.x;
+    double t = 0.0;
+    if (i < P.nf) {
+        const int key = cellIndex[i];
+        const int cx = key / (P.C[1] * P.C[2]) % P.C[0], cy = key / (P.C[2]) % P.C[1], cz = key % P.C[2];
+        const int g = grid_id(L, cx - L.mn[0], cy - L.mn[1], cz - L.mn[2]);
+        const double* S0 = L.S[L.cur];
+        double uu[3], qq[3];
+        for (int r = 0; r < 3; ++r) {
+            qq[r] = q[3 * i + r];
+            double y = u[3 * i + r];
+            y *= 0.0;
+            y += 1.0 * invDiagA[3 * i + r] * qq[r];
+            double sum = 0.0;
+            sum += 1.0 * 1.0 * S0[3 * g + r];
+            y *= 1.0;
+            y += sum;
+            uu[r] = y;
+            u[3 * i + r] = y;
+        }
+        t = (qq[0] * uu[0] + qq[1] * uu[1]) + qq[2] * uu[2];
+    }
+    if (part) {
+        const double v = block_tree_sum(t, sh);
+        if (threadIdx.x == 0) part[chunk] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// conjugate residual (solveWithConjugatedGradient, 4270-4421)
+
+// initial guess x = v - F/m dt (4299-4309)
+__device__ inline void cr_init_particle(const DevParams& P, const double* __restrict__ vel,
+                                        const double* __restrict__ force, const double* __restrict__ mass,
+                                        double* __restrict__ x, int i) {
+    for (int d = 0; d < 3; ++d) x[3 * i + d] = vel[3 * i + d] - force[3 * i + d] / mass[i] * P.dt;
+}
+
+// y = beta*yin + sum_k (alpha*a_k)*x_k over the sliced A (myDcsrmvForA, 4033-4050);
+// alpha is +1 or -1, so alpha*a is exact and is applied as a sign.  Optional second
+// output y2 = beta*y2 + sum (same sum), optional copy p = x, and the chunk partial
+// of dotx.y.  One chunk = 4 slices = 256 particles.  The row sums run over the
+// blocks in list order, entries s = 0,1,2 within a block, as the CSR.
+template <bool kNeg, class VT>
+__device__ inline void spmv_chunk(int nf, const int* __restrict__ nFluid, const long long* __restrict__ start,
+                                  const int* __restrict__ col, const VT* __restrict__ val,
+                                  const double* __restrict__ x, double beta, const double* yin, double* y,
+                                  double* y2, double* p_copy, const double* __restrict__ dotx,
+                                  double* __restrict__ part, int chunk, double* sh) {
+    const int i = chunk * 256 + threadIdx.x;
+    double t = 0.0;
+    if (i < nf) {
+        const int w = nFluid[i];
+        const int lane = i % kSlice;
+        const long long sb = start[i / kSlice];
+        const int* cp = col + sb * kSlice + lane;
+        const VT* vp = val + sb * 9 * kSlice + lane;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        // the matrix is read once per launch: streaming (nontemporal) loads; groups of
+        // kUn blocks have all their loads issued before the first use
+        constexpr int kUn = 4;
+        int k = 0;
+        for (; k + kUn <= w; k += kUn) {
+            int j[kUn];
+            VT a[kUn][9];
+#pragma unroll
+            for (int u = 0; u < kUn; ++u) j[u] = __builtin_nontemporal_load(cp + (long long)(k + u) * kSlice);
+#pragma unroll
+            for (int u = 0; u < kUn; ++u)
+#pragma unroll
+                for (int e = 0; e < 9; ++e)
+                    a[u][e] = __builtin_nontemporal_load(vp + ((long long)(k + u) * 9 + e) * kSlice);
+#pragma unroll
+            for (int u = 0; u < kUn; ++u) {
+                const double x0 = x[3 * j[u]], x1 = x[3 * j[u] + 1], x2 = x[3 * j[u] + 2];
+                double c[9];
+#pragma unroll
+                for (int e = 0; e < 9; ++e) c[e] = kNeg ? -(double)a[u][e] : (double)a[u][e];
+                s0 += c[0] * x0; s0 += c[1] * x1; s0 += c[2] * x2;
+                s1 += c[3] * x0; s1 += c[4] * x1; s1 += c[5] * x2;
+                s2 += c[6] * x0; s2 += c[7] * x1; s2 += c[8] * x2;
+            }
+        }
+        for (; k < w; ++k) {
+            const int j = __builtin_nontemporal_load(cp + (long long)k * kSlice);
+            const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
+            double c[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) {
+                const double v = (double)__builtin_nontemporal_load(vp + ((long long)k * 9 + e) * kSlice);
+                c[e] = kNeg ? -v : v;
+            }
+            s0 += c[0] * x0; s0 += c[1] * x1; s0 += c[2] * x2;
+            s1 += c[3] * x0; s1 += c[4] * x1; s1 += c[5] * x2;
+            s2 += c[6] * x0; s2 += c[7] * x1; s2 += c[8] * x2;
+        }
+        const double sm[3] = {s0, s1, s2};
+        double out[3];
+        for (int r = 0; r < 3; ++r) {
+            double yy = yin[3 * i + r];
+            yy *= beta;
+            yy += sm[r];
+            out[r] = yy;
+        }
+        if (y2)
+            for (int r = 0; r < 3; ++r) {
+                double yy = y2[3 * i + r];
+                yy *= beta;
+                yy += sm[r];
+                y2[3 * i + r] = yy;
+            }
+        for (int r = 0; r < 3; ++r) y[3 * i + r] = out[r];
+        if (p_copy)
+            for (int r = 0; r < 3; ++r) p_copy[3 * i + r] = x[3 * i + r];
+        if (part) {
+            const double d0 = dotx[3 * i], d1 = dotx[3 * i + 1], d2 = dotx[3 * i + 2];
+            t = (d0 * out[0] + d1 * out[1]) + d2 * out[2];
+        }
+    }
+    if (part) {
+        const double v = block_tree_sum(t, sh);
+        if (threadIdx.x == 0) part[chunk] = v;
+    }
+}
+
+// chunk partial of x.y: per particle (x0 y0 + x1 y1) + x2 y2, then the chunk tree
+__device__ inline void dot_partial_chunk(int nf, const double* __restrict__ x, const double* __restrict__ y,
+                                         double* __restrict__ part, int chunk, double* sh) {
+    const int i = chunk * 256 + threadIdx.x;
+    double t = 0.0;
+    if (i < nf) t = (x[3 * i] * y[3 * i] + x[3 * i + 1] * y[3 * i + 1]) + x[3 * i + 2] * y[3 * i + 2];
+    const double v = block_tree_sum(t, sh);
+    if (threadIdx.x == 0) part[chunk] = v;
+}
+
+// fold the chunk partials: lane l sums chunks l, l+256, ... from 0.0, then the tree.
+// scal[shift_slot] receives the previous scal[slot] first (rho -> rho_prev).
+__device__ inline void dot_final_body(const double* __restrict__ part, int nchunks, double* scal, int slot,
+                                      int shift_slot, double* sh) {
+    double acc = 0.0;
+    for (int c = threadIdx.x; c < nchunks; c += 256) acc = acc + part[c];
+    const double v = block_tree_sum(acc, sh);
+    if (threadIdx.x == 0) {
+        if (shift_slot >= 0) scal[shift_slot] = scal[slot];
+        scal[slot] = v;
+    }
+}
+
+// p = beta*p + 1.0*s, q = beta*q + 1.0*y with beta = rho/rho_prev (4360-4364)
+__device__ inline void pq_update_elem(const double* __restrict__ scal, const double* __restrict__ s,
+                                      const double* __restrict__ y, double* __restrict__ p, double* __restrict__ q,
+                                      long long k) {
+    const double beta = scal[kRHO] / scal[kRHOP];
+    double pp = p[k];
+    pp *= beta;
+    pp += 1.0 * s[k];
+    p[k] = pp;
+    double qq = q[k];
+    qq *= beta;
+    qq += 1.0 * y[k];
+    q[k] = qq;
+}
+
+// alpha = rho/tmp; x += alpha p; r -= alpha q; s -= alpha u (4372-4375) + r.r partial
+__device__ inline void xrs_update_chunk(int nf, const double* __restrict__ scal, const double* __restrict__ p,
+                                        const double* __restrict__ q, const double* __restrict__ u,
+                                        double* __restrict__ x, double* __restrict__ r, double* __restrict__ s,
+                                        double* __restrict__ part, int chunk, double* sh) {
+    const int i = chunk * 256 + threadIdx.x;
+    double t = 0.0;
+    if (i < nf) {
+        const double alpha = scal[kRHO] / scal[kTMP];
+        const double ma = -alpha;
+        double rr[3];
+        for (int d = 0; d < 3; ++d) {
+            const int k = 3 * i + d;
+            x[k] += alpha * p[k];
+            rr[d] = r[k] + ma * q[k];
+            r[k] = rr[d];
+            s[k] += ma * u[k];
+        }
+        t = (rr[0] * rr[0] + rr[1] * rr[1]) + rr[2] * rr[2];
+    }
+    const double v = block_tree_sum(t, sh);
+    if (threadIdx.x == 0) part[chunk] = v;
+}
+
+}  // namespace body
+}  // namespace mph

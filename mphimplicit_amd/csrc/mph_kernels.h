@@ -103,6 +103,7 @@ void launch_mg_galerkin0(hipStream_t st, const DevParams& P, const Level& L, con
 void launch_mg_galerkinL(hipStream_t st, const Level& S, const Level& L);
 void launch_mg_invd(hipStream_t st, const Level& L);
 void launch_fill(hipStream_t st, double* x, long long n, double v);
+void launch_fill_i32(hipStream_t st, int* x, long long n, int v);  // a kernel, not a memset node
 void launch_mg_p2g(hipStream_t st, const DevParams& P, Level& L, const int* cellStart, const double* q);
 void launch_mg_begin(hipStream_t st, Level& L);        // S *= 0; R = Q
 void launch_mg_update(hipStream_t st, Level& L);       // S += invD R

@@ -14,12 +14,13 @@
 #include <string>
 #include <vector>
 
-#include "../../include/mph_abi.h"
-#include "mph_kernels.h"
+#include "mph_scene_impl.h"
 
 using namespace mph;
+using namespace mph::host;
 
-namespace {
+namespace mph {
+namespace host {
 
 thread_local std::string g_err;
 
@@ -27,106 +28,6 @@ int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
-
-#define HIPCHK(expr)                                                                                   \
-    do {                                                                                               \
-        hipError_t e_ = (expr);                                                                        \
-        if (e_ != hipSuccess)                                                                          \
-            return fail(MPH_E_DEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_));         \
-    } while (0)
-
-template <class T>
-struct DBuf {
-    T* p = nullptr;
-    size_t cap = 0;  // elements
-    int ensure(size_t n) {
-        if (n <= cap && p) return MPH_OK;
-        size_t want = n + n / 4 + 64;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        if (hipMalloc((void**)&p, want * sizeof(T)) != hipSuccess) {
-            p = nullptr;
-            return fail(MPH_E_NOMEM, "device allocation of " + std::to_string(want * sizeof(T)) + " bytes failed");
-        }
-        cap = want;
-        return MPH_OK;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-struct HostLevel {
-    Level L{};
-    DBuf<double> A, invD, S0, S1, R, Q;
-    DBuf<float> A32;
-};
-
-enum { kRR0 = 0, kRS0 = 1, kRR = 2, kRHO = 3, kRHOP = 4, kTMP = 5, kRS = 6 };
-
-}  // namespace
-
-struct mph_scene {
-    mph_params prm{};
-    DevParams P{};
-    double dmax[3] = {0, 0, 0};
-    int device = 0;
-    hipStream_t st = nullptr;
-    // host-side state of the reference's globals
-    double time = 0, outputNext = 0, vtkOutputNext = 0;
-    int iStep = 0;
-    bool vtkStep = false;
-    WallKin wk{};
-    double Swa = 1, Swg = 1, Swp = 1, Swv = 1;
-    // particles
-    DBuf<int> type, type2, perm, perm2;
-    DBuf<double> pos, pos2, vel, vel2, force, mass, kappa, lambda, mu, densA, gcen, pressA, volStrainP, divP,
-        pressP, virialP, virialStress;
-    // cells
-    DBuf<uint64_t> keys;
-    DBuf<int> cellIndex, cellStart, bbox;
-    // neighbours
-    DBuf<int> nCount, nFluid, nCountP, nInd, nIndP, maxP;
-    DBuf<long long> nPtr, nPtrP, scanScratch;
-    DBuf<unsigned short> runStart;
-    long long E = 0, EP = 0;
-    int hmaxP = 0;
-    // A and C
-    int nslices = 0;
-    DBuf<int> width, selfIdx, acol;
-    DBuf<long long> sliceStart;
-    DBuf<double> aval, b, cval, dval, vecP;
-    DBuf<float> aval32;  // FP32 copy of A (mixed precision only)
-    long long slots = 0;
-    // multigrid
-    int hbbox[6] = {0, 0, 0, 0, 0, 0};
-    int depth = 0;
-    std::vector<HostLevel> lv;
-    DBuf<double> invDiagA;
-    // CR
-    DBuf<double> x, r, s, y, p, q, u, part, scal, rOut, dvec;
-    int irOuter = 0;
-    double* hpin = nullptr;  // pinned host scalars
-    int crIter = 0;
-    double rr0 = 0, rr = 0, rs0 = 0, rs = 0;
-    // timing
-    bool timing = false;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    double msPhase[12] = {0};
-    // per-launch timing of the particle-matrix SpMV (events on the scene stream)
-    bool spmvTiming = false;
-    std::vector<hipEvent_t> spmvEv;
-    std::vector<char> spmvEvF32;  // per event pair: FP32-matrix launch
-    int spmvUsed = 0;
-    double spmvMs = 0, spmvMs32 = 0;
-    long long spmvLaunches = 0, spmvLaunches32 = 0;
-    int spmvTimingErr = 0;
-};
-
-namespace {
 
 // ---------------------------------------------------------------------------
 // constants, as initializeWeight/Fluid/Wall/Domain (source/main.cpp:886-1162)
@@ -309,9 +210,9 @@ int phase_cells(mph_scene& S) {
     return MPH_OK;
 }
 
-int phase_neighbor(mph_scene& S) {
+int phase_neighbor_launch(mph_scene& S) {
     const DevParams& P = S.P;
-    HIPCHK(hipMemsetAsync(S.maxP.p, 0, sizeof(int), S.st));
+    launch_fill_i32(S.st, S.maxP.p, 1, 0);
     launch_neighbor_count(S.st, P, S.pos.p, S.cellIndex.p, S.cellStart.p, S.nCount.p, S.nFluid.p, S.nCountP.p,
                           S.maxP.p);
     launch_scan_i64(S.st, S.nCount.p, S.nPtr.p, P.np, S.scanScratch.p, (int)S.scanScratch.cap);
@@ -324,8 +225,14 @@ int phase_neighbor(mph_scene& S) {
     HIPCHK(hipMemcpyAsync(&h[2], S.sliceStart.p + S.nslices, 8, hipMemcpyDeviceToHost, S.st));
     HIPCHK(hipMemcpyAsync(&h[3], S.bbox.p, 6 * sizeof(int), hipMemcpyDeviceToHost, S.st));
     HIPCHK(hipMemcpyAsync(&h[6], S.maxP.p, sizeof(int), hipMemcpyDeviceToHost, S.st));
-    HIPCHK(hipStreamSynchronize(S.st));
-    S.hmaxP = *(int*)&h[6];
+    return MPH_OK;
+}
+
+// the stream must have reached the copies of phase_neighbor_launch
+int phase_neighbor_finish(mph_scene& S) {
+    const DevParams& P = S.P;
+    const long long* h = (const long long*)S.hpin;
+    S.hmaxP = *(const int*)&h[6];
     S.E = h[0];
     S.EP = h[1];
     S.slots = h[2] * kSlice;
@@ -335,6 +242,13 @@ int phase_neighbor(mph_scene& S) {
     launch_neighbor_fill(S.st, P, S.pos.p, S.cellIndex.p, S.cellStart.p, S.nCount.p, S.nPtr.p, S.nPtrP.p, S.nInd.p,
                          S.nIndP.p, S.runStart.p);
     return MPH_OK;
+}
+
+int phase_neighbor(mph_scene& S) {
+    int rc;
+    if ((rc = phase_neighbor_launch(S))) return rc;
+    HIPCHK(hipStreamSynchronize(S.st));
+    return phase_neighbor_finish(S);
 }
 
 int phase_explicit(mph_scene& S) {
@@ -715,6 +629,20 @@ int run_phase(mph_scene& S, int ph) {
     return MPH_OK;
 }
 
+bool step_wanted(const mph_scene& S) { return S.time < S.prm.end_time + 1.0e-5 * S.P.dt; }
+
+void step_begin(mph_scene& S) {
+    if (S.time + 1.0e-5 * S.P.dt >= S.outputNext) S.outputNext += S.prm.output_interval;
+    S.vtkStep = false;
+}
+
+bool step_vtk_due(const mph_scene& S) { return S.time + 1.0e-5 * S.P.dt >= S.vtkOutputNext; }
+
+void step_vtk_done(mph_scene& S) {
+    S.vtkOutputNext += S.prm.vtk_output_interval;
+    S.vtkStep = true;
+}
+
 int allocate(mph_scene& S) {
     const DevParams& P = S.P;
     const size_t np = P.np, nf = P.nf;
@@ -936,7 +864,8 @@ int host_field(mph_scene& S, const std::string& n, Blob& out, bool& found) {
     return MPH_OK;
 }
 
-}  // namespace
+}  // namespace host
+}  // namespace mph
 
 // ===========================================================================
 // C ABI
@@ -1041,16 +970,13 @@ int mph_scene_step(mph_scene* S, int nsteps, mph_step_stats* st) {
     (void)hipEventRecord(t0, S->st);
     int rc = MPH_OK;
     for (int k = 0; k < nsteps && rc == MPH_OK; ++k) {
-        const double dt = S->P.dt;
-        if (!(S->time < S->prm.end_time + 1.0e-5 * dt)) break;
-        if (S->time + 1.0e-5 * dt >= S->outputNext) S->outputNext += S->prm.output_interval;
+        if (!step_wanted(*S)) break;
+        step_begin(*S);
         for (int ph = MPH_PHASE_KINEMATICS; ph <= MPH_PHASE_SOLVE && rc == MPH_OK; ++ph) rc = run_phase(*S, ph);
         if (rc) break;
-        S->vtkStep = false;
-        if (S->time + 1.0e-5 * dt >= S->vtkOutputNext) {
+        if (step_vtk_due(*S)) {
             if ((rc = run_phase(*S, MPH_PHASE_DIAGNOSTICS))) break;
-            S->vtkOutputNext += S->prm.vtk_output_interval;
-            S->vtkStep = true;
+            step_vtk_done(*S);
             local.vtk_steps++;
         }
         run_phase(*S, MPH_PHASE_ADVANCE);
@@ -1198,6 +1124,31 @@ int mph_scene_set_timing(mph_scene* S, int on) {
     S->spmvTiming = (on & 2) != 0;
     S->spmvMs = 0;
     S->spmvLaunches = 0;
+    return MPH_OK;
+}
+
+int mph_scene_device_bytes(mph_scene* S, unsigned long long* bytes) {
+    if (!S || !bytes) return fail(MPH_E_ARG, "null argument");
+    unsigned long long n = 0;
+    for (DBuf<int>* b : {&S->type, &S->type2, &S->perm, &S->perm2, &S->cellIndex, &S->cellStart, &S->bbox,
+                         &S->nCount, &S->nFluid, &S->nCountP, &S->nInd, &S->nIndP, &S->width, &S->selfIdx, &S->acol,
+                         &S->maxP})
+        n += b->cap * sizeof(int);
+    n += S->runStart.cap * sizeof(unsigned short);
+    for (DBuf<double>* b : {&S->pos, &S->pos2, &S->vel, &S->vel2, &S->force, &S->mass, &S->kappa, &S->lambda,
+                            &S->mu, &S->densA, &S->gcen, &S->pressA, &S->volStrainP, &S->divP, &S->pressP,
+                            &S->virialP, &S->virialStress, &S->aval, &S->b, &S->cval, &S->dval, &S->vecP,
+                            &S->invDiagA, &S->rOut, &S->dvec, &S->x, &S->r, &S->s, &S->y, &S->p, &S->q, &S->u,
+                            &S->part, &S->scal})
+        n += b->cap * sizeof(double);
+    for (DBuf<long long>* b : {&S->nPtr, &S->nPtrP, &S->scanScratch, &S->sliceStart}) n += b->cap * sizeof(long long);
+    n += S->keys.cap * sizeof(uint64_t);
+    for (HostLevel& H : S->lv) {
+        for (DBuf<double>* b : {&H.A, &H.invD, &H.S0, &H.S1, &H.R, &H.Q}) n += b->cap * sizeof(double);
+        n += H.A32.cap * sizeof(float);
+    }
+    n += S->aval32.cap * sizeof(float);
+    *bytes = n;
     return MPH_OK;
 }
 

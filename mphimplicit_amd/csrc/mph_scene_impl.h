@@ -1,0 +1,137 @@
+// mph_scene_impl.h -- internal: the scene record behind the C ABI handle and the
+// phase drivers that the batched executor (mph_batch.cpp) shares with the
+// single-scene path (mph_scene.cpp).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mph_abi.h"
+#include "mph_kernels.h"
+
+namespace mph {
+namespace host {
+
+// record the message of the failing call (mph_last_error) and return the code
+int fail(int code, const std::string& msg);
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return ::mph::host::fail(MPH_E_DEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;  // elements
+    int ensure(size_t n) {
+        if (n <= cap && p) return MPH_OK;
+        size_t want = n + n / 4 + 64;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipMalloc((void**)&p, want * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return fail(MPH_E_NOMEM, "device allocation of " + std::to_string(want * sizeof(T)) + " bytes failed");
+        }
+        cap = want;
+        return MPH_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct HostLevel {
+    Level L{};
+    DBuf<double> A, invD, S0, S1, R, Q;
+    DBuf<float> A32;
+};
+
+enum { kRR0 = 0, kRS0 = 1, kRR = 2, kRHO = 3, kRHOP = 4, kTMP = 5, kRS = 6 };
+
+}  // namespace host
+}  // namespace mph
+
+struct mph_scene {
+    mph_params prm{};
+    mph::DevParams P{};
+    double dmax[3] = {0, 0, 0};
+    int device = 0;
+    hipStream_t st = nullptr;
+    // host-side state of the reference's globals
+    double time = 0, outputNext = 0, vtkOutputNext = 0;
+    int iStep = 0;
+    bool vtkStep = false;
+    mph::WallKin wk{};
+    double Swa = 1, Swg = 1, Swp = 1, Swv = 1;
+    // particles
+    mph::host::DBuf<int> type, type2, perm, perm2;
+    mph::host::DBuf<double> pos, pos2, vel, vel2, force, mass, kappa, lambda, mu, densA, gcen, pressA, volStrainP, divP,
+        pressP, virialP, virialStress;
+    // cells
+    mph::host::DBuf<uint64_t> keys;
+    mph::host::DBuf<int> cellIndex, cellStart, bbox;
+    // neighbours
+    mph::host::DBuf<int> nCount, nFluid, nCountP, nInd, nIndP, maxP;
+    mph::host::DBuf<long long> nPtr, nPtrP, scanScratch;
+    mph::host::DBuf<unsigned short> runStart;
+    long long E = 0, EP = 0;
+    int hmaxP = 0;
+    // A and C
+    int nslices = 0;
+    mph::host::DBuf<int> width, selfIdx, acol;
+    mph::host::DBuf<long long> sliceStart;
+    mph::host::DBuf<double> aval, b, cval, dval, vecP;
+    mph::host::DBuf<float> aval32;  // FP32 copy of A (mixed precision only)
+    long long slots = 0;
+    // multigrid
+    int hbbox[6] = {0, 0, 0, 0, 0, 0};
+    int depth = 0;
+    std::vector<mph::host::HostLevel> lv;
+    mph::host::DBuf<double> invDiagA;
+    // CR
+    mph::host::DBuf<double> x, r, s, y, p, q, u, part, scal, rOut, dvec;
+    int irOuter = 0;
+    double* hpin = nullptr;  // pinned host scalars
+    int crIter = 0;
+    double rr0 = 0, rr = 0, rs0 = 0, rs = 0;
+    // timing
+    bool timing = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double msPhase[12] = {0};
+    // per-launch timing of the particle-matrix SpMV (events on the scene stream)
+    bool spmvTiming = false;
+    std::vector<hipEvent_t> spmvEv;
+    std::vector<char> spmvEvF32;  // per event pair: FP32-matrix launch
+    int spmvUsed = 0;
+    double spmvMs = 0, spmvMs32 = 0;
+    long long spmvLaunches = 0, spmvLaunches32 = 0;
+    int spmvTimingErr = 0;
+};
+
+
+namespace mph {
+namespace host {
+
+bool mixed(const mph_scene& S);
+ASlices aslices(mph_scene& S);
+int run_phase(mph_scene& S, int ph);
+// calculateNeighbor in two halves around the host readback of the list totals:
+// launch = counts, scans and the async copies; finish = read them, grow, fill
+int phase_neighbor_launch(mph_scene& S);
+int phase_neighbor_finish(mph_scene& S);
+// the bookkeeping of one time step around the phases (main.cpp:386-474)
+bool step_wanted(const mph_scene& S);
+void step_begin(mph_scene& S);
+bool step_vtk_due(const mph_scene& S);
+void step_vtk_done(mph_scene& S);
+
+}  // namespace host
+}  // namespace mph

@@ -8,44 +8,18 @@
 // Dot products use one fixed tree (chunks of 256 particles, then 256 lanes over the
 // chunk sums), independent of launch configuration; the CPU oracle implements the
 // same tree so the solve can be compared bit for bit.
-#include "mph_kernels.h"
+#include "mph_cr_body.h"
 
 namespace mph {
 
 namespace {
 
-inline int nblk(long long n, int bs) { return (int)((n + bs - 1) / bs); }
-
-__device__ inline int cell_id(const DevParams& P, int cx, int cy, int cz) {
-    const int* C = P.C;
-    if (P.dim == 2) return ((cx) % C[0] + C[0]) % C[0] * C[1] + ((cy) % C[1] + C[1]) % C[1];
-    return ((cx) % C[0] + C[0]) % C[0] * C[1] * C[2] + ((cy) % C[1] + C[1]) % C[1] * C[2] +
-           ((cz) % C[2] + C[2]) % C[2];
-}
-
-__device__ inline void grid_xyz(const Level& L, int g, int& gx, int& gy, int& gz) {
-    gx = (g / (L.cnt[1] * L.cnt[2])) % L.cnt[0];
-    gy = (g / L.cnt[2]) % L.cnt[1];
-    gz = g % L.cnt[2];
-}
-__device__ inline int grid_id(const Level& L, int gx, int gy, int gz) {
-    return (L.cnt[1] * L.cnt[2]) * gx + L.cnt[2] * gy + gz;
-}
-
-// pairwise tree over the 256 values of a workgroup (t[l] += t[l+h], h = 128..1);
-// the result is valid in thread 0
-__device__ inline double block_tree_sum(double t, double* sh) {
-    sh[threadIdx.x] = t;
-    __syncthreads();
-    if (threadIdx.x < 128) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + 128];
-    __syncthreads();
-    double v = 0.0;
-    if (threadIdx.x < 64) {
-        v = sh[threadIdx.x] + sh[threadIdx.x + 64];
-        for (int h = 32; h >= 1; h >>= 1) v = v + __shfl_down(v, h, 64);
-    }
-    return v;
-}
+using body::block_tree_sum;
+using body::cell_id;
+using body::grid_id;
+using body::grid_xyz;
+using body::nblk;
+using namespace body;  // CR scalar slots
 
 // ---------------------------------------------------------------------------
 // hierarchy build
@@ -195,171 +169,58 @@ __global__ void k_fill(double* x, long long n, double v) {
     if (k < n) x[k] = v;
 }
 
+__global__ void k_fill_i32(int* x, long long n, int v) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] = v;
+}
+
 // ---------------------------------------------------------------------------
 // V-cycle (preconditionWithMultiGrid).  Every step is myDcsrmv / myDdmv /
 // myDcopy / myDscal of the reference: y = beta*y + sum_k (alpha*a_k)*x_k with the
 // sum formed first, y = beta*y + (alpha*d)*x.
 
-// P2G: Q0 = 0*Q0 + sum over the cell's fluid particles of q (2964-2997)
 __global__ void k_mg_p2g(DevParams P, Level L, const int* __restrict__ cellStart, const double* __restrict__ q) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= L.nrow) return;
-    const int g = row / 3, rd = row % 3;
-    int gx, gy, gz;
-    grid_xyz(L, g, gx, gy, gz);
-    const int ic = cell_id(P, gx + L.mn[0], gy + L.mn[1], gz + L.mn[2]);
-    double sum = 0.0;
-    for (int ip = cellStart[ic]; ip < cellStart[ic + 1]; ++ip) sum += 1.0 * 1.0 * q[3 * ip + rd];
-    double y = L.Q[row];
-    y *= 0.0;
-    y += sum;
-    L.Q[row] = y;
+    if (row < L.nrow) mg_p2g_row(P, L, cellStart, q, row);
 }
 
-__global__ void k_mg_begin(Level L) {  // myDscal(0.0, S); myDcopy(Q, R)
+__global__ void k_mg_begin(Level L) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= L.nrow) return;
-    double* S = L.S[L.cur];
-    S[row] *= 0.0;
-    L.R[row] = L.Q[row];
+    if (row < L.nrow) mg_begin_row(L, row);
 }
 
-__global__ void k_mg_update(Level L) {  // myDdmv(1.0, invD, R, 1.0, S)
+__global__ void k_mg_update(Level L) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= L.nrow) return;
-    double* S = L.S[L.cur];
-    double y = S[row];
-    y *= 1.0;
-    y += 1.0 * L.invD[row] * L.R[row];
-    S[row] = y;
+    if (row < L.nrow) mg_update_row(L, row);
 }
 
 template <class VT>
-__device__ inline double mg_row_sum(const Level& L, const VT* __restrict__ A, const double* __restrict__ x, int row) {
-    const int g = row / 3;
-    int gx, gy, gz;
-    grid_xyz(L, g, gx, gy, gz);
-    double sum = 0.0;
-    int sh = 0;
-    for (int ex = 0; ex < L.cs[0]; ++ex) {
-        const int jx = (gx + ex - 1 + L.cnt[0]) % L.cnt[0];
-        for (int ey = 0; ey < L.cs[1]; ++ey) {
-            const int jy = (gy + ey - 1 + L.cnt[1]) % L.cnt[1];
-            for (int ez = 0; ez < L.cs[2]; ++ez) {
-                const int jz = (gz + ez - 1 + L.cnt[2]) % L.cnt[2];
-                const int col = 3 * grid_id(L, jx, jy, jz);
-                for (int sd = 0; sd < 3; ++sd, ++sh)
-                    sum += -1.0 * (double)A[(long long)sh * L.nrow + row] * x[col + sd];
-            }
-        }
-    }
-    return sum;
-}
-
-template <class VT>
-__global__ void k_mg_residual(Level L) {  // myDcopy(Q, R); myDcsrmv(-1.0, A, S, 1.0, R)
+__global__ void k_mg_residual(Level L) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= L.nrow) return;
-    const VT* A = sizeof(VT) == 8 ? (const VT*)L.A : (const VT*)L.A32;
-    const double sum = mg_row_sum(L, A, L.S[L.cur], row);
-    double y = L.Q[row];
-    y *= 1.0;
-    y += sum;
-    L.R[row] = y;
+    if (row < L.nrow) mg_residual_row<VT>(L, row);
 }
 
-// residual then smoother update of the same row; the new iterate goes to the other
-// S buffer so every thread still reads the old one
 template <class VT>
 __global__ void k_mg_residual_update(Level L) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= L.nrow) return;
-    const double* So = L.S[L.cur];
-    const VT* A = sizeof(VT) == 8 ? (const VT*)L.A : (const VT*)L.A32;
-    const double sum = mg_row_sum(L, A, So, row);
-    double y = L.Q[row];
-    y *= 1.0;
-    y += sum;
-    L.R[row] = y;
-    double s = So[row];
-    s *= 1.0;
-    s += 1.0 * L.invD[row] * y;
-    L.S[L.cur ^ 1][row] = s;
+    if (row < L.nrow) mg_residual_update_row<VT>(L, row);
 }
 
-// restriction (R of 3115-3175): Q_L = 0*Q_L + sum of the clipped 2x2x2 children of R_S
 __global__ void k_mg_restrict(Level S, Level L) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= L.nrow) return;
-    const int g = row / 3, rd = row % 3;
-    int lx, ly, lz;
-    grid_xyz(L, g, lx, ly, lz);
-    const int lcx = lx + L.mn[0], lcy = ly + L.mn[1], lcz = lz + L.mn[2];
-    const int fmx = S.mn[0] > 2 * lcx, fmy = S.mn[1] > 2 * lcy, fmz = S.mn[2] > 2 * lcz;
-    const int fMx = S.mx[0] < 2 * (lcx + 1), fMy = S.mx[1] < 2 * (lcy + 1), fMz = S.mx[2] < 2 * (lcz + 1);
-    double sum = 0.0;
-    for (int ax = 0; ax < 2 - fmx - fMx; ++ax)
-        for (int ay = 0; ay < 2 - fmy - fMy; ++ay)
-            for (int az = 0; az < 2 - fmz - fMz; ++az) {
-                const int sg = grid_id(S, (2 * lcx + ax + fmx) - S.mn[0], (2 * lcy + ay + fmy) - S.mn[1],
-                                       (2 * lcz + az + fmz) - S.mn[2]);
-                sum += 1.0 * 1.0 * S.R[3 * sg + rd];
-            }
-    double y = L.Q[row];
-    y *= 0.0;
-    y += sum;
-    L.Q[row] = y;
+    if (row < L.nrow) mg_restrict_row(S, L, row);
 }
 
-// prolongation (P of 3216-3241): S_S += injection of the parent's S_L
 __global__ void k_mg_prolong(Level L, Level S) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= S.nrow) return;
-    const int g = row / 3, rd = row % 3;
-    int sx, sy, sz;
-    grid_xyz(S, g, sx, sy, sz);
-    const int lg = grid_id(L, ((sx + S.mn[0]) / 2) - L.mn[0], ((sy + S.mn[1]) / 2) - L.mn[1],
-                           ((sz + S.mn[2]) / 2) - L.mn[2]);
-    double sum = 0.0;
-    sum += 1.0 * 1.0 * L.S[L.cur][3 * lg + rd];
-    double* Sv = S.S[S.cur];
-    double y = Sv[row];
-    y *= 1.0;
-    y += sum;
-    Sv[row] = y;
+    if (row < S.nrow) mg_prolong_row(L, S, row);
 }
 
-// u = D^-1 q (4144-4148) + G2P S0 (4260), and the chunk partial of q.u
 __global__ void __launch_bounds__(256) k_mg_g2p(DevParams P, Level L, const int* __restrict__ cellIndex,
                                                 const double* __restrict__ invDiagA, const double* __restrict__ q,
                                                 double* __restrict__ u, double* __restrict__ part) {
     __shared__ double sh[256];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double t = 0.0;
-    if (i < P.nf) {
-        const int key = cellIndex[i];
-        const int cx = key / (P.C[1] * P.C[2]) % P.C[0], cy = key / (P.C[2]) % P.C[1], cz = key % P.C[2];
-        const int g = grid_id(L, cx - L.mn[0], cy - L.mn[1], cz - L.mn[2]);
-        const double* S0 = L.S[L.cur];
-        double uu[3], qq[3];
-        for (int r = 0; r < 3; ++r) {
-            qq[r] = q[3 * i + r];
-            double y = u[3 * i + r];
-            y *= 0.0;
-            y += 1.0 * invDiagA[3 * i + r] * qq[r];
-            double sum = 0.0;
-            sum += 1.0 * 1.0 * S0[3 * g + r];
-            y *= 1.0;
-            y += sum;
-            uu[r] = y;
-            u[3 * i + r] = y;
-        }
-        t = (qq[0] * uu[0] + qq[1] * uu[1]) + qq[2] * uu[2];
-    }
-    if (part) {
-        const double v = block_tree_sum(t, sh);
-        if (threadIdx.x == 0) part[blockIdx.x] = v;
-    }
+    mg_g2p_chunk(P, L, cellIndex, invDiagA, q, u, part, blockIdx.x, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -368,161 +229,45 @@ __global__ void __launch_bounds__(256) k_mg_g2p(DevParams P, Level L, const int*
 __global__ void k_cr_init(DevParams P, const double* __restrict__ vel, const double* __restrict__ force,
                           const double* __restrict__ mass, double* __restrict__ x) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.nf) return;
-    for (int d = 0; d < 3; ++d) x[3 * i + d] = vel[3 * i + d] - force[3 * i + d] / mass[i] * P.dt;
+    if (i < P.nf) cr_init_particle(P, vel, force, mass, x, i);
 }
 
-// Workgroup -> 256-particle chunk, arranged so that each XCD (hardware assigns
-// workgroup b to XCD b % 8) walks a contiguous range of chunks: neighbouring
-// particles' vector entries are then gathered through the same L2.
-__device__ inline int xcd_chunk(int b, int nb) {
-    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
-    return x * q + (x < r ? x : r) + k;
-}
-
-// y = beta*yin + sum_k (alpha*a_k)*x_k over the sliced A (myDcsrmvForA, 4033-4050);
-// alpha is +1 or -1, so alpha*a is exact and is applied as a sign.  Optional second
-// output y2 = beta*y2 + sum (same sum), optional copy p = x, and the chunk partial
-// of dotx.y.  Workgroup = 4 slices = one 256-particle dot chunk.  The row sums run
-// over the blocks in list order, entries s = 0,1,2 within a block, as the CSR.
 template <bool kNeg, class VT>
-__global__ void __launch_bounds__(256) k_spmv(DevParams P, const int* __restrict__ nFluid,
+__global__ void __launch_bounds__(256) k_spmv(int nf, const int* __restrict__ nFluid,
                                               const long long* __restrict__ start, const int* __restrict__ col,
                                               const VT* __restrict__ val, const double* __restrict__ x,
                                               double beta, const double* yin, double* y, double* y2,
                                               double* p_copy, const double* __restrict__ dotx,
                                               double* __restrict__ part) {
     __shared__ double sh[256];
-    const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
-    const int i = chunk * 256 + threadIdx.x;
-    double t = 0.0;
-    if (i < P.nf) {
-        const int w = nFluid[i];
-        const int lane = i % kSlice;
-        const long long sb = start[i / kSlice];
-        const int* cp = col + sb * kSlice + lane;
-        const VT* vp = val + sb * 9 * kSlice + lane;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-#define MPH_LD(k, e) (kNeg ? -(double)__builtin_nontemporal_load(vp + ((long long)(k) * 9 + (e)) * kSlice) \
-                           : (double)__builtin_nontemporal_load(vp + ((long long)(k) * 9 + (e)) * kSlice))
-#define MPH_BLOCK(k, j)                                                                  \
-    {                                                                                    \
-        const double x0 = x[3 * (j)], x1 = x[3 * (j) + 1], x2 = x[3 * (j) + 2];          \
-        const double a0 = MPH_LD(k, 0), a1 = MPH_LD(k, 1), a2 = MPH_LD(k, 2);            \
-        const double a3 = MPH_LD(k, 3), a4 = MPH_LD(k, 4), a5 = MPH_LD(k, 5);            \
-        const double a6 = MPH_LD(k, 6), a7 = MPH_LD(k, 7), a8 = MPH_LD(k, 8);            \
-        s0 += a0 * x0; s0 += a1 * x1; s0 += a2 * x2;                                     \
-        s1 += a3 * x0; s1 += a4 * x1; s1 += a5 * x2;                                     \
-        s2 += a6 * x0; s2 += a7 * x1; s2 += a8 * x2;                                     \
-    }
-        int k = 0;
-        for (; k + 2 <= w; k += 2) {
-            const int j0 = __builtin_nontemporal_load(cp + (long long)k * kSlice);
-            const int j1 = __builtin_nontemporal_load(cp + (long long)(k + 1) * kSlice);
-            MPH_BLOCK(k, j0)
-            MPH_BLOCK(k + 1, j1)
-        }
-        if (k < w) {
-            const int j0 = __builtin_nontemporal_load(cp + (long long)k * kSlice);
-            MPH_BLOCK(k, j0)
-        }
-#undef MPH_BLOCK
-#undef MPH_LD
-        const double sm[3] = {s0, s1, s2};
-        double out[3];
-        for (int r = 0; r < 3; ++r) {
-            double yy = yin[3 * i + r];
-            yy *= beta;
-            yy += sm[r];
-            out[r] = yy;
-        }
-        if (y2)
-            for (int r = 0; r < 3; ++r) {
-                double yy = y2[3 * i + r];
-                yy *= beta;
-                yy += sm[r];
-                y2[3 * i + r] = yy;
-            }
-        for (int r = 0; r < 3; ++r) y[3 * i + r] = out[r];
-        if (p_copy)
-            for (int r = 0; r < 3; ++r) p_copy[3 * i + r] = x[3 * i + r];
-        if (part) {
-            const double d0 = dotx[3 * i], d1 = dotx[3 * i + 1], d2 = dotx[3 * i + 2];
-            t = (d0 * out[0] + d1 * out[1]) + d2 * out[2];
-        }
-    }
-    if (part) {
-        const double v = block_tree_sum(t, sh);
-        if (threadIdx.x == 0) part[chunk] = v;
-    }
+    spmv_chunk<kNeg, VT>(nf, nFluid, start, col, val, x, beta, yin, y, y2, p_copy, dotx, part, blockIdx.x, sh);
 }
 
 __global__ void __launch_bounds__(256) k_dot_partial(int nf, const double* __restrict__ x,
                                                      const double* __restrict__ y, double* __restrict__ part) {
     __shared__ double sh[256];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double t = 0.0;
-    if (i < nf) t = (x[3 * i] * y[3 * i] + x[3 * i + 1] * y[3 * i + 1]) + x[3 * i + 2] * y[3 * i + 2];
-    const double v = block_tree_sum(t, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = v;
+    dot_partial_chunk(nf, x, y, part, blockIdx.x, sh);
 }
 
-// fold the chunk partials: lane l sums chunks l, l+256, ... from 0.0, then the tree.
-// scal[shift_slot] receives the previous scal[slot] first (rho -> rho_prev).
 __global__ void __launch_bounds__(256) k_dot_final(const double* __restrict__ part, int nchunks, double* scal,
                                                    int slot, int shift_slot) {
     __shared__ double sh[256];
-    double acc = 0.0;
-    for (int c = threadIdx.x; c < nchunks; c += 256) acc = acc + part[c];
-    const double v = block_tree_sum(acc, sh);
-    if (threadIdx.x == 0) {
-        if (shift_slot >= 0) scal[shift_slot] = scal[slot];
-        scal[slot] = v;
-    }
+    dot_final_body(part, nchunks, scal, slot, shift_slot, sh);
 }
 
-enum { kRR0 = 0, kRS0 = 1, kRR = 2, kRHO = 3, kRHOP = 4, kTMP = 5, kRS = 6 };
-
-// p = beta*p + 1.0*s, q = beta*q + 1.0*y with beta = rho/rho_prev (4360-4364)
 __global__ void k_pq_update(int nf, const double* __restrict__ scal, const double* __restrict__ s,
                             const double* __restrict__ y, double* __restrict__ p, double* __restrict__ q) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= 3 * nf) return;
-    const double beta = scal[kRHO] / scal[kRHOP];
-    double pp = p[k];
-    pp *= beta;
-    pp += 1.0 * s[k];
-    p[k] = pp;
-    double qq = q[k];
-    qq *= beta;
-    qq += 1.0 * y[k];
-    q[k] = qq;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < 3LL * nf) pq_update_elem(scal, s, y, p, q, k);
 }
 
-// alpha = rho/tmp; x += alpha p; r -= alpha q; s -= alpha u (4372-4375) + r.r partial
 __global__ void __launch_bounds__(256) k_xrs_update(int nf, const double* __restrict__ scal,
                                                     const double* __restrict__ p, const double* __restrict__ q,
                                                     const double* __restrict__ u, double* __restrict__ x,
                                                     double* __restrict__ r, double* __restrict__ s,
                                                     double* __restrict__ part) {
     __shared__ double sh[256];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double t = 0.0;
-    if (i < nf) {
-        const double alpha = scal[kRHO] / scal[kTMP];
-        const double ma = -alpha;
-        double rr[3];
-        for (int d = 0; d < 3; ++d) {
-            const int k = 3 * i + d;
-            x[k] += alpha * p[k];
-            rr[d] = r[k] + ma * q[k];
-            r[k] = rr[d];
-            s[k] += ma * u[k];
-        }
-        t = (rr[0] * rr[0] + rr[1] * rr[1]) + rr[2] * rr[2];
-    }
-    const double v = block_tree_sum(t, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = v;
+    xrs_update_chunk(nf, scal, p, q, u, x, r, s, part, blockIdx.x, sh);
 }
 
 __global__ void k_store_velocity(int nf, const double* __restrict__ x, double* __restrict__ vel) {
@@ -708,6 +453,9 @@ void launch_mg_invd(hipStream_t st, const Level& L) {
 void launch_fill(hipStream_t st, double* x, long long n, double v) {
     if (n > 0) hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, st, x, n, v);
 }
+void launch_fill_i32(hipStream_t st, int* x, long long n, int v) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, st, x, n, v);
+}
 void launch_mg_p2g(hipStream_t st, const DevParams& P, Level& L, const int* cellStart, const double* q) {
     hipLaunchKernelGGL(k_mg_p2g, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, P, L, cellStart, q);
 }
@@ -747,17 +495,17 @@ void launch_spmv(hipStream_t st, const DevParams& P, ASlices A, const int* nFlui
     const dim3 g(nblk(P.nf, kChunk)), b(kChunk);
     if (f32) {
         if (alpha < 0)
-            hipLaunchKernelGGL((k_spmv<true, float>), g, b, 0, st, P, nFluid, A.start, A.col, A.val32, x, beta, yin,
+            hipLaunchKernelGGL((k_spmv<true, float>), g, b, 0, st, P.nf, nFluid, A.start, A.col, A.val32, x, beta, yin,
                                y, y2, p_copy, dotx, part);
         else
-            hipLaunchKernelGGL((k_spmv<false, float>), g, b, 0, st, P, nFluid, A.start, A.col, A.val32, x, beta, yin,
+            hipLaunchKernelGGL((k_spmv<false, float>), g, b, 0, st, P.nf, nFluid, A.start, A.col, A.val32, x, beta, yin,
                                y, y2, p_copy, dotx, part);
     } else {
         if (alpha < 0)
-            hipLaunchKernelGGL((k_spmv<true, double>), g, b, 0, st, P, nFluid, A.start, A.col, A.val, x, beta, yin,
+            hipLaunchKernelGGL((k_spmv<true, double>), g, b, 0, st, P.nf, nFluid, A.start, A.col, A.val, x, beta, yin,
                                y, y2, p_copy, dotx, part);
         else
-            hipLaunchKernelGGL((k_spmv<false, double>), g, b, 0, st, P, nFluid, A.start, A.col, A.val, x, beta, yin,
+            hipLaunchKernelGGL((k_spmv<false, double>), g, b, 0, st, P.nf, nFluid, A.start, A.col, A.val, x, beta, yin,
                                y, y2, p_copy, dotx, part);
     }
 }

@@ -53,6 +53,13 @@ def lib():
         L.mph_scene_write_vtk.argtypes = [vp, cp]
         L.mph_scene_set_timing.argtypes = [vp, i]
         L.mph_scene_destroy.argtypes = [vp]
+        L.mph_scene_device_bytes.argtypes = [vp, C.POINTER(C.c_ulonglong)]
+        L.mph_batch_create.argtypes = [i, C.POINTER(vp)]
+        L.mph_batch_add.argtypes = [vp, vp]
+        L.mph_batch_remove.argtypes = [vp, vp]
+        L.mph_batch_size.argtypes = [vp]
+        L.mph_batch_step.argtypes = [vp, vp]
+        L.mph_batch_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -174,3 +181,56 @@ class Scene:
 
     def write_prof(self, path: str) -> None:
         _check(lib().mph_scene_write_prof(self.h, path.encode()))
+
+    def device_bytes(self) -> int:
+        v = C.c_ulonglong()
+        _check(lib().mph_scene_device_bytes(self.h, C.byref(v)))
+        return v.value
+
+
+class Batch:
+    """mph_batch_*: many scenes advanced one time step together (lockstep CR).
+
+    The batch does not own its scenes; keep them alive while they are members."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().mph_batch_create(int(device), C.byref(h)))
+        self.h = h
+        self.members: list[Scene] = []
+
+    def add(self, scene: Scene) -> None:
+        _check(lib().mph_batch_add(self.h, scene.h))
+        self.members.append(scene)
+
+    def remove(self, scene: Scene) -> None:
+        _check(lib().mph_batch_remove(self.h, scene.h))
+        self.members.remove(scene)
+
+    def __len__(self) -> int:
+        return int(lib().mph_batch_size(self.h))
+
+    def step(self) -> list:
+        """One time step of every member; returns one StepStats per member."""
+        n = len(self.members)
+        arr = (StepStats * max(n, 1))()
+        _check(lib().mph_batch_step(self.h, C.cast(arr, C.c_void_p)))
+        return [arr[k] for k in range(n)]
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().mph_batch_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

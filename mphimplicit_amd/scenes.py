@@ -155,6 +155,72 @@ def tiny_3d(seed=5, steps=5, lift=2):
     return config_3d_drop(n=6, box=14, lift=lift, seed=seed, steps=steps)
 
 
+def box_scene(seed: int, dim: int, nf_target: int, mu: float, gravity, dt=5.0e-4, steps=20):
+    """A fluid square (2D) or cube (3D) of about nf_target particles resting on the
+    floor, centred, in an open-top 4-layer box whose interior is 2.5x the fluid
+    extent per axis (the batched and serving workloads)."""
+    l0 = 0.01
+    n = max(2, int(round(nf_target ** (1.0 / dim))))
+    inner = int(math.ceil(2.5 * n))
+    lo = [(inner - n) / 2.0] * dim
+    lo[dim - 1] = 0.0
+    hi = [a + n for a in lo]
+    fluid = lattice(lo + [0.0] * (3 - dim), hi + [0.0] * (3 - dim), l0, dim)
+    walls = box_walls([0] * dim, [inner] * dim, 4, l0, dim, open_top=True)
+    pad = 5
+    dmin = [-(4 + pad) * l0] * 3
+    dmax = [(inner + 4 + pad) * l0] * 3
+    dmax[dim - 1] = (inner + n + pad) * l0  # headroom above the open top
+    if dim == 2:
+        dmin[2], dmax[2] = -0.05, 0.05
+    g = assemble(fluid, walls, 0.0, l0, dmin, dmax, seed, dim)
+    p = base_params(dim, mu, gravity, dt, steps)
+    return p, g
+
+
+def batch_sweep(count=64, seed0=1000, steps=20, nf_range=(2000, 200000), mu_range=(0.01, 10.0)):
+    """The batched workload: scene s has seed seed0+s, is 2D or 3D with
+    probability 1/2, has a log-uniform fluid count in nf_range and viscosity in
+    mu_range, and gravity 9.8 along a random unit vector (in-plane in 2D)."""
+    out = []
+    for s in range(count):
+        u = splitmix64_uniform(seed0 + s + 0x5EED, 6)
+        dim = 2 if u[0] < 0.5 else 3
+        nf = math.exp(math.log(nf_range[0]) + u[1] * (math.log(nf_range[1]) - math.log(nf_range[0])))
+        mu = math.exp(math.log(mu_range[0]) + u[2] * (math.log(mu_range[1]) - math.log(mu_range[0])))
+        if dim == 2:
+            a = 2.0 * math.pi * u[3]
+            g = (9.8 * math.cos(a), 9.8 * math.sin(a), 0.0)
+        else:
+            z = 2.0 * u[3] - 1.0
+            a = 2.0 * math.pi * u[4]
+            r = math.sqrt(max(0.0, 1.0 - z * z))
+            g = (9.8 * r * math.cos(a), 9.8 * r * math.sin(a), 9.8 * z)
+        out.append(box_scene(seed0 + s, dim, int(nf), mu, g, steps=steps))
+    return out
+
+
+def serve_stream(count: int, rate=200.0, seed=7, median=50000, sigma=1.0, clip=(5000, 500000),
+                 mu_range=(0.01, 10.0), steps_range=(1, 20)):
+    """Requests of the serving workload: Poisson arrivals at `rate` per second,
+    1..20 steps each, lognormal fluid counts (median, sigma) clipped to `clip`,
+    log-uniform viscosity, 2D or 3D with probability 1/2.  Returns a list of
+    (arrival_s, steps, scene_seed, dim, nf, mu) tuples; box_scene builds each."""
+    u = splitmix64_uniform(seed, 6 * count).reshape(count, 6)
+    t = 0.0
+    out = []
+    for k in range(count):
+        t += -math.log(1.0 - u[k, 0]) / rate
+        steps = steps_range[0] + int(u[k, 1] * (steps_range[1] - steps_range[0] + 1))
+        # Box-Muller normal from two uniforms
+        z = math.sqrt(-2.0 * math.log(1.0 - u[k, 2])) * math.cos(2.0 * math.pi * u[k, 3])
+        nf = int(min(clip[1], max(clip[0], median * math.exp(sigma * z))))
+        mu = math.exp(math.log(mu_range[0]) + u[k, 4] * (math.log(mu_range[1]) - math.log(mu_range[0])))
+        dim = 2 if u[k, 5] < 0.5 else 3
+        out.append((t, steps, 20000 + k, dim, nf, mu))
+    return out
+
+
 CONFIGS = {
     "2d_40x40": config_2d,
     "3d_64k": config_3d_drop,

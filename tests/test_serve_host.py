@@ -1,0 +1,169 @@
+"""Host logic of the serving loop and the batched / serving workload generators
+(CPU only: the scheduler runs against stand-in Scene and Batch objects)."""
+import math
+
+import numpy as np
+import pytest
+
+from mphimplicit_amd import scenes, serve
+
+
+class FakeScene:
+    live = 0
+
+    def __init__(self, params, grid, device=0):
+        self.params, self.grid = params, grid
+        self.time = grid.time
+        FakeScene.live += 1
+
+    def scalar(self, name):
+        assert name == "Time"
+        return self.time
+
+    def device_bytes(self):
+        return 3000 * self.grid.np
+
+    def read(self, name):
+        return np.array([self.time])
+
+    def close(self):
+        FakeScene.live -= 1
+
+
+class FakeStats:
+    def __init__(self, done):
+        self.steps_done = done
+
+
+class FakeBatch:
+    def __init__(self, device=0):
+        self.members = []
+        self.max_members = 0
+
+    def add(self, s):
+        self.members.append(s)
+        self.max_members = max(self.max_members, len(self.members))
+
+    def remove(self, s):
+        self.members.remove(s)
+
+    def step(self):
+        out = []
+        for s in self.members:
+            p = s.params
+            if s.time < p.end_time + 1e-5 * p.dt:
+                s.time += p.dt
+                out.append(FakeStats(1))
+            else:
+                out.append(FakeStats(0))
+        return out
+
+    def close(self):
+        self.members = []
+
+
+@pytest.fixture
+def fake(monkeypatch):
+    monkeypatch.setattr(serve, "Scene", FakeScene)
+    monkeypatch.setattr(serve, "Batch", FakeBatch)
+    FakeScene.live = 0
+    yield
+
+
+def small_requests(n=8, arrival_gap=0.0):
+    out = []
+    for k in range(n):
+        p, g = scenes.box_scene(100 + k, 2 if k % 2 else 3, 200 + 50 * k, 0.1, (0, -9.8, 0) if k % 2 else (0, 0, -9.8),
+                                steps=1 + k % 4)
+        out.append(serve.Request(rid=k, arrival=k * arrival_gap, steps=1 + k % 4, params=p, grid=g,
+                                 nf=int((g.type < 2).sum()), dim=p.dim))
+    return out
+
+
+def test_all_requests_served_once(fake):
+    reqs = small_requests()
+    srv = serve.Server(keep_fields=("Velocity",))
+    res = srv.run(reqs, duration_s=30)
+    assert res["requests_done"] == len(reqs)
+    assert sorted(r.rid for r in srv.finished) == list(range(len(reqs)))
+    for r in srv.finished:
+        assert r.steps_done == r.steps and r.done >= r.admitted >= 0 and r.latency >= 0
+    assert FakeScene.live == 0                      # every scene was released
+    assert srv.batch.max_members == len(reqs)       # all arrived at once: all in one batch
+    assert res["batch_steps"] == max(r.steps for r in reqs)
+
+
+def test_budget_limits_concurrency(fake):
+    """After every admission the admitted footprint fits the budget (or one scene
+    runs alone), and the budget does bind: not all requests run together."""
+    reqs = small_requests()
+    est = serve.FootprintModel().estimate
+    one = max(est(r.dim, r.nf, r.grid.np, True) for r in reqs)
+    budget = int(1.5 * one)
+    srv = serve.Server(budget_bytes=budget)
+    seen = []
+    admit = srv._admit
+
+    def checked(now):
+        admit(now)
+        seen.append((len(srv.active), srv.used_bytes()))
+        assert srv.used_bytes() <= budget or len(srv.active) == 1
+
+    srv._admit = checked
+    res = srv.run(reqs, duration_s=30)
+    assert res["requests_done"] == len(reqs)
+    assert 1 <= srv.batch.max_members < len(reqs)
+    assert max(n for n, _ in seen) == srv.batch.max_members
+
+
+def test_tiny_budget_serialises(fake):
+    reqs = small_requests(5)
+    srv = serve.Server(budget_bytes=1)
+    res = srv.run(reqs, duration_s=30)
+    assert res["requests_done"] == 5
+    assert srv.batch.max_members == 1
+    assert res["batch_steps"] == sum(r.steps for r in reqs)
+
+
+def test_footprint_model_learns_upwards():
+    m = serve.FootprintModel()
+    e = m.estimate(3, 1000, 3000, True)
+    m.observe(3, e, 2 * e)
+    assert m.estimate(3, 1000, 3000, True) == pytest.approx(2 * e, rel=1e-6)
+    m.observe(3, 2 * e, e)  # never shrinks
+    assert m.estimate(3, 1000, 3000, True) == pytest.approx(2 * e, rel=1e-6)
+
+
+def test_batch_sweep_distribution():
+    sw = scenes.batch_sweep(16, steps=3)
+    assert len(sw) == 16
+    dims = [p.dim for p, _ in sw]
+    assert set(dims) <= {2, 3}
+    for p, g in sw:
+        nf = int((g.type < 2).sum())
+        assert 1500 <= nf <= 230000
+        assert 0.01 <= p.shear_viscosity[0] <= 10.0
+        gn = math.sqrt(sum(p.gravity[d] ** 2 for d in range(3)))
+        assert gn == pytest.approx(9.8)
+        if p.dim == 2:
+            assert p.gravity[2] == 0.0
+        assert p.end_time == pytest.approx(2 * p.dt)
+        # particles inside the domain, walls around the fluid
+        lo, hi = np.asarray(g.domain_min), np.asarray(g.domain_max)
+        assert (g.pos >= lo).all() and (g.pos <= hi).all()
+    assert scenes.batch_sweep(4, steps=3)[2][1].pos.tobytes() == sw[2][1].pos.tobytes()  # seeded
+
+
+def test_serve_stream_statistics():
+    st = scenes.serve_stream(4000, rate=200.0)
+    t = np.array([x[0] for x in st])
+    assert (np.diff(t) > 0).all()
+    assert 4000 / t[-1] == pytest.approx(200.0, rel=0.08)          # Poisson rate
+    steps = np.array([x[1] for x in st])
+    assert steps.min() == 1 and steps.max() == 20
+    nf = np.array([x[4] for x in st])
+    assert nf.min() >= 5000 and nf.max() <= 500000
+    assert np.median(nf) == pytest.approx(50000, rel=0.1)          # lognormal median
+    mu = np.array([x[5] for x in st])
+    assert mu.min() >= 0.01 and mu.max() <= 10.0
+    assert 0.4 < np.mean([x[3] == 2 for x in st]) < 0.6
