@@ -123,6 +123,8 @@ def run_gpu(args, precision="fp64"):
                     b64, r64, traffic if precision == "fp64" else None)
     rf32 = roofline("k_spmv (particle matrix, sliced 3x3 blocks, FP32 values)", scene.scalar("SpmvMsF32"), l32_prev,
                     b32, r32, None)
+    tuned = {"spmv_fp64_variant": int(scene.scalar("TunedSpmv64")), "spmv_fp32_variant": int(scene.scalar("TunedSpmv32")),
+             "cac_variant": int(scene.scalar("TunedCac"))}
     scene.run_phase(abi.PHASE_DIAGNOSTICS)
     last = {"Velocity": scene.read("Velocity"), "PressureP": scene.read("PressureP")}
     scene.close()
@@ -138,6 +140,7 @@ def run_gpu(args, precision="fp64"):
         "setup_s": setup_s,
         "first": first,
         "last": last,
+        "tuned": tuned,
     }
 
 
@@ -337,6 +340,7 @@ def main():
         "p50_step_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_step_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "cr_iterations": g["iters"],
+        "autotuned": g["tuned"],
         "roofline": g["roofline"],
         "cpu_baseline": None,
     }

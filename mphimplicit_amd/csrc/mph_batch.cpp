@@ -39,6 +39,8 @@ struct mph_batch {
     size_t hscalCap = 0;
     std::vector<BScene> htab;
     std::vector<int> hoff;
+    Autotune tune64, tune32;  // SpMV load-group depth of the scene-table launches
+    Autotune tuneFuse;        // V-cycle levels fused into one workgroup per scene
 };
 
 namespace {
@@ -100,6 +102,7 @@ int build_tables(mph_batch& B, const std::vector<mph_scene*>& sc, Tables& T) {
         b.scal = B.scal.p + 8 * (size_t)s;
         b.mg = S.prm.multigrid ? 1 : 0;
         b.depth = b.mg ? S.depth : 0;
+        for (int v = 0; v < kFuseVariants; ++v) b.fuseOf[v] = b.mg ? S.fuseOf[v] : 0;
         for (int l = 0; l < b.depth; ++l) b.lv[l] = S.lv[l].L;
         auto put = [&](int row, int blocks) { B.hoff[(size_t)row * (ns + 1) + s + 1] = blocks; };
         put(kOffChunk, (S.P.nf + 255) / 256);
@@ -121,16 +124,30 @@ int build_tables(mph_batch& B, const std::vector<mph_scene*>& sc, Tables& T) {
 int read_scalars(mph_batch& B, int ns) {
     HIPCHK(hipMemcpyAsync(B.hscal, B.scal.p, (size_t)ns * 8 * sizeof(double), hipMemcpyDeviceToHost, B.st));
     HIPCHK(hipStreamSynchronize(B.st));
+    B.tune64.collect();
+    B.tune32.collect();
+    B.tuneFuse.collect();
     return MPH_OK;
+}
+
+// the scene-table SpMV with the batch's tuned load-group depth
+void bspmv(mph_batch& B, const Tables& T, const SceneSet& on, const SceneSet& first, bool f32, double alpha, int x,
+           double beta, int yin, int y, int y2, int p_copy, int dot) {
+    Autotune& A = f32 ? B.tune32 : B.tune64;
+    const int v = A.pick();
+    A.begin(B.st, v);
+    spmv(T, on, first, f32, alpha, x, beta, yin, y, y2, p_copy, dot, v);
+    A.end(B.st);
 }
 
 // the V-cycle of every scene in `on`, level by level; cur[s][l] is the ping-pong
 // state of scene s at level l and advances with every smoothing sweep
 struct VCycle {
+    mph_batch& B;
     const Tables& T;
     const std::vector<mph_scene*>& sc;
     std::vector<std::vector<int>> cur;
-    VCycle(const Tables& t, const std::vector<mph_scene*>& s) : T(t), sc(s), cur(s.size()) {
+    VCycle(mph_batch& b, const Tables& t, const std::vector<mph_scene*>& s) : B(b), T(t), sc(s), cur(s.size()) {
         for (size_t k = 0; k < s.size(); ++k) cur[k].assign(kMaxLevels, 0);
     }
     SceneSet curset(const Set& on, int l) const {
@@ -148,6 +165,9 @@ struct VCycle {
     // precondition (main.cpp:4122-4266) of in -> out for the multigrid scenes in `on`
     void apply(const Set& on, int in, int out, bool part, bool f32) {
         if (on.empty()) return;
+        const int v = B.tuneFuse.pick();
+        B.tuneFuse.begin(T.st, v);
+        auto F = [&](int s) { return sc[s]->fuseOf[v]; };
         int maxD = 0;
         for (int s = 0; s < (int)sc.size(); ++s)
             if (on.has(s)) maxD = std::max(maxD, sc[s]->depth);
@@ -156,10 +176,10 @@ struct VCycle {
         mg_update(T, on.m, 0, curset(on, 0));  // pre-smooth x2
         ru(on, 0, f32);
         mg_residual(T, on.m, 0, curset(on, 0), f32);
-        for (int l = 1; l <= maxD - 1; ++l) {
+        for (int l = 1; l <= maxD - 1; ++l) {  // levels below each scene's fused part
             Set inner, coarsest, both;
             for (int s = 0; s < (int)sc.size(); ++s) {
-                if (!on.has(s)) continue;
+                if (!on.has(s) || l >= F(s)) continue;
                 const int D = sc[s]->depth;
                 if (D - 1 > l) inner.add(s), both.add(s);
                 else if (D - 1 == l) coarsest.add(s), both.add(s);
@@ -175,10 +195,24 @@ struct VCycle {
             ru(coarsest, l, f32);                                                  // the prolongation
             if (!coarsest.empty()) mg_prolong(T, coarsest.m, l, curset(coarsest, l), curset(coarsest, l - 1));
         }
+        Set fused;  // levels F..depth-1 of every scene in one launch
+        CurBits cb{};
+        for (int s = 0; s < (int)sc.size(); ++s)
+            if (on.has(s) && F(s) < sc[s]->depth) {
+                fused.add(s);
+                for (int l = F(s) - 1; l < sc[s]->depth; ++l)
+                    if (cur[s][l]) cb.lv[l].w[s >> 6] |= 1ull << (s & 63);
+            }
+        if (!fused.empty()) {
+            mg_coarse(T, fused.m, cb, v, f32);
+            for (int s = 0; s < (int)sc.size(); ++s)  // five sweeps per fused level, four at the coarsest
+                if (fused.has(s))
+                    for (int l = F(s); l < sc[s]->depth - 1; ++l) cur[s][l] ^= 1;
+        }
         for (int l = maxD - 2; l >= 1; --l) {
             Set up;
             for (int s = 0; s < (int)sc.size(); ++s)
-                if (on.has(s) && sc[s]->depth - 2 >= l) up.add(s);
+                if (on.has(s) && sc[s]->depth - 2 >= l && l < F(s)) up.add(s);
             if (up.empty()) continue;
             for (int k = 0; k < 3; ++k) ru(up, l, f32);  // post-smooth x3
             mg_prolong(T, up.m, l, curset(up, l), curset(up, l - 1));
@@ -186,6 +220,7 @@ struct VCycle {
         ru(on, 0, f32);  // post-smooth x2
         ru(on, 0, f32);
         mg_g2p(T, on.m, curset(on, 0), in, out, part);
+        B.tuneFuse.end(T.st);
     }
 };
 
@@ -227,13 +262,13 @@ int solve_fp64(mph_batch& B, const std::vector<mph_scene*>& sc) {
     if ((rc = build_tables(B, sc, T))) return rc;
     Set all;
     for (int s = 0; s < ns; ++s) all.add(s);
-    VCycle V(T, sc);
+    VCycle V(B, T, sc);
     solve_prologue(T, sc, all);
     apply_m(V, all, kB, kS, true, false);  // s = M^-1 b; rs0 = b.s; rr0 = b.b
     dot_final(T, all.m, kRS0, -1);
     dot_partial(T, all.m, kB, kB);
     dot_final(T, all.m, kRR0, -1);
-    spmv(T, all.m, SceneSet{}, false, -1.0, kX, 1.0, kB, kR, -1, -1, -1);  // r = b - A x; s = M^-1 r
+    bspmv(B, T, all.m, SceneSet{}, false, -1.0, kX, 1.0, kB, kR, -1, -1, -1);  // r = b - A x; s = M^-1 r
     apply_m(V, all, kR, kS, false, false);
     dot_partial(T, all.m, kR, kR);
     dot_final(T, all.m, kRR, -1);
@@ -258,7 +293,7 @@ int solve_fp64(mph_batch& B, const std::vector<mph_scene*>& sc) {
         active = now;
         if (active.empty()) break;
         // y = A s (first iteration also q = y, p = s); rho = s.y
-        spmv(T, active.m, first.m, false, 1.0, kS, 0.0, kY, kY, kQ, kP, kS);
+        bspmv(B, T, active.m, first.m, false, 1.0, kS, 0.0, kY, kY, kQ, kP, kS);
         dot_final(T, active.m, kRHO, kRHOP);
         pq_update(T, rest.m);
         apply_m(V, active, kQ, kU, true, false);  // u = M^-1 q; tmp = q.u
@@ -295,11 +330,11 @@ int solve_mixed(mph_batch& B, const std::vector<mph_scene*>& sc) {
     if ((rc = build_tables(B, sc, T))) return rc;
     Set all;
     for (int s = 0; s < ns; ++s) all.add(s);
-    VCycle V(T, sc);
+    VCycle V(B, T, sc);
     solve_prologue(T, sc, all);
     dot_partial(T, all.m, kB, kB);
     dot_final(T, all.m, kRR0, -1);
-    spmv(T, all.m, SceneSet{}, false, -1.0, kX, 1.0, kB, kROut, -1, -1, -1);
+    bspmv(B, T, all.m, SceneSet{}, false, -1.0, kX, 1.0, kB, kROut, -1, -1, -1);
     dot_partial(T, all.m, kROut, kROut);
     dot_final(T, all.m, kRR, -1);
     if ((rc = read_scalars(B, ns))) return rc;
@@ -358,7 +393,7 @@ int solve_mixed(mph_batch& B, const std::vector<mph_scene*>& sc) {
         if (update.empty() && inner.empty()) break;
         if (!update.empty()) {  // x += d; rOut = b - A64 x; rr = rOut.rOut
             axpy(T, update.m, kD, kX);
-            spmv(T, update.m, SceneSet{}, false, -1.0, kX, 1.0, kB, kROut, -1, -1, -1);
+            bspmv(B, T, update.m, SceneSet{}, false, -1.0, kX, 1.0, kB, kROut, -1, -1, -1);
             dot_partial(T, update.m, kROut, kROut);
             dot_final(T, update.m, kRR, -1);
         }
@@ -368,7 +403,7 @@ int solve_mixed(mph_batch& B, const std::vector<mph_scene*>& sc) {
             apply_m(V, restart, kR, kS, false, true);
         }
         if (!inner.empty()) {
-            spmv(T, inner.m, restart.m, true, 1.0, kS, 0.0, kY, kY, kQ, kP, kS);
+            bspmv(B, T, inner.m, restart.m, true, 1.0, kS, 0.0, kY, kY, kQ, kP, kS);
             dot_final(T, inner.m, kRHO, kRHOP);
             pq_update(T, rest.m);
             apply_m(V, inner, kQ, kU, true, true);
@@ -419,6 +454,9 @@ int mph_batch_create(int device, mph_batch** out) {
     HIPCHK(hipSetDevice(device));
     mph_batch* B = new mph_batch;
     B->device = device;
+    B->tune64.init(kSpmvVariants64, 3);
+    B->tune32.init(kSpmvVariants32, 3);
+    B->tuneFuse.init(kFuseVariants, 3);
     if (hipStreamCreateWithFlags(&B->st, hipStreamNonBlocking) != hipSuccess) {
         delete B;
         return fail(MPH_E_DEVICE, "stream creation failed");
@@ -520,6 +558,9 @@ int mph_batch_destroy(mph_batch* B) {
     B->off.release();
     B->scal.release();
     if (B->hscal) (void)hipHostFree(B->hscal);
+    B->tune64.release();
+    B->tune32.release();
+    B->tuneFuse.release();
     if (B->st) (void)hipStreamDestroy(B->st);
     delete B;
     return MPH_OK;

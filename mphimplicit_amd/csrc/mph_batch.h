@@ -13,7 +13,6 @@ namespace mph {
 namespace batch {
 
 constexpr int kMaxScenes = 256;  // bits of a SceneSet
-constexpr int kMaxLevels = 20;   // multigrid depth = floor(log2(cells per axis)) + 1
 
 // CR vectors by role
 enum { kX = 0, kR, kS, kY, kP, kQ, kU, kB, kROut, kD, kNVec };
@@ -24,6 +23,10 @@ enum { kOffChunk = 0, kOffElem = 1, kOffLevel = 2, kOffRows = kOffLevel + kMaxLe
 
 struct SceneSet {
     unsigned long long w[kMaxScenes / 64];
+};
+
+struct CurBits {  // per level, the scenes whose ping-pong bit is set
+    SceneSet lv[kMaxLevels];
 };
 
 struct BScene {
@@ -44,6 +47,7 @@ struct BScene {
     double* scal;
     int depth;
     int mg;
+    int fuseOf[kFuseVariants];  // first level of the one-workgroup coarse cycle per variant
     Level lv[kMaxLevels];  // .cur is ignored: it comes with each launch
 };
 
@@ -65,7 +69,7 @@ void axpy(const Tables& T, const SceneSet& on, int src, int dst);  // dst += 1.0
 // y = beta*yin + sum (alpha*a) x with alpha = +-1; y2 and p_copy only for scenes in
 // `first`; dot partial of dot.y into the scene's part when dot >= 0
 void spmv(const Tables& T, const SceneSet& on, const SceneSet& first, bool f32, double alpha, int x, double beta,
-          int yin, int y, int y2, int p_copy, int dot);
+          int yin, int y, int y2, int p_copy, int dot, int variant);
 void dot_partial(const Tables& T, const SceneSet& on, int a, int b);
 void dot_final(const Tables& T, const SceneSet& on, int slot, int shift_slot);
 void pq_update(const Tables& T, const SceneSet& on);
@@ -79,6 +83,8 @@ void mg_residual_update(const Tables& T, const SceneSet& on, int l, const SceneS
 void mg_restrict(const Tables& T, const SceneSet& on, int l);  // level l-1 -> l
 void mg_prolong(const Tables& T, const SceneSet& on, int l, const SceneSet& curL, const SceneSet& curS);  // l -> l-1
 void mg_g2p(const Tables& T, const SceneSet& on, const SceneSet& cur0, int in, int out, bool part);
+// levels fuseOf[variant]..depth-1 of each scene in one workgroup
+void mg_coarse(const Tables& T, const SceneSet& on, const CurBits& cb, int variant, bool f32);
 
 }  // namespace batch
 }  // namespace mph

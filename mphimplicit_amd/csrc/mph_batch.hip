@@ -68,7 +68,7 @@ __global__ void kb_axpy(const BScene* tab, const int* off, int ns, SceneSet on, 
     if (k < 3LL * B.P.nf) B.vec[dst][k] += 1.0 * B.vec[src][k];
 }
 
-template <bool kNeg, class VT>
+template <bool kNeg, class VT, int kUn>
 __global__ void __launch_bounds__(256) kb_spmv(const BScene* tab, const int* off, int ns, SceneSet on,
                                                SceneSet first, int x, double beta, int yin, int y, int y2,
                                                int p_copy, int dot) {
@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(256) kb_spmv(const BScene* tab, const int* off
     MPH_SCENE(kOffChunk)
     const bool f = in_set(first, s);
     const VT* val = sizeof(VT) == 8 ? (const VT*)B.val : (const VT*)B.val32;
-    spmv_chunk<kNeg, VT>(B.P.nf, B.nFluid, B.start, B.col, val, B.vec[x], beta, B.vec[yin], B.vec[y],
+    spmv_chunk<kNeg, VT, kUn>(B.P.nf, B.nFluid, B.start, B.col, val, B.vec[x], beta, B.vec[yin], B.vec[y],
                          f && y2 >= 0 ? B.vec[y2] : nullptr, f && p_copy >= 0 ? B.vec[p_copy] : nullptr,
                          dot >= 0 ? B.vec[dot] : nullptr, dot >= 0 ? B.part : nullptr, lb, sh);
 }
@@ -192,6 +192,19 @@ __global__ void __launch_bounds__(256) kb_mg_g2p(const BScene* tab, const int* o
     mg_g2p_chunk(B.P, L, B.cellIndex, B.invDiagA, B.vec[in], B.vec[out], part ? B.part : nullptr, lb, sh);
 }
 
+// levels fuse..depth-1 of each scene in its own workgroup; cur.lv[l] holds the
+// ping-pong bit of level l per scene
+__global__ void __launch_bounds__(1024) kb_mg_coarse(const BScene* tab, int ns, SceneSet on, CurBits cb, int variant,
+                                                    int f32) {
+    const int s = blockIdx.x;
+    if (s >= ns || !in_set(on, s)) return;
+    const BScene& B = tab[s];
+    unsigned cur = 0;
+#pragma unroll
+    for (int l = 0; l < kMaxLevels; ++l) cur |= (in_set(cb.lv[l], s) ? 1u : 0u) << l;
+    mg_coarse_cycle(B.lv, B.fuseOf[variant], B.depth, cur, f32 != 0);
+}
+
 #undef MPH_SCENE
 
 bool any(const SceneSet& m) {
@@ -220,15 +233,23 @@ void cr_init(const Tables& T, const SceneSet& on) {
 void copy(const Tables& T, const SceneSet& on, int src, int dst) { MPH_LAUNCH(kb_copy, kOffElem, src, dst); }
 void axpy(const Tables& T, const SceneSet& on, int src, int dst) { MPH_LAUNCH(kb_axpy, kOffElem, src, dst); }
 void spmv(const Tables& T, const SceneSet& on, const SceneSet& first, bool f32, double alpha, int x, double beta,
-          int yin, int y, int y2, int p_copy, int dot) {
-    // alpha is +1 or -1 at every call site (main.cpp:4337, 4353, 4357)
+          int yin, int y, int y2, int p_copy, int dot, int variant) {
+    // alpha is +1 or -1 at every call site (main.cpp:4337, 4353, 4357); variants as
+    // launch_spmv: FP64 2/3/4 block pairs per load group, FP32 1/2
+#define MPH_SPMV(NEG, VT, UN) MPH_LAUNCH((kb_spmv<NEG, VT, UN>), kOffChunk, first, x, beta, yin, y, y2, p_copy, dot)
     if (f32) {
-        if (alpha < 0) MPH_LAUNCH((kb_spmv<true, float>), kOffChunk, first, x, beta, yin, y, y2, p_copy, dot);
-        else MPH_LAUNCH((kb_spmv<false, float>), kOffChunk, first, x, beta, yin, y, y2, p_copy, dot);
+        if (alpha < 0) { if (variant == 1) MPH_SPMV(true, float, 2); else MPH_SPMV(true, float, 1); }
+        else { if (variant == 1) MPH_SPMV(false, float, 2); else MPH_SPMV(false, float, 1); }
+    } else if (alpha < 0) {
+        if (variant == 1) MPH_SPMV(true, double, 3);
+        else if (variant == 2) MPH_SPMV(true, double, 4);
+        else MPH_SPMV(true, double, 2);
     } else {
-        if (alpha < 0) MPH_LAUNCH((kb_spmv<true, double>), kOffChunk, first, x, beta, yin, y, y2, p_copy, dot);
-        else MPH_LAUNCH((kb_spmv<false, double>), kOffChunk, first, x, beta, yin, y, y2, p_copy, dot);
+        if (variant == 1) MPH_SPMV(false, double, 3);
+        else if (variant == 2) MPH_SPMV(false, double, 4);
+        else MPH_SPMV(false, double, 2);
     }
+#undef MPH_SPMV
 }
 void dot_partial(const Tables& T, const SceneSet& on, int a, int b) { MPH_LAUNCH(kb_dot_partial, kOffChunk, a, b); }
 void dot_final(const Tables& T, const SceneSet& on, int slot, int shift_slot) {
@@ -264,6 +285,11 @@ void mg_prolong(const Tables& T, const SceneSet& on, int l, const SceneSet& curL
 }
 void mg_g2p(const Tables& T, const SceneSet& on, const SceneSet& cur0, int in, int out, bool part) {
     MPH_LAUNCH(kb_mg_g2p, kOffChunk, cur0, in, out, part ? 1 : 0);
+}
+
+void mg_coarse(const Tables& T, const SceneSet& on, const CurBits& cb, int variant, bool f32) {
+    if (T.ns > 0 && any(on))
+        hipLaunchKernelGGL(kb_mg_coarse, dim3(T.ns), dim3(1024), 0, T.st, T.tab, T.ns, on, cb, variant, f32 ? 1 : 0);
 }
 
 #undef MPH_LAUNCH

@@ -321,7 +321,7 @@ __global__ void k_slice_width(int nf, int nslices, const int* __restrict__ nFlui
         const int i = s * kSlice + l;
         if (i < nf && nFluid[i] > w) w = nFluid[i];
     }
-    width[s] = w;
+    width[s] = (w + 1) & ~1;  // block pairs: every slice starts on an even slot
 }
 
 inline int nblk(long long n, int bs) { return (int)((n + bs - 1) / bs); }

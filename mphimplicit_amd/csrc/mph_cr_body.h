@@ -81,22 +81,78 @@ __device__ inline void mg_update_row(const Level& L, int row) {  // myDdmv(1.0, 
     S[row] = y;
 }
 
+// (v mod n) for 0 <= v < 3n without a division
+__device__ inline int wrap3(int v, int n) {
+    v -= v >= n ? n : 0;
+    v -= v >= n ? n : 0;
+    return v;
+}
+
+// sum over the 27-point (9 in 2D) block stencil of -1.0*a*x in shift order; the
+// wrapped neighbour index (g + e - 1 + cnt) % cnt of the reference's periodic
+// stencil (main.cpp:3323-3325) is formed by subtraction (its argument is < 3 cnt).
+// Full stencils (3x3x3, 3x3x1) take an unrolled path that issues the loads of nine
+// cells at a time before summing them in order; smaller levels take the loop.
+template <class VT, int C0, int C1, int C2>
+__device__ inline double mg_row_sum_full(const Level& L, const VT* __restrict__ A, const double* __restrict__ x,
+                                         int row, int gx, int gy, int gz) {
+    constexpr int N = C0 * C1 * C2;
+    const int c0 = L.cnt[0], c1 = L.cnt[1], c2 = L.cnt[2];
+    const long long nrow = L.nrow;
+    int jx[C0], jy[C1], jz[C2];
+#pragma unroll
+    for (int e = 0; e < C0; ++e) jx[e] = wrap3(gx + e - 1 + c0, c0);
+#pragma unroll
+    for (int e = 0; e < C1; ++e) jy[e] = wrap3(gy + e - 1 + c1, c1);
+#pragma unroll
+    for (int e = 0; e < C2; ++e) jz[e] = wrap3(gz + e - 1 + c2, c2);
+    double sum = 0.0;
+#pragma unroll
+    for (int q0 = 0; q0 < N; q0 += 9) {
+        VT av[27];
+        double xv[27];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int q = q0 + t;
+            if (q < N) {
+                const int ex = q / (C1 * C2), ey = (q / C2) % C1, ez = q % C2;
+                const double* xc = x + 3 * ((c1 * c2) * jx[ex] + c2 * jy[ey] + jz[ez]);
+#pragma unroll
+                for (int sd = 0; sd < 3; ++sd) {
+                    av[3 * t + sd] = A[(3 * q + sd) * nrow + row];
+                    xv[3 * t + sd] = xc[sd];
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+            if (q0 + t < N)
+#pragma unroll
+                for (int sd = 0; sd < 3; ++sd) sum += -1.0 * (double)av[3 * t + sd] * xv[3 * t + sd];
+    }
+    return sum;
+}
+
 template <class VT>
 __device__ inline double mg_row_sum(const Level& L, const VT* __restrict__ A, const double* __restrict__ x, int row) {
     const int g = row / 3;
     int gx, gy, gz;
     grid_xyz(L, g, gx, gy, gz);
+    if (L.cs[0] == 3 && L.cs[1] == 3 && L.cs[2] == 3) return mg_row_sum_full<VT, 3, 3, 3>(L, A, x, row, gx, gy, gz);
+    if (L.cs[0] == 3 && L.cs[1] == 3 && L.cs[2] == 1) return mg_row_sum_full<VT, 3, 3, 1>(L, A, x, row, gx, gy, gz);
+    const int c0 = L.cnt[0], c1 = L.cnt[1], c2 = L.cnt[2];
+    const int nrow = L.nrow;
+    const VT* a = A + row;  // entry sh of the row at a[sh * nrow]
     double sum = 0.0;
-    int sh = 0;
     for (int ex = 0; ex < L.cs[0]; ++ex) {
-        const int jx = (gx + ex - 1 + L.cnt[0]) % L.cnt[0];
+        const int jx = wrap3(gx + ex - 1 + c0, c0);
         for (int ey = 0; ey < L.cs[1]; ++ey) {
-            const int jy = (gy + ey - 1 + L.cnt[1]) % L.cnt[1];
+            const int jy = wrap3(gy + ey - 1 + c1, c1);
+            const int bxy = (c1 * c2) * jx + c2 * jy;
             for (int ez = 0; ez < L.cs[2]; ++ez) {
-                const int jz = (gz + ez - 1 + L.cnt[2]) % L.cnt[2];
-                const int col = 3 * grid_id(L, jx, jy, jz);
-                for (int sd = 0; sd < 3; ++sd, ++sh)
-                    sum += -1.0 * (double)A[(long long)sh * L.nrow + row] * x[col + sd];
+                const int jz = wrap3(gz + ez - 1 + c2, c2);
+                const double* xc = x + 3 * (bxy + jz);
+                for (int sd = 0; sd < 3; ++sd, a += nrow) sum += -1.0 * (double)a[0] * xc[sd];
             }
         }
     }
@@ -169,6 +225,73 @@ __device__ inline void mg_prolong_row(const Level& L, const Level& S, int row) {
     Sv[row] = y;
 }
 
+// Levels lc..D-1 of the V-cycle inside ONE workgroup: restriction into lc, the
+// down sweep (S = invD R, then two residual-update sweeps and a residual per
+// level), the coarsest level (S = invD R, then four residual-update sweeps), the
+// up sweep (three residual-update sweeps per level) and the prolongations back
+// into lc-1 -- the order of preconditionWithMultiGrid (4160-4255) for those levels.  Each step runs over the level's rows with a
+// stride of blockDim.x and ends at a workgroup barrier, so every row sees the
+// previous step complete.  `cur` holds the ping-pong state of each level (bit l)
+// on entry, lc-1 included; it advances as the separate launches would advance it
+// (afterwards every fused level but the coarsest has flipped).  f32: use a level's FP32
+// stencil where it has one.
+__device__ inline void mg_coarse_cycle(const Level* lvs, int lc, int D, unsigned cur, bool f32) {
+    auto lev = [&](int l) {
+        Level L = lvs[l];
+        L.cur = (cur >> l) & 1u;
+        return L;
+    };
+    auto ru = [&](int l) {
+        const Level L = lev(l);
+        for (int row = threadIdx.x; row < L.nrow; row += blockDim.x) {
+            if (f32 && L.A32) mg_residual_update_row<float>(L, row);
+            else mg_residual_update_row<double>(L, row);
+        }
+        __syncthreads();
+        cur ^= 1u << l;
+    };
+    auto restrict_into = [&](int l) {
+        const Level L = lev(l);
+        for (int row = threadIdx.x; row < L.nrow; row += blockDim.x) mg_restrict_row(lvs[l - 1], L, row);
+        __syncthreads();
+    };
+    auto begin_update = [&](int l) {
+        const Level L = lev(l);
+        for (int row = threadIdx.x; row < L.nrow; row += blockDim.x) mg_begin_row(L, row);
+        __syncthreads();
+        for (int row = threadIdx.x; row < L.nrow; row += blockDim.x) mg_update_row(L, row);
+        __syncthreads();
+    };
+    auto prolong_from = [&](int l) {  // l -> l-1
+        const Level L = lev(l), S = lev(l - 1);
+        for (int row = threadIdx.x; row < S.nrow; row += blockDim.x) mg_prolong_row(L, S, row);
+        __syncthreads();
+    };
+    for (int l = lc; l < D - 1; ++l) {
+        restrict_into(l);
+        begin_update(l);
+        ru(l);
+        ru(l);
+        const Level L = lev(l);
+        for (int row = threadIdx.x; row < L.nrow; row += blockDim.x) {
+            if (f32 && L.A32) mg_residual_row<float>(L, row);
+            else mg_residual_row<double>(L, row);
+        }
+        __syncthreads();
+    }
+    {
+        const int c = D - 1;
+        restrict_into(c);
+        begin_update(c);
+        for (int k = 0; k < 4; ++k) ru(c);
+        prolong_from(c);
+    }
+    for (int l = D - 2; l >= lc; --l) {
+        for (int k = 0; k < 3; ++k) ru(l);
+        prolong_from(l);
+    }
+}
+
 // u = D^-1 q (4144-4148) + G2P S0 (4260), and the chunk partial of q.u (when part)
 __device__ inline void mg_g2p_chunk(const DevParams& P, const Level& L, const int* __restrict__ cellIndex,
                                     const double* __restrict__ invDiagA, const double* __restrict__ q,
@@ -216,58 +339,62 @@ __device__ inline void cr_init_particle(const DevParams& P, const double* __rest
 // output y2 = beta*y2 + sum (same sum), optional copy p = x, and the chunk partial
 // of dotx.y.  One chunk = 4 slices = 256 particles.  The row sums run over the
 // blocks in list order, entries s = 0,1,2 within a block, as the CSR.
-template <bool kNeg, class VT>
+template <bool kNeg, class VT, int kUn>
 __device__ inline void spmv_chunk(int nf, const int* __restrict__ nFluid, const long long* __restrict__ start,
                                   const int* __restrict__ col, const VT* __restrict__ val,
                                   const double* __restrict__ x, double beta, const double* yin, double* y,
                                   double* y2, double* p_copy, const double* __restrict__ dotx,
                                   double* __restrict__ part, int chunk, double* sh) {
+    typedef VT vt2 __attribute__((ext_vector_type(2)));
+    typedef int it2 __attribute__((ext_vector_type(2)));
     const int i = chunk * 256 + threadIdx.x;
     double t = 0.0;
     if (i < nf) {
         const int w = nFluid[i];
         const int lane = i % kSlice;
-        const long long sb = start[i / kSlice];
-        const int* cp = col + sb * kSlice + lane;
-        const VT* vp = val + sb * 9 * kSlice + lane;
+        const long long sp = start[i / kSlice] / 2;  // first block pair of the slice
+        // the matrix is read once per launch: streaming (nontemporal) loads of whole
+        // block pairs (a_col_index / a_val_index), kUn pairs per group with all their
+        // loads issued before the first use
+        const it2* cp = (const it2*)col + sp * kSlice + lane;
+        const vt2* vp = (const vt2*)val + sp * 9 * kSlice + lane;
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        // the matrix is read once per launch: streaming (nontemporal) loads; groups of
-        // kUn blocks have all their loads issued before the first use
-        constexpr int kUn = 4;
-        int k = 0;
-        for (; k + kUn <= w; k += kUn) {
-            int j[kUn];
-            VT a[kUn][9];
+        auto block = [&](int j, const vt2* a, int h) {
+            const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
+            double c[9];
 #pragma unroll
-            for (int u = 0; u < kUn; ++u) j[u] = __builtin_nontemporal_load(cp + (long long)(k + u) * kSlice);
+            for (int e = 0; e < 9; ++e) c[e] = kNeg ? -(double)a[e][h] : (double)a[e][h];
+            s0 += c[0] * x0; s0 += c[1] * x1; s0 += c[2] * x2;
+            s1 += c[3] * x0; s1 += c[4] * x1; s1 += c[5] * x2;
+            s2 += c[6] * x0; s2 += c[7] * x1; s2 += c[8] * x2;
+        };
+        const int wp = w / 2;
+        int m = 0;
+        for (; m + kUn <= wp; m += kUn) {
+            it2 j[kUn];
+            vt2 a[kUn][9];
+#pragma unroll
+            for (int u = 0; u < kUn; ++u) j[u] = __builtin_nontemporal_load(cp + (long long)(m + u) * kSlice);
 #pragma unroll
             for (int u = 0; u < kUn; ++u)
 #pragma unroll
                 for (int e = 0; e < 9; ++e)
-                    a[u][e] = __builtin_nontemporal_load(vp + ((long long)(k + u) * 9 + e) * kSlice);
+                    a[u][e] = __builtin_nontemporal_load(vp + ((long long)(m + u) * 9 + e) * kSlice);
 #pragma unroll
             for (int u = 0; u < kUn; ++u) {
-                const double x0 = x[3 * j[u]], x1 = x[3 * j[u] + 1], x2 = x[3 * j[u] + 2];
-                double c[9];
-#pragma unroll
-                for (int e = 0; e < 9; ++e) c[e] = kNeg ? -(double)a[u][e] : (double)a[u][e];
-                s0 += c[0] * x0; s0 += c[1] * x1; s0 += c[2] * x2;
-                s1 += c[3] * x0; s1 += c[4] * x1; s1 += c[5] * x2;
-                s2 += c[6] * x0; s2 += c[7] * x1; s2 += c[8] * x2;
+                block(j[u][0], a[u], 0);
+                block(j[u][1], a[u], 1);
             }
         }
-        for (; k < w; ++k) {
-            const int j = __builtin_nontemporal_load(cp + (long long)k * kSlice);
-            const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
-            double c[9];
+        for (; m <= wp; ++m) {  // the last full pairs, then the odd last block
+            const int h = (m < wp) ? 2 : (w & 1);
+            if (h == 0) break;
+            const it2 j = __builtin_nontemporal_load(cp + (long long)m * kSlice);
+            vt2 a[9];
 #pragma unroll
-            for (int e = 0; e < 9; ++e) {
-                const double v = (double)__builtin_nontemporal_load(vp + ((long long)k * 9 + e) * kSlice);
-                c[e] = kNeg ? -v : v;
-            }
-            s0 += c[0] * x0; s0 += c[1] * x1; s0 += c[2] * x2;
-            s1 += c[3] * x0; s1 += c[4] * x1; s1 += c[5] * x2;
-            s2 += c[6] * x0; s2 += c[7] * x1; s2 += c[8] * x2;
+            for (int e = 0; e < 9; ++e) a[e] = __builtin_nontemporal_load(vp + ((long long)m * 9 + e) * kSlice);
+            block(j[0], a, 0);
+            if (h == 2) block(j[1], a, 1);
         }
         const double sm[3] = {s0, s1, s2};
         double out[3];

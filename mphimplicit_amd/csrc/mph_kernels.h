@@ -7,10 +7,13 @@
 //                int cellIndex[np] (sorted keys), int cellStart[2T+1]
 //   neighbours   int64 ptr[np+1], int ind[E]; same for the RadiusP lists
 //   A            sliced 3x3-block ELL: slice = 64 consecutive fluid particles,
-//                width = max fluid-neighbour count in the slice, slot
-//                (slice, k, lane) = (sliceStart[slice] + k)*64 + lane;
-//                acol[slot] = neighbour particle; aval[(slot/64*9 + e)*64 + lane]
-//                holds entry e = 3r+s of the block (coalesced per entry)
+//                width = max fluid-neighbour count in the slice rounded up to
+//                even, sliceStart = prefix sum of widths (always even).  Blocks
+//                k = 2m, 2m+1 of a particle form a pair: its two columns are
+//                adjacent ints and entry e = 3r+s of both blocks are adjacent
+//                values (a_col_index / a_val_index), so a wavefront reads 512 B
+//                of columns and 1 KiB (FP64) per entry of a block pair
+//                with one 8/16-byte load per lane
 //   C            cval[3*(ptrP[i] + jn) + r] for fluid i, P-neighbour jn
 //   multigrid    per level: A stored [shift][row], vectors [row]
 //   CR vectors   double[3*nf], row 3i+r as the reference
@@ -50,9 +53,22 @@ struct ASlices {
 };
 
 constexpr int kSlice = 64;    // particles per A slice (one wavefront)
+
+// block k of particle lane `lane` of the slice whose first slot is sb (even)
+__host__ __device__ inline long long a_col_index(long long sb, int k, int lane) {
+    return ((sb / 2 + k / 2) * 64 + lane) * 2 + (k & 1);
+}
+__host__ __device__ inline long long a_val_index(long long sb, int k, int e, int lane) {
+    return (((sb / 2 + k / 2) * 9 + e) * 64 + lane) * 2 + (k & 1);
+}
 constexpr int kChunk = 256;   // particles per dot-product chunk / CR workgroup
 constexpr int kRuns = 28;     // run starts per fluid particle: 27 neighbour cells + end
 constexpr int kMaxP = 40;     // P-neighbours staged in LDS per particle by the C Lambda C^T kernel
+constexpr int kMaxLevels = 20;   // multigrid depth = floor(log2(cells per axis)) + 1
+// the V-cycle levels with at most this many rows may run in the one-workgroup
+// coarse cycle; the choice among these thresholds is autotuned (0: none)
+constexpr int kFuseVariants = 3;
+constexpr int kFuseRowsOf[kFuseVariants] = {0, 256, 2048};
 
 // ---- cells (calculateCellParticle) ----
 void launch_hash(hipStream_t st, const DevParams& P, const double* pos, const int* type, uint64_t* keys);
@@ -92,9 +108,11 @@ void launch_vector_p(hipStream_t st, const DevParams& P, const int* type, const 
                      const long long* nPtrP, const int* nIndP, double* vecP);
 void launch_b_minus_cp(hipStream_t st, const DevParams& P, const int* nCountP, const long long* nPtrP,
                        const int* nIndP, const double* cval, const double* vecP, double* b);
+// variant 0: P lists from global memory; 1: the slice's index lists in LDS (needs
+// every fluid particle's P-neighbour count <= kMaxP).  Same bits either way.
+constexpr int kCacVariants = 2;
 void launch_cac(hipStream_t st, const DevParams& P, const int* nFluid, const int* nCountP, const long long* nPtrP,
-                const int* nIndP, const double* cval, const double* dval, const double* lambda, ASlices A,
-                bool staged);  // also writes A.val32 when it is non-null
+                const int* nIndP, const double* cval, const double* dval, ASlices A, int variant);  // also writes A.val32 when it is non-null
 // ---- multigrid ----
 void launch_fluid_bbox(hipStream_t st, const DevParams& P, const int* cellStart, int* bbox);
 void launch_inv_diag(hipStream_t st, const DevParams& P, const int* selfIdx, ASlices A, double* invDiagA);
@@ -111,14 +129,21 @@ void launch_mg_residual(hipStream_t st, Level& L);     // R = Q - A S
 void launch_mg_residual_update(hipStream_t st, Level& L);  // R = Q - A S; S += invD R
 void launch_mg_restrict(hipStream_t st, const Level& S, Level& L);
 void launch_mg_prolong(hipStream_t st, const Level& L, Level& S);
+// levels lc..D-1 of the V-cycle in one workgroup; lvs = device copy of the levels,
+// cur = their ping-pong bits (bit l, lc-1 included).  Afterwards every level in
+// lc..D-2 has flipped (5 sweeps) and the coarsest has not (4 sweeps).
+void launch_mg_coarse(hipStream_t st, const Level* lvs, int lc, int D, unsigned cur, bool f32);
 void launch_mg_g2p(hipStream_t st, const DevParams& P, const Level& L0, const int* cellIndex,
                    const double* invDiagA, const double* q, double* u, double* part);
 // ---- conjugate residual ----
 void launch_cr_init(hipStream_t st, const DevParams& P, const double* vel, const double* force,
                     const double* mass, double* x);
+// SpMV variants differ only in how many block pairs are loaded per group (same
+// bits): FP64 2, 3 or 4 pairs; FP32 1 or 2 pairs
+constexpr int kSpmvVariants64 = 3, kSpmvVariants32 = 2;
 void launch_spmv(hipStream_t st, const DevParams& P, ASlices A, const int* nFluid, double alpha, const double* x,
                  double beta, const double* yin, double* y, double* y2, double* p_copy, const double* dotx,
-                 double* part, bool f32 = false);
+                 double* part, bool f32 = false, int variant = 0);
 void launch_convert_f32(hipStream_t st, const double* in, float* out, long long n);
 void launch_axpy(hipStream_t st, long long n, const double* x, double* y);  // y += 1.0*x
 void launch_dot_partial(hipStream_t st, const DevParams& P, const double* x, const double* y, double* part);

@@ -25,16 +25,6 @@ __device__ inline Rel rel(const DevParams& P, const double* __restrict__ pos, in
     return o;
 }
 
-// value (r,s) of block k of fluid particle i in the sliced A layout
-__device__ inline long long aval_index(const long long* __restrict__ start, int i, int k, int e) {
-    const int s = i / kSlice, lane = i % kSlice;
-    return ((start[s] + k) * 9 + e) * kSlice + lane;
-}
-__device__ inline long long acol_index(const long long* __restrict__ start, int i, int k) {
-    const int s = i / kSlice, lane = i % kSlice;
-    return (start[s] + k) * kSlice + lane;
-}
-
 // calculateConvection, calculateWall, calculatePeriodicBoundary, resetForce
 __global__ void k_kinematics(DevParams P, WallKin wk, const int* __restrict__ type, double* __restrict__ pos,
                              double* __restrict__ vel, double* __restrict__ force) {
@@ -200,12 +190,25 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
     const double mui = mu[i];
     const long long b0 = nPtr[i];
     const int nfl = nFluid[i], nc = nCount[i];
+    const long long sb = start[i / kSlice];
+    const int lane = i % kSlice;
+    // blocks 2m and 2m+1 are adjacent in memory: an even block waits in `pend` and
+    // is stored together with its odd partner as one 16-byte store per entry
+    double pend[9];
+    int pendCol = 0;
+    bool pending = false;
+    auto store1 = [&](int k, const double* v, int col) {
+        acol[a_col_index(sb, k, lane)] = col;
+        for (int e9 = 0; e9 < 9; ++e9) aval[a_val_index(sb, k, e9, lane)] = v[e9];
+    };
     for (int jn = 0; jn < nc; ++jn) {
         const int j = nInd[b0 + jn];
         const bool fluidj = jn < nfl;
-        if (fluidj) acol[acol_index(start, i, jn)] = j;
         if (i == j) {
             iN = jn;
+            acol[a_col_index(sb, jn, lane)] = j;
+            if (pending) store1(jn - 1, pend, pendCol);  // the self block is stored last
+            pending = false;
             continue;
         }
         double c9[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -230,14 +233,30 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
                 sv2 += c9[6] * v0; sv2 += c9[7] * v1; sv2 += c9[8] * v2;
             }
         }
-        if (fluidj)
-            for (int e9 = 0; e9 < 9; ++e9) aval[aval_index(start, i, jn, e9)] = inside ? -c9[e9] : 0.0;
+        if (fluidj) {
+            double v[9];
+            for (int e9 = 0; e9 < 9; ++e9) v[e9] = inside ? -c9[e9] : 0.0;
+            if ((jn & 1) == 0) {
+                for (int e9 = 0; e9 < 9; ++e9) pend[e9] = v[e9];
+                pendCol = j;
+                pending = true;
+            } else if (pending) {
+                typedef double d2 __attribute__((ext_vector_type(2)));
+                typedef int i2 __attribute__((ext_vector_type(2)));
+                *(i2*)&acol[a_col_index(sb, jn - 1, lane)] = i2{pendCol, j};
+                for (int e9 = 0; e9 < 9; ++e9) *(d2*)&aval[a_val_index(sb, jn - 1, e9, lane)] = d2{pend[e9], v[e9]};
+                pending = false;
+            } else {
+                store1(jn, v, j);
+            }
+        }
     }
+    if (pending) store1(nfl - 1, pend, pendCol);
     const double coef = mass[i] / P.V / P.dt;
     self[0] += coef;
     self[4] += coef;
     self[8] += coef;
-    for (int e9 = 0; e9 < 9; ++e9) aval[aval_index(start, i, iN, e9)] = self[e9];
+    for (int e9 = 0; e9 < 9; ++e9) aval[a_val_index(sb, iN, e9, lane)] = self[e9];
     selfIdx[i] = iN;
     const double bv[3] = {0.0 + sv0, 0.0 + sv1, 0.0 + sv2};
     for (int d = 0; d < 3; ++d) b[3 * i + d] = bv[d] + coef * vel[3 * i + d];
@@ -343,90 +362,43 @@ __global__ void k_b_minus_cp(DevParams P, const int* __restrict__ nCountP, const
     b[3 * i + 2] = b2;
 }
 
-// A += C Lambda C^T (multiplyMatrixC, 2587-2642).  One workgroup per A slice:
-// wave w handles blocks k = w, w+4, ... for the slice's 64 particles (lanes), so
-// the read-modify-write of every block entry is coalesced.  The two P-neighbour
-// lists are merged in ascending order, as in the reference.
+// A += C Lambda C^T (multiplyMatrixC, 2586-2642): for each fluid i and each of its
+// fluid neighbours j, merge the sorted P-neighbour lists of i and j and sum
+// D_ik (x) C_jk over the shared k (D = C*Lambda, left to right as the reference),
+// then add the 3x3 sum to block (i, j).  Workgroup = one slice: lane = particle,
+// wave w takes the blocks k = w, w+4, ...  kLds: the slice's own index lists are
+// staged in LDS (odd row stride, no bank conflicts); otherwise they are read from
+// global memory.  Both give the same bits; the faster one is picked at run time.
+// The FP32 copy of the finished block is written when aval32 is non-null.
+template <bool kLds>
 __global__ void __launch_bounds__(256) k_cac(DevParams P, const int* __restrict__ nFluid,
                                              const int* __restrict__ nCountP, const long long* __restrict__ nPtrP,
                                              const int* __restrict__ nIndP, const double* __restrict__ cval,
-                                             const double* __restrict__ lambda, const int* __restrict__ width,
-                                             const long long* __restrict__ start, const int* __restrict__ acol,
-                                             double* __restrict__ aval) {
-    const int slice = blockIdx.x;
-    const int lane = threadIdx.x % kSlice, wave = threadIdx.x / kSlice;
-    const int i = slice * kSlice + lane;
-    if (i >= P.nf) return;
-    const int nfl = nFluid[i];
-    const long long pi = nPtrP[i];
-    const int ni = nCountP[i];
-    const int w = width[slice];
-    for (int k = wave; k < w; k += 4) {
-        if (k >= nfl) break;
-        const long long slot = (start[slice] + k) * kSlice + lane;
-        const int j = acol[slot];
-        const long long pj = nPtrP[j];
-        const int nj = nCountP[j];
-        double sum[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        int a = 0, c = 0;
-        while (a < ni && c < nj) {
-            const int ka = nIndP[pi + a], kc = nIndP[pj + c];
-            if (ka == kc) {
-                const double lk = lambda[ka];
-                const double* ci = &cval[3 * (pi + a)];
-                const double* cj = &cval[3 * (pj + c)];
-                for (int r = 0; r < 3; ++r)
-                    for (int s = 0; s < 3; ++s) sum[3 * r + s] += ci[r] * lk * cj[s];
-                a++;
-                c++;
-            } else if (ka < kc) {
-                a++;
-            } else {
-                c++;
-            }
-        }
-        for (int e9 = 0; e9 < 9; ++e9) {
-            const long long idx = ((start[slice] + k) * 9 + e9) * kSlice + lane;
-            aval[idx] += sum[e9];
-        }
-    }
-}
-
-// the same sums with the slice's own P-neighbour lists (indices and D = C*Lambda)
-// staged in LDS; the neighbour j's list is read from global memory
-__global__ void __launch_bounds__(256) k_cac_lds(DevParams P, const int* __restrict__ nFluid,
-                                                 const int* __restrict__ nCountP,
-                                                 const long long* __restrict__ nPtrP, const int* __restrict__ nIndP,
-                                                 const double* __restrict__ cval, const double* __restrict__ dval,
-                                                 const int* __restrict__ width, const long long* __restrict__ start,
-                                                 const int* __restrict__ acol, double* __restrict__ aval,
-                                                 float* __restrict__ aval32) {
-    __shared__ int sk[kSlice * kMaxP];
-    __shared__ double sd[kSlice * kMaxP * 3];
+                                             const double* __restrict__ dval, const long long* __restrict__ start,
+                                             const int* __restrict__ acol, double* __restrict__ aval,
+                                             float* __restrict__ aval32) {
+    constexpr int kStride = kMaxP + 1;
+    __shared__ int sk[kLds ? kSlice * kStride : 1];
     const int slice = blockIdx.x;
     const int base = slice * kSlice;
-    for (int t = threadIdx.x; t < kSlice * kMaxP; t += blockDim.x) {
-        const int l = t / kMaxP, a = t % kMaxP, i = base + l;
-        if (i < P.nf && a < nCountP[i]) {
-            const long long e = nPtrP[i] + a;
-            sk[t] = nIndP[e];
-            sd[3 * t] = dval[3 * e];
-            sd[3 * t + 1] = dval[3 * e + 1];
-            sd[3 * t + 2] = dval[3 * e + 2];
+    if (kLds) {
+        for (int t = threadIdx.x; t < kSlice * kMaxP; t += blockDim.x) {
+            const int l = t / kMaxP, a = t % kMaxP, i = base + l;
+            if (i < P.nf && a < nCountP[i]) sk[l * kStride + a] = nIndP[nPtrP[i] + a];
         }
+        __syncthreads();
     }
-    __syncthreads();
     const int lane = threadIdx.x % kSlice, wave = threadIdx.x / kSlice;
     const int i = base + lane;
     if (i >= P.nf) return;
     const int nfl = nFluid[i];
+    const long long pi = nPtrP[i];
     const int ni = nCountP[i];
-    const int* ki = sk + lane * kMaxP;
-    const double* di = sd + 3 * lane * kMaxP;
+    const int* ki = kLds ? sk + lane * kStride : nIndP + pi;
+    const double* di = dval + 3 * pi;
     const long long sb = start[slice];
     for (int k = wave; k < nfl; k += 4) {
-        const long long slot = (sb + k) * kSlice + lane;
-        const int j = acol[slot];
+        const int j = acol[a_col_index(sb, k, lane)];
         const long long pj = nPtrP[j];
         const int nj = nCountP[j];
         double sum[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -454,7 +426,7 @@ __global__ void __launch_bounds__(256) k_cac_lds(DevParams P, const int* __restr
             }
         }
         for (int e9 = 0; e9 < 9; ++e9) {
-            const long long at = ((sb + k) * 9 + e9) * kSlice + lane;
+            const long long at = a_val_index(sb, k, e9, lane);
             const double v = aval[at] + sum[e9];
             aval[at] = v;
             if (aval32) aval32[at] = (float)v;
@@ -516,15 +488,14 @@ void launch_b_minus_cp(hipStream_t st, const DevParams& P, const int* nCountP, c
 }
 
 void launch_cac(hipStream_t st, const DevParams& P, const int* nFluid, const int* nCountP, const long long* nPtrP,
-                const int* nIndP, const double* cval, const double* dval, const double* lambda, ASlices A,
-                bool staged) {
+                const int* nIndP, const double* cval, const double* dval, ASlices A, int variant) {
     if (A.nslices == 0) return;
-    if (staged)
-        hipLaunchKernelGGL(k_cac_lds, dim3(A.nslices), dim3(256), 0, st, P, nFluid, nCountP, nPtrP, nIndP, cval,
-                           dval, A.width, A.start, A.col, A.val, A.val32);
+    if (variant == 1)
+        hipLaunchKernelGGL(k_cac<true>, dim3(A.nslices), dim3(256), 0, st, P, nFluid, nCountP, nPtrP, nIndP, cval,
+                           dval, A.start, A.col, A.val, A.val32);
     else
-        hipLaunchKernelGGL(k_cac, dim3(A.nslices), dim3(256), 0, st, P, nFluid, nCountP, nPtrP, nIndP, cval, lambda,
-                           A.width, A.start, A.col, A.val);
+        hipLaunchKernelGGL(k_cac<false>, dim3(A.nslices), dim3(256), 0, st, P, nFluid, nCountP, nPtrP, nIndP, cval,
+                           dval, A.start, A.col, A.val, A.val32);
 }
 
 }  // namespace mph

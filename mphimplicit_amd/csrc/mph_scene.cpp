@@ -283,11 +283,16 @@ int phase_multiply_c(mph_scene& S) {
     int rc;
     if (mixed(S) && (rc = S.aval32.ensure(S.slots > 0 ? 9 * S.slots : 1))) return rc;
     launch_b_minus_cp(S.st, S.P, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.vecP.p, S.b.p);
-    const bool staged = S.hmaxP <= kMaxP;
-    launch_cac(S.st, S.P, S.nFluid.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.dval.p, S.lambda.p, aslices(S),
-               staged);
-    // the staged kernel writes the FP32 copy while it rewrites A; otherwise convert
-    if (mixed(S) && !staged) launch_convert_f32(S.st, S.aval.p, S.aval32.p, 9 * S.slots);
+    // C Lambda C^T also writes the FP32 copy of A in mixed precision (aval32)
+    int v = S.tuneCac.pick();
+    const bool lds_ok = S.hmaxP <= kMaxP;
+    if (v == 1 && !lds_ok) {
+        launch_cac(S.st, S.P, S.nFluid.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.dval.p, aslices(S), 0);
+    } else {
+        S.tuneCac.begin(S.st, v);
+        launch_cac(S.st, S.P, S.nFluid.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.dval.p, aslices(S), v);
+        S.tuneCac.end(S.st);
+    }
     return MPH_OK;
 }
 
@@ -303,6 +308,7 @@ int phase_multigrid(mph_scene& S) {
     while ((1 << depth) <= mgc) depth++;
     if (depth < 2)
         return fail(MPH_E_STATE, "multigrid depth < 2: all fluid in one cell (the reference V-cycle reads level -1)");
+    if (depth > kMaxLevels) return fail(MPH_E_STATE, "multigrid depth above " + std::to_string(kMaxLevels));
     S.depth = depth;
     if ((int)S.lv.size() < depth) S.lv.resize(depth);
     int rc;
@@ -336,6 +342,20 @@ int phase_multigrid(mph_scene& S) {
             L.A32 = H.A32.p;
         }
     }
+    // the small levels may run in one workgroup (launch_mg_coarse): first fused level
+    // per row threshold
+    for (int v = 0; v < kFuseVariants; ++v) {
+        S.fuseOf[v] = depth;
+        for (int l = 1; l < depth; ++l)
+            if (S.lv[l].L.nrow <= kFuseRowsOf[v]) {
+                S.fuseOf[v] = l;
+                break;
+            }
+    }
+    S.hlv.resize(depth);
+    for (int l = 0; l < depth; ++l) S.hlv[l] = S.lv[l].L;
+    if ((rc = S.lvDev.ensure(kMaxLevels))) return rc;
+    HIPCHK(hipMemcpyAsync(S.lvDev.p, S.hlv.data(), depth * sizeof(Level), hipMemcpyHostToDevice, S.st));
     launch_inv_diag(S.st, S.P, S.selfIdx.p, aslices(S), S.invDiagA.p);
     launch_mg_galerkin0(S.st, S.P, S.lv[0].L, S.cellStart.p, S.runStart.p, aslices(S));
     for (int l = 1; l < depth; ++l) launch_mg_galerkinL(S.st, S.lv[l - 1].L, S.lv[l].L);
@@ -350,14 +370,16 @@ int phase_multigrid(mph_scene& S) {
 // receives the chunk partials of in.out when non-null
 void precondition(mph_scene& S, const double* in, double* out, double* part) {
     hipStream_t st = S.st;
-    const int D = S.depth;
+    const int v = S.tuneFuse.pick();
+    const int D = S.depth, F = S.fuseOf[v];  // levels F..D-1 run fused (F = D: none)
+    S.tuneFuse.begin(st, v);
     Level& L0 = S.lv[0].L;
     launch_mg_p2g(st, S.P, L0, S.cellStart.p, in);
     launch_mg_begin(st, L0);
     launch_mg_update(st, L0);  // pre-smooth x2: {S += invD R; R = Q - A S}
     launch_mg_residual_update(st, L0);
     launch_mg_residual(st, L0);
-    for (int l = 1; l < D - 1; ++l) {
+    for (int l = 1; l < D - 1 && l < F; ++l) {
         Level& L = S.lv[l].L;
         launch_mg_restrict(st, S.lv[l - 1].L, L);
         launch_mg_begin(st, L);
@@ -366,7 +388,12 @@ void precondition(mph_scene& S, const double* in, double* out, double* part) {
         launch_mg_residual_update(st, L);
         launch_mg_residual(st, L);
     }
-    {
+    if (F < D) {
+        unsigned cur = 0;
+        for (int l = F - 1; l < D; ++l) cur |= (unsigned)S.lv[l].L.cur << l;
+        launch_mg_coarse(st, S.lvDev.p, F, D, cur, S.lv[0].L.f32 != 0);
+        for (int l = F; l < D - 1; ++l) S.lv[l].L.cur ^= 1;  // five sweeps each; the coarsest four
+    } else {
         Level& L = S.lv[D - 1].L;
         launch_mg_restrict(st, S.lv[D - 2].L, L);
         launch_mg_begin(st, L);
@@ -374,13 +401,14 @@ void precondition(mph_scene& S, const double* in, double* out, double* part) {
         for (int k = 0; k < 4; ++k) launch_mg_residual_update(st, L);
         launch_mg_prolong(st, L, S.lv[D - 2].L);
     }
-    for (int l = D - 2; l > 0; --l) {
+    for (int l = (D - 2 < F - 1 ? D - 2 : F - 1); l > 0; --l) {
         Level& L = S.lv[l].L;
         for (int k = 0; k < 3; ++k) launch_mg_residual_update(st, L);  // post-smooth x3
         launch_mg_prolong(st, L, S.lv[l - 1].L);
     }
     for (int k = 0; k < 2; ++k) launch_mg_residual_update(st, L0);  // post-smooth x2
     launch_mg_g2p(st, S.P, L0, S.cellIndex.p, S.invDiagA.p, in, out, part);
+    S.tuneFuse.end(st);
 }
 
 // without MULTIGRID_SOLVER the preconditioner is a copy (main.cpp:4331, 4341, 4369)
@@ -394,10 +422,18 @@ int apply_m(mph_scene& S, const double* in, double* out, double* part) {
     return MPH_OK;
 }
 
+void collect_tuning(mph_scene& S) {  // the stream is idle
+    S.tuneSpmv64.collect();
+    S.tuneSpmv32.collect();
+    S.tuneCac.collect();
+    S.tuneFuse.collect();
+}
+
 int read_scalar(mph_scene& S, int slot, double* v) {
     HIPCHK(hipMemcpyAsync(S.hpin + 8, S.scal.p + slot, 8, hipMemcpyDeviceToHost, S.st));
     HIPCHK(hipStreamSynchronize(S.st));
     *v = S.hpin[8];
+    collect_tuning(S);
     return MPH_OK;
 }
 
@@ -418,7 +454,11 @@ void spmv(mph_scene& S, double alpha, const double* x, double beta, const double
         hipError_t e = hipEventRecord(S.spmvEv[S.spmvUsed], S.st);
         if (e != hipSuccess) S.spmvTimingErr = (int)e;
     }
-    launch_spmv(S.st, S.P, aslices(S), S.nFluid.p, alpha, x, beta, yin, y, y2, p_copy, dotx, part, f32);
+    Autotune& T = f32 ? S.tuneSpmv32 : S.tuneSpmv64;
+    const int v = T.pick();
+    T.begin(S.st, v);
+    launch_spmv(S.st, S.P, aslices(S), S.nFluid.p, alpha, x, beta, yin, y, y2, p_copy, dotx, part, f32, v);
+    T.end(S.st);
     if (S.spmvTiming) {
         hipError_t e = hipEventRecord(S.spmvEv[S.spmvUsed + 1], S.st);
         if (e != hipSuccess) S.spmvTimingErr = (int)e;
@@ -497,6 +537,7 @@ int phase_solve(mph_scene& S) {
     HIPCHK(hipMemcpyAsync(S.hpin + 16, S.scal.p, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (S.spmvTiming) collect_spmv_timing(S);
+    collect_tuning(S);
     S.crIter = (int)iter;
     S.rr0 = S.hpin[16 + kRR0];
     S.rs0 = S.hpin[16 + kRS0];
@@ -580,6 +621,7 @@ int phase_solve_mixed(mph_scene& S) {
     launch_store_velocity(st, P, S.x.p, S.vel.p);
     HIPCHK(hipStreamSynchronize(st));
     if (S.spmvTiming) collect_spmv_timing(S);
+    collect_tuning(S);
     S.crIter = (int)total;
     S.irOuter = outer;
     S.rr = rrOut;
@@ -693,6 +735,11 @@ void destroy(mph_scene* S) {
         H.A32.release();
     }
     S->aval32.release();
+    S->lvDev.release();
+    S->tuneSpmv64.release();
+    S->tuneSpmv32.release();
+    S->tuneCac.release();
+    S->tuneFuse.release();
     if (S->hpin) (void)hipHostFree(S->hpin);
     for (hipEvent_t& e : S->ev)
         if (e) (void)hipEventDestroy(e);
@@ -806,12 +853,12 @@ int host_field(mph_scene& S, const std::string& n, Blob& out, bool& found) {
         for (int i = 0; i < nf; ++i) {
             const int sl = i / kSlice, lane = i % kSlice;
             for (int k = 0; k < nfl[i]; ++k) {
-                const long long slot = (start[sl] + k) * kSlice + lane;
+                const long long slot = a_col_index(start[sl], k, lane);
                 for (int r = 0; r < 3; ++r)
                     for (int c = 0; c < 3; ++c) {
                         const long long at = ptr[3 * i + r] + 3LL * k + c;
                         ind[at] = 3 * col[slot] + c;
-                        cof[at] = val[((start[sl] + k) * 9 + 3 * r + c) * kSlice + lane];
+                        cof[at] = val[a_val_index(start[sl], k, 3 * r + c, lane)];
                     }
             }
         }
@@ -943,6 +990,10 @@ int mph_scene_create(const mph_params* p, const mph_grid* g, int device, mph_sce
         return fail(MPH_E_DEVICE, "init sequence failed");
     }
     S->iStep = (int)(S->time / S->P.dt);
+    S->tuneSpmv64.init(kSpmvVariants64, 3);
+    S->tuneSpmv32.init(kSpmvVariants32, 3);
+    S->tuneCac.init(kCacVariants, 1);
+    S->tuneFuse.init(kFuseVariants, 3);
     S->outputNext = S->time;
     S->vtkOutputNext = S->time;
     *out = S;
@@ -1089,6 +1140,10 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
     else if (n == "SpmvMs") *v = S->spmvMs;
     else if (n == "SpmvLaunches") *v = (double)S->spmvLaunches;
     else if (n == "SpmvMsF32") *v = S->spmvMs32;
+    else if (n == "TunedSpmv64") *v = S->tuneSpmv64.best;
+    else if (n == "TunedSpmv32") *v = S->tuneSpmv32.best;
+    else if (n == "TunedCac") *v = S->tuneCac.best;
+    else if (n == "TunedFuse") *v = S->tuneFuse.best;
     else if (n == "SpmvLaunchesF32") *v = (double)S->spmvLaunches32;
     else if (n == "SpmvTimingError") *v = S->spmvTimingErr;
     else if (n == "ABlocks" || n == "CBlocks") {

@@ -56,6 +56,81 @@ struct HostLevel {
 
 enum { kRR0 = 0, kRS0 = 1, kRR = 2, kRHO = 3, kRHOP = 4, kTMP = 5, kRS = 6 };
 
+// Run-time choice among kernel variants that compute identical bits: while
+// exploring, launches cycle through the variants with HIP events around them;
+// once every variant has `trials` timings the fastest mean is kept for good.
+struct Autotune {
+    int nvar = 1, trials = 1, best = 0;
+    std::vector<double> sum;
+    std::vector<int> cnt;
+    struct Pend {
+        int v;
+        hipEvent_t a, b;
+    };
+    std::vector<Pend> pend;
+    std::vector<hipEvent_t> pool;
+    void init(int n, int t) {
+        nvar = n;
+        trials = t;
+        best = n == 1 ? 0 : -1;
+        sum.assign(n, 0.0);
+        cnt.assign(n, 0);
+    }
+    int pick() const {  // the variant for the next launch
+        if (best >= 0) return best;
+        int v = 0;
+        for (int k = 1; k < nvar; ++k)
+            if (cnt[k] < cnt[v]) v = k;
+        return v;
+    }
+    bool exploring() const { return best < 0; }
+    hipEvent_t event() {
+        if (pool.empty()) {
+            hipEvent_t e = nullptr;
+            (void)hipEventCreate(&e);
+            return e;
+        }
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    void begin(hipStream_t st, int v) {
+        if (!exploring()) return;
+        Pend p{v, event(), event()};
+        (void)hipEventRecord(p.a, st);
+        pend.push_back(p);
+    }
+    void end(hipStream_t st) {
+        if (!exploring() || pend.empty()) return;
+        (void)hipEventRecord(pend.back().b, st);
+    }
+    // after the stream has been synchronised past the recorded launches
+    void collect() {
+        for (Pend& p : pend) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess && p.v >= 0 && p.v < nvar) {
+                sum[p.v] += ms;
+                cnt[p.v]++;
+            }
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pend.clear();
+        if (best >= 0) return;
+        for (int k = 0; k < nvar; ++k)
+            if (cnt[k] < trials) return;
+        best = 0;
+        for (int k = 1; k < nvar; ++k)
+            if (sum[k] / cnt[k] < sum[best] / cnt[best]) best = k;
+    }
+    void release() {
+        for (Pend& p : pend) pool.push_back(p.a), pool.push_back(p.b);
+        pend.clear();
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+        pool.clear();
+    }
+};
+
 }  // namespace host
 }  // namespace mph
 
@@ -94,7 +169,10 @@ struct mph_scene {
     // multigrid
     int hbbox[6] = {0, 0, 0, 0, 0, 0};
     int depth = 0;
+    int fuseOf[mph::kFuseVariants] = {0, 0, 0};  // first level of the one-workgroup coarse cycle
     std::vector<mph::host::HostLevel> lv;
+    std::vector<mph::Level> hlv;           // level records as uploaded to lvDev
+    mph::host::DBuf<mph::Level> lvDev;
     mph::host::DBuf<double> invDiagA;
     // CR
     mph::host::DBuf<double> x, r, s, y, p, q, u, part, scal, rOut, dvec;
@@ -114,6 +192,9 @@ struct mph_scene {
     double spmvMs = 0, spmvMs32 = 0;
     long long spmvLaunches = 0, spmvLaunches32 = 0;
     int spmvTimingErr = 0;
+    // run-time variant choice (SpMV load groups per precision, C Lambda C^T schedule,
+    // V-cycle levels fused into one workgroup)
+    mph::host::Autotune tuneSpmv64, tuneSpmv32, tuneCac, tuneFuse;
 };
 
 

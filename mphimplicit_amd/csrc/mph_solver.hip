@@ -46,27 +46,30 @@ __global__ void k_inv_diag(DevParams P, const int* __restrict__ selfIdx, const l
     if (i >= P.nf) return;
     const int s = i / kSlice, lane = i % kSlice, k = selfIdx[i];
     for (int r = 0; r < 3; ++r) {
-        const double v = aval[((start[s] + k) * 9 + 4 * r) * kSlice + lane];
+        const double v = aval[a_val_index(start[s], k, 4 * r, lane)];
         invDiagA[3 * i + r] = v != 0.0 ? 1.0 / v : 0.0;
     }
 }
 
 // level-0 matrix = sum of the particle blocks of every cell pair, in the order of
 // calculateMultiGridCsrA (3351-3410): particles of the row cell ascending, then the
-// blocks of each particle ascending.  Thread = (cell g, stencil cell c).  The blocks
-// of a particle that fall in one neighbour cell form a contiguous run of its sorted
-// fluid-neighbour list; the neighbour search recorded where each run starts.
-__global__ void k_mg_galerkin0(DevParams P, Level L, const int* __restrict__ cellStart,
-                               const unsigned short* __restrict__ runStart, const long long* __restrict__ start,
-                               const double* __restrict__ aval) {
+// blocks of each particle ascending.  Workgroup = row cell g; thread = (stencil
+// cell c, block entry e), so each thread runs one sequential sum and the cell's
+// particles (adjacent lanes of the A slices) share cache lines across the
+// workgroup.  The blocks of a particle that fall in one neighbour cell form a
+// contiguous run of its sorted fluid-neighbour list; the neighbour search recorded
+// where each run starts.
+__global__ void __launch_bounds__(256) k_mg_galerkin0(DevParams P, Level L, const int* __restrict__ cellStart,
+                                                      const unsigned short* __restrict__ runStart,
+                                                      const long long* __restrict__ start,
+                                                      const double* __restrict__ aval) {
     const int ncs = L.cs[0] * L.cs[1] * L.cs[2];
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)L.cells * ncs) return;
-    const int g = (int)(t / ncs), c = (int)(t % ncs);
+    const int g = blockIdx.x;
+    if ((int)threadIdx.x >= 9 * ncs) return;
+    const int c = threadIdx.x / 9, e = threadIdx.x % 9;
     int gx, gy, gz;
     grid_xyz(L, g, gx, gy, gz);
     const int cx = gx + L.mn[0], cy = gy + L.mn[1], cz = gz + L.mn[2];
-    double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int ic = cell_id(P, cx, cy, cz);
     const int p0 = cellStart[ic], p1 = cellStart[ic + 1];
     // visiting order of the neighbour search (ascending CellId); which runs map to c
@@ -94,22 +97,20 @@ __global__ void k_mg_galerkin0(DevParams P, Level L, const int* __restrict__ cel
                     if (L.cs[2] * L.cs[1] * ix + L.cs[2] * iy + iz == c) mask |= 1u << q;
                 }
     }
+    double acc = 0.0;
     if (mask)
         for (int ip = p0; ip < p1; ++ip) {
             const unsigned short* rs = runStart + (long long)ip * kRuns;
             const int lane = ip % kSlice;
-            const double* vb = aval + (start[ip / kSlice] * 9) * kSlice + lane;
+            const long long sb = start[ip / kSlice];
             for (unsigned m = mask; m; m &= m - 1) {
                 const int q = __builtin_ctz(m);
                 const int k1 = rs[q + 1];
-                for (int k = rs[q]; k < k1; ++k) {
-                    const double* v = vb + (long long)k * 9 * kSlice;
-                    for (int e = 0; e < 9; ++e) acc[e] += v[e * kSlice];
-                }
+                for (int k = rs[q]; k < k1; ++k) acc += aval[a_val_index(sb, k, e, lane)];
             }
         }
-    for (int r = 0; r < 3; ++r)
-        for (int sd = 0; sd < 3; ++sd) L.A[(long long)(3 * c + sd) * L.nrow + 3 * g + r] = acc[3 * r + sd];
+    const int r = e / 3, sd = e % 3;
+    L.A[(long long)(3 * c + sd) * L.nrow + 3 * g + r] = acc;
 }
 
 // coarse level = R A_fine P by summation (3414-3507): thread = (coarse cell, stencil
@@ -216,6 +217,12 @@ __global__ void k_mg_prolong(Level L, Level S) {
     if (row < S.nrow) mg_prolong_row(L, S, row);
 }
 
+// cur: ping-pong bits of levels lc-1..D-1 (bit l), tracked by the host
+__global__ void __launch_bounds__(1024) k_mg_coarse(const Level* __restrict__ lvs, int lc, int D, unsigned cur,
+                                                   int f32) {
+    mg_coarse_cycle(lvs, lc, D, cur, f32 != 0);
+}
+
 __global__ void __launch_bounds__(256) k_mg_g2p(DevParams P, Level L, const int* __restrict__ cellIndex,
                                                 const double* __restrict__ invDiagA, const double* __restrict__ q,
                                                 double* __restrict__ u, double* __restrict__ part) {
@@ -232,7 +239,7 @@ __global__ void k_cr_init(DevParams P, const double* __restrict__ vel, const dou
     if (i < P.nf) cr_init_particle(P, vel, force, mass, x, i);
 }
 
-template <bool kNeg, class VT>
+template <bool kNeg, class VT, int kUn>
 __global__ void __launch_bounds__(256) k_spmv(int nf, const int* __restrict__ nFluid,
                                               const long long* __restrict__ start, const int* __restrict__ col,
                                               const VT* __restrict__ val, const double* __restrict__ x,
@@ -240,7 +247,7 @@ __global__ void __launch_bounds__(256) k_spmv(int nf, const int* __restrict__ nF
                                               double* p_copy, const double* __restrict__ dotx,
                                               double* __restrict__ part) {
     __shared__ double sh[256];
-    spmv_chunk<kNeg, VT>(nf, nFluid, start, col, val, x, beta, yin, y, y2, p_copy, dotx, part, blockIdx.x, sh);
+    spmv_chunk<kNeg, VT, kUn>(nf, nFluid, start, col, val, x, beta, yin, y, y2, p_copy, dotx, part, blockIdx.x, sh);
 }
 
 __global__ void __launch_bounds__(256) k_dot_partial(int nf, const double* __restrict__ x,
@@ -439,9 +446,10 @@ void launch_inv_diag(hipStream_t st, const DevParams& P, const int* selfIdx, ASl
 }
 void launch_mg_galerkin0(hipStream_t st, const DevParams& P, const Level& L, const int* cellStart,
                          const unsigned short* runStart, ASlices A) {
-    const long long n = (long long)L.cells * L.cs[0] * L.cs[1] * L.cs[2];
-    hipLaunchKernelGGL(k_mg_galerkin0, dim3(nblk(n, 128)), dim3(128), 0, st, P, L, cellStart, runStart, A.start,
-                       A.val);
+    const int threads = 9 * L.cs[0] * L.cs[1] * L.cs[2];  // <= 243
+    if (L.cells > 0)
+        hipLaunchKernelGGL(k_mg_galerkin0, dim3(L.cells), dim3((threads + 63) / 64 * 64), 0, st, P, L, cellStart,
+                           runStart, A.start, A.val);
 }
 void launch_mg_galerkinL(hipStream_t st, const Level& S, const Level& L) {
     const long long n = (long long)L.cells * L.cs[0] * L.cs[1] * L.cs[2];
@@ -480,6 +488,9 @@ void launch_mg_restrict(hipStream_t st, const Level& S, Level& L) {
 void launch_mg_prolong(hipStream_t st, const Level& L, Level& S) {
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(S.nrow, 256)), dim3(256), 0, st, L, S);
 }
+void launch_mg_coarse(hipStream_t st, const Level* lvs, int lc, int D, unsigned cur, bool f32) {
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, st, lvs, lc, D, cur, f32 ? 1 : 0);
+}
 void launch_mg_g2p(hipStream_t st, const DevParams& P, const Level& L0, const int* cellIndex,
                    const double* invDiagA, const double* q, double* u, double* part) {
     hipLaunchKernelGGL(k_mg_g2p, dim3(nblk(P.nf, 256)), dim3(256), 0, st, P, L0, cellIndex, invDiagA, q, u, part);
@@ -488,26 +499,37 @@ void launch_cr_init(hipStream_t st, const DevParams& P, const double* vel, const
                     const double* mass, double* x) {
     hipLaunchKernelGGL(k_cr_init, dim3(nblk(P.nf, 256)), dim3(256), 0, st, P, vel, force, mass, x);
 }
+template <bool kNeg, class VT, int kUn>
+static void spmv_launch(hipStream_t st, const DevParams& P, const ASlices& A, const int* nFluid, const VT* val,
+                        const double* x, double beta, const double* yin, double* y, double* y2, double* p_copy,
+                        const double* dotx, double* part) {
+    hipLaunchKernelGGL((k_spmv<kNeg, VT, kUn>), dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P.nf, nFluid, A.start,
+                       A.col, val, x, beta, yin, y, y2, p_copy, dotx, part);
+}
+
 void launch_spmv(hipStream_t st, const DevParams& P, ASlices A, const int* nFluid, double alpha, const double* x,
                  double beta, const double* yin, double* y, double* y2, double* p_copy, const double* dotx,
-                 double* part, bool f32) {
+                 double* part, bool f32, int variant) {
     // alpha is +1 or -1 at every call site (main.cpp:4337, 4353, 4357)
-    const dim3 g(nblk(P.nf, kChunk)), b(kChunk);
+    if (P.nf == 0) return;
+#define MPH_SPMV(NEG, VT, UN, VAL) spmv_launch<NEG, VT, UN>(st, P, A, nFluid, VAL, x, beta, yin, y, y2, p_copy, dotx, part)
     if (f32) {
-        if (alpha < 0)
-            hipLaunchKernelGGL((k_spmv<true, float>), g, b, 0, st, P.nf, nFluid, A.start, A.col, A.val32, x, beta, yin,
-                               y, y2, p_copy, dotx, part);
-        else
-            hipLaunchKernelGGL((k_spmv<false, float>), g, b, 0, st, P.nf, nFluid, A.start, A.col, A.val32, x, beta, yin,
-                               y, y2, p_copy, dotx, part);
+        const int un = variant == 1 ? 2 : 1;
+        if (alpha < 0) { if (un == 2) MPH_SPMV(true, float, 2, A.val32); else MPH_SPMV(true, float, 1, A.val32); }
+        else { if (un == 2) MPH_SPMV(false, float, 2, A.val32); else MPH_SPMV(false, float, 1, A.val32); }
     } else {
-        if (alpha < 0)
-            hipLaunchKernelGGL((k_spmv<true, double>), g, b, 0, st, P.nf, nFluid, A.start, A.col, A.val, x, beta, yin,
-                               y, y2, p_copy, dotx, part);
-        else
-            hipLaunchKernelGGL((k_spmv<false, double>), g, b, 0, st, P.nf, nFluid, A.start, A.col, A.val, x, beta, yin,
-                               y, y2, p_copy, dotx, part);
+        const int un = variant == 1 ? 3 : variant == 2 ? 4 : 2;
+        if (alpha < 0) {
+            if (un == 3) MPH_SPMV(true, double, 3, A.val);
+            else if (un == 4) MPH_SPMV(true, double, 4, A.val);
+            else MPH_SPMV(true, double, 2, A.val);
+        } else {
+            if (un == 3) MPH_SPMV(false, double, 3, A.val);
+            else if (un == 4) MPH_SPMV(false, double, 4, A.val);
+            else MPH_SPMV(false, double, 2, A.val);
+        }
     }
+#undef MPH_SPMV
 }
 
 __global__ void k_convert_f32_(const double* __restrict__ in, float* __restrict__ out, long long n) {
