@@ -314,6 +314,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (scenes shard by scene, "
               "not by particle). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    from mphimplicit_amd.scene import probe_bandwidth
+    read_gbs, copy_gbs = probe_bandwidth(0, 8 << 30, 5)
+    print("[bench] HBM ceilings: read %.0f GB/s, copy %.0f GB/s" % (read_gbs, copy_gbs), file=sys.stderr, flush=True)
     g = run_gpu(args)
     lat_ms = np.array(g["lat"]) * 1e3
     out = {
@@ -342,6 +345,10 @@ def main():
         "cr_iterations": g["iters"],
         "autotuned": g["tuned"],
         "roofline": g["roofline"],
+        "hbm_ceiling": {"read_gbs": round(read_gbs, 1), "copy_gbs": round(copy_gbs, 1),
+                        "spmv_frac_of_read": round(g["roofline"]["achieved"] / read_gbs, 4)
+                        if g["roofline"]["achieved"] else None,
+                        "probe": "mph_probe_bandwidth: best of 5 passes over 8 GiB; read with 16-byte nontemporal loads, copy with 16-byte loads and stores"},
         "cpu_baseline": None,
     }
     m = None

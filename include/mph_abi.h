@@ -143,8 +143,22 @@ int mph_scene_run_phase(mph_scene* s, int phase);
 
 /* Copy a named field to or from the host.  Sizes are in bytes and must equal the
  * field's current size (mph_scene_field_size).  Field names follow the
- * reference's globals: "Position", "Velocity", "Property", "Force", "NeighborInd",
- * "CsrCofA", "VectorB", ... (listed in mphimplicit_amd/csrc/mph_fields.inc). */
+ * reference's globals:
+ *   particles   Position Velocity Property ParticleIndex Force Mass Kappa Lambda Mu
+ *               DensityA GravityCenter PressureA VolStrainP DivergenceP PressureP
+ *               VirialPressureAtParticle VirialStressAtParticle
+ *   cells       CellIndex CellParticle
+ *   neighbours  NeighborCount NeighborFluidCount NeighborCountP NeighborPtr
+ *               NeighborPtrP NeighborInd NeighborIndP
+ *   matrices    CsrPtrA CsrIndA CsrCofA VectorB CsrPtrC CsrIndC CsrCofC VectorP
+ *               InvDiagA, the multigrid levels MGCofA_L<l> / MGInvD_L<l>, and the layout of
+ *               A on the device as ACol / AVal
+ * Scalars (mph_scene_scalar): Time iStep Dt OutputNext VtkOutputNext VtkStep N0a N0p
+ * R2g Swa Swg Swp Swv MaxRadius ParticleCount FluidCount PowerParticleCount
+ * CellCount0..2 TotalCellCount NeighborIndCount NeighborIndCountP MultiGridDepth
+ * CRIterations rr0 rr rs0 rs RefinementSteps ABlocks ASlots CBlocks, the timing
+ * scalars of mph_scene_set_timing, and TunedSpmv64 TunedSpmv32 TunedCac TunedFuse
+ * (the variants the run-time tuner kept, -1 while still exploring). */
 int mph_scene_field_size(mph_scene* s, const char* field, size_t* bytes, int* elem_bytes);
 int mph_scene_read(mph_scene* s, const char* field, void* host, size_t bytes);
 int mph_scene_write(mph_scene* s, const char* field, const void* host, size_t bytes);
@@ -196,6 +210,12 @@ int mph_batch_remove(mph_batch* b, mph_scene* s);
 int mph_batch_size(const mph_batch* b);
 int mph_batch_step(mph_batch* b, mph_step_stats* stats);
 int mph_batch_destroy(mph_batch* b);
+
+/* Measured HBM ceilings of `device`, for the roofline (no counterpart in the
+ * reference): the best of `reps` timed passes of a streaming read of `bytes` and of a
+ * device-to-device copy of `bytes` (read + write counted), with the 16-byte
+ * nontemporal loads the SpMV uses.  Allocates and frees 2*bytes of device memory. */
+int mph_probe_bandwidth(int device, unsigned long long bytes, int reps, double* read_gbs, double* copy_gbs);
 
 /* Device bytes held by the scene (all persistent buffers, as now grown). */
 int mph_scene_device_bytes(mph_scene* s, unsigned long long* bytes);

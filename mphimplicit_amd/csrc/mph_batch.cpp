@@ -171,10 +171,8 @@ struct VCycle {
         int maxD = 0;
         for (int s = 0; s < (int)sc.size(); ++s)
             if (on.has(s)) maxD = std::max(maxD, sc[s]->depth);
-        mg_p2g(T, on.m, in);
-        mg_begin(T, on.m, 0, curset(on, 0));
-        mg_update(T, on.m, 0, curset(on, 0));  // pre-smooth x2
-        ru(on, 0, f32);
+        mg_p2g_bu(T, on.m, in, curset(on, 0));  // P2G, S = 0, R = Q, S += invD R
+        ru(on, 0, f32);                          // pre-smooth x2
         mg_residual(T, on.m, 0, curset(on, 0), f32);
         for (int l = 1; l <= maxD - 1; ++l) {  // levels below each scene's fused part
             Set inner, coarsest, both;
@@ -185,9 +183,7 @@ struct VCycle {
                 else if (D - 1 == l) coarsest.add(s), both.add(s);
             }
             if (both.empty()) continue;
-            mg_restrict(T, both.m, l);
-            mg_begin(T, both.m, l, curset(both, l));
-            mg_update(T, both.m, l, curset(both, l));
+            mg_restrict_bu(T, both.m, l, curset(both, l));
             ru(both, l, f32);
             ru(both, l, f32);
             if (!inner.empty()) mg_residual(T, inner.m, l, curset(inner, l), f32);  // pre-smooth x3

@@ -76,6 +76,8 @@ void pq_update(const Tables& T, const SceneSet& on);
 void xrs_update(const Tables& T, const SceneSet& on, int x);
 void store_velocity(const Tables& T, const SceneSet& on);
 void mg_p2g(const Tables& T, const SceneSet& on, int in);
+void mg_p2g_bu(const Tables& T, const SceneSet& on, int in, const SceneSet& cur);  // P2G, begin, update
+void mg_restrict_bu(const Tables& T, const SceneSet& on, int l, const SceneSet& cur);  // l-1 -> l, begin, update
 void mg_begin(const Tables& T, const SceneSet& on, int l, const SceneSet& cur);
 void mg_update(const Tables& T, const SceneSet& on, int l, const SceneSet& cur);
 void mg_residual(const Tables& T, const SceneSet& on, int l, const SceneSet& cur, bool f32);

@@ -133,6 +133,24 @@ __global__ void kb_mg_p2g(const BScene* tab, const int* off, int ns, SceneSet on
     if (row < L.nrow) mg_p2g_row(B.P, L, B.cellStart, B.vec[in], row);
 }
 
+__global__ void kb_mg_p2g_bu(const BScene* tab, const int* off, int ns, SceneSet on, int in, SceneSet cur) {
+    MPH_SCENE(kOffLevel)
+    const int row = lb * 256 + threadIdx.x;
+    const Level L = level(B, 0, cur, s);
+    if (row >= L.nrow) return;
+    mg_p2g_row(B.P, L, B.cellStart, B.vec[in], row);
+    mg_begin_update_row(L, row);
+}
+
+__global__ void kb_mg_restrict_bu(const BScene* tab, const int* off, int ns, SceneSet on, int l, SceneSet cur) {
+    MPH_SCENE(kOffLevel + l)
+    const int row = lb * 256 + threadIdx.x;
+    const Level L = level(B, l, cur, s);
+    if (row >= L.nrow) return;
+    mg_restrict_row(B.lv[l - 1], L, row);
+    mg_begin_update_row(L, row);
+}
+
 __global__ void kb_mg_begin(const BScene* tab, const int* off, int ns, SceneSet on, int l, SceneSet cur) {
     MPH_SCENE(kOffLevel + l)
     const int row = lb * 256 + threadIdx.x;
@@ -235,7 +253,7 @@ void axpy(const Tables& T, const SceneSet& on, int src, int dst) { MPH_LAUNCH(kb
 void spmv(const Tables& T, const SceneSet& on, const SceneSet& first, bool f32, double alpha, int x, double beta,
           int yin, int y, int y2, int p_copy, int dot, int variant) {
     // alpha is +1 or -1 at every call site (main.cpp:4337, 4353, 4357); variants as
-    // launch_spmv: FP64 2/3/4 block pairs per load group, FP32 1/2
+    // launch_spmv: FP64 2/3/4 block pairs per load group, FP32 1/2 block quads
 #define MPH_SPMV(NEG, VT, UN) MPH_LAUNCH((kb_spmv<NEG, VT, UN>), kOffChunk, first, x, beta, yin, y, y2, p_copy, dot)
     if (f32) {
         if (alpha < 0) { if (variant == 1) MPH_SPMV(true, float, 2); else MPH_SPMV(true, float, 1); }
@@ -267,6 +285,12 @@ void store_velocity(const Tables& T, const SceneSet& on) {
         hipLaunchKernelGGL(kb_store_velocity, dim3(g), dim3(256), 0, T.st, T.tab, T.off, T.ns, on);
 }
 void mg_p2g(const Tables& T, const SceneSet& on, int in) { MPH_LAUNCH(kb_mg_p2g, kOffLevel, in); }
+void mg_p2g_bu(const Tables& T, const SceneSet& on, int in, const SceneSet& cur) {
+    MPH_LAUNCH(kb_mg_p2g_bu, kOffLevel, in, cur);
+}
+void mg_restrict_bu(const Tables& T, const SceneSet& on, int l, const SceneSet& cur) {
+    MPH_LAUNCH(kb_mg_restrict_bu, kOffLevel + l, l, cur);
+}
 void mg_begin(const Tables& T, const SceneSet& on, int l, const SceneSet& cur) {
     MPH_LAUNCH(kb_mg_begin, kOffLevel + l, l, cur);
 }

@@ -206,7 +206,10 @@ __global__ void k_neighbor_count(DevParams P, const double* __restrict__ pos, co
     nCount[i] = n;
     nFluid[i] = nfl;
     nCountP[i] = npp;
-    if (i < P.nf) atomicMax(maxFluidP, npp);
+    if (i < P.nf) {
+        atomicMax(maxFluidP, npp);      // [0]: largest P-neighbour count of a fluid particle
+        atomicMax(maxFluidP + 1, nfl);  // [1]: largest fluid-neighbour count (A row length)
+    }
 }
 
 // fill pass, main.cpp:1567-1666.  For fluid particles it also records where the
@@ -321,7 +324,7 @@ __global__ void k_slice_width(int nf, int nslices, const int* __restrict__ nFlui
         const int i = s * kSlice + l;
         if (i < nf && nFluid[i] > w) w = nFluid[i];
     }
-    width[s] = (w + 1) & ~1;  // block pairs: every slice starts on an even slot
+    width[s] = (w + 3) & ~3;  // block pairs / quads: every slice starts on a multiple of 4
 }
 
 inline int nblk(long long n, int bs) { return (int)((n + bs - 1) / bs); }

@@ -81,6 +81,18 @@ __device__ inline void mg_update_row(const Level& L, int row) {  // myDdmv(1.0, 
     S[row] = y;
 }
 
+// mg_begin_row then mg_update_row on the same row (both touch that row only)
+__device__ inline void mg_begin_update_row(const Level& L, int row) {
+    double* S = L.S[L.cur];
+    double y = S[row];
+    y *= 0.0;
+    const double r = L.Q[row];
+    L.R[row] = r;
+    y *= 1.0;
+    y += 1.0 * L.invD[row] * r;
+    S[row] = y;
+}
+
 // (v mod n) for 0 <= v < 3n without a division
 __device__ inline int wrap3(int v, int n) {
     v -= v >= n ? n : 0;
@@ -257,9 +269,7 @@ __device__ inline void mg_coarse_cycle(const Level* lvs, int lc, int D, unsigned
     };
     auto begin_update = [&](int l) {
         const Level L = lev(l);
-        for (int row = threadIdx.x; row < L.nrow; row += blockDim.x) mg_begin_row(L, row);
-        __syncthreads();
-        for (int row = threadIdx.x; row < L.nrow; row += blockDim.x) mg_update_row(L, row);
+        for (int row = threadIdx.x; row < L.nrow; row += blockDim.x) mg_begin_update_row(L, row);
         __syncthreads();
     };
     auto prolong_from = [&](int l) {  // l -> l-1
@@ -345,56 +355,102 @@ __device__ inline void spmv_chunk(int nf, const int* __restrict__ nFluid, const 
                                   const double* __restrict__ x, double beta, const double* yin, double* y,
                                   double* y2, double* p_copy, const double* __restrict__ dotx,
                                   double* __restrict__ part, int chunk, double* sh) {
-    typedef VT vt2 __attribute__((ext_vector_type(2)));
     typedef int it2 __attribute__((ext_vector_type(2)));
     const int i = chunk * 256 + threadIdx.x;
     double t = 0.0;
     if (i < nf) {
         const int w = nFluid[i];
         const int lane = i % kSlice;
-        const long long sp = start[i / kSlice] / 2;  // first block pair of the slice
+        const long long sb = start[i / kSlice];
         // the matrix is read once per launch: streaming (nontemporal) loads of whole
-        // block pairs (a_col_index / a_val_index), kUn pairs per group with all their
-        // loads issued before the first use
-        const it2* cp = (const it2*)col + sp * kSlice + lane;
-        const vt2* vp = (const vt2*)val + sp * 9 * kSlice + lane;
+        // block pairs (FP64, a_val_index) or quads (FP32, a_val32_index), kUn of them
+        // per group with all their loads issued before the first use; columns are
+        // pairs (a_col_index)
+        const it2* cp = (const it2*)col + (sb / 2) * kSlice + lane;
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        auto block = [&](int j, const vt2* a, int h) {
+        auto block = [&](int j, const double* c9) {
             const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
             double c[9];
 #pragma unroll
-            for (int e = 0; e < 9; ++e) c[e] = kNeg ? -(double)a[e][h] : (double)a[e][h];
+            for (int e = 0; e < 9; ++e) c[e] = kNeg ? -c9[e] : c9[e];
             s0 += c[0] * x0; s0 += c[1] * x1; s0 += c[2] * x2;
             s1 += c[3] * x0; s1 += c[4] * x1; s1 += c[5] * x2;
             s2 += c[6] * x0; s2 += c[7] * x1; s2 += c[8] * x2;
         };
-        const int wp = w / 2;
-        int m = 0;
-        for (; m + kUn <= wp; m += kUn) {
-            it2 j[kUn];
-            vt2 a[kUn][9];
+        if constexpr (sizeof(VT) == 8) {
+            typedef VT vt2 __attribute__((ext_vector_type(2)));
+            const vt2* vp = (const vt2*)val + (sb / 2) * 9 * kSlice + lane;
+            auto pair = [&](const it2 j, const vt2* a, int h) {  // blocks 2m, 2m+1 (h of them)
+                for (int u = 0; u < h; ++u) {
+                    double c9[9];
 #pragma unroll
-            for (int u = 0; u < kUn; ++u) j[u] = __builtin_nontemporal_load(cp + (long long)(m + u) * kSlice);
+                    for (int e = 0; e < 9; ++e) c9[e] = (double)a[e][u];
+                    block(j[u], c9);
+                }
+            };
+            const int wp = w / 2;
+            int m = 0;
+            for (; m + kUn <= wp; m += kUn) {
+                it2 j[kUn];
+                vt2 a[kUn][9];
 #pragma unroll
-            for (int u = 0; u < kUn; ++u)
+                for (int u = 0; u < kUn; ++u) j[u] = __builtin_nontemporal_load(cp + (long long)(m + u) * kSlice);
 #pragma unroll
-                for (int e = 0; e < 9; ++e)
-                    a[u][e] = __builtin_nontemporal_load(vp + ((long long)(m + u) * 9 + e) * kSlice);
+                for (int u = 0; u < kUn; ++u)
 #pragma unroll
-            for (int u = 0; u < kUn; ++u) {
-                block(j[u][0], a[u], 0);
-                block(j[u][1], a[u], 1);
+                    for (int e = 0; e < 9; ++e)
+                        a[u][e] = __builtin_nontemporal_load(vp + ((long long)(m + u) * 9 + e) * kSlice);
+#pragma unroll
+                for (int u = 0; u < kUn; ++u) pair(j[u], a[u], 2);
             }
-        }
-        for (; m <= wp; ++m) {  // the last full pairs, then the odd last block
-            const int h = (m < wp) ? 2 : (w & 1);
-            if (h == 0) break;
-            const it2 j = __builtin_nontemporal_load(cp + (long long)m * kSlice);
-            vt2 a[9];
+            for (; m <= wp; ++m) {  // the last full pairs, then the odd last block
+                const int h = (m < wp) ? 2 : (w & 1);
+                if (h == 0) break;
+                const it2 j = __builtin_nontemporal_load(cp + (long long)m * kSlice);
+                vt2 a[9];
 #pragma unroll
-            for (int e = 0; e < 9; ++e) a[e] = __builtin_nontemporal_load(vp + ((long long)m * 9 + e) * kSlice);
-            block(j[0], a, 0);
-            if (h == 2) block(j[1], a, 1);
+                for (int e = 0; e < 9; ++e) a[e] = __builtin_nontemporal_load(vp + ((long long)m * 9 + e) * kSlice);
+                pair(j, a, h);
+            }
+        } else {
+            typedef VT vt4 __attribute__((ext_vector_type(4)));
+            const vt4* vp = (const vt4*)val + (sb / 4) * 9 * kSlice + lane;
+            auto quad = [&](const it2 j0, const it2 j1, const vt4* a, int h) {  // blocks 4q..4q+h-1
+                for (int u = 0; u < h; ++u) {
+                    double c9[9];
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) c9[e] = (double)a[e][u];
+                    block(u < 2 ? j0[u] : j1[u - 2], c9);
+                }
+            };
+            const int wq = w / 4;
+            int q = 0;
+            for (; q + kUn <= wq; q += kUn) {
+                it2 j[kUn][2];
+                vt4 a[kUn][9];
+#pragma unroll
+                for (int u = 0; u < kUn; ++u) {
+                    j[u][0] = __builtin_nontemporal_load(cp + (long long)(2 * (q + u)) * kSlice);
+                    j[u][1] = __builtin_nontemporal_load(cp + (long long)(2 * (q + u) + 1) * kSlice);
+                }
+#pragma unroll
+                for (int u = 0; u < kUn; ++u)
+#pragma unroll
+                    for (int e = 0; e < 9; ++e)
+                        a[u][e] = __builtin_nontemporal_load(vp + ((long long)(q + u) * 9 + e) * kSlice);
+#pragma unroll
+                for (int u = 0; u < kUn; ++u) quad(j[u][0], j[u][1], a[u], 4);
+            }
+            for (; q <= wq; ++q) {  // the last full quads, then the partial last one
+                const int h = (q < wq) ? 4 : (w & 3);
+                if (h == 0) break;
+                const it2 j0 = __builtin_nontemporal_load(cp + (long long)(2 * q) * kSlice);
+                const it2 j1 = __builtin_nontemporal_load(cp + (long long)(2 * q + 1) * kSlice);
+                vt4 a[9];
+#pragma unroll
+                for (int e = 0; e < 9; ++e) a[e] = __builtin_nontemporal_load(vp + ((long long)q * 9 + e) * kSlice);
+                quad(j0, j1, a, h);
+            }
         }
         const double sm[3] = {s0, s1, s2};
         double out[3];

@@ -8,12 +8,13 @@
 //   neighbours   int64 ptr[np+1], int ind[E]; same for the RadiusP lists
 //   A            sliced 3x3-block ELL: slice = 64 consecutive fluid particles,
 //                width = max fluid-neighbour count in the slice rounded up to
-//                even, sliceStart = prefix sum of widths (always even).  Blocks
+//                a multiple of 4, sliceStart = prefix sum of widths.  Blocks
 //                k = 2m, 2m+1 of a particle form a pair: its two columns are
 //                adjacent ints and entry e = 3r+s of both blocks are adjacent
 //                values (a_col_index / a_val_index), so a wavefront reads 512 B
 //                of columns and 1 KiB (FP64) per entry of a block pair
-//                with one 8/16-byte load per lane
+//                with one 8/16-byte load per lane; the FP32 copy groups four
+//                blocks per entry (a_val32_index)
 //   C            cval[3*(ptrP[i] + jn) + r] for fluid i, P-neighbour jn
 //   multigrid    per level: A stored [shift][row], vectors [row]
 //   CR vectors   double[3*nf], row 3i+r as the reference
@@ -60,6 +61,11 @@ __host__ __device__ inline long long a_col_index(long long sb, int k, int lane) 
 }
 __host__ __device__ inline long long a_val_index(long long sb, int k, int e, int lane) {
     return (((sb / 2 + k / 2) * 9 + e) * 64 + lane) * 2 + (k & 1);
+}
+// the FP32 copy of A (mixed precision) groups blocks 4q..4q+3 per entry instead, so
+// its loads are 16 bytes per lane as well
+__host__ __device__ inline long long a_val32_index(long long sb, int k, int e, int lane) {
+    return (((sb / 4 + k / 4) * 9 + e) * 64 + lane) * 4 + (k & 3);
 }
 constexpr int kChunk = 256;   // particles per dot-product chunk / CR workgroup
 constexpr int kRuns = 28;     // run starts per fluid particle: 27 neighbour cells + end
@@ -108,21 +114,27 @@ void launch_vector_p(hipStream_t st, const DevParams& P, const int* type, const 
                      const long long* nPtrP, const int* nIndP, double* vecP);
 void launch_b_minus_cp(hipStream_t st, const DevParams& P, const int* nCountP, const long long* nPtrP,
                        const int* nIndP, const double* cval, const double* vecP, double* b);
-// variant 0: P lists from global memory; 1: the slice's index lists in LDS (needs
-// every fluid particle's P-neighbour count <= kMaxP).  Same bits either way.
-constexpr int kCacVariants = 2;
+// variant 0: P lists from global memory; 1: the slice's index lists in LDS; 2: and
+// each neighbour's list copied to LDS before its merge (1 and 2 need every fluid
+// particle's P-neighbour count <= kMaxP).  Same bits in every variant.
+constexpr int kCacVariants = 3;
 void launch_cac(hipStream_t st, const DevParams& P, const int* nFluid, const int* nCountP, const long long* nPtrP,
                 const int* nIndP, const double* cval, const double* dval, ASlices A, int variant);  // also writes A.val32 when it is non-null
 // ---- multigrid ----
 void launch_fluid_bbox(hipStream_t st, const DevParams& P, const int* cellStart, int* bbox);
 void launch_inv_diag(hipStream_t st, const DevParams& P, const int* selfIdx, ASlices A, double* invDiagA);
+// variant 0: per-thread sums straight from A; 1: each particle's row staged in LDS
+// first (rows of at most 256 blocks).  Same bits.
+constexpr int kGalVariants = 2, kGalRowMax = 256;
 void launch_mg_galerkin0(hipStream_t st, const DevParams& P, const Level& L, const int* cellStart,
-                         const unsigned short* runStart, ASlices A);
+                         const unsigned short* runStart, const int* nFluid, ASlices A, int variant);
 void launch_mg_galerkinL(hipStream_t st, const Level& S, const Level& L);
 void launch_mg_invd(hipStream_t st, const Level& L);
 void launch_fill(hipStream_t st, double* x, long long n, double v);
 void launch_fill_i32(hipStream_t st, int* x, long long n, int v);  // a kernel, not a memset node
 void launch_mg_p2g(hipStream_t st, const DevParams& P, Level& L, const int* cellStart, const double* q);
+void launch_mg_p2g_bu(hipStream_t st, const DevParams& P, Level& L, const int* cellStart, const double* q);
+void launch_mg_restrict_bu(hipStream_t st, const Level& S, Level& L);  // restriction, begin, update
 void launch_mg_begin(hipStream_t st, Level& L);        // S *= 0; R = Q
 void launch_mg_update(hipStream_t st, Level& L);       // S += invD R
 void launch_mg_residual(hipStream_t st, Level& L);     // R = Q - A S
@@ -139,7 +151,7 @@ void launch_mg_g2p(hipStream_t st, const DevParams& P, const Level& L0, const in
 void launch_cr_init(hipStream_t st, const DevParams& P, const double* vel, const double* force,
                     const double* mass, double* x);
 // SpMV variants differ only in how many block pairs are loaded per group (same
-// bits): FP64 2, 3 or 4 pairs; FP32 1 or 2 pairs
+// bits): FP64 2, 3 or 4 block pairs; FP32 1 or 2 block quads
 constexpr int kSpmvVariants64 = 3, kSpmvVariants32 = 2;
 void launch_spmv(hipStream_t st, const DevParams& P, ASlices A, const int* nFluid, double alpha, const double* x,
                  double beta, const double* yin, double* y, double* y2, double* p_copy, const double* dotx,

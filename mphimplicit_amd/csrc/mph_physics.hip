@@ -366,11 +366,13 @@ __global__ void k_b_minus_cp(DevParams P, const int* __restrict__ nCountP, const
 // fluid neighbours j, merge the sorted P-neighbour lists of i and j and sum
 // D_ik (x) C_jk over the shared k (D = C*Lambda, left to right as the reference),
 // then add the 3x3 sum to block (i, j).  Workgroup = one slice: lane = particle,
-// wave w takes the blocks k = w, w+4, ...  kLds: the slice's own index lists are
-// staged in LDS (odd row stride, no bank conflicts); otherwise they are read from
-// global memory.  Both give the same bits; the faster one is picked at run time.
+// wave w takes the blocks k = w, w+4, ...  kMode >= 1: the slice's own index lists
+// are staged in LDS (odd row stride, no bank conflicts); otherwise they are read from
+// global memory; with kMode 2 each thread also copies j's index list into LDS
+// before merging (its loads all in flight at once instead of one per merge step).
+// All give the same bits; the fastest is picked at run time.
 // The FP32 copy of the finished block is written when aval32 is non-null.
-template <bool kLds>
+template <int kMode>
 __global__ void __launch_bounds__(256) k_cac(DevParams P, const int* __restrict__ nFluid,
                                              const int* __restrict__ nCountP, const long long* __restrict__ nPtrP,
                                              const int* __restrict__ nIndP, const double* __restrict__ cval,
@@ -378,10 +380,12 @@ __global__ void __launch_bounds__(256) k_cac(DevParams P, const int* __restrict_
                                              const int* __restrict__ acol, double* __restrict__ aval,
                                              float* __restrict__ aval32) {
     constexpr int kStride = kMaxP + 1;
-    __shared__ int sk[kLds ? kSlice * kStride : 1];
+    constexpr bool kLdsI = kMode >= 1, kLdsJ = kMode >= 2;
+    __shared__ int sk[kLdsI ? kSlice * kStride : 1];
+    __shared__ int sj[kLdsJ ? 256 * kStride : 1];
     const int slice = blockIdx.x;
     const int base = slice * kSlice;
-    if (kLds) {
+    if (kLdsI) {
         for (int t = threadIdx.x; t < kSlice * kMaxP; t += blockDim.x) {
             const int l = t / kMaxP, a = t % kMaxP, i = base + l;
             if (i < P.nf && a < nCountP[i]) sk[l * kStride + a] = nIndP[nPtrP[i] + a];
@@ -394,17 +398,23 @@ __global__ void __launch_bounds__(256) k_cac(DevParams P, const int* __restrict_
     const int nfl = nFluid[i];
     const long long pi = nPtrP[i];
     const int ni = nCountP[i];
-    const int* ki = kLds ? sk + lane * kStride : nIndP + pi;
+    const int* ki = kLdsI ? sk + lane * kStride : nIndP + pi;
+    int* kjs = sj + threadIdx.x * kStride;  // this thread's copy of j's list (kLdsJ)
     const double* di = dval + 3 * pi;
     const long long sb = start[slice];
     for (int k = wave; k < nfl; k += 4) {
         const int j = acol[a_col_index(sb, k, lane)];
         const long long pj = nPtrP[j];
         const int nj = nCountP[j];
+        const int* kj = nIndP + pj;
+        if (kLdsJ) {  // all loads of j's list in flight at once, then merge from LDS
+            for (int t = 0; t < nj; ++t) kjs[t] = kj[t];
+            kj = kjs;
+        }
         double sum[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         int a = 0, c = 0;
         int ka = ni > 0 ? ki[0] : 0;
-        int kc = nj > 0 ? nIndP[pj] : 0;
+        int kc = nj > 0 ? kj[0] : 0;
         while (a < ni && c < nj) {
             if (ka == kc) {
                 const double* cj = &cval[3 * (pj + c)];
@@ -416,23 +426,24 @@ __global__ void __launch_bounds__(256) k_cac(DevParams P, const int* __restrict_
                 a++;
                 c++;
                 if (a < ni) ka = ki[a];
-                if (c < nj) kc = nIndP[pj + c];
+                if (c < nj) kc = kj[c];
             } else if (ka < kc) {
                 a++;
                 if (a < ni) ka = ki[a];
             } else {
                 c++;
-                if (c < nj) kc = nIndP[pj + c];
+                if (c < nj) kc = kj[c];
             }
         }
         for (int e9 = 0; e9 < 9; ++e9) {
             const long long at = a_val_index(sb, k, e9, lane);
             const double v = aval[at] + sum[e9];
             aval[at] = v;
-            if (aval32) aval32[at] = (float)v;
+            if (aval32) aval32[a_val32_index(sb, k, e9, lane)] = (float)v;
         }
     }
 }
+
 
 }  // namespace
 
@@ -490,12 +501,13 @@ void launch_b_minus_cp(hipStream_t st, const DevParams& P, const int* nCountP, c
 void launch_cac(hipStream_t st, const DevParams& P, const int* nFluid, const int* nCountP, const long long* nPtrP,
                 const int* nIndP, const double* cval, const double* dval, ASlices A, int variant) {
     if (A.nslices == 0) return;
-    if (variant == 1)
-        hipLaunchKernelGGL(k_cac<true>, dim3(A.nslices), dim3(256), 0, st, P, nFluid, nCountP, nPtrP, nIndP, cval,
-                           dval, A.start, A.col, A.val, A.val32);
-    else
-        hipLaunchKernelGGL(k_cac<false>, dim3(A.nslices), dim3(256), 0, st, P, nFluid, nCountP, nPtrP, nIndP, cval,
-                           dval, A.start, A.col, A.val, A.val32);
+#define MPH_CAC(M)                                                                                            \
+    hipLaunchKernelGGL(k_cac<M>, dim3(A.nslices), dim3(256), 0, st, P, nFluid, nCountP, nPtrP, nIndP, cval, dval, \
+                       A.start, A.col, A.val, A.val32)
+    if (variant == 2) MPH_CAC(2);
+    else if (variant == 1) MPH_CAC(1);
+    else MPH_CAC(0);
+#undef MPH_CAC
 }
 
 }  // namespace mph

@@ -212,7 +212,7 @@ int phase_cells(mph_scene& S) {
 
 int phase_neighbor_launch(mph_scene& S) {
     const DevParams& P = S.P;
-    launch_fill_i32(S.st, S.maxP.p, 1, 0);
+    launch_fill_i32(S.st, S.maxP.p, 2, 0);
     launch_neighbor_count(S.st, P, S.pos.p, S.cellIndex.p, S.cellStart.p, S.nCount.p, S.nFluid.p, S.nCountP.p,
                           S.maxP.p);
     launch_scan_i64(S.st, S.nCount.p, S.nPtr.p, P.np, S.scanScratch.p, (int)S.scanScratch.cap);
@@ -224,7 +224,7 @@ int phase_neighbor_launch(mph_scene& S) {
     HIPCHK(hipMemcpyAsync(&h[1], S.nPtrP.p + P.np, 8, hipMemcpyDeviceToHost, S.st));
     HIPCHK(hipMemcpyAsync(&h[2], S.sliceStart.p + S.nslices, 8, hipMemcpyDeviceToHost, S.st));
     HIPCHK(hipMemcpyAsync(&h[3], S.bbox.p, 6 * sizeof(int), hipMemcpyDeviceToHost, S.st));
-    HIPCHK(hipMemcpyAsync(&h[6], S.maxP.p, sizeof(int), hipMemcpyDeviceToHost, S.st));
+    HIPCHK(hipMemcpyAsync(&h[6], S.maxP.p, 2 * sizeof(int), hipMemcpyDeviceToHost, S.st));
     return MPH_OK;
 }
 
@@ -232,7 +232,8 @@ int phase_neighbor_launch(mph_scene& S) {
 int phase_neighbor_finish(mph_scene& S) {
     const DevParams& P = S.P;
     const long long* h = (const long long*)S.hpin;
-    S.hmaxP = *(const int*)&h[6];
+    S.hmaxP = ((const int*)&h[6])[0];
+    S.hmaxW = ((const int*)&h[6])[1];
     S.E = h[0];
     S.EP = h[1];
     S.slots = h[2] * kSlice;
@@ -286,7 +287,7 @@ int phase_multiply_c(mph_scene& S) {
     // C Lambda C^T also writes the FP32 copy of A in mixed precision (aval32)
     int v = S.tuneCac.pick();
     const bool lds_ok = S.hmaxP <= kMaxP;
-    if (v == 1 && !lds_ok) {
+    if (v >= 1 && !lds_ok) {
         launch_cac(S.st, S.P, S.nFluid.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.dval.p, aslices(S), 0);
     } else {
         S.tuneCac.begin(S.st, v);
@@ -357,7 +358,17 @@ int phase_multigrid(mph_scene& S) {
     if ((rc = S.lvDev.ensure(kMaxLevels))) return rc;
     HIPCHK(hipMemcpyAsync(S.lvDev.p, S.hlv.data(), depth * sizeof(Level), hipMemcpyHostToDevice, S.st));
     launch_inv_diag(S.st, S.P, S.selfIdx.p, aslices(S), S.invDiagA.p);
-    launch_mg_galerkin0(S.st, S.P, S.lv[0].L, S.cellStart.p, S.runStart.p, aslices(S));
+    {
+        // the LDS variant holds one particle's row: its slice width must fit
+        int v = S.tuneGal.pick();
+        if (v == 1 && S.hmaxW > kGalRowMax) {
+            launch_mg_galerkin0(S.st, S.P, S.lv[0].L, S.cellStart.p, S.runStart.p, S.nFluid.p, aslices(S), 0);
+        } else {
+            S.tuneGal.begin(S.st, v);
+            launch_mg_galerkin0(S.st, S.P, S.lv[0].L, S.cellStart.p, S.runStart.p, S.nFluid.p, aslices(S), v);
+            S.tuneGal.end(S.st);
+        }
+    }
     for (int l = 1; l < depth; ++l) launch_mg_galerkinL(S.st, S.lv[l - 1].L, S.lv[l].L);
     for (int l = 0; l < depth; ++l) launch_mg_invd(S.st, S.lv[l].L);
     if (mixed(S))
@@ -374,16 +385,12 @@ void precondition(mph_scene& S, const double* in, double* out, double* part) {
     const int D = S.depth, F = S.fuseOf[v];  // levels F..D-1 run fused (F = D: none)
     S.tuneFuse.begin(st, v);
     Level& L0 = S.lv[0].L;
-    launch_mg_p2g(st, S.P, L0, S.cellStart.p, in);
-    launch_mg_begin(st, L0);
-    launch_mg_update(st, L0);  // pre-smooth x2: {S += invD R; R = Q - A S}
-    launch_mg_residual_update(st, L0);
+    launch_mg_p2g_bu(st, S.P, L0, S.cellStart.p, in);  // P2G, S = 0, R = Q, S += invD R
+    launch_mg_residual_update(st, L0);                  // pre-smooth x2
     launch_mg_residual(st, L0);
     for (int l = 1; l < D - 1 && l < F; ++l) {
         Level& L = S.lv[l].L;
-        launch_mg_restrict(st, S.lv[l - 1].L, L);
-        launch_mg_begin(st, L);
-        launch_mg_update(st, L);  // pre-smooth x3
+        launch_mg_restrict_bu(st, S.lv[l - 1].L, L);  // pre-smooth x3
         launch_mg_residual_update(st, L);
         launch_mg_residual_update(st, L);
         launch_mg_residual(st, L);
@@ -395,9 +402,7 @@ void precondition(mph_scene& S, const double* in, double* out, double* part) {
         for (int l = F; l < D - 1; ++l) S.lv[l].L.cur ^= 1;  // five sweeps each; the coarsest four
     } else {
         Level& L = S.lv[D - 1].L;
-        launch_mg_restrict(st, S.lv[D - 2].L, L);
-        launch_mg_begin(st, L);
-        launch_mg_update(st, L);  // x4, then one more S += invD R
+        launch_mg_restrict_bu(st, S.lv[D - 2].L, L);  // S = invD R, then four sweeps
         for (int k = 0; k < 4; ++k) launch_mg_residual_update(st, L);
         launch_mg_prolong(st, L, S.lv[D - 2].L);
     }
@@ -427,6 +432,7 @@ void collect_tuning(mph_scene& S) {  // the stream is idle
     S.tuneSpmv32.collect();
     S.tuneCac.collect();
     S.tuneFuse.collect();
+    S.tuneGal.collect();
 }
 
 int read_scalar(mph_scene& S, int slot, double* v) {
@@ -740,6 +746,7 @@ void destroy(mph_scene* S) {
     S->tuneSpmv32.release();
     S->tuneCac.release();
     S->tuneFuse.release();
+    S->tuneGal.release();
     if (S->hpin) (void)hipHostFree(S->hpin);
     for (hipEvent_t& e : S->ev)
         if (e) (void)hipEventDestroy(e);
@@ -994,6 +1001,7 @@ int mph_scene_create(const mph_params* p, const mph_grid* g, int device, mph_sce
     S->tuneSpmv32.init(kSpmvVariants32, 3);
     S->tuneCac.init(kCacVariants, 1);
     S->tuneFuse.init(kFuseVariants, 3);
+    S->tuneGal.init(kGalVariants, 1);
     S->outputNext = S->time;
     S->vtkOutputNext = S->time;
     *out = S;
@@ -1144,6 +1152,7 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
     else if (n == "TunedSpmv32") *v = S->tuneSpmv32.best;
     else if (n == "TunedCac") *v = S->tuneCac.best;
     else if (n == "TunedFuse") *v = S->tuneFuse.best;
+    else if (n == "TunedGalerkin") *v = S->tuneGal.best;
     else if (n == "SpmvLaunchesF32") *v = (double)S->spmvLaunches32;
     else if (n == "SpmvTimingError") *v = S->spmvTimingErr;
     else if (n == "ABlocks" || n == "CBlocks") {

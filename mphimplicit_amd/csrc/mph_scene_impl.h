@@ -159,6 +159,7 @@ struct mph_scene {
     mph::host::DBuf<unsigned short> runStart;
     long long E = 0, EP = 0;
     int hmaxP = 0;
+    int hmaxW = 0;  // largest fluid-neighbour count (A row length in blocks)
     // A and C
     int nslices = 0;
     mph::host::DBuf<int> width, selfIdx, acol;
@@ -194,7 +195,7 @@ struct mph_scene {
     int spmvTimingErr = 0;
     // run-time variant choice (SpMV load groups per precision, C Lambda C^T schedule,
     // V-cycle levels fused into one workgroup)
-    mph::host::Autotune tuneSpmv64, tuneSpmv32, tuneCac, tuneFuse;
+    mph::host::Autotune tuneSpmv64, tuneSpmv32, tuneCac, tuneFuse, tuneGal;
 };
 
 

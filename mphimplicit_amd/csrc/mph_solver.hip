@@ -59,6 +59,38 @@ __global__ void k_inv_diag(DevParams P, const int* __restrict__ selfIdx, const l
 // workgroup.  The blocks of a particle that fall in one neighbour cell form a
 // contiguous run of its sorted fluid-neighbour list; the neighbour search recorded
 // where each run starts.
+// which of the visited neighbour cells (ascending CellId, as the neighbour search)
+// of cell g map to stencil cell c of level 0: bit q = run q of every particle's list
+__device__ inline unsigned galerkin_mask(const DevParams& P, const Level& L, const int* __restrict__ cellStart,
+                                         int g, int c) {
+    int gx, gy, gz;
+    grid_xyz(L, g, gx, gy, gz);
+    const int cx = gx + L.mn[0], cy = gy + L.mn[1], cz = gz + L.mn[2];
+    unsigned mask = 0;
+    const int* C = P.C;
+    const int rx = P.range[0], ry = P.range[1], rz = P.range[2];
+    int jx[3], jy[3], jz[3];
+    for (int k = 0; k < 2 * rx + 1; ++k) jx[k] = ((cx - rx + k) % C[0] + C[0]) % C[0];
+    for (int k = 0; k < 2 * ry + 1; ++k) jy[k] = ((cy - ry + k) % C[1] + C[1]) % C[1];
+    for (int k = 0; k < 2 * rz + 1; ++k) jz[k] = ((cz - rz + k) % C[2] + C[2]) % C[2];
+    const int bx = (2 * rx) - (cx + rx) % C[0], by = (2 * ry) - (cy + ry) % C[1], bz = (2 * rz) - (cz + rz) % C[2];
+    const int x0 = bx > 0 ? bx : 0, y0 = by > 0 ? by : 0, z0 = bz > 0 ? bz : 0;
+    int q = 0;
+    for (int a = x0; a < x0 + (2 * rx + 1); ++a)
+        for (int b = y0; b < y0 + (2 * ry + 1); ++b)
+            for (int cc = z0; cc < z0 + (2 * rz + 1); ++cc, ++q) {
+                const int ax = jx[a % (2 * rx + 1)], ay = jy[b % (2 * ry + 1)], az = jz[cc % (2 * rz + 1)];
+                const int jc = cell_id(P, ax, ay, az);
+                if (cellStart[jc] == cellStart[jc + 1]) continue;  // no fluid there
+                const int dx = (ax - L.mn[0]) - gx, dy = (ay - L.mn[1]) - gy, dz = (az - L.mn[2]) - gz;
+                const int ix = (dx + 1 + L.cnt[0]) % L.cnt[0];
+                const int iy = (dy + 1 + L.cnt[1]) % L.cnt[1];
+                const int iz = (dz + 1 + L.cnt[2]) % L.cnt[2];
+                if (L.cs[2] * L.cs[1] * ix + L.cs[2] * iy + iz == c) mask |= 1u << q;
+            }
+    return mask;
+}
+
 __global__ void __launch_bounds__(256) k_mg_galerkin0(DevParams P, Level L, const int* __restrict__ cellStart,
                                                       const unsigned short* __restrict__ runStart,
                                                       const long long* __restrict__ start,
@@ -69,34 +101,9 @@ __global__ void __launch_bounds__(256) k_mg_galerkin0(DevParams P, Level L, cons
     const int c = threadIdx.x / 9, e = threadIdx.x % 9;
     int gx, gy, gz;
     grid_xyz(L, g, gx, gy, gz);
-    const int cx = gx + L.mn[0], cy = gy + L.mn[1], cz = gz + L.mn[2];
-    const int ic = cell_id(P, cx, cy, cz);
+    const int ic = cell_id(P, gx + L.mn[0], gy + L.mn[1], gz + L.mn[2]);
     const int p0 = cellStart[ic], p1 = cellStart[ic + 1];
-    // visiting order of the neighbour search (ascending CellId); which runs map to c
-    unsigned mask = 0;
-    {
-        const int* C = P.C;
-        const int rx = P.range[0], ry = P.range[1], rz = P.range[2];
-        int jx[3], jy[3], jz[3];
-        for (int k = 0; k < 2 * rx + 1; ++k) jx[k] = ((cx - rx + k) % C[0] + C[0]) % C[0];
-        for (int k = 0; k < 2 * ry + 1; ++k) jy[k] = ((cy - ry + k) % C[1] + C[1]) % C[1];
-        for (int k = 0; k < 2 * rz + 1; ++k) jz[k] = ((cz - rz + k) % C[2] + C[2]) % C[2];
-        const int bx = (2 * rx) - (cx + rx) % C[0], by = (2 * ry) - (cy + ry) % C[1], bz = (2 * rz) - (cz + rz) % C[2];
-        const int x0 = bx > 0 ? bx : 0, y0 = by > 0 ? by : 0, z0 = bz > 0 ? bz : 0;
-        int q = 0;
-        for (int a = x0; a < x0 + (2 * rx + 1); ++a)
-            for (int b = y0; b < y0 + (2 * ry + 1); ++b)
-                for (int cc = z0; cc < z0 + (2 * rz + 1); ++cc, ++q) {
-                    const int ax = jx[a % (2 * rx + 1)], ay = jy[b % (2 * ry + 1)], az = jz[cc % (2 * rz + 1)];
-                    const int jc = cell_id(P, ax, ay, az);
-                    if (cellStart[jc] == cellStart[jc + 1]) continue;  // no fluid there
-                    const int dx = (ax - L.mn[0]) - gx, dy = (ay - L.mn[1]) - gy, dz = (az - L.mn[2]) - gz;
-                    const int ix = (dx + 1 + L.cnt[0]) % L.cnt[0];
-                    const int iy = (dy + 1 + L.cnt[1]) % L.cnt[1];
-                    const int iz = (dz + 1 + L.cnt[2]) % L.cnt[2];
-                    if (L.cs[2] * L.cs[1] * ix + L.cs[2] * iy + iz == c) mask |= 1u << q;
-                }
-    }
+    const unsigned mask = galerkin_mask(P, L, cellStart, g, c);
     double acc = 0.0;
     if (mask)
         for (int ip = p0; ip < p1; ++ip) {
@@ -106,11 +113,67 @@ __global__ void __launch_bounds__(256) k_mg_galerkin0(DevParams P, Level L, cons
             for (unsigned m = mask; m; m &= m - 1) {
                 const int q = __builtin_ctz(m);
                 const int k1 = rs[q + 1];
-                for (int k = rs[q]; k < k1; ++k) acc += aval[a_val_index(sb, k, e, lane)];
+                for (int k = rs[q]; k < k1; k += 8) {  // eight loads in flight, added in order
+                    double v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) v[u] = k + u < k1 ? aval[a_val_index(sb, k + u, e, lane)] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        if (k + u < k1) acc += v[u];
+                }
             }
         }
     const int r = e / 3, sd = e % 3;
     L.A[(long long)(3 * c + sd) * L.nrow + 3 * g + r] = acc;
+}
+
+// the same sums with each particle's row of blocks staged in LDS first: the
+// workgroup loads the row with 16-byte loads (block pairs), whose cache lines also
+// hold the next three particles' rows, then every (c, e) thread adds its runs from
+// LDS.  Rows longer than kGalRow blocks are not supported (the caller checks).
+constexpr int kGalRow = 256;
+__global__ void __launch_bounds__(256) k_mg_galerkin0_lds(DevParams P, Level L, const int* __restrict__ cellStart,
+                                                          const unsigned short* __restrict__ runStart,
+                                                          const int* __restrict__ nFluid,
+                                                          const long long* __restrict__ start,
+                                                          const double* __restrict__ aval) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    __shared__ double row[kGalRow * 9];
+    const int ncs = L.cs[0] * L.cs[1] * L.cs[2];
+    const int g = blockIdx.x;
+    const bool active = (int)threadIdx.x < 9 * ncs;
+    const int c = threadIdx.x / 9, e = threadIdx.x % 9;
+    int gx, gy, gz;
+    grid_xyz(L, g, gx, gy, gz);
+    const int ic = cell_id(P, gx + L.mn[0], gy + L.mn[1], gz + L.mn[2]);
+    const int p0 = cellStart[ic], p1 = cellStart[ic + 1];
+    const unsigned mask = active ? galerkin_mask(P, L, cellStart, g, c) : 0u;
+    double acc = 0.0;
+    for (int ip = p0; ip < p1; ++ip) {
+        const int lane = ip % kSlice;
+        const long long sb = start[ip / kSlice];
+        const int npair = (nFluid[ip] + 1) / 2;
+        __syncthreads();  // the previous row has been used
+        for (int t = threadIdx.x; t < npair * 9; t += blockDim.x) {
+            const int m = t / 9, ee = t % 9;
+            const d2 v = *(const d2*)&aval[a_val_index(sb, 2 * m, ee, lane)];
+            row[(2 * m) * 9 + ee] = v.x;
+            row[(2 * m + 1) * 9 + ee] = v.y;
+        }
+        __syncthreads();
+        if (mask) {
+            const unsigned short* rs = runStart + (long long)ip * kRuns;
+            for (unsigned m = mask; m; m &= m - 1) {
+                const int q = __builtin_ctz(m);
+                const int k1 = rs[q + 1];
+                for (int k = rs[q]; k < k1; ++k) acc += row[k * 9 + e];
+            }
+        }
+    }
+    if (active) {
+        const int r = e / 3, sd = e % 3;
+        L.A[(long long)(3 * c + sd) * L.nrow + 3 * g + r] = acc;
+    }
 }
 
 // coarse level = R A_fine P by summation (3414-3507): thread = (coarse cell, stencil
@@ -183,6 +246,22 @@ __global__ void k_fill_i32(int* x, long long n, int v) {
 __global__ void k_mg_p2g(DevParams P, Level L, const int* __restrict__ cellStart, const double* __restrict__ q) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row < L.nrow) mg_p2g_row(P, L, cellStart, q, row);
+}
+
+// P2G, then begin + update of level 0 on the same row
+__global__ void k_mg_p2g_bu(DevParams P, Level L, const int* __restrict__ cellStart, const double* __restrict__ q) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= L.nrow) return;
+    mg_p2g_row(P, L, cellStart, q, row);
+    mg_begin_update_row(L, row);
+}
+
+// restriction into L, then begin + update of L on the same row
+__global__ void k_mg_restrict_bu(Level S, Level L) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= L.nrow) return;
+    mg_restrict_row(S, L, row);
+    mg_begin_update_row(L, row);
 }
 
 __global__ void k_mg_begin(Level L) {
@@ -445,9 +524,13 @@ void launch_inv_diag(hipStream_t st, const DevParams& P, const int* selfIdx, ASl
     hipLaunchKernelGGL(k_inv_diag, dim3(nblk(P.nf, 256)), dim3(256), 0, st, P, selfIdx, A.start, A.val, invDiagA);
 }
 void launch_mg_galerkin0(hipStream_t st, const DevParams& P, const Level& L, const int* cellStart,
-                         const unsigned short* runStart, ASlices A) {
+                         const unsigned short* runStart, const int* nFluid, ASlices A, int variant) {
     const int threads = 9 * L.cs[0] * L.cs[1] * L.cs[2];  // <= 243
-    if (L.cells > 0)
+    if (L.cells <= 0) return;
+    if (variant == 1)
+        hipLaunchKernelGGL(k_mg_galerkin0_lds, dim3(L.cells), dim3(256), 0, st, P, L, cellStart, runStart, nFluid,
+                           A.start, A.val);
+    else
         hipLaunchKernelGGL(k_mg_galerkin0, dim3(L.cells), dim3((threads + 63) / 64 * 64), 0, st, P, L, cellStart,
                            runStart, A.start, A.val);
 }
@@ -466,6 +549,12 @@ void launch_fill_i32(hipStream_t st, int* x, long long n, int v) {
 }
 void launch_mg_p2g(hipStream_t st, const DevParams& P, Level& L, const int* cellStart, const double* q) {
     hipLaunchKernelGGL(k_mg_p2g, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, P, L, cellStart, q);
+}
+void launch_mg_p2g_bu(hipStream_t st, const DevParams& P, Level& L, const int* cellStart, const double* q) {
+    hipLaunchKernelGGL(k_mg_p2g_bu, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, P, L, cellStart, q);
+}
+void launch_mg_restrict_bu(hipStream_t st, const Level& S, Level& L) {
+    hipLaunchKernelGGL(k_mg_restrict_bu, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, S, L);
 }
 void launch_mg_begin(hipStream_t st, Level& L) {
     hipLaunchKernelGGL(k_mg_begin, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, L);

@@ -60,6 +60,7 @@ def lib():
         L.mph_batch_size.argtypes = [vp]
         L.mph_batch_step.argtypes = [vp, vp]
         L.mph_batch_destroy.argtypes = [vp]
+        L.mph_probe_bandwidth.argtypes = [i, C.c_ulonglong, i, C.POINTER(d), C.POINTER(d)]
         _lib = L
     return _lib
 
@@ -186,6 +187,13 @@ class Scene:
         v = C.c_ulonglong()
         _check(lib().mph_scene_device_bytes(self.h, C.byref(v)))
         return v.value
+
+
+def probe_bandwidth(device: int = 0, nbytes: int = 8 << 30, reps: int = 5) -> tuple[float, float]:
+    """mph_probe_bandwidth: (streaming-read GB/s, copy GB/s) measured on `device`."""
+    r, c = C.c_double(), C.c_double()
+    _check(lib().mph_probe_bandwidth(int(device), int(nbytes), int(reps), C.byref(r), C.byref(c)))
+    return r.value, c.value
 
 
 class Batch:
