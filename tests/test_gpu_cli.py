@@ -67,5 +67,6 @@ def test_mph_run_files_match_oracle(tmp_path):
         time = o.scalar("Time")
         step += 1
     assert step == 13 and checked == 3
-    assert sorted(x.name for x in tmp_path.glob("out*")) == ["out000.prof", "out000.vtk", "out010.vtk"]
+    assert sorted(x.name for x in tmp_path.glob("out[0-9]*")) == ["out000.prof", "out000.vtk", "out010.vtk"]
+    assert (tmp_path / "output.vtk").exists()  # the initial state, as main.cpp:374
     assert gpu_iters == iters
