@@ -253,13 +253,14 @@ def run_batch(args):
     return out
 
 
-def run_serve(args):
+def run_serve(args, rate=None):
     """Serving stream: Poisson arrivals of scene-step requests against the wall
     clock for a bounded time, continuous batching over one resident batch."""
     from mphimplicit_amd import scenes
     from mphimplicit_amd.serve import Server, make_requests
 
-    stream = scenes.serve_stream(args.serve_requests, rate=args.serve_rate)
+    rate = args.serve_rate if rate is None else rate
+    stream = scenes.serve_stream(args.serve_requests, rate=rate)
     reqs = make_requests(stream, tol=args.rel_residual ** 2)
     srv = Server()
     try:
@@ -268,7 +269,7 @@ def run_serve(args):
     finally:
         srv.close()
     r["workload"] = ("Poisson arrivals at %g req/s, 1-20 steps, lognormal fluid counts (median 50k, sigma 1, "
-                     "clipped 5k-500k), 2D/3D, mixed precision, %g s run" % (args.serve_rate, args.serve_seconds))
+                     "clipped 5k-500k), 2D/3D, mixed precision, %g s run" % (rate, args.serve_seconds))
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
 
 
@@ -303,6 +304,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--serve-light-rate", type=float, default=5.0,
+                    help="a second serving run at this arrival rate, below the GPU's capacity (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the known-size kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -376,6 +379,8 @@ def main():
         out["batched"] = run_batch(args)
     if args.serve == "on":
         out["serving"] = run_serve(args)
+        if args.serve_light_rate > 0:
+            out["serving_below_capacity"] = run_serve(args, args.serve_light_rate)
     if args.cpu_baseline == "on":
         cb, cfields = run_cpu_baseline(args)
         if args.cpu_steps == 1:

@@ -114,18 +114,20 @@ void launch_vector_p(hipStream_t st, const DevParams& P, const int* type, const 
                      const long long* nPtrP, const int* nIndP, double* vecP);
 void launch_b_minus_cp(hipStream_t st, const DevParams& P, const int* nCountP, const long long* nPtrP,
                        const int* nIndP, const double* cval, const double* vecP, double* b);
-// variant 0: P lists from global memory; 1: the slice's index lists in LDS; 2: and
-// each neighbour's list copied to LDS before its merge (1 and 2 need every fluid
-// particle's P-neighbour count <= kMaxP).  Same bits in every variant.
-constexpr int kCacVariants = 3;
+// kernel modes: 0 P lists from global memory; 2 both lists staged in LDS; 3 staged,
+// and the merge records the matches before any arithmetic.  The tuner chooses
+// between modes 3 and 2 (variants 0, 1); mode 0 runs when a fluid particle has more
+// than kMaxP P-neighbours.  Same bits in every mode.
+constexpr int kCacVariants = 2;
+constexpr int kCacModeOf[kCacVariants] = {3, 2};
 void launch_cac(hipStream_t st, const DevParams& P, const int* nFluid, const int* nCountP, const long long* nPtrP,
-                const int* nIndP, const double* cval, const double* dval, ASlices A, int variant);  // also writes A.val32 when it is non-null
+                const int* nIndP, const double* cval, const double* dval, ASlices A, int mode);  // also writes A.val32 when it is non-null
 // ---- multigrid ----
 void launch_fluid_bbox(hipStream_t st, const DevParams& P, const int* cellStart, int* bbox);
 void launch_inv_diag(hipStream_t st, const DevParams& P, const int* selfIdx, ASlices A, double* invDiagA);
-// variant 0: per-thread sums straight from A; 1: each particle's row staged in LDS
-// first (rows of at most 256 blocks).  Same bits.
-constexpr int kGalVariants = 2, kGalRowMax = 256;
+// variant 0: per-thread sums straight from A; 1: the rows of four adjacent
+// particles staged in LDS per round (rows of at most 216 blocks).  Same bits.
+constexpr int kGalVariants = 2, kGalRowMax = 216;
 void launch_mg_galerkin0(hipStream_t st, const DevParams& P, const Level& L, const int* cellStart,
                          const unsigned short* runStart, const int* nFluid, ASlices A, int variant);
 void launch_mg_galerkinL(hipStream_t st, const Level& S, const Level& L);

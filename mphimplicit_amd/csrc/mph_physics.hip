@@ -366,10 +366,12 @@ __global__ void k_b_minus_cp(DevParams P, const int* __restrict__ nCountP, const
 // fluid neighbours j, merge the sorted P-neighbour lists of i and j and sum
 // D_ik (x) C_jk over the shared k (D = C*Lambda, left to right as the reference),
 // then add the 3x3 sum to block (i, j).  Workgroup = one slice: lane = particle,
-// wave w takes the blocks k = w, w+4, ...  kMode >= 1: the slice's own index lists
-// are staged in LDS (odd row stride, no bank conflicts); otherwise they are read from
-// global memory; with kMode 2 each thread also copies j's index list into LDS
-// before merging (its loads all in flight at once instead of one per merge step).
+// wave w takes the blocks k = w, w+4, ...  kMode >= 2: the slice's own index lists
+// are staged in LDS (odd row stride, no bank conflicts) and each thread copies j's
+// index list into LDS before merging (its loads all in flight at once instead of
+// one per merge step); with kMode 0 both are read from global memory;
+// kMode 3 in addition first merges without arithmetic, recording the matches, and
+// then adds them (the merge's three-way branch no longer carries the 9 products).
 // All give the same bits; the fastest is picked at run time.
 // The FP32 copy of the finished block is written when aval32 is non-null.
 template <int kMode>
@@ -380,7 +382,7 @@ __global__ void __launch_bounds__(256) k_cac(DevParams P, const int* __restrict_
                                              const int* __restrict__ acol, double* __restrict__ aval,
                                              float* __restrict__ aval32) {
     constexpr int kStride = kMaxP + 1;
-    constexpr bool kLdsI = kMode >= 1, kLdsJ = kMode >= 2;
+    constexpr bool kLdsI = kMode >= 2, kLdsJ = kMode >= 2;  // mode 3: both + two-phase merge
     __shared__ int sk[kLdsI ? kSlice * kStride : 1];
     __shared__ int sj[kLdsJ ? 256 * kStride : 1];
     const int slice = blockIdx.x;
@@ -412,27 +414,47 @@ __global__ void __launch_bounds__(256) k_cac(DevParams P, const int* __restrict_
             kj = kjs;
         }
         double sum[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        int a = 0, c = 0;
-        int ka = ni > 0 ? ki[0] : 0;
-        int kc = nj > 0 ? kj[0] : 0;
-        while (a < ni && c < nj) {
-            if (ka == kc) {
-                const double* cj = &cval[3 * (pj + c)];
-                const double c0 = cj[0], c1 = cj[1], c2 = cj[2];
-                const double d0 = di[3 * a], d1 = di[3 * a + 1], d2 = di[3 * a + 2];
-                sum[0] += d0 * c0; sum[1] += d0 * c1; sum[2] += d0 * c2;
-                sum[3] += d1 * c0; sum[4] += d1 * c1; sum[5] += d1 * c2;
-                sum[6] += d2 * c0; sum[7] += d2 * c1; sum[8] += d2 * c2;
-                a++;
-                c++;
-                if (a < ni) ka = ki[a];
-                if (c < nj) kc = kj[c];
-            } else if (ka < kc) {
-                a++;
-                if (a < ni) ka = ki[a];
-            } else {
-                c++;
-                if (c < nj) kc = kj[c];
+        auto add = [&](int a, int c) {  // sum += D_i,k (x) C_j,k for the shared k
+            const double* cj = &cval[3 * (pj + c)];
+            const double c0 = cj[0], c1 = cj[1], c2 = cj[2];
+            const double d0 = di[3 * a], d1 = di[3 * a + 1], d2 = di[3 * a + 2];
+            sum[0] += d0 * c0; sum[1] += d0 * c1; sum[2] += d0 * c2;
+            sum[3] += d1 * c0; sum[4] += d1 * c1; sum[5] += d1 * c2;
+            sum[6] += d2 * c0; sum[7] += d2 * c1; sum[8] += d2 * c2;
+        };
+        if (kMode == 3) {
+            // merge without work in the loop: the matches (a, c), ascending, go over
+            // the already consumed front of j's list in LDS; then add them in order
+            int a = 0, c = 0, n = 0;
+            int ka = ni > 0 ? ki[0] : 0;
+            int kc = nj > 0 ? kjs[0] : 0;
+            while (a < ni && c < nj) {
+                const bool eq = ka == kc, le = ka <= kc, ge = ka >= kc;
+                if (eq) kjs[n++] = a | (c << 16);
+                a += le;
+                c += ge;
+                if (le && a < ni) ka = ki[a];
+                if (ge && c < nj) kc = kjs[c];
+            }
+            for (int t = 0; t < n; ++t) add(kjs[t] & 0xffff, kjs[t] >> 16);
+        } else {
+            int a = 0, c = 0;
+            int ka = ni > 0 ? ki[0] : 0;
+            int kc = nj > 0 ? kj[0] : 0;
+            while (a < ni && c < nj) {
+                if (ka == kc) {
+                    add(a, c);
+                    a++;
+                    c++;
+                    if (a < ni) ka = ki[a];
+                    if (c < nj) kc = kj[c];
+                } else if (ka < kc) {
+                    a++;
+                    if (a < ni) ka = ki[a];
+                } else {
+                    c++;
+                    if (c < nj) kc = kj[c];
+                }
             }
         }
         for (int e9 = 0; e9 < 9; ++e9) {
@@ -499,13 +521,13 @@ void launch_b_minus_cp(hipStream_t st, const DevParams& P, const int* nCountP, c
 }
 
 void launch_cac(hipStream_t st, const DevParams& P, const int* nFluid, const int* nCountP, const long long* nPtrP,
-                const int* nIndP, const double* cval, const double* dval, ASlices A, int variant) {
+                const int* nIndP, const double* cval, const double* dval, ASlices A, int mode) {
     if (A.nslices == 0) return;
 #define MPH_CAC(M)                                                                                            \
     hipLaunchKernelGGL(k_cac<M>, dim3(A.nslices), dim3(256), 0, st, P, nFluid, nCountP, nPtrP, nIndP, cval, dval, \
                        A.start, A.col, A.val, A.val32)
-    if (variant == 2) MPH_CAC(2);
-    else if (variant == 1) MPH_CAC(1);
+    if (mode == 3) MPH_CAC(3);
+    else if (mode == 2) MPH_CAC(2);
     else MPH_CAC(0);
 #undef MPH_CAC
 }

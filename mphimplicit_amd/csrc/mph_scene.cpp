@@ -285,13 +285,13 @@ int phase_multiply_c(mph_scene& S) {
     if (mixed(S) && (rc = S.aval32.ensure(S.slots > 0 ? 9 * S.slots : 1))) return rc;
     launch_b_minus_cp(S.st, S.P, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.vecP.p, S.b.p);
     // C Lambda C^T also writes the FP32 copy of A in mixed precision (aval32)
-    int v = S.tuneCac.pick();
-    const bool lds_ok = S.hmaxP <= kMaxP;
-    if (v >= 1 && !lds_ok) {
+    if (S.hmaxP > kMaxP) {  // P lists too long for the LDS modes
         launch_cac(S.st, S.P, S.nFluid.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.dval.p, aslices(S), 0);
     } else {
+        const int v = S.tuneCac.pick();
         S.tuneCac.begin(S.st, v);
-        launch_cac(S.st, S.P, S.nFluid.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.dval.p, aslices(S), v);
+        launch_cac(S.st, S.P, S.nFluid.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.dval.p, aslices(S),
+                   kCacModeOf[v]);
         S.tuneCac.end(S.st);
     }
     return MPH_OK;

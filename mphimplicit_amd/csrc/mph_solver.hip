@@ -127,18 +127,21 @@ __global__ void __launch_bounds__(256) k_mg_galerkin0(DevParams P, Level L, cons
     L.A[(long long)(3 * c + sd) * L.nrow + 3 * g + r] = acc;
 }
 
-// the same sums with each particle's row of blocks staged in LDS first: the
-// workgroup loads the row with 16-byte loads (block pairs), whose cache lines also
-// hold the next three particles' rows, then every (c, e) thread adds its runs from
-// LDS.  Rows longer than kGalRow blocks are not supported (the caller checks).
-constexpr int kGalRow = 256;
+// the same sums with the rows of kGalGroup adjacent particles staged in LDS per
+// round: a 128-byte line of A holds one entry pair of 8 adjacent particles, so
+// loading the group's rows together (one coalesced 64-byte piece per entry pair)
+// fetches each line at most twice, where walking one particle at a time across
+// thousands of resident workgroups re-fetched it up to eight times.  Every (c, e)
+// thread then adds its runs particle by particle from LDS, in the same order.
+// Rows longer than kGalRow blocks are not supported (the caller checks).
+constexpr int kGalGroup = 4, kGalRow = 216;
 __global__ void __launch_bounds__(256) k_mg_galerkin0_lds(DevParams P, Level L, const int* __restrict__ cellStart,
                                                           const unsigned short* __restrict__ runStart,
                                                           const int* __restrict__ nFluid,
                                                           const long long* __restrict__ start,
                                                           const double* __restrict__ aval) {
     typedef double d2 __attribute__((ext_vector_type(2)));
-    __shared__ double row[kGalRow * 9];
+    __shared__ double row[kGalGroup * kGalRow * 9];
     const int ncs = L.cs[0] * L.cs[1] * L.cs[2];
     const int g = blockIdx.x;
     const bool active = (int)threadIdx.x < 9 * ncs;
@@ -149,26 +152,33 @@ __global__ void __launch_bounds__(256) k_mg_galerkin0_lds(DevParams P, Level L, 
     const int p0 = cellStart[ic], p1 = cellStart[ic + 1];
     const unsigned mask = active ? galerkin_mask(P, L, cellStart, g, c) : 0u;
     double acc = 0.0;
-    for (int ip = p0; ip < p1; ++ip) {
-        const int lane = ip % kSlice;
-        const long long sb = start[ip / kSlice];
-        const int npair = (nFluid[ip] + 1) / 2;
-        __syncthreads();  // the previous row has been used
-        for (int t = threadIdx.x; t < npair * 9; t += blockDim.x) {
-            const int m = t / 9, ee = t % 9;
-            const d2 v = *(const d2*)&aval[a_val_index(sb, 2 * m, ee, lane)];
-            row[(2 * m) * 9 + ee] = v.x;
-            row[(2 * m + 1) * 9 + ee] = v.y;
+    for (int ip0 = p0; ip0 < p1;) {
+        // groups aligned to kGalGroup particles never straddle a 64-particle slice
+        const int gend = min(p1, (ip0 / kGalGroup + 1) * kGalGroup);
+        const int G = gend - ip0;
+        const long long sb = start[ip0 / kSlice];
+        const int lane0 = ip0 % kSlice;
+        int npair = 0;
+        for (int p = 0; p < G; ++p) npair = max(npair, (nFluid[ip0 + p] + 1) / 2);
+        __syncthreads();  // the previous group has been used
+        for (int t = threadIdx.x; t < npair * 9 * G; t += blockDim.x) {
+            const int p = t % G, ee = (t / G) % 9, m = t / (G * 9);
+            const d2 v = *(const d2*)&aval[a_val_index(sb, 2 * m, ee, lane0 + p)];
+            row[(p * kGalRow + 2 * m) * 9 + ee] = v.x;
+            row[(p * kGalRow + 2 * m + 1) * 9 + ee] = v.y;
         }
         __syncthreads();
-        if (mask) {
-            const unsigned short* rs = runStart + (long long)ip * kRuns;
-            for (unsigned m = mask; m; m &= m - 1) {
-                const int q = __builtin_ctz(m);
-                const int k1 = rs[q + 1];
-                for (int k = rs[q]; k < k1; ++k) acc += row[k * 9 + e];
+        if (mask)
+            for (int p = 0; p < G; ++p) {
+                const unsigned short* rs = runStart + (long long)(ip0 + p) * kRuns;
+                const double* rp = row + p * kGalRow * 9 + e;
+                for (unsigned m = mask; m; m &= m - 1) {
+                    const int q = __builtin_ctz(m);
+                    const int k1 = rs[q + 1];
+                    for (int k = rs[q]; k < k1; ++k) acc += rp[k * 9];
+                }
             }
-        }
+        ip0 = gend;
     }
     if (active) {
         const int r = e / 3, sd = e % 3;
