@@ -260,9 +260,10 @@ def run_serve(args, rate=None):
     from mphimplicit_amd.serve import Server, make_requests
 
     rate = args.serve_rate if rate is None else rate
-    stream = scenes.serve_stream(args.serve_requests, rate=rate)
+    count = min(args.serve_requests, int(rate * args.serve_seconds * 1.5) + 10)  # enough to outlast the run
+    stream = scenes.serve_stream(count, rate=rate)
     reqs = make_requests(stream, tol=args.rel_residual ** 2)
-    srv = Server()
+    srv = Server(lazy_mixed=True)
     try:
         r = srv.run(reqs, duration_s=args.serve_seconds,
                     log=lambda m: print(m, file=sys.stderr, flush=True))

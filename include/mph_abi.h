@@ -194,7 +194,8 @@ int mph_scene_destroy(mph_scene* s);
  * No counterpart in the reference, which advances one scene per process
  * (main.cpp:386-474); a batch replaces running many such processes side by side.
  * A batch holds up to 256 caller-owned scenes of one device (they stay owned by the
- * caller; destroy the batch or remove a scene before destroying that scene).
+ * caller; mph_scene_destroy returns MPH_E_STATE while the scene is in a batch:
+ * remove it or destroy the batch first).
  * mph_batch_step advances every member whose time is below its end_time by one
  * time step: the phases before the solve on each scene's stream, then the CR
  * solves of all members in lockstep (one launch per kernel for the whole batch,

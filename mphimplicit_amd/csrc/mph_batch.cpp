@@ -469,6 +469,7 @@ int mph_batch_add(mph_batch* B, mph_scene* S) {
     if ((int)B->scenes.size() >= kMaxScenes)
         return fail(MPH_E_ARG, "batch is full (" + std::to_string(kMaxScenes) + " scenes)");
     B->scenes.push_back(S);
+    S->batches++;
     return MPH_OK;
 }
 
@@ -477,6 +478,7 @@ int mph_batch_remove(mph_batch* B, mph_scene* S) {
     auto it = std::find(B->scenes.begin(), B->scenes.end(), S);
     if (it == B->scenes.end()) return fail(MPH_E_ARG, "scene is not in the batch");
     B->scenes.erase(it);
+    S->batches--;
     return MPH_OK;
 }
 
@@ -550,6 +552,7 @@ int mph_batch_destroy(mph_batch* B) {
     if (!B) return MPH_OK;
     (void)hipSetDevice(B->device);
     if (B->st) (void)hipStreamSynchronize(B->st);
+    for (mph_scene* S : B->scenes) S->batches--;
     B->tab.release();
     B->off.release();
     B->scal.release();

@@ -813,7 +813,7 @@ DevField device_field(mph_scene& S, const std::string& n) {
     if (n == "NeighborIndP") return F(S.nIndP.p, (size_t)S.EP);
     if (n == "VectorB") return F(S.b.p, 3 * nf);
     if (n == "VectorP") return F(S.vecP.p, np);
-    if (n == "InvDiagA") return F(S.invDiagA.p, 3 * nf);
+    if (n == "InvDiagA") return F(S.invDiagA.p, S.prm.multigrid ? 3 * nf : 0);  // built by the multigrid phase only
     return DevField{nullptr, 0, 0};
 }
 
@@ -1218,6 +1218,7 @@ int mph_scene_device_bytes(mph_scene* S, unsigned long long* bytes) {
 
 int mph_scene_destroy(mph_scene* S) {
     if (!S) return MPH_OK;
+    if (S->batches > 0) return fail(MPH_E_STATE, "scene is still a member of a batch: remove it first");
     (void)hipSetDevice(S->device);
     destroy(S);
     return MPH_OK;

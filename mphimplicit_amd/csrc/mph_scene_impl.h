@@ -196,6 +196,7 @@ struct mph_scene {
     // run-time variant choice (SpMV load groups per precision, C Lambda C^T schedule,
     // V-cycle levels fused into one workgroup)
     mph::host::Autotune tuneSpmv64, tuneSpmv32, tuneCac, tuneFuse, tuneGal;
+    int batches = 0;  // batches holding this scene (it cannot be destroyed meanwhile)
 };
 
 

@@ -129,7 +129,7 @@ class Scene:
 
     def close(self) -> None:
         if getattr(self, "h", None):
-            lib().mph_scene_destroy(self.h)
+            _check(lib().mph_scene_destroy(self.h))
             self.h = None
 
     def __del__(self):
@@ -227,7 +227,7 @@ class Batch:
 
     def close(self) -> None:
         if getattr(self, "h", None):
-            lib().mph_batch_destroy(self.h)
+            lib().mph_batch_destroy(self.h)  # releases the members' membership
             self.h = None
             self.members = []
 

@@ -178,6 +178,17 @@ def box_scene(seed: int, dim: int, nf_target: int, mu: float, gravity, dt=5.0e-4
     return p, g
 
 
+def box_scene_counts(dim: int, nf_target: int) -> tuple[int, int]:
+    """(fluid, total) particle counts of box_scene(dim, nf_target) without building it."""
+    n = max(2, int(round(nf_target ** (1.0 / dim))))
+    inner = int(math.ceil(2.5 * n))
+    if dim == 2:
+        walls = (inner + 8) * (inner + 4) - inner * inner
+    else:
+        walls = (inner + 8) * (inner + 8) * (inner + 4) - inner ** 3
+    return n ** dim, n ** dim + walls
+
+
 def batch_sweep(count=64, seed0=1000, steps=20, nf_range=(2000, 200000), mu_range=(0.01, 10.0)):
     """The batched workload: scene s has seed seed0+s, is 2D or 3D with
     probability 1/2, has a log-uniform fluid count in nf_range and viscosity in

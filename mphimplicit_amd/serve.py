@@ -29,10 +29,12 @@ class Request:
     rid: int
     arrival: float          # seconds since the start of the run
     steps: int
-    params: Params
-    grid: object            # files.GridData
+    params: Params | None   # None until built by `make` (at admission)
+    grid: object            # files.GridData, or None until built
     nf: int = 0
     dim: int = 3
+    np: int = 0             # total particles (for the footprint estimate before building)
+    make: object = None     # () -> (params, grid), called at admission when params is None
     # filled in by the server
     admitted: float = -1.0
     done: float = -1.0
@@ -70,11 +72,12 @@ class Server:
     """Continuous-batching scheduler of scene-step requests on one device."""
 
     def __init__(self, device: int = 0, budget_bytes: int = 200 << 30, max_active: int = 256,
-                 keep_fields: tuple = ()):
+                 keep_fields: tuple = (), lazy_mixed: bool = True):
         self.device = device
         self.budget = int(budget_bytes)
         self.max_active = max_active
         self.keep_fields = keep_fields
+        self.lazy_mixed = lazy_mixed  # precision assumed for requests not yet built
         self.batch = Batch(device)
         self.model = FootprintModel()
         self.pending: list[Request] = []
@@ -92,10 +95,14 @@ class Server:
     def _admit(self, now: float) -> None:
         while self.pending and len(self.active) < self.max_active:
             r = self.pending[0]
-            est = self.model.estimate(r.dim, r.nf, r.grid.np, r.params.precision == PRECISION_MIXED)
+            np_ = r.grid.np if r.grid is not None else r.np
+            mixed = r.params.precision == PRECISION_MIXED if r.params is not None else self.lazy_mixed
+            est = self.model.estimate(r.dim, r.nf, np_, mixed)
             if self.active and self.used_bytes() + est > self.budget:
                 break  # wait for memory (an empty server admits anything)
             self.pending.pop(0)
+            if r.params is None:
+                r.params, r.grid = r.make()
             r.scene = Scene(r.params, r.grid, self.device)
             r.est_bytes = est
             r.admitted = now
@@ -108,6 +115,8 @@ class Server:
             r.result[name] = r.scene.read(name)
         r.scene.close()
         r.scene = None
+        if r.make is not None:  # rebuilt on demand; keep finished requests small
+            r.params = r.grid = None
         r.done = now
         self.active.remove(r)
         self.finished.append(r)
@@ -180,13 +189,22 @@ class Server:
 
 
 def make_requests(stream, precision=PRECISION_MIXED, tol=1e-16) -> list[Request]:
-    """Requests from scenes.serve_stream tuples (arrival, steps, seed, dim, nf, mu)."""
+    """Requests from scenes.serve_stream tuples (arrival, steps, seed, dim, nf, mu);
+    each scene is built when the server admits its request."""
     from . import scenes
+
+    def factory(seed, dim, nf, mu, steps):
+        def make():
+            g_dir = (0.0, -9.8, 0.0) if dim == 2 else (0.0, 0.0, -9.8)
+            p, g = scenes.box_scene(seed, dim, nf, mu, g_dir, steps=steps)
+            p.precision = precision
+            p.cr_tolerance = tol
+            return p, g
+        return make
+
     out = []
     for k, (t, steps, seed, dim, nf, mu) in enumerate(stream):
-        g_dir = (0.0, -9.8, 0.0) if dim == 2 else (0.0, 0.0, -9.8)
-        p, g = scenes.box_scene(seed, dim, nf, mu, g_dir, steps=steps)
-        p.precision = precision
-        p.cr_tolerance = tol
-        out.append(Request(rid=k, arrival=t, steps=steps, params=p, grid=g, nf=int((g.type < 2).sum()), dim=dim))
+        nfl, npart = scenes.box_scene_counts(dim, nf)
+        out.append(Request(rid=k, arrival=t, steps=steps, params=None, grid=None, nf=nfl, dim=dim, np=npart,
+                           make=factory(seed, dim, nf, mu, steps)))
     return out

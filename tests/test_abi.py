@@ -119,3 +119,25 @@ def test_writers_byte_identical(name, tmp_path):
         vtk = str(tmp_path / ("%s.vtk" % tag))
         write_vtk(vtk, o)
         assert open(vtk, "rb").read() == fx["%s/file/ref.vtk" % tag].tobytes(), tag
+
+
+def test_no_device_fails_loudly():
+    """Without a usable HIP device the library reports an error: there is no CPU
+    path behind the boundary.  (Skipped where a GPU is present.)"""
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    L = lib()
+    h = C.c_void_p()
+    assert L.mph_batch_create(0, C.byref(h)) < 0
+    assert L.mph_last_error().decode() != ""
+    fx = fixture("tiny3d")
+    p = files.parse_data_text(str(fx["data_text"]), 3)
+    hdr = fx["in_hdr"]
+    g = files.GridData(float(hdr[0]), float(hdr[1]), hdr[2:5].copy(), hdr[5:8].copy(), fx["in_type"].copy(),
+                       fx["in_pos"].copy(), fx["in_vel"].copy())
+    from mphimplicit_amd.scene import MphError, Scene
+
+    with pytest.raises(MphError):
+        Scene(p, g)

@@ -88,3 +88,18 @@ def test_batch_membership_changes():
     same(stay, alone("3d_4096", prec, 3)[0])
     same(leave, alone("cube3d_sample", prec, 1)[0])
     same(late, alone("bridge2d_sample", prec, 2)[0])
+
+
+def test_member_cannot_be_destroyed():
+    """mph_scene_destroy refuses a scene that a batch still holds (MPH_E_STATE)."""
+    from mphimplicit_amd.scene import MphError
+
+    s = Scene(*make("tiny2d", abi.PRECISION_FP64))
+    b = Batch()
+    b.add(s)
+    with pytest.raises(MphError) as e:
+        s.close()
+    assert e.value.code == -5
+    b.remove(s)
+    s.close()
+    b.close()

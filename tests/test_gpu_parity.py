@@ -35,6 +35,16 @@ def golden_scene(name):
     return p, g
 
 
+def _with(make, **kw):
+    """A scene with some parameters changed (paths the default scenes do not reach)."""
+    def f():
+        p, g = make()
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p, g
+    return f
+
+
 SCENES = {
     "tiny2d": lambda: scenes.tiny_2d(),
     "tiny3d": lambda: scenes.tiny_3d(),
@@ -43,6 +53,14 @@ SCENES = {
     "cube3d_sample": lambda: golden_scene("cube3d"),
     "bridge2d_sample": lambda: golden_scene("bridge2d"),
     "glued2d_sample": lambda: golden_scene("glued2d"),
+    # plain CR without the multigrid preconditioner (the reference built without
+    # MULTIGRID_SOLVER)
+    "3d_4096_nomg": _with(lambda: scenes.config_3d_drop(n=16, box=30, lift=4, steps=30), multigrid=0),
+    # ~70 P-neighbours per particle: more than the LDS staging of C Lambda C^T holds
+    "tiny3d_wideP": _with(lambda: scenes.tiny_3d(), radius_ratio_p=2.6),
+    # ~400 fluid neighbours per particle: rows longer than the Galerkin LDS staging
+    "3d_4096_wideV": _with(lambda: scenes.config_3d_drop(n=16, box=30, lift=4, steps=30), radius_ratio_a=4.6,
+                           radius_ratio_v=4.6),
 }
 
 

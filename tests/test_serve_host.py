@@ -167,3 +167,19 @@ def test_serve_stream_statistics():
     mu = np.array([x[5] for x in st])
     assert mu.min() >= 0.01 and mu.max() <= 10.0
     assert 0.4 < np.mean([x[3] == 2 for x in st]) < 0.6
+
+
+@pytest.mark.parametrize("dim,nf", [(2, 300), (2, 5000), (3, 700), (3, 20000)])
+def test_box_scene_counts(dim, nf):
+    """The counts the footprint estimate uses before a request's scene is built."""
+    p, g = scenes.box_scene(1, dim, nf, 0.1, (0.0, -9.8, 0.0) if dim == 2 else (0.0, 0.0, -9.8))
+    assert scenes.box_scene_counts(dim, nf) == (int((g.type < 2).sum()), g.np)
+
+
+def test_lazy_requests_built_at_admission(fake):
+    reqs = serve.make_requests(scenes.serve_stream(6, rate=1000.0))
+    assert all(r.params is None and r.grid is None and r.np > r.nf > 0 for r in reqs)
+    srv = serve.Server()
+    res = srv.run(reqs, duration_s=30)
+    assert res["requests_done"] == 6
+    assert all(r.params is None for r in srv.finished)  # released again after retiring
