@@ -66,7 +66,7 @@ def roofline(kernel, ms, launches, bytes_sum, ref_sum, traffic):
     }
 
 
-def run_gpu(args, precision="fp64"):
+def run_gpu(args, precision="fp64", form=None):
     """Create the scene, run warmup + timed steps; return timings, the SpMV roofline
     and the fields used for the precision comparison (Velocity and PressureP after
     the first step, whose output-step diagnostics run inside it, and after the last)."""
@@ -77,12 +77,15 @@ def run_gpu(args, precision="fp64"):
     params.cr_tolerance = args.rel_residual ** 2
     params.end_time = 1.0e30
     params.precision = abi.PRECISION_MIXED if precision == "mixed" else abi.PRECISION_FP64
+    form = form or args.operator
+    factored = form == "factored"
+    params.operator_form = abi.OPERATOR_FACTORED if factored else abi.OPERATOR_ASSEMBLED
     nf = int((grid.type < 2).sum())
     t = time.perf_counter()
     scene = Scene(params, grid, device=0)
     setup_s = time.perf_counter() - t
-    print("[bench] %s scene %s: %d particles (%d fluid), created in %.1f s"
-          % (precision, args.config, grid.np, nf, setup_s), file=sys.stderr, flush=True)
+    print("[bench] %s %s scene %s: %d particles (%d fluid), created in %.1f s"
+          % (precision, form, args.config, grid.np, nf, setup_s), file=sys.stderr, flush=True)
     first = None
     for k in range(args.warmup):
         st = scene.step(1)
@@ -102,9 +105,13 @@ def run_gpu(args, precision="fp64"):
         l64, l32 = scene.scalar("SpmvLaunches"), scene.scalar("SpmvLaunchesF32")
         blocks = scene.scalar("ABlocks")
         mine, ref = spmv_bytes(blocks, nf, 8)
+        if factored:
+            mine = scene.scalar("FactoredBytes")
         b64 += mine * (l64 - l64_prev)
         r64 += ref * (l64 - l64_prev)
         mine, ref = spmv_bytes(blocks, nf, 4)
+        if factored and l32 > l32_prev:
+            mine = scene.scalar("FactoredBytesF32")
         b32 += mine * (l32 - l32_prev)
         r32 += ref * (l32 - l32_prev)
         l64_prev, l32_prev = l64, l32
@@ -119,12 +126,18 @@ def run_gpu(args, precision="fp64"):
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("bytes_per_launch")
-    rf64 = roofline("k_spmv (particle matrix, sliced 3x3 blocks, FP64 values)", scene.scalar("SpmvMs"), l64_prev,
-                    b64, r64, traffic if precision == "fp64" else None)
-    rf32 = roofline("k_spmv (particle matrix, sliced 3x3 blocks, FP32 values)", scene.scalar("SpmvMsF32"), l32_prev,
-                    b32, r32, None)
+    if factored:
+        k64 = "k_fac_t + k_fac_y (factored particle matrix, FP64 scalars)"
+        k32 = "k_fac_t + k_fac_y (factored particle matrix, FP32 scalars)"
+    else:
+        k64 = "k_spmv (particle matrix, sliced 3x3 blocks, FP64 values)"
+        k32 = "k_spmv (particle matrix, sliced 3x3 blocks, FP32 values)"
+    rf64 = roofline(k64, scene.scalar("SpmvMs"), l64_prev, b64, r64,
+                    traffic if precision == "fp64" and not factored else None)
+    rf32 = roofline(k32, scene.scalar("SpmvMsF32"), l32_prev, b32, r32, None)
     tuned = {"spmv_fp64_variant": int(scene.scalar("TunedSpmv64")), "spmv_fp32_variant": int(scene.scalar("TunedSpmv32")),
-             "cac_variant": int(scene.scalar("TunedCac"))}
+             "cac_variant": int(scene.scalar("TunedCac")), "fuse_variant": int(scene.scalar("TunedFuse")),
+             "galerkin_variant": int(scene.scalar("TunedGalerkin"))}
     scene.run_phase(abi.PHASE_DIAGNOSTICS)
     last = {"Velocity": scene.read("Velocity"), "PressureP": scene.read("PressureP")}
     scene.close()
@@ -294,6 +307,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="3d_1m")
     ap.add_argument("--rel-residual", type=float, default=1.0e-8, help="CR stop: sqrt(rr/rr0) below this")
+    ap.add_argument("--operator", choices=["assembled", "factored"], default="assembled",
+                    help="how the solve applies the particle matrix (mph_params.operator_form)")
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--mixed", choices=["on", "off"], default="on",

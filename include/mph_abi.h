@@ -44,6 +44,17 @@ extern "C" {
 #define MPH_PRECISION_FP64 0  /* plain FP64, bit-compatible with the reference   */
 #define MPH_PRECISION_MIXED 1 /* FP32 SpMV + multigrid inside FP64 refinement    */
 
+/* How the solve applies the particle matrix (mph_params.operator_form).  Both build
+ * the assembled matrix, which the multigrid preconditioner reads.
+ * ASSEMBLED multiplies by the stored 3x3 blocks, the reference's CSR product
+ * (main.cpp:4033-4050), bit for bit.
+ * FACTORED multiplies by A = viscous part + C Lambda C^T, rebuilt from one scalar per
+ * block and per RadiusP pair plus the positions.  It reads about a sixth of the bytes
+ * per product and equals the assembled product up to rounding, not bit for bit.
+ * A batch may hold scenes of both forms. */
+#define MPH_OPERATOR_ASSEMBLED 0
+#define MPH_OPERATOR_FACTORED 1
+
 /*
  * Parameters of a run: the keywords of the reference's *.data file
  * (readDataFile, source/main.cpp:523-563) plus the compile-time switches of the
@@ -74,6 +85,7 @@ typedef struct mph_params {
     int multigrid;       /* 1: MG-preconditioned CR (MULTIGRID_SOLVER), 0: CR    */
     int precision;       /* MPH_PRECISION_*                                      */
     int max_iterations;  /* <=0: 3*fluid count, as the reference                 */
+    int operator_form;   /* MPH_OPERATOR_*                                       */
 } mph_params;
 
 /* Particles and domain, the content of a *.grid / *.prof file
@@ -157,8 +169,10 @@ int mph_scene_run_phase(mph_scene* s, int phase);
  * R2g Swa Swg Swp Swv MaxRadius ParticleCount FluidCount PowerParticleCount
  * CellCount0..2 TotalCellCount NeighborIndCount NeighborIndCountP MultiGridDepth
  * CRIterations rr0 rr rs0 rs RefinementSteps ABlocks ASlots CBlocks, the timing
- * scalars of mph_scene_set_timing, and TunedSpmv64 TunedSpmv32 TunedCac TunedFuse
- * (the variants the run-time tuner kept, -1 while still exploring). */
+ * scalars of mph_scene_set_timing, TunedSpmv64 TunedSpmv32 TunedCac TunedFuse
+ * TunedGalerkin (the variants the run-time tuner kept, -1 while still exploring),
+ * and for the factored operator FactoredBytes and FactoredBytesF32 (algorithmic
+ * bytes of one product with FP64 / FP32 scalars). */
 int mph_scene_field_size(mph_scene* s, const char* field, size_t* bytes, int* elem_bytes);
 int mph_scene_read(mph_scene* s, const char* field, void* host, size_t bytes);
 int mph_scene_write(mph_scene* s, const char* field, const void* host, size_t bytes);
@@ -187,6 +201,14 @@ int mph_write_vtk_arrays(const char* path, int n, const int* type, const double*
  * the launch count are the scalars "SpmvMs" and "SpmvLaunches" (FP64 matrix) and
  * "SpmvMsF32" and "SpmvLaunchesF32" (FP32 matrix, mixed precision only). */
 int mph_scene_set_timing(mph_scene* s, int on);
+
+/* y = A x with the particle matrix of the last assembly, in the scene's operator
+ * form (myDcsrmvForA, source/main.cpp:4033-4050, as called with alpha 1, beta 0).
+ * x and y are host arrays of 3*FluidCount doubles in the current particle order.
+ * precision MPH_PRECISION_MIXED applies the FP32-stored values that a
+ * mixed-precision scene's inner solve uses.  MPH_E_STATE: no matrix since the
+ * particles last moved, or FP32 values asked of an FP64 scene. */
+int mph_scene_apply_matrix(mph_scene* s, int precision, const double* x, double* y);
 
 int mph_scene_destroy(mph_scene* s);
 

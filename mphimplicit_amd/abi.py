@@ -20,6 +20,8 @@ E_NOTIMPL = -7
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+OPERATOR_ASSEMBLED = 0
+OPERATOR_FACTORED = 1
 
 PHASE_KINEMATICS = 1
 PHASE_CELLS = 2
@@ -79,6 +81,7 @@ class Params(C.Structure):
         ("multigrid", C.c_int),
         ("precision", C.c_int),
         ("max_iterations", C.c_int),
+        ("operator_form", C.c_int),
     ]
 
     def copy(self) -> "Params":
@@ -128,4 +131,5 @@ def default_params(dim: int = 3) -> Params:
     p.multigrid = 1
     p.precision = PRECISION_FP64
     p.max_iterations = 0
+    p.operator_form = OPERATOR_ASSEMBLED
     return p

@@ -40,6 +40,7 @@ struct mph_batch {
     std::vector<BScene> htab;
     std::vector<int> hoff;
     Autotune tune64, tune32;  // SpMV load-group depth of the scene-table launches
+    std::vector<char> fac;      // per scene of the current solve: factored operator
     Autotune tuneFuse;        // V-cycle levels fused into one workgroup per scene
 };
 
@@ -74,6 +75,7 @@ int build_tables(mph_batch& B, const std::vector<mph_scene*>& sc, Tables& T) {
         (rc = B.scal.ensure((size_t)ns * 8)) || (rc = ensure_pinned(B, (size_t)ns * 8)))
         return rc;
     B.htab.assign(ns, BScene{});
+    B.fac.assign(ns, 0);
     B.hoff.assign((size_t)kOffRows * (ns + 1), 0);
     for (int s = 0; s < ns; ++s) {
         mph_scene& S = *sc[s];
@@ -90,6 +92,12 @@ int build_tables(mph_batch& B, const std::vector<mph_scene*>& sc, Tables& T) {
         b.col = A.col;
         b.val = A.val;
         b.val32 = A.val32;
+        b.factored = factored(S) ? 1 : 0;
+        B.fac[s] = (char)b.factored;
+        if (b.factored) {
+            b.fac[0] = fac_op(S, false);
+            if (mixed(S)) b.fac[1] = fac_op(S, true);
+        }
         b.cellStart = S.cellStart.p;
         b.cellIndex = S.cellIndex.p;
         b.invDiagA = S.invDiagA.p;
@@ -107,6 +115,7 @@ int build_tables(mph_batch& B, const std::vector<mph_scene*>& sc, Tables& T) {
         auto put = [&](int row, int blocks) { B.hoff[(size_t)row * (ns + 1) + s + 1] = blocks; };
         put(kOffChunk, (S.P.nf + 255) / 256);
         put(kOffElem, (int)((3LL * S.P.nf + 255) / 256));
+        put(kOffPart, (S.P.np + 255) / 256);
         for (int l = 0; l < b.depth; ++l) put(kOffLevel + l, (b.lv[l].nrow + 255) / 256);
     }
     for (int row = 0; row < kOffRows; ++row)
@@ -133,11 +142,26 @@ int read_scalars(mph_batch& B, int ns) {
 // the scene-table SpMV with the batch's tuned load-group depth
 void bspmv(mph_batch& B, const Tables& T, const SceneSet& on, const SceneSet& first, bool f32, double alpha, int x,
            double beta, int yin, int y, int y2, int p_copy, int dot) {
-    Autotune& A = f32 ? B.tune32 : B.tune64;
-    const int v = A.pick();
-    A.begin(B.st, v);
-    spmv(T, on, first, f32, alpha, x, beta, yin, y, y2, p_copy, dot, v);
-    A.end(B.st);
+    SceneSet onA = on, onF{};  // assembled and factored members
+    bool anyA = false, anyF = false;
+    for (int s = 0; s < T.ns; ++s) {
+        if (!((on.w[s >> 6] >> (s & 63)) & 1ull)) continue;
+        if (B.fac[s]) {
+            onA.w[s >> 6] &= ~(1ull << (s & 63));
+            onF.w[s >> 6] |= 1ull << (s & 63);
+            anyF = true;
+        } else {
+            anyA = true;
+        }
+    }
+    if (anyA) {
+        Autotune& A = f32 ? B.tune32 : B.tune64;
+        const int v = A.pick();
+        A.begin(B.st, v);
+        spmv(T, onA, first, f32, alpha, x, beta, yin, y, y2, p_copy, dot, v);
+        A.end(B.st);
+    }
+    if (anyF) spmv_fac(T, onF, first, f32, alpha, x, beta, yin, y, y2, p_copy, dot);
 }
 
 // the V-cycle of every scene in `on`, level by level; cur[s][l] is the ping-pong

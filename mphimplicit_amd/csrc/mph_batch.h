@@ -17,9 +17,10 @@ constexpr int kMaxScenes = 256;  // bits of a SceneSet
 // CR vectors by role
 enum { kX = 0, kR, kS, kY, kP, kQ, kU, kB, kROut, kD, kNVec };
 
-// rows of the block-offset table: 256-particle chunks, 256-element blocks of the
-// 3*nf vectors, then 256-row blocks of each multigrid level
-enum { kOffChunk = 0, kOffElem = 1, kOffLevel = 2, kOffRows = kOffLevel + kMaxLevels };
+// rows of the block-offset table: 256-particle chunks of the fluid particles,
+// 256-element blocks of the 3*nf vectors, 256-particle blocks of all particles,
+// then 256-row blocks of each multigrid level
+enum { kOffChunk = 0, kOffElem = 1, kOffPart = 2, kOffLevel = 3, kOffRows = kOffLevel + kMaxLevels };
 
 struct SceneSet {
     unsigned long long w[kMaxScenes / 64];
@@ -48,6 +49,8 @@ struct BScene {
     int depth;
     int mg;
     int fuseOf[kFuseVariants];  // first level of the one-workgroup coarse cycle per variant
+    int factored;               // MPH_OPERATOR_FACTORED: products through fac, not val/val32
+    FacOp fac[2];               // FP64 and FP32 scalars (FP32: mixed precision only)
     Level lv[kMaxLevels];  // .cur is ignored: it comes with each launch
 };
 
@@ -70,6 +73,9 @@ void axpy(const Tables& T, const SceneSet& on, int src, int dst);  // dst += 1.0
 // `first`; dot partial of dot.y into the scene's part when dot >= 0
 void spmv(const Tables& T, const SceneSet& on, const SceneSet& first, bool f32, double alpha, int x, double beta,
           int yin, int y, int y2, int p_copy, int dot, int variant);
+// the same product for scenes of the factored form
+void spmv_fac(const Tables& T, const SceneSet& on, const SceneSet& first, bool f32, double alpha, int x, double beta,
+              int yin, int y, int y2, int p_copy, int dot);
 void dot_partial(const Tables& T, const SceneSet& on, int a, int b);
 void dot_final(const Tables& T, const SceneSet& on, int slot, int shift_slot);
 void pq_update(const Tables& T, const SceneSet& on);

@@ -81,6 +81,31 @@ __global__ void __launch_bounds__(256) kb_spmv(const BScene* tab, const int* off
                          dot >= 0 ? B.vec[dot] : nullptr, dot >= 0 ? B.part : nullptr, lb, sh);
 }
 
+template <class VT>
+__global__ void __launch_bounds__(256) kb_fac_t(const BScene* tab, const int* off, int ns, SceneSet on, int x) {
+    MPH_SCENE(kOffPart)
+    const FacOp& F = B.fac[sizeof(VT) == 4 ? 1 : 0];
+    if (*F.wrap) fac_t_chunk<VT, true>(B.P, F, B.vec[x], lb);
+    else fac_t_chunk<VT, false>(B.P, F, B.vec[x], lb);
+}
+
+template <bool kNeg, class VT>
+__global__ void __launch_bounds__(256) kb_fac_y(const BScene* tab, const int* off, int ns, SceneSet on,
+                                                SceneSet first, int x, double beta, int yin, int y, int y2,
+                                                int p_copy, int dot) {
+    __shared__ double sh[256];
+    __shared__ double rows[3 * 256];
+    MPH_SCENE(kOffChunk)
+    const bool f = in_set(first, s);
+    const FacOp& F = B.fac[sizeof(VT) == 4 ? 1 : 0];
+    double* y2p = f && y2 >= 0 ? B.vec[y2] : nullptr;
+    double* pc = f && p_copy >= 0 ? B.vec[p_copy] : nullptr;
+    const double* dx = dot >= 0 ? B.vec[dot] : nullptr;
+    double* pt = dot >= 0 ? B.part : nullptr;
+    if (*F.wrap) fac_y_chunk<kNeg, VT, true>(B.P, F, B.vec[x], beta, B.vec[yin], B.vec[y], y2p, pc, dx, pt, lb, sh, rows);
+    else fac_y_chunk<kNeg, VT, false>(B.P, F, B.vec[x], beta, B.vec[yin], B.vec[y], y2p, pc, dx, pt, lb, sh, rows);
+}
+
 __global__ void __launch_bounds__(256) kb_dot_partial(const BScene* tab, const int* off, int ns, SceneSet on,
                                                       int a, int b) {
     __shared__ double sh[256];
@@ -268,6 +293,20 @@ void spmv(const Tables& T, const SceneSet& on, const SceneSet& first, bool f32, 
         else MPH_SPMV(false, double, 2);
     }
 #undef MPH_SPMV
+}
+void spmv_fac(const Tables& T, const SceneSet& on, const SceneSet& first, bool f32, double alpha, int x, double beta,
+              int yin, int y, int y2, int p_copy, int dot) {
+    if (f32) MPH_LAUNCH(kb_fac_t<float>, kOffPart, x);
+    else MPH_LAUNCH(kb_fac_t<double>, kOffPart, x);
+#define MPH_FAC(NEG, VT) MPH_LAUNCH((kb_fac_y<NEG, VT>), kOffChunk, first, x, beta, yin, y, y2, p_copy, dot)
+    if (f32) {
+        if (alpha < 0) MPH_FAC(true, float);
+        else MPH_FAC(false, float);
+    } else {
+        if (alpha < 0) MPH_FAC(true, double);
+        else MPH_FAC(false, double);
+    }
+#undef MPH_FAC
 }
 void dot_partial(const Tables& T, const SceneSet& on, int a, int b) { MPH_LAUNCH(kb_dot_partial, kOffChunk, a, b); }
 void dot_final(const Tables& T, const SceneSet& on, int slot, int shift_slot) {

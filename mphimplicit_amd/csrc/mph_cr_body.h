@@ -344,6 +344,45 @@ __device__ inline void cr_init_particle(const DevParams& P, const double* __rest
     for (int d = 0; d < 3; ++d) x[3 * i + d] = vel[3 * i + d] - force[3 * i + d] / mass[i] * P.dt;
 }
 
+// The tail of every particle-matrix product, given particle i's row sums (s0, s1,
+// s2): y = beta*yin + sum (the sum formed first, as myDcsrmvForA), optional second
+// output y2 = beta*y2 + sum, optional copy p = x, and the chunk partial of dotx.y.
+// Every thread of the workgroup calls it (i >= nf: no row).
+__device__ inline void spmv_store(int nf, int i, double s0, double s1, double s2, const double* __restrict__ x,
+                                  double beta, const double* yin, double* y, double* y2, double* p_copy,
+                                  const double* __restrict__ dotx, double* __restrict__ part, int chunk,
+                                  double* sh) {
+    double t = 0.0;
+    if (i < nf) {
+        const double sm[3] = {s0, s1, s2};
+        double out[3];
+        for (int r = 0; r < 3; ++r) {
+            double yy = yin[3 * i + r];
+            yy *= beta;
+            yy += sm[r];
+            out[r] = yy;
+        }
+        if (y2)
+            for (int r = 0; r < 3; ++r) {
+                double yy = y2[3 * i + r];
+                yy *= beta;
+                yy += sm[r];
+                y2[3 * i + r] = yy;
+            }
+        for (int r = 0; r < 3; ++r) y[3 * i + r] = out[r];
+        if (p_copy)
+            for (int r = 0; r < 3; ++r) p_copy[3 * i + r] = x[3 * i + r];
+        if (part) {
+            const double d0 = dotx[3 * i], d1 = dotx[3 * i + 1], d2 = dotx[3 * i + 2];
+            t = (d0 * out[0] + d1 * out[1]) + d2 * out[2];
+        }
+    }
+    if (part) {
+        const double v = block_tree_sum(t, sh);
+        if (threadIdx.x == 0) part[chunk] = v;
+    }
+}
+
 // y = beta*yin + sum_k (alpha*a_k)*x_k over the sliced A (myDcsrmvForA, 4033-4050);
 // alpha is +1 or -1, so alpha*a is exact and is applied as a sign.  Optional second
 // output y2 = beta*y2 + sum (same sum), optional copy p = x, and the chunk partial
@@ -357,7 +396,7 @@ __device__ inline void spmv_chunk(int nf, const int* __restrict__ nFluid, const 
                                   double* __restrict__ part, int chunk, double* sh) {
     typedef int it2 __attribute__((ext_vector_type(2)));
     const int i = chunk * 256 + threadIdx.x;
-    double t = 0.0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     if (i < nf) {
         const int w = nFluid[i];
         const int lane = i % kSlice;
@@ -367,7 +406,6 @@ __device__ inline void spmv_chunk(int nf, const int* __restrict__ nFluid, const 
         // per group with all their loads issued before the first use; columns are
         // pairs (a_col_index)
         const it2* cp = (const it2*)col + (sb / 2) * kSlice + lane;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
         auto block = [&](int j, const double* c9) {
             const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
             double c[9];
@@ -452,33 +490,230 @@ __device__ inline void spmv_chunk(int nf, const int* __restrict__ nFluid, const 
                 quad(j0, j1, a, h);
             }
         }
-        const double sm[3] = {s0, s1, s2};
-        double out[3];
-        for (int r = 0; r < 3; ++r) {
-            double yy = yin[3 * i + r];
-            yy *= beta;
-            yy += sm[r];
-            out[r] = yy;
-        }
-        if (y2)
-            for (int r = 0; r < 3; ++r) {
-                double yy = y2[3 * i + r];
-                yy *= beta;
-                yy += sm[r];
-                y2[3 * i + r] = yy;
+    }
+    spmv_store(nf, i, s0, s1, s2, x, beta, yin, y, y2, p_copy, dotx, part, chunk, sh);
+}
+
+// ---------------------------------------------------------------------------
+// the particle matrix in factored form (FacOp; the derivation is at the top of
+// mph_factored.hip).  Not bit-identical to the assembled product: the same sums,
+// associated differently, with multiply-adds contracted.  Both passes give each particle a group of G lanes
+// (kFacLanes) that walk its neighbour list together: the list is in
+// cell order, so the group's column and scalar loads are coalesced and its
+// gathers of neighbours' records mostly hit consecutive records; each lane keeps a
+// batch of entries in flight.  The group's partial sums are combined by a
+// butterfly, in the same order for every launch.
+
+// minimum-image displacement pos_j - pos_i (the reference's Mod form, used where
+// the scalars are built)
+__device__ inline void fac_disp(const DevParams& P, const double* __restrict__ pos, int j, double pi0, double pi1,
+                                double pi2, double& r0, double& r1, double& r2) {
+    r0 = min_image(pos[3 * j], pi0, P.W[0], P.hW[0]);
+    r1 = min_image(pos[3 * j + 1], pi1, P.W[1], P.hW[1]);
+    r2 = min_image(pos[3 * j + 2], pi2, P.W[2], P.hW[2]);
+}
+
+// the same displacement for the products: positions lie inside the domain, so one
+// wrap by the domain width gives the minimum image (equal to the Mod form up to
+// rounding).  Only scenes with pairs across the periodic boundary (FacOp::wrap)
+// take it; the others use the plain difference (kWrap false).
+template <bool kWrap>
+__device__ inline double fac_wrap(double r, double w, double hw) {
+    if (!kWrap) return r;
+    return r > hw ? r - w : (r < -hw ? r + w : r);
+}
+
+// sum over the G lanes of each aligned group (G a power of two, <= 64); every lane
+// of the wavefront calls it
+__device__ inline double group_sum(double v, int G) {
+    for (int m = G >> 1; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// lanes per particle in both passes, and the list entries each lane has in flight
+// per batch (viscous entries; P entries)
+constexpr int kFacLanes = 8, kFacBatch = 8, kFacBatchP = 4;
+
+// first pass over the 256 particles of `chunk` (fluid or wall):
+//   t_k = lambda_k (C_kk . x_k [k fluid] - sum_{j fluid, j != k} beta_kj r_kj . x_j),
+// written to k's record together with x_k (fluid k).  Every thread calls it.
+template <class VT, bool kWrap>
+__device__ inline void fac_t_chunk(const DevParams& P, const FacOp& F, const double* __restrict__ x, int chunk) {
+#pragma clang fp contract(fast)
+    constexpr int G = kFacLanes, U = kFacBatchP;
+    const int g = threadIdx.x & (G - 1);
+    const VT* __restrict__ beta = (const VT*)F.beta;
+    const double W0 = P.W[0], W1 = P.W[1], W2 = P.W[2], h0 = P.hW[0], h1 = P.hW[1], h2 = P.hW[2];
+    for (int r = threadIdx.x / G; r < 256; r += 256 / G) {
+        const int k = chunk * 256 + r;
+        double acc = 0.0;
+        const int run = k < P.np ? F.runP[k] : 0;
+        if (run > 0) {
+            const double pk0 = F.pos[3 * k], pk1 = F.pos[3 * k + 1], pk2 = F.pos[3 * k + 2];
+            const long long c0 = F.nPtrP[k];
+            for (int jb = 0; jb < run; jb += G * U) {
+                int jx[U];
+                double bb[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int jn = jb + g + u * G;
+                    const int j = jn < run ? F.nIndP[c0 + jn] : k;
+                    const bool use = jn < run && j < P.nf && j != k;
+                    jx[u] = use ? j : 0;  // unused entries read particle 0 with weight 0
+                    bb[u] = use ? (double)beta[c0 + jn] : 0.0;
+                }
+                double v[U][6];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const long long j3 = 3 * (long long)jx[u];
+                    v[u][0] = F.pos[j3];
+                    v[u][1] = F.pos[j3 + 1];
+                    v[u][2] = F.pos[j3 + 2];
+                    v[u][3] = x[j3];
+                    v[u][4] = x[j3 + 1];
+                    v[u][5] = x[j3 + 2];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const double r0 = fac_wrap<kWrap>(v[u][0] - pk0, W0, h0);
+                    const double r1 = fac_wrap<kWrap>(v[u][1] - pk1, W1, h1);
+                    const double r2 = fac_wrap<kWrap>(v[u][2] - pk2, W2, h2);
+                    acc -= bb[u] * ((r0 * v[u][3] + r1 * v[u][4]) + r2 * v[u][5]);
+                }
             }
-        for (int r = 0; r < 3; ++r) y[3 * i + r] = out[r];
-        if (p_copy)
-            for (int r = 0; r < 3; ++r) p_copy[3 * i + r] = x[3 * i + r];
-        if (part) {
-            const double d0 = dotx[3 * i], d1 = dotx[3 * i + 1], d2 = dotx[3 * i + 2];
-            t = (d0 * out[0] + d1 * out[1]) + d2 * out[2];
+        }
+        acc = group_sum(acc, G);
+        if (g == 0 && k < P.np) {
+            double* rk = F.rec + 8 * (long long)k;
+            if (k < P.nf) {
+                const double x0 = x[3 * k], x1 = x[3 * k + 1], x2 = x[3 * k + 2];
+                acc += (F.selfC[3 * k] * x0 + F.selfC[3 * k + 1] * x1) + F.selfC[3 * k + 2] * x2;
+                rk[0] = x0;
+                rk[1] = x1;
+                rk[2] = x2;
+            }
+            rk[6] = run > 0 ? acc * F.lambda[k] : 0.0;
         }
     }
-    if (part) {
-        const double v = block_tree_sum(t, sh);
-        if (threadIdx.x == 0) part[chunk] = v;
+}
+
+// second pass over the 256 fluid rows of `chunk`:
+//   y_i = D_i x_i + C_ii t_i - sum_j alpha_ij r_ij (r_ij . x_j) + sum_{k != i} beta_ik r_ik t_k,
+// the row sums staged in rows[3*256] (LDS), then the tail of spmv_store.  Every
+// thread calls it.
+template <bool kNeg, class VT, bool kWrap>
+__device__ inline void fac_y_chunk(const DevParams& P, const FacOp& F, const double* __restrict__ x, double beta,
+                                   const double* yin, double* y, double* y2, double* p_copy,
+                                   const double* __restrict__ dotx, double* __restrict__ part, int chunk,
+                                   double* sh, double* rows) {
+#pragma clang fp contract(fast)
+    typedef double dv2 __attribute__((ext_vector_type(2)));
+    constexpr int G = kFacLanes;
+    const int g = threadIdx.x & (G - 1);
+    const dv2* __restrict__ rec = (const dv2*)F.rec;  // 4 x 16 B per particle
+    const VT* __restrict__ alpha = (const VT*)F.alpha;
+    const VT* __restrict__ bet = (const VT*)F.beta;
+    const double W0 = P.W[0], W1 = P.W[1], W2 = P.W[2], h0 = P.hW[0], h1 = P.hW[1], h2 = P.hW[2];
+    for (int r = threadIdx.x / G; r < 256; r += 256 / G) {
+        const int i = chunk * 256 + r;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        if (i < P.nf) {
+            const double pi0 = F.pos[3 * i], pi1 = F.pos[3 * i + 1], pi2 = F.pos[3 * i + 2];
+            // viscous part over the fluid neighbours (alpha 0: self, beyond RadiusV);
+            // unused entries of a batch read i's own record with weight 0
+            const long long b0 = F.nPtr[i];
+            const int w = F.nFluid[i];
+            for (int jb = 0; jb < w; jb += G * kFacBatch) {
+                constexpr int U = kFacBatch;
+                int j[U];
+                double a[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int jn = jb + g + u * G;
+                    j[u] = jn < w ? F.nInd[b0 + jn] : i;
+                    a[u] = jn < w ? (double)alpha[b0 + jn] : 0.0;
+                }
+                dv2 q0[U], q1[U], q2[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    q0[u] = rec[4 * (long long)j[u]];
+                    q1[u] = rec[4 * (long long)j[u] + 1];
+                    q2[u] = rec[4 * (long long)j[u] + 2];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const double r0 = fac_wrap<kWrap>(q1[u][1] - pi0, W0, h0);
+                    const double r1 = fac_wrap<kWrap>(q2[u][0] - pi1, W1, h1);
+                    const double r2 = fac_wrap<kWrap>(q2[u][1] - pi2, W2, h2);
+                    const double f = a[u] * ((r0 * q0[u][0] + r1 * q0[u][1]) + r2 * q1[u][0]);
+                    s0 -= f * r0;
+                    s1 -= f * r1;
+                    s2 -= f * r2;
+                }
+            }
+            // C Lambda C^T part over the P neighbours (beta 0: self)
+            const long long c0 = F.nPtrP[i];
+            const int n = F.nCountP[i];
+            for (int jb = 0; jb < n; jb += G * kFacBatchP) {
+                constexpr int U = kFacBatchP;
+                int k[U];
+                double bb[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int jn = jb + g + u * G;
+                    k[u] = jn < n ? F.nIndP[c0 + jn] : i;
+                    bb[u] = jn < n ? (double)bet[c0 + jn] : 0.0;
+                }
+                dv2 q1[U], q2[U], q3[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    q1[u] = rec[4 * (long long)k[u] + 1];
+                    q2[u] = rec[4 * (long long)k[u] + 2];
+                    q3[u] = rec[4 * (long long)k[u] + 3];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const double r0 = fac_wrap<kWrap>(q1[u][1] - pi0, W0, h0);
+                    const double r1 = fac_wrap<kWrap>(q2[u][0] - pi1, W1, h1);
+                    const double r2 = fac_wrap<kWrap>(q2[u][1] - pi2, W2, h2);
+                    const double f = bb[u] * q3[u][0];
+                    s0 += f * r0;
+                    s1 += f * r1;
+                    s2 += f * r2;
+                }
+            }
+        }
+        s0 = group_sum(s0, G);
+        s1 = group_sum(s1, G);
+        s2 = group_sum(s2, G);
+        if (g == 0 && i < P.nf) {
+            const double* D = F.diag + 9 * i;
+            const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
+            const double ti = F.rec[8 * (long long)i + 6];
+            double d0 = (D[0] * xi0 + D[1] * xi1) + D[2] * xi2;
+            double d1 = (D[3] * xi0 + D[4] * xi1) + D[5] * xi2;
+            double d2 = (D[6] * xi0 + D[7] * xi1) + D[8] * xi2;
+            d0 += F.selfC[3 * i] * ti;
+            d1 += F.selfC[3 * i + 1] * ti;
+            d2 += F.selfC[3 * i + 2] * ti;
+            s0 += d0;
+            s1 += d1;
+            s2 += d2;
+            rows[3 * r] = kNeg ? -s0 : s0;
+            rows[3 * r + 1] = kNeg ? -s1 : s1;
+            rows[3 * r + 2] = kNeg ? -s2 : s2;
+        }
     }
+    __syncthreads();
+    const int i = chunk * 256 + threadIdx.x;
+    const int t = threadIdx.x;
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+    if (i < P.nf) {
+        r0 = rows[3 * t];
+        r1 = rows[3 * t + 1];
+        r2 = rows[3 * t + 2];
+    }
+    spmv_store(P.nf, i, r0, r1, r2, x, beta, yin, y, y2, p_copy, dotx, part, chunk, sh);
 }
 
 // chunk partial of x.y: per particle (x0 y0 + x1 y1) + x2 y2, then the chunk tree

@@ -53,6 +53,34 @@ struct ASlices {
     float* val32;              // [slots*9] (mixed precision copy) or null
 };
 
+// The particle matrix in factored form (one scalar per neighbour-list entry instead
+// of nine per block; derivation in mph_factored.hip).  The scalars sit beside the
+// neighbour lists, in their order: alpha[nPtr[i] + jn] for the fluid neighbours
+// jn < nFluid[i] of fluid i, beta[nPtrP[k] + jn] for the P neighbours of every k.
+struct FacOp {
+    const int* nFluid;        // [np] fluid neighbours (they lead each list)
+    const long long* nPtr;    // neighbour lists
+    const int* nInd;
+    const void* alpha;        // viscous scalar (double, or float in FP32 form)
+    const double* diag;       // [nf][9] viscous self block + inertia
+    const double* selfC;      // [nf][3] C_ii
+    const int* nCountP;       // [np] P entries
+    const int* runP;          // [np] P entries up to and including the last fluid one
+    const long long* nPtrP;   // P lists
+    const int* nIndP;
+    const void* beta;         // -w_p'(d)/d (double, or float in FP32 form)
+    const double* lambda;     // [np]
+    const double* pos;        // [np][3]
+    // [np][8] per particle: x0 x1 x2 p0 p1 p2 t 0 -- the operand x (fluid), the
+    // position and t = lambda_k C_k^T x, so the second pass gathers a neighbour's
+    // operands with three aligned 16-byte loads.  Positions are written with the P
+    // slices; x and t by every first pass.
+    double* rec;
+    // 1 when some neighbour pair interacts through the periodic boundary (then the
+    // displacements take the minimum image); set by the matrix-A launch
+    const int* wrap;
+};
+
 constexpr int kSlice = 64;    // particles per A slice (one wavefront)
 
 // block k of particle lane `lane` of the slice whose first slot is sb (even)
@@ -104,9 +132,14 @@ void launch_forces(hipStream_t st, const DevParams& P, const int* type, const do
                    const long long* nPtr, const int* nInd, const double* pressA, const double* gcen,
                    const double* mass, double* force, double* vel);
 // ---- implicit assembly ----
+// alpha/alpha32/diag/wrap (each may be null): the factored form's viscous scalar per
+// fluid-neighbour entry (FacOp order), the diagonal block without C Lambda C^T, and
+// the flag set when a pair interacts through the periodic boundary (wrap must be 0
+// before the launch)
 void launch_matrix_a(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
                      const double* mass, const double* mu, const int* nCount, const int* nFluid,
-                     const long long* nPtr, const int* nInd, ASlices A, int* selfIdx, double* b);
+                     const long long* nPtr, const int* nInd, ASlices A, int* selfIdx, double* b,
+                     double* alpha = nullptr, float* alpha32 = nullptr, double* diag = nullptr, int* wrap = nullptr);
 void launch_matrix_c(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP,
                      const long long* nPtrP, const int* nIndP, const double* lambda, double* cval, double* dval);
 void launch_vector_p(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
@@ -158,6 +191,15 @@ constexpr int kSpmvVariants64 = 3, kSpmvVariants32 = 2;
 void launch_spmv(hipStream_t st, const DevParams& P, ASlices A, const int* nFluid, double alpha, const double* x,
                  double beta, const double* yin, double* y, double* y2, double* p_copy, const double* dotx,
                  double* part, bool f32 = false, int variant = 0);
+// factored form: beta, runP, C_ii and the records' positions (after matrix A has
+// written alpha and the diagonal blocks), and y = beta*yin +- A x with the same
+// outputs as launch_spmv (two launches: t over every particle, then the fluid rows)
+void launch_fac_p(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP, const long long* nPtrP,
+                  const int* nIndP, double* beta, float* beta32, int* runP, double* selfC, double* rec);
+void launch_fac_spmv(hipStream_t st, const DevParams& P, const FacOp& F, double alpha, const double* x, double beta,
+                     const double* yin, double* y, double* y2, double* p_copy, const double* dotx, double* part,
+                     bool f32);
+
 void launch_convert_f32(hipStream_t st, const double* in, float* out, long long n);
 void launch_axpy(hipStream_t st, long long n, const double* x, double* y);  // y += 1.0*x
 void launch_dot_partial(hipStream_t st, const DevParams& P, const double* x, const double* y, double* part);

@@ -181,7 +181,9 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
                            const double* __restrict__ mu, const int* __restrict__ nCount,
                            const int* __restrict__ nFluid, const long long* __restrict__ nPtr,
                            const int* __restrict__ nInd, const long long* __restrict__ start, int* __restrict__ acol,
-                           double* __restrict__ aval, int* __restrict__ selfIdx, double* __restrict__ b) {
+                           double* __restrict__ aval, int* __restrict__ selfIdx, double* __restrict__ b,
+                           double* __restrict__ falpha, float* __restrict__ falpha32, double* __restrict__ fdiag,
+                           int* __restrict__ fwrap) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.nf) return;
     int iN = 0;
@@ -197,6 +199,12 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
     double pend[9];
     int pendCol = 0;
     bool pending = false;
+    // factored form: alpha_ij = visc mu_ij (-w_v'(d)) / d^3, the viscous block being
+    // -alpha_ij r r^T (beside the neighbour list; 0 for self and beyond RadiusV)
+    auto store_alpha = [&](int k, double a) {
+        if (falpha) falpha[b0 + k] = a;
+        if (falpha32) falpha32[b0 + k] = (float)a;
+    };
     auto store1 = [&](int k, const double* v, int col) {
         acol[a_col_index(sb, k, lane)] = col;
         for (int e9 = 0; e9 < 9; ++e9) aval[a_val_index(sb, k, e9, lane)] = v[e9];
@@ -209,10 +217,16 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
             acol[a_col_index(sb, jn, lane)] = j;
             if (pending) store1(jn - 1, pend, pendCol);  // the self block is stored last
             pending = false;
+            if (jn < nfl) store_alpha(jn, 0.0);
             continue;
         }
         double c9[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double alpha = 0.0;
         const Rel q = rel(P, pos, i, j);
+        // factored form: does any pair interact through the periodic boundary
+        if (fwrap)
+            for (int d = 0; d < 3; ++d)
+                if (fabs(pos[3 * j + d] - pos[3 * i + d]) > P.hW[d]) atomicOr(fwrap, 1);
         const bool inside = q.r2 != 0.0 && P.radV2 - q.r2 > 0;
         if (inside) {
             const double dij = sqrt(q.r2);
@@ -220,6 +234,7 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
             const double e[3] = {q.x[0] / dij, q.x[1] / dij, q.x[2] / dij};
             const double muj = mu[j];
             const double muij = 2.0 * (mui * muj) / (mui + muj);
+            if (falpha || falpha32) alpha = P.visc * muij * wdrij / (dij * dij * dij);
             for (int r = 0; r < 3; ++r)
                 for (int s = 0; s < 3; ++s) {
                     const double coef = +P.visc * muij * e[s] * e[r] * wdrij / dij;
@@ -234,6 +249,7 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
             }
         }
         if (fluidj) {
+            store_alpha(jn, alpha);
             double v[9];
             for (int e9 = 0; e9 < 9; ++e9) v[e9] = inside ? -c9[e9] : 0.0;
             if ((jn & 1) == 0) {
@@ -257,6 +273,8 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
     self[4] += coef;
     self[8] += coef;
     for (int e9 = 0; e9 < 9; ++e9) aval[a_val_index(sb, iN, e9, lane)] = self[e9];
+    if (fdiag)
+        for (int e9 = 0; e9 < 9; ++e9) fdiag[9 * i + e9] = self[e9];
     selfIdx[i] = iN;
     const double bv[3] = {0.0 + sv0, 0.0 + sv1, 0.0 + sv2};
     for (int d = 0; d < 3; ++d) b[3 * i + d] = bv[d] + coef * vel[3 * i + d];
@@ -496,9 +514,10 @@ void launch_forces(hipStream_t st, const DevParams& P, const int* type, const do
 
 void launch_matrix_a(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
                      const double* mass, const double* mu, const int* nCount, const int* nFluid,
-                     const long long* nPtr, const int* nInd, ASlices A, int* selfIdx, double* b) {
+                     const long long* nPtr, const int* nInd, ASlices A, int* selfIdx, double* b, double* alpha,
+                     float* alpha32, double* diag, int* wrap) {
     hipLaunchKernelGGL(k_matrix_a, dim3(nblk(P.nf, kSlice)), dim3(kSlice), 0, st, P, type, pos, vel, mass, mu,
-                       nCount, nFluid, nPtr, nInd, A.start, A.col, A.val, selfIdx, b);
+                       nCount, nFluid, nPtr, nInd, A.start, A.col, A.val, selfIdx, b, alpha, alpha32, diag, wrap);
 }
 
 void launch_matrix_c(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP,

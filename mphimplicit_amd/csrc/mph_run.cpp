@@ -2,13 +2,15 @@
 // executable's interface (source/main.cpp:313-521):
 //
 //   mph_run data grid prof_pattern vtk_pattern log [threads] [--dim 2|3]
-//           [--rel-residual R] [--device D]
+//           [--rel-residual R] [--device D] [--mixed] [--factored]
 //
 // It reads the same *.data / *.grid files, runs the same time loop, writes the same
 // *.prof (every OutputInterval) and *.vtk (every VtkOutputInterval) files named by
 // the printf patterns and the step number, and logs the CR summary line of every
 // step in the reference's format (main.cpp:4381).  `threads` is accepted and ignored
-// (the reference's OpenMP thread count).
+// (the reference's OpenMP thread count).  --mixed and --factored select the
+// mixed-precision solve and the factored particle matrix (mph_params); without
+// them the run is the reference's arithmetic.
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -50,7 +52,7 @@ double scalar(mph_scene* s, const char* n) {
 
 int main(int argc, char** argv) {
     std::vector<std::string> pos;
-    int dim = 3, device = 0;
+    int dim = 3, device = 0, mixedp = 0, fact = 0;
     double relres = 0;
 #ifdef MPH_RUN_2D
     dim = 2;
@@ -59,6 +61,8 @@ int main(int argc, char** argv) {
         if (!std::strcmp(argv[k], "--dim") && k + 1 < argc) dim = std::atoi(argv[++k]);
         else if (!std::strcmp(argv[k], "--rel-residual") && k + 1 < argc) relres = std::atof(argv[++k]);
         else if (!std::strcmp(argv[k], "--device") && k + 1 < argc) device = std::atoi(argv[++k]);
+        else if (!std::strcmp(argv[k], "--mixed")) mixedp = 1;
+        else if (!std::strcmp(argv[k], "--factored")) fact = 1;
         else pos.push_back(argv[k]);
     }
     // defaults of main.cpp:58-62
@@ -73,6 +77,8 @@ int main(int argc, char** argv) {
     mph_params_default(&p);
     p.dim = dim;
     if (relres > 0) p.cr_tolerance = relres * relres;
+    if (mixedp) p.precision = MPH_PRECISION_MIXED;
+    if (fact) p.operator_form = MPH_OPERATOR_FACTORED;
     if (mph_read_data_file(data.c_str(), &p)) return die("reading data file");
     int np = 0;
     if (mph_read_grid_header(grid.c_str(), &np)) return die("reading grid header");

@@ -173,6 +173,7 @@ int init_constants(mph_scene& S, const mph_grid& g) {
 // phases
 
 bool mixed(const mph_scene& S) { return S.prm.precision == MPH_PRECISION_MIXED; }
+bool factored(const mph_scene& S) { return S.prm.operator_form == MPH_OPERATOR_FACTORED; }
 
 ASlices aslices(mph_scene& S) {
     ASlices A;
@@ -181,11 +182,12 @@ ASlices aslices(mph_scene& S) {
     A.start = S.sliceStart.p;
     A.col = S.acol.p;
     A.val = S.aval.p;
-    A.val32 = mixed(S) ? S.aval32.p : nullptr;
+    A.val32 = mixed(S) && !factored(S) ? S.aval32.p : nullptr;
     return A;
 }
 
 int phase_kinematics(mph_scene& S) {
+    S.assembled = false;
     launch_kinematics(S.st, S.P, S.wk, S.type.p, S.pos.p, S.vel.p, S.force.p);
     for (int w = 2; w < 4; ++w)  // WallCenter += WallVelocity*Dt (main.cpp:2123-2127)
         for (int d = 0; d < 3; ++d) S.wk.center[w][d] += S.wk.vel[w][d] * S.P.dt;
@@ -193,6 +195,7 @@ int phase_kinematics(mph_scene& S) {
 }
 
 int phase_cells(mph_scene& S) {
+    S.assembled = false;
     const DevParams& P = S.P;
     launch_hash(S.st, P, S.pos.p, S.type.p, S.keys.p);
     launch_bitonic(S.st, P, S.keys.p);
@@ -266,8 +269,17 @@ int phase_matrix_a(mph_scene& S) {
     int rc;
     if ((rc = S.acol.ensure(S.slots > 0 ? S.slots : 1)) || (rc = S.aval.ensure(S.slots > 0 ? 9 * S.slots : 1)))
         return rc;
+    if (factored(S)) {
+        const long long ne = S.E > 0 ? S.E : 1;
+        if ((rc = S.falpha.ensure(ne)) || (rc = S.fdiag.ensure(9 * (size_t)S.P.nf)) ||
+            (mixed(S) && (rc = S.falpha32.ensure(ne))) || (rc = S.fwrap.ensure(1)))
+            return rc;
+        launch_fill_i32(S.st, S.fwrap.p, 1, 0);
+    }
     launch_matrix_a(S.st, S.P, S.type.p, S.pos.p, S.vel.p, S.mass.p, S.mu.p, S.nCount.p, S.nFluid.p, S.nPtr.p,
-                    S.nInd.p, aslices(S), S.selfIdx.p, S.b.p);
+                    S.nInd.p, aslices(S), S.selfIdx.p, S.b.p, factored(S) ? S.falpha.p : nullptr,
+                    factored(S) && mixed(S) ? S.falpha32.p : nullptr, factored(S) ? S.fdiag.p : nullptr,
+                    factored(S) ? S.fwrap.p : nullptr);
     return MPH_OK;
 }
 
@@ -277,14 +289,23 @@ int phase_matrix_c(mph_scene& S) {
     launch_matrix_c(S.st, S.P, S.pos.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.lambda.p, S.cval.p, S.dval.p);
     launch_vector_p(S.st, S.P, S.type.p, S.pos.p, S.vel.p, S.kappa.p, S.lambda.p, S.volStrainP.p, S.nCountP.p,
                     S.nPtrP.p, S.nIndP.p, S.vecP.p);
+    if (factored(S)) {
+        const long long ne = S.EP > 0 ? S.EP : 1;
+        if ((rc = S.pbeta.ensure(ne)) || (mixed(S) && (rc = S.pbeta32.ensure(ne))) || (rc = S.runP.ensure(S.P.np)) ||
+            (rc = S.selfC.ensure(3 * (size_t)S.P.nf)) || (rc = S.frec.ensure(8 * (size_t)S.P.np)))
+            return rc;
+        launch_fac_p(S.st, S.P, S.pos.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.pbeta.p,
+                     mixed(S) ? S.pbeta32.p : nullptr, S.runP.p, S.selfC.p, S.frec.p);
+    }
     return MPH_OK;
 }
 
 int phase_multiply_c(mph_scene& S) {
     int rc;
-    if (mixed(S) && (rc = S.aval32.ensure(S.slots > 0 ? 9 * S.slots : 1))) return rc;
+    if (mixed(S) && !factored(S) && (rc = S.aval32.ensure(S.slots > 0 ? 9 * S.slots : 1))) return rc;
     launch_b_minus_cp(S.st, S.P, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.vecP.p, S.b.p);
-    // C Lambda C^T also writes the FP32 copy of A in mixed precision (aval32)
+    // C Lambda C^T also writes the FP32 copy of A in mixed precision (aval32) unless
+    // the solve applies the factored form
     if (S.hmaxP > kMaxP) {  // P lists too long for the LDS modes
         launch_cac(S.st, S.P, S.nFluid.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.dval.p, aslices(S), 0);
     } else {
@@ -294,6 +315,7 @@ int phase_multiply_c(mph_scene& S) {
                    kCacModeOf[v]);
         S.tuneCac.end(S.st);
     }
+    S.assembled = true;
     return MPH_OK;
 }
 
@@ -443,6 +465,26 @@ int read_scalar(mph_scene& S, int slot, double* v) {
     return MPH_OK;
 }
 
+FacOp fac_op(mph_scene& S, bool f32) {
+    FacOp F;
+    F.nFluid = S.nFluid.p;
+    F.nPtr = S.nPtr.p;
+    F.nInd = S.nInd.p;
+    F.alpha = f32 ? (const void*)S.falpha32.p : (const void*)S.falpha.p;
+    F.diag = S.fdiag.p;
+    F.selfC = S.selfC.p;
+    F.nCountP = S.nCountP.p;
+    F.runP = S.runP.p;
+    F.nPtrP = S.nPtrP.p;
+    F.nIndP = S.nIndP.p;
+    F.beta = f32 ? (const void*)S.pbeta32.p : (const void*)S.pbeta.p;
+    F.lambda = S.lambda.p;
+    F.pos = S.pos.p;
+    F.rec = S.frec.p;
+    F.wrap = S.fwrap.p;
+    return F;
+}
+
 // the particle-matrix SpMV, bracketed by events when per-launch timing is on
 void spmv(mph_scene& S, double alpha, const double* x, double beta, const double* yin, double* y, double* y2,
           double* p_copy, const double* dotx, double* part, bool f32 = false) {
@@ -460,11 +502,15 @@ void spmv(mph_scene& S, double alpha, const double* x, double beta, const double
         hipError_t e = hipEventRecord(S.spmvEv[S.spmvUsed], S.st);
         if (e != hipSuccess) S.spmvTimingErr = (int)e;
     }
-    Autotune& T = f32 ? S.tuneSpmv32 : S.tuneSpmv64;
-    const int v = T.pick();
-    T.begin(S.st, v);
-    launch_spmv(S.st, S.P, aslices(S), S.nFluid.p, alpha, x, beta, yin, y, y2, p_copy, dotx, part, f32, v);
-    T.end(S.st);
+    if (factored(S)) {
+        launch_fac_spmv(S.st, S.P, fac_op(S, f32), alpha, x, beta, yin, y, y2, p_copy, dotx, part, f32);
+    } else {
+        Autotune& T = f32 ? S.tuneSpmv32 : S.tuneSpmv64;
+        const int v = T.pick();
+        T.begin(S.st, v);
+        launch_spmv(S.st, S.P, aslices(S), S.nFluid.p, alpha, x, beta, yin, y, y2, p_copy, dotx, part, f32, v);
+        T.end(S.st);
+    }
     if (S.spmvTiming) {
         hipError_t e = hipEventRecord(S.spmvEv[S.spmvUsed + 1], S.st);
         if (e != hipSuccess) S.spmvTimingErr = (int)e;
@@ -725,9 +771,12 @@ void destroy(mph_scene* S) {
     if (S->st) (void)hipStreamSynchronize(S->st);
     for (DBuf<int>* b : {&S->type, &S->type2, &S->perm, &S->perm2, &S->cellIndex, &S->cellStart, &S->bbox,
                          &S->nCount, &S->nFluid, &S->nCountP, &S->nInd, &S->nIndP, &S->width, &S->selfIdx, &S->acol,
-                         &S->maxP})
+                         &S->maxP, &S->runP, &S->fwrap})
         b->release();
     S->runStart.release();
+    for (DBuf<double>* b : {&S->falpha, &S->fdiag, &S->selfC, &S->pbeta, &S->frec}) b->release();
+    S->falpha32.release();
+    S->pbeta32.release();
     for (DBuf<double>* b : {&S->pos, &S->pos2, &S->vel, &S->vel2, &S->force, &S->mass, &S->kappa, &S->lambda,
                             &S->mu, &S->densA, &S->gcen, &S->pressA, &S->volStrainP, &S->divP, &S->pressP,
                             &S->virialP, &S->virialStress, &S->aval, &S->b, &S->cval, &S->dval, &S->vecP,
@@ -940,6 +989,7 @@ void mph_params_default(mph_params* p) {
     p->multigrid = 1;
     p->precision = MPH_PRECISION_FP64;
     p->max_iterations = 0;
+    p->operator_form = MPH_OPERATOR_ASSEMBLED;
 }
 
 int mph_scene_create(const mph_params* p, const mph_grid* g, int device, mph_scene** out) {
@@ -949,6 +999,8 @@ int mph_scene_create(const mph_params* p, const mph_grid* g, int device, mph_sce
     if (g->np <= 0 || !g->type || !g->pos || !g->vel) return fail(MPH_E_ARG, "empty particle set");
     if (p->precision != MPH_PRECISION_FP64 && p->precision != MPH_PRECISION_MIXED)
         return fail(MPH_E_ARG, "unknown precision mode");
+    if (p->operator_form != MPH_OPERATOR_ASSEMBLED && p->operator_form != MPH_OPERATOR_FACTORED)
+        return fail(MPH_E_ARG, "unknown operator form");
     int nf = 0;
     for (int i = 0; i < g->np; ++i) {
         if (g->type[i] < 0 || g->type[i] > 3) return fail(MPH_E_ARG, "particle type outside 0..3");
@@ -1153,6 +1205,29 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
     else if (n == "TunedCac") *v = S->tuneCac.best;
     else if (n == "TunedFuse") *v = S->tuneFuse.best;
     else if (n == "TunedGalerkin") *v = S->tuneGal.best;
+    else if (n == "FactoredBytes" || n == "FactoredBytesF32") {
+        // algorithmic bytes of one factored product: per A slot a column and a scalar,
+        // per P entry read a column and a scalar (first pass: the entries up to the
+        // last fluid one; second pass: all entries of fluid rows), the diagonal block,
+        // C_ii, lambda, the position and t once each, the counts, and the CR vectors
+        // as the assembled product (x read, y read and written: 72 B per fluid
+        // particle).  The gathers of neighbours' x, t and positions are cache reads.
+        if (!factored(*S)) return fail(MPH_E_STATE, "the scene applies the assembled operator");
+        const double vb = n == "FactoredBytes" ? 8.0 : 4.0;
+        std::vector<int> nfl, ncp, run;
+        int rc;
+        if ((rc = d2h(*S, nfl, S->nFluid.p, (size_t)P.nf)) || (rc = d2h(*S, ncp, S->nCountP.p, (size_t)P.np)) ||
+            (rc = d2h(*S, run, S->runP.p, (size_t)P.np)))
+            return rc;
+        double t = 0;
+        for (int k = 0; k < P.np; ++k) {
+            t += 4.0 + 8.0;  // run count, t written
+            if (run[k] > 0) t += (4.0 + vb) * run[k] + 8.0 + 24.0;  // entries, lambda, position
+        }
+        for (int i = 0; i < P.nf; ++i)
+            t += (4.0 + vb) * nfl[i] + (4.0 + vb) * ncp[i] + 4.0 + 4.0 + 72.0 + 24.0 + 8.0 + 72.0;
+        *v = t;
+    }
     else if (n == "SpmvLaunchesF32") *v = (double)S->spmvLaunches32;
     else if (n == "SpmvTimingError") *v = S->spmvTimingErr;
     else if (n == "ABlocks" || n == "CBlocks") {
@@ -1191,13 +1266,34 @@ int mph_scene_set_timing(mph_scene* S, int on) {
     return MPH_OK;
 }
 
+int mph_scene_apply_matrix(mph_scene* S, int precision, const double* x, double* y) {
+    if (!S || !x || !y) return fail(MPH_E_ARG, "null argument");
+    if (precision != MPH_PRECISION_FP64 && precision != MPH_PRECISION_MIXED)
+        return fail(MPH_E_ARG, "unknown precision mode");
+    if (!S->assembled) return fail(MPH_E_STATE, "no particle matrix for the current particles");
+    const bool f32 = precision == MPH_PRECISION_MIXED;
+    if (f32 && !mixed(*S)) return fail(MPH_E_STATE, "FP32 values exist in mixed-precision scenes only");
+    HIPCHK(hipSetDevice(S->device));
+    const size_t n = 3 * (size_t)S->P.nf;
+    HIPCHK(hipMemcpyAsync(S->s.p, x, n * sizeof(double), hipMemcpyHostToDevice, S->st));
+    launch_fill(S->st, S->y.p, (long long)n, 0.0);
+    spmv(*S, 1.0, S->s.p, 0.0, S->y.p, S->y.p, nullptr, nullptr, nullptr, nullptr, f32);
+    HIPCHK(hipMemcpyAsync(y, S->y.p, n * sizeof(double), hipMemcpyDeviceToHost, S->st));
+    HIPCHK(hipStreamSynchronize(S->st));
+    if (S->spmvTiming) collect_spmv_timing(*S);
+    collect_tuning(*S);
+    return MPH_OK;
+}
+
 int mph_scene_device_bytes(mph_scene* S, unsigned long long* bytes) {
     if (!S || !bytes) return fail(MPH_E_ARG, "null argument");
     unsigned long long n = 0;
     for (DBuf<int>* b : {&S->type, &S->type2, &S->perm, &S->perm2, &S->cellIndex, &S->cellStart, &S->bbox,
                          &S->nCount, &S->nFluid, &S->nCountP, &S->nInd, &S->nIndP, &S->width, &S->selfIdx, &S->acol,
-                         &S->maxP})
+                         &S->maxP, &S->runP, &S->fwrap})
         n += b->cap * sizeof(int);
+    for (DBuf<double>* b : {&S->falpha, &S->fdiag, &S->selfC, &S->pbeta, &S->frec}) n += b->cap * sizeof(double);
+    n += (S->falpha32.cap + S->pbeta32.cap) * sizeof(float);
     n += S->runStart.cap * sizeof(unsigned short);
     for (DBuf<double>* b : {&S->pos, &S->pos2, &S->vel, &S->vel2, &S->force, &S->mass, &S->kappa, &S->lambda,
                             &S->mu, &S->densA, &S->gcen, &S->pressA, &S->volStrainP, &S->divP, &S->pressP,

@@ -165,8 +165,13 @@ struct mph_scene {
     mph::host::DBuf<int> width, selfIdx, acol;
     mph::host::DBuf<long long> sliceStart;
     mph::host::DBuf<double> aval, b, cval, dval, vecP;
-    mph::host::DBuf<float> aval32;  // FP32 copy of A (mixed precision only)
+    mph::host::DBuf<float> aval32;  // FP32 copy of A (mixed precision, assembled operator only)
     long long slots = 0;
+    bool assembled = false;  // A (and the factored scalars) match the current particles
+    // factored operator (MPH_OPERATOR_FACTORED): scalars beside the neighbour lists
+    mph::host::DBuf<int> runP, fwrap;
+    mph::host::DBuf<double> falpha, fdiag, selfC, pbeta, frec;  // frec: FacOp::rec
+    mph::host::DBuf<float> falpha32, pbeta32;  // FP32 forms (mixed precision)
     // multigrid
     int hbbox[6] = {0, 0, 0, 0, 0, 0};
     int depth = 0;
@@ -204,6 +209,8 @@ namespace mph {
 namespace host {
 
 bool mixed(const mph_scene& S);
+bool factored(const mph_scene& S);
+mph::FacOp fac_op(mph_scene& S, bool f32);  // the factored form's pointers (FP64 or FP32 scalars)
 ASlices aslices(mph_scene& S);
 int run_phase(mph_scene& S, int ph);
 // calculateNeighbor in two halves around the host readback of the list totals:

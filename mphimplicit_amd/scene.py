@@ -10,6 +10,7 @@ import os
 
 import numpy as np
 
+from . import abi
 from .abi import Grid, Params, StepStats
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -54,6 +55,7 @@ def lib():
         L.mph_scene_set_timing.argtypes = [vp, i]
         L.mph_scene_destroy.argtypes = [vp]
         L.mph_scene_device_bytes.argtypes = [vp, C.POINTER(C.c_ulonglong)]
+        L.mph_scene_apply_matrix.argtypes = [vp, i, C.POINTER(d), C.POINTER(d)]
         L.mph_batch_create.argtypes = [i, C.POINTER(vp)]
         L.mph_batch_add.argtypes = [vp, vp]
         L.mph_batch_remove.argtypes = [vp, vp]
@@ -187,6 +189,15 @@ class Scene:
         v = C.c_ulonglong()
         _check(lib().mph_scene_device_bytes(self.h, C.byref(v)))
         return v.value
+
+    def apply_matrix(self, x: np.ndarray, precision: int = abi.PRECISION_FP64) -> np.ndarray:
+        """y = A x with the last assembled particle matrix, in the scene's operator
+        form (mph_scene_apply_matrix); x has 3*FluidCount entries."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        _check(lib().mph_scene_apply_matrix(self.h, int(precision), x.ctypes.data_as(C.POINTER(C.c_double)),
+                                            y.ctypes.data_as(C.POINTER(C.c_double))))
+        return y
 
 
 def probe_bandwidth(device: int = 0, nbytes: int = 8 << 30, reps: int = 5) -> tuple[float, float]:

@@ -124,6 +124,20 @@ def config_3d_drop(n=40, box=80, lift=16, seed=1, steps=100, mu=0.1):
     return p, g
 
 
+def periodic_3d(n=12, depth=6, seed=9, steps=5, mu=0.1):
+    """A fluid layer of n x n x depth particles on a 4-layer floor, filling the
+    domain in x and y: the domain is periodic there, so particles at opposite
+    sides are neighbours through the boundary (test scene)."""
+    l0 = 0.01
+    fluid = lattice([0, 0, 4], [n, n, 4 + depth], l0, 3)
+    walls = lattice([0, 0, 0], [n, n, 4], l0, 3)
+    dmin = [0.0, 0.0, -0.05]
+    dmax = [n * l0, n * l0, (4 + depth + 10) * l0]
+    g = assemble(fluid, walls, 0.0, l0, dmin, dmax, seed, 3)
+    p = base_params(3, mu, (0.0, 0.0, -9.8), 5.0e-4, steps)
+    return p, g
+
+
 def config_3d_slab(n=100, inner=140, seed=1, steps=50, mu=1.0, amplitude=0.1):
     """Config 3: n^3 fluid resting on the floor of a closed 4-layer box with an
     inner^3 interior, seeded Taylor-Green initial velocity of the given amplitude."""

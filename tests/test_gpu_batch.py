@@ -5,8 +5,10 @@ Scenes of different dimensions, sizes, multigrid depths and physics (surface
 tension, two fluids) are advanced together; the same inputs are advanced alone
 with mph_scene_step; every step's CR iteration count and residual scalars and the
 final fields must be identical.  Members that stop early (end_time), leave the
-batch or join later must not change the others.  The single-scene path itself is
-pinned to the oracle by test_gpu_parity.py.
+batch or join later must not change the others.  Scenes of the factored operator
+form share a batch with assembled ones (the products split into one launch set per
+form).  The single-scene path itself is pinned to the oracle by test_gpu_parity.py
+and test_gpu_factored.py.
 """
 import numpy as np
 import pytest
@@ -21,10 +23,11 @@ FIELDS = ["Position", "Velocity", "ParticleIndex", "NeighborInd", "CsrCofA", "Ve
 MIX = ["tiny2d", "2d_40x40", "3d_4096", "cube3d_sample", "bridge2d_sample", "glued2d_sample", "tiny3d"]
 
 
-def make(name, precision, tol=1e-12, end_steps=None):
+def make(name, precision, tol=1e-12, end_steps=None, form=abi.OPERATOR_ASSEMBLED):
     p, g = SCENES[name]()
     p.cr_tolerance = tol
     p.precision = precision
+    p.operator_form = form
     if end_steps is not None:
         p.end_time = g.time + (end_steps - 1) * p.dt
     else:
@@ -32,8 +35,8 @@ def make(name, precision, tol=1e-12, end_steps=None):
     return p, g
 
 
-def alone(name, precision, steps, end_steps=None):
-    p, g = make(name, precision, end_steps=end_steps)
+def alone(name, precision, steps, end_steps=None, form=abi.OPERATOR_ASSEMBLED):
+    p, g = make(name, precision, end_steps=end_steps, form=form)
     s = Scene(p, g)
     stats = [s.step(1) for _ in range(steps)]
     return s, stats
@@ -45,18 +48,27 @@ def same(a, b):
         assert x.shape == y.shape and np.array_equal(x.view(np.uint8), y.view(np.uint8)), n
 
 
+def forms_of(kind):
+    """Operator form per member: all assembled, all factored, or alternating (a
+    batch whose launches split between the two forms)."""
+    A, F = abi.OPERATOR_ASSEMBLED, abi.OPERATOR_FACTORED
+    return [A if kind == "assembled" else F if kind == "factored" else (A, F)[k % 2] for k in range(len(MIX))]
+
+
+@pytest.mark.parametrize("kind", ["assembled", "alternating"])
 @pytest.mark.parametrize("precision", [abi.PRECISION_FP64, abi.PRECISION_MIXED])
-def test_batch_equals_alone(precision):
+def test_batch_equals_alone(precision, kind):
     steps = 3
-    print("[batch] precision %d, %d scenes" % (precision, len(MIX)), flush=True)
-    members = [Scene(*make(n, precision)) for n in MIX]
+    forms = forms_of(kind)
+    print("[batch] precision %d, %d scenes, %s operator forms" % (precision, len(MIX), kind), flush=True)
+    members = [Scene(*make(n, precision, form=f)) for n, f in zip(MIX, forms)]
     with Batch() as b:
         for s in members:
             b.add(s)
         assert len(b) == len(MIX)
         hist = [b.step() for _ in range(steps)]
     for k, name in enumerate(MIX):
-        ref, rstats = alone(name, precision, steps)
+        ref, rstats = alone(name, precision, steps, form=forms[k])
         for t in range(steps):
             a, r = hist[t][k], rstats[t]
             assert a.steps_done == 1

@@ -61,6 +61,8 @@ SCENES = {
     # ~400 fluid neighbours per particle: rows longer than the Galerkin LDS staging
     "3d_4096_wideV": _with(lambda: scenes.config_3d_drop(n=16, box=30, lift=4, steps=30), radius_ratio_a=4.6,
                            radius_ratio_v=4.6),
+    # fluid filling the domain in x and y: neighbours through the periodic boundary
+    "periodic3d": lambda: scenes.periodic_3d(),
 }
 
 

@@ -242,6 +242,113 @@ def test_implicit_system_identities(maker):
     assert np.allclose(A, A.T, rtol=1e-9, atol=1e-11 * scale)
 
 
+def factored_pieces(o, p, g):
+    """The factored form's scalars from positions and neighbour lists alone:
+    alpha_ij = visc mu_ij (-w_v'(d))/d^3, beta_ij = -w_p'(d)/d, the diagonal block
+    D_i = sum_j alpha_ij r r^T + m_i/(V dt) I and C_ii = sum_k beta_ik r_ik."""
+    dim, l0 = p.dim, g.spacing
+    nf, np_ = int(o.scalar("FluidCount")), g.np
+    x = o.read("Position").reshape(-1, 3)
+    mu, m = o.read("Mu"), o.read("Mass")
+    W = np.array(g.domain_max) - np.array(g.domain_min)
+    hV, hP = p.radius_ratio_v * l0, p.radius_ratio_p * l0
+
+    def disp(i, j):
+        return np.mod(x[j] - x[i] + 0.5 * W, W) - 0.5 * W
+
+    def wdr(d, h, sw):  # -w'(d) of the (1 - r/h)^2 kernel with normalisation sw
+        return 2.0 / (sw * h ** dim * h) * (1.0 - d / h)
+
+    swv, swp = o.scalar("Swv"), o.scalar("Swp")
+    visc = 10.0 if dim == 3 else 8.0
+    ptr, cnt, ind = o.read("NeighborPtr"), o.read("NeighborCount"), o.read("NeighborInd")
+    nfl = o.read("NeighborFluidCount")
+    alpha, D = {}, np.zeros((nf, 3, 3))
+    for i in range(nf):
+        for jn in range(cnt[i]):
+            j = ind[ptr[i] + jn]
+            r = disp(i, j)
+            d = math.sqrt(r @ r)
+            a = 0.0
+            if j != i and d != 0.0 and d < hV:
+                muij = 2.0 * mu[i] * mu[j] / (mu[i] + mu[j])
+                a = visc * muij * wdr(d, hV, swv) / d ** 3
+                D[i] += a * np.outer(r, r)
+            if jn < nfl[i]:
+                alpha[(i, j)] = a
+        D[i] += m[i] / l0 ** dim / p.dt * np.eye(3)
+    ptrP, cntP, indP = o.read("NeighborPtrP"), o.read("NeighborCountP"), o.read("NeighborIndP")
+    beta, selfC = {}, np.zeros((nf, 3))
+    for k in range(np_):
+        for jn in range(cntP[k]):
+            j = indP[ptrP[k] + jn]
+            r = disp(k, j)
+            d = math.sqrt(r @ r)
+            b = wdr(d, hP, swp) / d if (j != k and d != 0.0 and d < hP) else 0.0
+            beta[(k, j)] = b
+            if k < nf:
+                selfC[k] += b * r
+    return nf, np_, disp, alpha, D, beta, selfC
+
+
+@pytest.mark.parametrize("maker", [scenes.tiny_2d, scenes.tiny_3d, lambda: scenes.periodic_3d(depth=3)])
+def test_factored_form_identity(maker):
+    """The particle matrix rebuilt from the factored form's scalars equals the
+    assembled A (calculateMatrixA + multiplyMatrixC), and the two-pass product
+    t_k = lambda_k sum_j C_jk . x_j, y_i = D_i x_i - sum_j alpha_ij r (r . x_j) +
+    sum_k C_ik t_k equals A x.  A sign, radius or index mistake in the derivation
+    fails one of the two comparisons."""
+    p, g = maker()
+    o = oracle(p, g)
+    for ph in (abi.PHASE_KINEMATICS, abi.PHASE_CELLS, abi.PHASE_NEIGHBOR, abi.PHASE_EXPLICIT, abi.PHASE_MATRIX_A,
+               abi.PHASE_MATRIX_C, abi.PHASE_MULTIPLY_C):
+        o.run_phase(ph)
+    nf, np_, disp, alpha, D, beta, selfC = factored_pieces(o, p, g)
+    lam = o.read("Lambda")
+    A, _ = dense_system(o, nf, np_)
+    # C from beta (rows: fluid particles, columns: every particle)
+    C = np.zeros((nf, np_, 3))
+    for (k, j), b in beta.items():
+        if k < nf and b != 0.0:
+            C[k, j] = b * disp(k, j)
+    C[np.arange(nf), np.arange(nf)] = selfC
+    Af = np.zeros((nf, 3, nf, 3))
+    for (i, j), a in alpha.items():
+        if j != i:
+            r = disp(i, j)
+            Af[i, :, j, :] -= a * np.outer(r, r)
+    Af[np.arange(nf), :, np.arange(nf), :] += D
+    Af += np.einsum("ikr,k,jks->irjs", C, lam, C)
+    Af = Af.reshape(3 * nf, 3 * nf)
+    scale = np.abs(A).max()
+    assert np.abs(A - Af).max() <= 1e-12 * scale
+    # the two passes, over k's own P list with C_jk = -beta_kj r_kj
+    rng = np.random.default_rng(7)
+    xv = rng.standard_normal((nf, 3))
+    ptrP, cntP, indP = o.read("NeighborPtrP"), o.read("NeighborCountP"), o.read("NeighborIndP")
+    t = np.zeros(np_)
+    for k in range(np_):
+        acc = selfC[k] @ xv[k] if k < nf else 0.0
+        for jn in range(cntP[k]):
+            j = indP[ptrP[k] + jn]
+            if j < nf and j != k:
+                acc -= beta[(k, j)] * (disp(k, j) @ xv[j])
+        t[k] = lam[k] * acc
+    y = np.einsum("irs,is->ir", D, xv)
+    for (i, j), a in alpha.items():
+        if j != i:
+            r = disp(i, j)
+            y[i] -= a * r * (r @ xv[j])
+    y += selfC * t[:nf, None]
+    for i in range(nf):
+        for jn in range(cntP[i]):
+            k = indP[ptrP[i] + jn]
+            if k != i:
+                y[i] += beta[(i, k)] * disp(i, k) * t[k]
+    ref = A @ xv.ravel()
+    assert np.abs(y.ravel() - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 def level_dense(L_vals, cnt, nshift):
     """Dense level matrix from [row][shift] values and the wrapped stencil."""
     cs = [min(3, c) for c in cnt]
