@@ -9,10 +9,13 @@ conjugate residual in FP64.  A step is one pass of the whole time step (kinemati
 cell sort, neighbour search, explicit forces, assembly, multigrid build, solve).
 
 Prints ONE JSON line (see the fields below).  ``value`` is fluid-particle-steps/s
-with all state resident in HBM.  ``roofline`` is measured live on the dominant
-kernel (the particle-matrix SpMV of the CR loop) with HIP events on the scene's
-stream.  ``cpu_baseline`` is the CPU oracle (OpenMP) timed on a bounded sample of
-the same scene on this host.
+with all state resident in HBM, with the particle matrix applied in the form that
+--operator names (default auto: the factored form, the same matrix from one scalar
+per neighbour entry, for FP64 legs, and the assembled blocks for mixed-precision
+legs; ``fp64_assembled`` reports the bit-exact assembled FP64 form beside it).
+``roofline`` is measured live on the dominant kernel (the particle-matrix product
+of the CR loop) with HIP events on the scene's stream.  ``cpu_baseline`` is the CPU
+oracle (OpenMP) timed on a bounded sample of the same scene on this host.
 """
 from __future__ import annotations
 
@@ -29,7 +32,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s, 1M 3D to 1e-8 res; mixed/batched vs plain FP64; p99 step"
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spmv_traffic_3d_1m.json")
+TRAFFIC_FILE = {"assembled": os.path.join(ROOT, "profiles", "spmv_traffic_3d_1m.json"),
+                "factored": os.path.join(ROOT, "profiles", "fac_traffic_3d_1m.json")}
 
 
 def spmv_bytes(blocks: float, nf: int, value_bytes: int = 8) -> tuple[float, float]:
@@ -66,6 +70,16 @@ def roofline(kernel, ms, launches, bytes_sum, ref_sum, traffic):
     }
 
 
+def form_for(args, precision):
+    """Operator form of a leg: --operator, or with "auto" the faster form per
+    precision on this GPU (factored for FP64; assembled for mixed precision, whose
+    FP32 block copy halves the assembled bytes while the factored product stays bound
+    by its FP64 record gathers)."""
+    if args.operator != "auto":
+        return args.operator
+    return "factored" if precision == "fp64" else "assembled"
+
+
 def run_gpu(args, precision="fp64", form=None):
     """Create the scene, run warmup + timed steps; return timings, the SpMV roofline
     and the fields used for the precision comparison (Velocity and PressureP after
@@ -77,7 +91,7 @@ def run_gpu(args, precision="fp64", form=None):
     params.cr_tolerance = args.rel_residual ** 2
     params.end_time = 1.0e30
     params.precision = abi.PRECISION_MIXED if precision == "mixed" else abi.PRECISION_FP64
-    form = form or args.operator
+    form = form or form_for(args, precision)
     factored = form == "factored"
     params.operator_form = abi.OPERATOR_FACTORED if factored else abi.OPERATOR_ASSEMBLED
     nf = int((grid.type < 2).sum())
@@ -123,8 +137,8 @@ def run_gpu(args, precision="fp64", form=None):
           % (scene.scalar("SpmvMs"), l64_prev, scene.scalar("SpmvMsF32"), l32_prev,
              scene.scalar("SpmvTimingError")), file=sys.stderr, flush=True)
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
+    if os.path.exists(TRAFFIC_FILE[form]):
+        with open(TRAFFIC_FILE[form]) as f:
             traffic = json.load(f).get("bytes_per_launch")
     if factored:
         k64 = "k_fac_t + k_fac_y (factored particle matrix, FP64 scalars)"
@@ -132,8 +146,11 @@ def run_gpu(args, precision="fp64", form=None):
     else:
         k64 = "k_spmv (particle matrix, sliced 3x3 blocks, FP64 values)"
         k32 = "k_spmv (particle matrix, sliced 3x3 blocks, FP32 values)"
-    rf64 = roofline(k64, scene.scalar("SpmvMs"), l64_prev, b64, r64,
-                    traffic if precision == "fp64" and not factored else None)
+    rf64 = roofline(k64, scene.scalar("SpmvMs"), l64_prev, b64, r64, traffic if precision == "fp64" else None)
+    if factored:
+        rf64["note"] = ("the factored product moves ~2.8 GB of algorithmic HBM bytes where the assembled one moves "
+                        "13.2 GB; its limit is the cache traffic of gathering neighbours' 64-byte records (x, "
+                        "position, t) through L2 and the Infinity Cache, not HBM")
     rf32 = roofline(k32, scene.scalar("SpmvMsF32"), l32_prev, b32, r32, None)
     tuned = {"spmv_fp64_variant": int(scene.scalar("TunedSpmv64")), "spmv_fp32_variant": int(scene.scalar("TunedSpmv32")),
              "cac_variant": int(scene.scalar("TunedCac")), "fuse_variant": int(scene.scalar("TunedFuse")),
@@ -213,12 +230,17 @@ def run_batch(args):
     nf = [int((g.type < 2).sum()) for _, g in sweep]
     npart = [int(g.np) for _, g in sweep]
 
+    def form_of(precision):
+        f = form_for(args, "mixed" if precision == abi.PRECISION_MIXED else "fp64")
+        return abi.OPERATOR_FACTORED if f == "factored" else abi.OPERATOR_ASSEMBLED
+
     def mk(precision):
         out = []
         for p, g in sweep:
             q = p.copy()
             q.precision = precision
             q.cr_tolerance = args.rel_residual ** 2
+            q.operator_form = form_of(precision)
             out.append(Scene(q, g))
         return out
 
@@ -227,6 +249,7 @@ def run_batch(args):
     for k, (p, g) in enumerate(sweep):
         q = p.copy()
         q.cr_tolerance = args.rel_residual ** 2
+        q.operator_form = form_of(abi.PRECISION_FP64)
         s = Scene(q, g)
         t = time.perf_counter()
         for _ in range(args.batch_steps):
@@ -252,8 +275,10 @@ def run_batch(args):
     work = sum(nf) * args.batch_steps
     v_seq = work / t_seq
     out = {
-        "workload": "%d seeded scenes (2D/3D, %d..%d fluid particles, %d fluid / %d total), %d steps each"
-                    % (args.batch_count, min(nf), max(nf), sum(nf), sum(npart), args.batch_steps),
+        "workload": "%d seeded scenes (2D/3D, %d..%d fluid particles, %d fluid / %d total), %d steps each; "
+                    "particle matrix %s in FP64, %s in mixed precision"
+                    % (args.batch_count, min(nf), max(nf), sum(nf), sum(npart), args.batch_steps,
+                       form_for(args, "fp64"), form_for(args, "mixed")),
         "unit": "fluid-particle-steps/s",
         "sequential_fp64": round(v_seq, 1),
     }
@@ -269,13 +294,14 @@ def run_batch(args):
 def run_serve(args, rate=None):
     """Serving stream: Poisson arrivals of scene-step requests against the wall
     clock for a bounded time, continuous batching over one resident batch."""
-    from mphimplicit_amd import scenes
+    from mphimplicit_amd import abi, scenes
     from mphimplicit_amd.serve import Server, make_requests
 
     rate = args.serve_rate if rate is None else rate
     count = min(args.serve_requests, int(rate * args.serve_seconds * 1.5) + 10)  # enough to outlast the run
     stream = scenes.serve_stream(count, rate=rate)
-    reqs = make_requests(stream, tol=args.rel_residual ** 2)
+    form = abi.OPERATOR_FACTORED if form_for(args, "mixed") == "factored" else abi.OPERATOR_ASSEMBLED
+    reqs = make_requests(stream, tol=args.rel_residual ** 2, operator_form=form)
     srv = Server(lazy_mixed=True)
     try:
         r = srv.run(reqs, duration_s=args.serve_seconds,
@@ -283,7 +309,8 @@ def run_serve(args, rate=None):
     finally:
         srv.close()
     r["workload"] = ("Poisson arrivals at %g req/s, 1-20 steps, lognormal fluid counts (median 50k, sigma 1, "
-                     "clipped 5k-500k), 2D/3D, mixed precision, %g s run" % (rate, args.serve_seconds))
+                     "clipped 5k-500k), 2D/3D, mixed precision, %s particle matrix, %g s run"
+                     % (rate, form_for(args, "mixed"), args.serve_seconds))
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
 
 
@@ -307,8 +334,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="3d_1m")
     ap.add_argument("--rel-residual", type=float, default=1.0e-8, help="CR stop: sqrt(rr/rr0) below this")
-    ap.add_argument("--operator", choices=["assembled", "factored"], default="assembled",
-                    help="how the solve applies the particle matrix (mph_params.operator_form)")
+    ap.add_argument("--operator", choices=["auto", "assembled", "factored"], default="auto",
+                    help="how the solve applies the particle matrix (mph_params.operator_form); auto: "
+                         "factored for FP64 legs, assembled for mixed-precision legs")
+    ap.add_argument("--assembled-leg", choices=["on", "off"], default="on",
+                    help="when the FP64 leg is factored: also run plain FP64 with the assembled matrix (the "
+                         "bit-exact form)")
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--mixed", choices=["on", "off"], default="on",
@@ -357,6 +388,7 @@ def main():
             "total_particles": g["np"],
             "rel_residual": args.rel_residual,
             "precision": "fp64",
+            "operator": form_for(args, "fp64"),
         },
         "total_particle_steps_per_s": round(g["np"] * args.steps / g["total"], 1),
         "p50_step_ms": round(float(np.percentile(lat_ms, 50)), 3),
@@ -370,6 +402,25 @@ def main():
                         "probe": "mph_probe_bandwidth: best of 5 passes over 8 GiB; read with 16-byte nontemporal loads, copy with 16-byte loads and stores"},
         "cpu_baseline": None,
     }
+    if form_for(args, "fp64") == "factored" and args.assembled_leg == "on":
+        a = run_gpu(args, "fp64", "assembled")
+        alat = np.array(a["lat"]) * 1e3
+        av = a["nf"] * args.steps / a["total"]
+        out["fp64_assembled"] = {
+            "value": round(av, 1),
+            "unit": "fluid-particle-steps/s",
+            "ms_per_step": round(float(alat.mean()), 3),
+            "p50_step_ms": round(float(np.percentile(alat, 50)), 3),
+            "p99_step_ms": round(float(np.percentile(alat, 99)), 3),
+            "cr_iterations": a["iters"],
+            "arithmetic": "the reference's CSR product on the stored 3x3 blocks, bit-identical to the reference",
+            "factored_speedup": round(out["value"] / av, 3),
+            "roofline": a["roofline"],
+            "rel_error_factored_vs_assembled_after_first_step": field_error(g["first"], a["first"]),
+            "rel_error_factored_vs_assembled_after_last_step": field_error(g["last"], a["last"]),
+        }
+        if a["roofline"]["achieved"]:
+            out["hbm_ceiling"]["assembled_spmv_frac_of_read"] = round(a["roofline"]["achieved"] / read_gbs, 4)
     m = None
     if args.mixed == "on":
         m = run_gpu(args, "mixed")
@@ -384,8 +435,9 @@ def main():
             "cr_iterations_inner": m["iters"],
             "refinement_steps": m["refine"],
             "speedup_vs_plain_fp64": round(mv / out["value"], 3),
-            "arithmetic": "FP32 particle matrix and multigrid stencils (coarsest level FP64), FP64 vectors, dots and outer residual; "
-                          "same stop rule ||b - A x||/||b|| < %g with the FP64 matrix" % args.rel_residual,
+            "arithmetic": "FP32 particle-matrix values (%s form) and multigrid stencils (coarsest level FP64), FP64 "
+                          "vectors, dots and outer residual; same stop rule ||b - A x||/||b|| < %g with the FP64 "
+                          "matrix" % (form_for(args, "mixed"), args.rel_residual),
             "roofline_fp32_spmv": m["roofline_f32"],
             "roofline_fp64_residual_spmv": m["roofline"],
             "rel_error_vs_fp64_gpu_after_first_step": field_error(m["first"], g["first"]),

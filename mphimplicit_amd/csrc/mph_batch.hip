@@ -85,8 +85,9 @@ template <class VT>
 __global__ void __launch_bounds__(256) kb_fac_t(const BScene* tab, const int* off, int ns, SceneSet on, int x) {
     MPH_SCENE(kOffPart)
     const FacOp& F = B.fac[sizeof(VT) == 4 ? 1 : 0];
-    if (*F.wrap) fac_t_chunk<VT, true>(B.P, F, B.vec[x], lb);
-    else fac_t_chunk<VT, false>(B.P, F, B.vec[x], lb);
+    if constexpr (sizeof(VT) == 4) fac_t32_chunk(B.P, F, B.vec[x], lb);
+    else if (*F.wrap) fac_t_chunk<true>(B.P, F, B.vec[x], lb);
+    else fac_t_chunk<false>(B.P, F, B.vec[x], lb);
 }
 
 template <bool kNeg, class VT>
@@ -102,7 +103,10 @@ __global__ void __launch_bounds__(256) kb_fac_y(const BScene* tab, const int* of
     double* pc = f && p_copy >= 0 ? B.vec[p_copy] : nullptr;
     const double* dx = dot >= 0 ? B.vec[dot] : nullptr;
     double* pt = dot >= 0 ? B.part : nullptr;
-    if (*F.wrap) fac_y_chunk<kNeg, VT, true>(B.P, F, B.vec[x], beta, B.vec[yin], B.vec[y], y2p, pc, dx, pt, lb, sh, rows);
+    if constexpr (sizeof(VT) == 4)
+        fac_y32_chunk<kNeg>(B.P, F, B.vec[x], beta, B.vec[yin], B.vec[y], y2p, pc, dx, pt, lb, sh, rows);
+    else if (*F.wrap)
+        fac_y_chunk<kNeg, VT, true>(B.P, F, B.vec[x], beta, B.vec[yin], B.vec[y], y2p, pc, dx, pt, lb, sh, rows);
     else fac_y_chunk<kNeg, VT, false>(B.P, F, B.vec[x], beta, B.vec[yin], B.vec[y], y2p, pc, dx, pt, lb, sh, rows);
 }
 

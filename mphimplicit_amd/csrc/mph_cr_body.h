@@ -537,12 +537,13 @@ constexpr int kFacLanes = 8, kFacBatch = 8, kFacBatchP = 4;
 // first pass over the 256 particles of `chunk` (fluid or wall):
 //   t_k = lambda_k (C_kk . x_k [k fluid] - sum_{j fluid, j != k} beta_kj r_kj . x_j),
 // written to k's record together with x_k (fluid k).  Every thread calls it.
-template <class VT, bool kWrap>
+// (Streaming a stored v = beta r instead of gathering positions measured slower:
+// as many load instructions, more bytes.)
+template <bool kWrap>
 __device__ inline void fac_t_chunk(const DevParams& P, const FacOp& F, const double* __restrict__ x, int chunk) {
 #pragma clang fp contract(fast)
     constexpr int G = kFacLanes, U = kFacBatchP;
     const int g = threadIdx.x & (G - 1);
-    const VT* __restrict__ beta = (const VT*)F.beta;
     const double W0 = P.W[0], W1 = P.W[1], W2 = P.W[2], h0 = P.hW[0], h1 = P.hW[1], h2 = P.hW[2];
     for (int r = threadIdx.x / G; r < 256; r += 256 / G) {
         const int k = chunk * 256 + r;
@@ -560,7 +561,7 @@ __device__ inline void fac_t_chunk(const DevParams& P, const FacOp& F, const dou
                     const int j = jn < run ? F.nIndP[c0 + jn] : k;
                     const bool use = jn < run && j < P.nf && j != k;
                     jx[u] = use ? j : 0;  // unused entries read particle 0 with weight 0
-                    bb[u] = use ? (double)beta[c0 + jn] : 0.0;
+                    bb[u] = use ? F.beta[c0 + jn] : 0.0;
                 }
                 double v[U][6];
 #pragma unroll
@@ -612,7 +613,6 @@ __device__ inline void fac_y_chunk(const DevParams& P, const FacOp& F, const dou
     const int g = threadIdx.x & (G - 1);
     const dv2* __restrict__ rec = (const dv2*)F.rec;  // 4 x 16 B per particle
     const VT* __restrict__ alpha = (const VT*)F.alpha;
-    const VT* __restrict__ bet = (const VT*)F.beta;
     const double W0 = P.W[0], W1 = P.W[1], W2 = P.W[2], h0 = P.hW[0], h1 = P.hW[1], h2 = P.hW[2];
     for (int r = threadIdx.x / G; r < 256; r += 256 / G) {
         const int i = chunk * 256 + r;
@@ -662,7 +662,7 @@ __device__ inline void fac_y_chunk(const DevParams& P, const FacOp& F, const dou
                 for (int u = 0; u < U; ++u) {
                     const int jn = jb + g + u * G;
                     k[u] = jn < n ? F.nIndP[c0 + jn] : i;
-                    bb[u] = jn < n ? (double)bet[c0 + jn] : 0.0;
+                    bb[u] = jn < n ? F.beta[c0 + jn] : 0.0;
                 }
                 dv2 q1[U], q2[U], q3[U];
 #pragma unroll
@@ -690,6 +690,190 @@ __device__ inline void fac_y_chunk(const DevParams& P, const FacOp& F, const dou
             const double* D = F.diag + 9 * i;
             const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
             const double ti = F.rec[8 * (long long)i + 6];
+            double d0 = (D[0] * xi0 + D[1] * xi1) + D[2] * xi2;
+            double d1 = (D[3] * xi0 + D[4] * xi1) + D[5] * xi2;
+            double d2 = (D[6] * xi0 + D[7] * xi1) + D[8] * xi2;
+            d0 += F.selfC[3 * i] * ti;
+            d1 += F.selfC[3 * i + 1] * ti;
+            d2 += F.selfC[3 * i + 2] * ti;
+            s0 += d0;
+            s1 += d1;
+            s2 += d2;
+            rows[3 * r] = kNeg ? -s0 : s0;
+            rows[3 * r + 1] = kNeg ? -s1 : s1;
+            rows[3 * r + 2] = kNeg ? -s2 : s2;
+        }
+    }
+    __syncthreads();
+    const int i = chunk * 256 + threadIdx.x;
+    const int t = threadIdx.x;
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+    if (i < P.nf) {
+        r0 = rows[3 * t];
+        r1 = rows[3 * t + 1];
+        r2 = rows[3 * t + 2];
+    }
+    spmv_store(P.nf, i, r0, r1, r2, x, beta, yin, y, y2, p_copy, dotx, part, chunk, sh);
+}
+
+// FP32 form of the two passes (mixed-precision inner solves): the displacements are
+// folded into the stored vectors (v_kj = beta_kj r_kj, u_ij = sqrt(alpha_ij) r_ij),
+// so a pass gathers only a neighbour's 16-byte record (x, t as floats).  Arithmetic
+// is FP64 on the FP32-stored values.
+__device__ inline void fac_t32_chunk(const DevParams& P, const FacOp& F, const double* __restrict__ x, int chunk) {
+#pragma clang fp contract(fast)
+    constexpr int G = kFacLanes, U = kFacBatchP;
+    typedef float fv4 __attribute__((ext_vector_type(4)));
+    const int g = threadIdx.x & (G - 1);
+    for (int r = threadIdx.x / G; r < 256; r += 256 / G) {
+        const int k = chunk * 256 + r;
+        double acc = 0.0;
+        const int run = k < P.np ? F.runP[k] : 0;
+        if (run > 0) {
+            const long long c0 = F.nPtrP[k];
+            for (int jb = 0; jb < run; jb += G * U) {
+                int jx[U];
+                float v[U][3];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int jn = jb + g + u * G;
+                    const int j = jn < run ? F.nIndP[c0 + jn] : k;
+                    const bool use = jn < run && j < P.nf && j != k;
+                    jx[u] = use ? j : 0;  // unused entries read particle 0 with weight 0
+                    const long long e = 3 * (c0 + (use ? jn : 0));
+                    v[u][0] = use ? F.v32[e] : 0.0f;
+                    v[u][1] = use ? F.v32[e + 1] : 0.0f;
+                    v[u][2] = use ? F.v32[e + 2] : 0.0f;
+                }
+                double xv[U][3];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const long long j3 = 3 * (long long)jx[u];
+                    xv[u][0] = x[j3];
+                    xv[u][1] = x[j3 + 1];
+                    xv[u][2] = x[j3 + 2];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    acc -= ((double)v[u][0] * xv[u][0] + (double)v[u][1] * xv[u][1]) + (double)v[u][2] * xv[u][2];
+            }
+        }
+        acc = group_sum(acc, G);
+        if (g == 0 && k < P.np) {
+            fv4 rk = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (k < P.nf) {
+                const double x0 = x[3 * k], x1 = x[3 * k + 1], x2 = x[3 * k + 2];
+                acc += (F.selfC[3 * k] * x0 + F.selfC[3 * k + 1] * x1) + F.selfC[3 * k + 2] * x2;
+                rk[0] = (float)x0;
+                rk[1] = (float)x1;
+                rk[2] = (float)x2;
+            }
+            rk[3] = run > 0 ? (float)(acc * F.lambda[k]) : 0.0f;
+            ((fv4*)F.rec32)[k] = rk;
+        }
+    }
+}
+
+template <bool kNeg>
+__device__ inline void fac_y32_chunk(const DevParams& P, const FacOp& F, const double* __restrict__ x, double beta,
+                                     const double* yin, double* y, double* y2, double* p_copy,
+                                     const double* __restrict__ dotx, double* __restrict__ part, int chunk,
+                                     double* sh, double* rows) {
+#pragma clang fp contract(fast)
+    typedef float fv4 __attribute__((ext_vector_type(4)));
+    constexpr int G = kFacLanes, U = kFacBatch;
+    const int g = threadIdx.x & (G - 1);
+    const fv4* __restrict__ rec = (const fv4*)F.rec32;
+    // the viscous loop is software-pipelined: the columns and u of the next batch
+    // (or of the next row's first batch) load while the current batch's records are
+    // gathered, so the two dependent round trips of a batch overlap
+    struct Batch {
+        int j[U];
+        float a[U][3];
+    };
+    auto vload = [&](int i, int w, long long b0, int jb, Batch& b) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int jn = jb + g + u * G;
+            const bool in = jn < w;
+            b.j[u] = in ? F.nInd[b0 + jn] : i;  // unused entries: i's own record, u = 0
+            const long long e = 3 * (b0 + (in ? jn : 0));
+            b.a[u][0] = in ? F.u32[e] : 0.0f;
+            b.a[u][1] = in ? F.u32[e + 1] : 0.0f;
+            b.a[u][2] = in ? F.u32[e + 2] : 0.0f;
+        }
+    };
+    Batch nb;
+    {
+        const int i0 = chunk * 256 + threadIdx.x / G;
+        if (i0 < P.nf) vload(i0, F.nFluid[i0], F.nPtr[i0], 0, nb);
+    }
+    for (int r = threadIdx.x / G; r < 256; r += 256 / G) {
+        const int i = chunk * 256 + r;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        if (i < P.nf) {
+            // viscous part: -u (u . x_j)
+            const long long b0 = F.nPtr[i];
+            const int w = F.nFluid[i];
+            Batch cb = nb;
+            fv4 q[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) q[u] = rec[cb.j[u]];
+            for (int jb = 0; jb < w; jb += G * U) {
+                const bool more = jb + G * U < w;
+                const int rn = r + 256 / G, in_ = chunk * 256 + rn;
+                if (more) vload(i, w, b0, jb + G * U, nb);
+                else if (rn < 256 && in_ < P.nf) vload(in_, F.nFluid[in_], F.nPtr[in_], 0, nb);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const double a0 = cb.a[u][0], a1 = cb.a[u][1], a2 = cb.a[u][2];
+                    const double f = (a0 * (double)q[u][0] + a1 * (double)q[u][1]) + a2 * (double)q[u][2];
+                    s0 -= f * a0;
+                    s1 -= f * a1;
+                    s2 -= f * a2;
+                }
+                if (more) {
+                    cb = nb;
+#pragma unroll
+                    for (int u = 0; u < U; ++u) q[u] = rec[cb.j[u]];
+                }
+            }
+            // C Lambda C^T part: v_ik t_k
+            const long long c0 = F.nPtrP[i];
+            const int n = F.nCountP[i];
+            for (int jb = 0; jb < n; jb += G * kFacBatchP) {
+                constexpr int U = kFacBatchP;
+                int k[U];
+                float vv[U][3];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int jn = jb + g + u * G;
+                    const bool in = jn < n;
+                    k[u] = in ? F.nIndP[c0 + jn] : i;
+                    const long long e = 3 * (c0 + (in ? jn : 0));
+                    vv[u][0] = in ? F.v32[e] : 0.0f;
+                    vv[u][1] = in ? F.v32[e + 1] : 0.0f;
+                    vv[u][2] = in ? F.v32[e + 2] : 0.0f;
+                }
+                fv4 q[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) q[u] = rec[k[u]];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const double t = q[u][3];
+                    s0 += (double)vv[u][0] * t;
+                    s1 += (double)vv[u][1] * t;
+                    s2 += (double)vv[u][2] * t;
+                }
+            }
+        }
+        s0 = group_sum(s0, G);
+        s1 = group_sum(s1, G);
+        s2 = group_sum(s2, G);
+        if (g == 0 && i < P.nf) {
+            const double* D = F.diag + 9 * i;
+            const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
+            const double ti = rec[i][3];
             double d0 = (D[0] * xi0 + D[1] * xi1) + D[2] * xi2;
             double d1 = (D[3] * xi0 + D[4] * xi1) + D[5] * xi2;
             double d2 = (D[6] * xi0 + D[7] * xi1) + D[8] * xi2;

@@ -43,11 +43,12 @@ namespace {
 
 using body::nblk;
 
-// beta beside the P lists, the extent of their fluid entries, C_ii of fluid
-// particles, and the records (positions; x and t zero until the first pass)
+// beta beside the P lists (and v = beta r in FP32 for the FP32 form), the extent of
+// their fluid entries, C_ii of fluid particles, and the records (positions; x and t
+// zero until the first pass)
 __global__ void k_fac_p(DevParams P, const double* __restrict__ pos, const int* __restrict__ nCountP,
                         const long long* __restrict__ nPtrP, const int* __restrict__ nIndP, double* __restrict__ beta,
-                        float* __restrict__ beta32, int* __restrict__ runP, double* __restrict__ selfC,
+                        float* __restrict__ v32, int* __restrict__ runP, double* __restrict__ selfC,
                         double* __restrict__ rec) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P.np) return;
@@ -58,9 +59,8 @@ __global__ void k_fac_p(DevParams P, const double* __restrict__ pos, const int* 
     int run = 0;
     for (int jn = 0; jn < n; ++jn) {
         const int j = nIndP[b0 + jn];
-        double b = 0.0;
+        double b = 0.0, r0 = 0.0, r1 = 0.0, r2 = 0.0;
         if (j != k) {
-            double r0, r1, r2;
             body::fac_disp(P, pos, j, pk0, pk1, pk2, r0, r1, r2);
             const double r2s = (r0 * r0 + r1 * r1 + r2 * r2);
             if (r2s != 0.0 && P.radP2 - r2s > 0) {
@@ -72,8 +72,12 @@ __global__ void k_fac_p(DevParams P, const double* __restrict__ pos, const int* 
             }
         }
         if (j < P.nf) run = jn + 1;
-        if (beta) beta[b0 + jn] = b;
-        if (beta32) beta32[b0 + jn] = (float)b;
+        beta[b0 + jn] = b;
+        if (v32) {
+            v32[3 * (b0 + jn)] = (float)(b * r0);
+            v32[3 * (b0 + jn) + 1] = (float)(b * r1);
+            v32[3 * (b0 + jn) + 2] = (float)(b * r2);
+        }
     }
     runP[k] = run;
     double* rk = rec + 8 * (long long)k;
@@ -85,11 +89,37 @@ __global__ void k_fac_p(DevParams P, const double* __restrict__ pos, const int* 
     }
 }
 
-// the wrap flag is uniform, so the branch is taken by the whole launch
+// u = sqrt(alpha) r per fluid-neighbour entry of fluid rows, eight lanes per row
+// (coalesced writes beside the list)
+__global__ void __launch_bounds__(256) k_fac_u(DevParams P, const int* __restrict__ nFluid,
+                                               const long long* __restrict__ nPtr, const int* __restrict__ nInd,
+                                               const double* __restrict__ alpha, const double* __restrict__ pos,
+                                               float* __restrict__ u32) {
+    constexpr int G = 8;
+    const int g = threadIdx.x & (G - 1);
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+    if (i >= P.nf) return;
+    const double pi0 = pos[3 * i], pi1 = pos[3 * i + 1], pi2 = pos[3 * i + 2];
+    const long long b0 = nPtr[i];
+    const int w = nFluid[i];
+    for (int jn = g; jn < w; jn += G) {
+        const int j = nInd[b0 + jn];
+        double r0, r1, r2;
+        body::fac_disp(P, pos, j, pi0, pi1, pi2, r0, r1, r2);
+        const double s = sqrt(alpha[b0 + jn]);  // alpha >= 0: visc, mu and -w' are not negative
+        u32[3 * (b0 + jn)] = (float)(s * r0);
+        u32[3 * (b0 + jn) + 1] = (float)(s * r1);
+        u32[3 * (b0 + jn) + 2] = (float)(s * r2);
+    }
+}
+
+// the wrap flag is uniform, so the branch is taken by the whole launch; the FP32
+// form has no displacements to wrap
 template <class VT>
 __global__ void __launch_bounds__(256) k_fac_t(DevParams P, FacOp F, const double* __restrict__ x) {
-    if (*F.wrap) body::fac_t_chunk<VT, true>(P, F, x, blockIdx.x);
-    else body::fac_t_chunk<VT, false>(P, F, x, blockIdx.x);
+    if constexpr (sizeof(VT) == 4) body::fac_t32_chunk(P, F, x, blockIdx.x);
+    else if (*F.wrap) body::fac_t_chunk<true>(P, F, x, blockIdx.x);
+    else body::fac_t_chunk<false>(P, F, x, blockIdx.x);
 }
 
 template <bool kNeg, class VT>
@@ -98,7 +128,10 @@ __global__ void __launch_bounds__(256) k_fac_y(DevParams P, FacOp F, const doubl
                                                const double* __restrict__ dotx, double* __restrict__ part) {
     __shared__ double sh[256];
     __shared__ double rows[3 * 256];
-    if (*F.wrap) body::fac_y_chunk<kNeg, VT, true>(P, F, x, beta, yin, y, y2, p_copy, dotx, part, blockIdx.x, sh, rows);
+    if constexpr (sizeof(VT) == 4)
+        body::fac_y32_chunk<kNeg>(P, F, x, beta, yin, y, y2, p_copy, dotx, part, blockIdx.x, sh, rows);
+    else if (*F.wrap)
+        body::fac_y_chunk<kNeg, VT, true>(P, F, x, beta, yin, y, y2, p_copy, dotx, part, blockIdx.x, sh, rows);
     else body::fac_y_chunk<kNeg, VT, false>(P, F, x, beta, yin, y, y2, p_copy, dotx, part, blockIdx.x, sh, rows);
 }
 
@@ -113,10 +146,17 @@ void fac_launch(hipStream_t st, const DevParams& P, const FacOp& F, const double
 }  // namespace
 
 void launch_fac_p(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP, const long long* nPtrP,
-                  const int* nIndP, double* beta, float* beta32, int* runP, double* selfC, double* rec) {
+                  const int* nIndP, double* beta, float* v32, int* runP, double* selfC, double* rec) {
     if (P.np == 0) return;
-    hipLaunchKernelGGL(k_fac_p, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, pos, nCountP, nPtrP, nIndP, beta, beta32,
+    hipLaunchKernelGGL(k_fac_p, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, pos, nCountP, nPtrP, nIndP, beta, v32,
                        runP, selfC, rec);
+}
+
+void launch_fac_u(hipStream_t st, const DevParams& P, const int* nFluid, const long long* nPtr, const int* nInd,
+                  const double* alpha, const double* pos, float* u32) {
+    if (P.nf == 0) return;
+    hipLaunchKernelGGL(k_fac_u, dim3(nblk(8LL * P.nf, 256)), dim3(256), 0, st, P, nFluid, nPtr, nInd, alpha, pos,
+                       u32);
 }
 
 void launch_fac_spmv(hipStream_t st, const DevParams& P, const FacOp& F, double alpha, const double* x, double beta,

@@ -53,10 +53,11 @@ struct ASlices {
     float* val32;              // [slots*9] (mixed precision copy) or null
 };
 
-// The particle matrix in factored form (one scalar per neighbour-list entry instead
-// of nine per block; derivation in mph_factored.hip).  The scalars sit beside the
-// neighbour lists, in their order: alpha[nPtr[i] + jn] for the fluid neighbours
-// jn < nFluid[i] of fluid i, beta[nPtrP[k] + jn] for the P neighbours of every k.
+// The particle matrix in factored form (a scalar or a vector per neighbour-list
+// entry instead of nine values per block; derivation in mph_factored.hip).  The
+// values sit beside the neighbour lists, in their order: alpha[nPtr[i] + jn] for the
+// fluid neighbours jn < nFluid[i] of fluid i, beta[nPtrP[k] + jn] for the P
+// neighbours of every k.
 struct FacOp {
     const int* nFluid;        // [np] fluid neighbours (they lead each list)
     const long long* nPtr;    // neighbour lists
@@ -68,7 +69,7 @@ struct FacOp {
     const int* runP;          // [np] P entries up to and including the last fluid one
     const long long* nPtrP;   // P lists
     const int* nIndP;
-    const void* beta;         // -w_p'(d)/d (double, or float in FP32 form)
+    const double* beta;       // -w_p'(d)/d per P entry
     const double* lambda;     // [np]
     const double* pos;        // [np][3]
     // [np][8] per particle: x0 x1 x2 p0 p1 p2 t 0 -- the operand x (fluid), the
@@ -79,6 +80,13 @@ struct FacOp {
     // 1 when some neighbour pair interacts through the periodic boundary (then the
     // displacements take the minimum image); set by the matrix-A launch
     const int* wrap;
+    // FP32 form (the inner solves of mixed precision), which needs no positions:
+    // u = sqrt(alpha) r per fluid-neighbour entry and v = beta r per P entry, three
+    // floats each beside the lists (u32[3*(nPtr[i]+jn)+c], v32[3*(nPtrP[k]+jn)+c]),
+    // and a 16-byte record x0 x1 x2 t per particle
+    const float* u32;
+    const float* v32;
+    float* rec32;
 };
 
 constexpr int kSlice = 64;    // particles per A slice (one wavefront)
@@ -132,14 +140,14 @@ void launch_forces(hipStream_t st, const DevParams& P, const int* type, const do
                    const long long* nPtr, const int* nInd, const double* pressA, const double* gcen,
                    const double* mass, double* force, double* vel);
 // ---- implicit assembly ----
-// alpha/alpha32/diag/wrap (each may be null): the factored form's viscous scalar per
+// alpha/diag/wrap (each may be null): the factored form's viscous scalar per
 // fluid-neighbour entry (FacOp order), the diagonal block without C Lambda C^T, and
 // the flag set when a pair interacts through the periodic boundary (wrap must be 0
 // before the launch)
 void launch_matrix_a(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
                      const double* mass, const double* mu, const int* nCount, const int* nFluid,
                      const long long* nPtr, const int* nInd, ASlices A, int* selfIdx, double* b,
-                     double* alpha = nullptr, float* alpha32 = nullptr, double* diag = nullptr, int* wrap = nullptr);
+                     double* alpha = nullptr, double* diag = nullptr, int* wrap = nullptr);
 void launch_matrix_c(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP,
                      const long long* nPtrP, const int* nIndP, const double* lambda, double* cval, double* dval);
 void launch_vector_p(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
@@ -195,7 +203,10 @@ void launch_spmv(hipStream_t st, const DevParams& P, ASlices A, const int* nFlui
 // written alpha and the diagonal blocks), and y = beta*yin +- A x with the same
 // outputs as launch_spmv (two launches: t over every particle, then the fluid rows)
 void launch_fac_p(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP, const long long* nPtrP,
-                  const int* nIndP, double* beta, float* beta32, int* runP, double* selfC, double* rec);
+                  const int* nIndP, double* beta, float* v32, int* runP, double* selfC, double* rec);
+// u32 of the FP32 form from alpha and the positions (after launch_fac_p)
+void launch_fac_u(hipStream_t st, const DevParams& P, const int* nFluid, const long long* nPtr, const int* nInd,
+                  const double* alpha, const double* pos, float* u32);
 void launch_fac_spmv(hipStream_t st, const DevParams& P, const FacOp& F, double alpha, const double* x, double beta,
                      const double* yin, double* y, double* y2, double* p_copy, const double* dotx, double* part,
                      bool f32);

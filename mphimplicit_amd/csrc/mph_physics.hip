@@ -182,8 +182,7 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
                            const int* __restrict__ nFluid, const long long* __restrict__ nPtr,
                            const int* __restrict__ nInd, const long long* __restrict__ start, int* __restrict__ acol,
                            double* __restrict__ aval, int* __restrict__ selfIdx, double* __restrict__ b,
-                           double* __restrict__ falpha, float* __restrict__ falpha32, double* __restrict__ fdiag,
-                           int* __restrict__ fwrap) {
+                           double* __restrict__ falpha, double* __restrict__ fdiag, int* __restrict__ fwrap) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.nf) return;
     int iN = 0;
@@ -201,9 +200,9 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
     bool pending = false;
     // factored form: alpha_ij = visc mu_ij (-w_v'(d)) / d^3, the viscous block being
     // -alpha_ij r r^T (beside the neighbour list; 0 for self and beyond RadiusV)
+    // (streaming stores: each lane writes its own list, 64 lists per store)
     auto store_alpha = [&](int k, double a) {
-        if (falpha) falpha[b0 + k] = a;
-        if (falpha32) falpha32[b0 + k] = (float)a;
+        if (falpha) __builtin_nontemporal_store(a, falpha + b0 + k);
     };
     auto store1 = [&](int k, const double* v, int col) {
         acol[a_col_index(sb, k, lane)] = col;
@@ -234,7 +233,7 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
             const double e[3] = {q.x[0] / dij, q.x[1] / dij, q.x[2] / dij};
             const double muj = mu[j];
             const double muij = 2.0 * (mui * muj) / (mui + muj);
-            if (falpha || falpha32) alpha = P.visc * muij * wdrij / (dij * dij * dij);
+            if (falpha) alpha = P.visc * muij * wdrij / (dij * dij * dij);
             for (int r = 0; r < 3; ++r)
                 for (int s = 0; s < 3; ++s) {
                     const double coef = +P.visc * muij * e[s] * e[r] * wdrij / dij;
@@ -515,9 +514,9 @@ void launch_forces(hipStream_t st, const DevParams& P, const int* type, const do
 void launch_matrix_a(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
                      const double* mass, const double* mu, const int* nCount, const int* nFluid,
                      const long long* nPtr, const int* nInd, ASlices A, int* selfIdx, double* b, double* alpha,
-                     float* alpha32, double* diag, int* wrap) {
+                     double* diag, int* wrap) {
     hipLaunchKernelGGL(k_matrix_a, dim3(nblk(P.nf, kSlice)), dim3(kSlice), 0, st, P, type, pos, vel, mass, mu,
-                       nCount, nFluid, nPtr, nInd, A.start, A.col, A.val, selfIdx, b, alpha, alpha32, diag, wrap);
+                       nCount, nFluid, nPtr, nInd, A.start, A.col, A.val, selfIdx, b, alpha, diag, wrap);
 }
 
 void launch_matrix_c(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP,

@@ -271,15 +271,13 @@ int phase_matrix_a(mph_scene& S) {
         return rc;
     if (factored(S)) {
         const long long ne = S.E > 0 ? S.E : 1;
-        if ((rc = S.falpha.ensure(ne)) || (rc = S.fdiag.ensure(9 * (size_t)S.P.nf)) ||
-            (mixed(S) && (rc = S.falpha32.ensure(ne))) || (rc = S.fwrap.ensure(1)))
+        if ((rc = S.falpha.ensure(ne)) || (rc = S.fdiag.ensure(9 * (size_t)S.P.nf)) || (rc = S.fwrap.ensure(1)))
             return rc;
         launch_fill_i32(S.st, S.fwrap.p, 1, 0);
     }
     launch_matrix_a(S.st, S.P, S.type.p, S.pos.p, S.vel.p, S.mass.p, S.mu.p, S.nCount.p, S.nFluid.p, S.nPtr.p,
                     S.nInd.p, aslices(S), S.selfIdx.p, S.b.p, factored(S) ? S.falpha.p : nullptr,
-                    factored(S) && mixed(S) ? S.falpha32.p : nullptr, factored(S) ? S.fdiag.p : nullptr,
-                    factored(S) ? S.fwrap.p : nullptr);
+                    factored(S) ? S.fdiag.p : nullptr, factored(S) ? S.fwrap.p : nullptr);
     return MPH_OK;
 }
 
@@ -291,11 +289,16 @@ int phase_matrix_c(mph_scene& S) {
                     S.nPtrP.p, S.nIndP.p, S.vecP.p);
     if (factored(S)) {
         const long long ne = S.EP > 0 ? S.EP : 1;
-        if ((rc = S.pbeta.ensure(ne)) || (mixed(S) && (rc = S.pbeta32.ensure(ne))) || (rc = S.runP.ensure(S.P.np)) ||
-            (rc = S.selfC.ensure(3 * (size_t)S.P.nf)) || (rc = S.frec.ensure(8 * (size_t)S.P.np)))
+        if ((rc = S.fbeta.ensure(ne)) || (rc = S.runP.ensure(S.P.np)) || (rc = S.selfC.ensure(3 * (size_t)S.P.nf)) ||
+            (rc = S.frec.ensure(8 * (size_t)S.P.np)))
             return rc;
-        launch_fac_p(S.st, S.P, S.pos.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.pbeta.p,
-                     mixed(S) ? S.pbeta32.p : nullptr, S.runP.p, S.selfC.p, S.frec.p);
+        if (mixed(S) && ((rc = S.fv32.ensure(3 * ne)) || (rc = S.fu32.ensure(3 * (S.E > 0 ? S.E : 1))) ||
+                         (rc = S.frec32.ensure(4 * (size_t)S.P.np))))
+            return rc;
+        launch_fac_p(S.st, S.P, S.pos.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.fbeta.p, mixed(S) ? S.fv32.p : nullptr,
+                     S.runP.p, S.selfC.p, S.frec.p);
+        if (mixed(S))  // alpha was written by the matrix-A phase
+            launch_fac_u(S.st, S.P, S.nFluid.p, S.nPtr.p, S.nInd.p, S.falpha.p, S.pos.p, S.fu32.p);
     }
     return MPH_OK;
 }
@@ -470,18 +473,21 @@ FacOp fac_op(mph_scene& S, bool f32) {
     F.nFluid = S.nFluid.p;
     F.nPtr = S.nPtr.p;
     F.nInd = S.nInd.p;
-    F.alpha = f32 ? (const void*)S.falpha32.p : (const void*)S.falpha.p;
+    F.alpha = S.falpha.p;
     F.diag = S.fdiag.p;
     F.selfC = S.selfC.p;
     F.nCountP = S.nCountP.p;
     F.runP = S.runP.p;
     F.nPtrP = S.nPtrP.p;
     F.nIndP = S.nIndP.p;
-    F.beta = f32 ? (const void*)S.pbeta32.p : (const void*)S.pbeta.p;
+    F.beta = S.fbeta.p;
     F.lambda = S.lambda.p;
     F.pos = S.pos.p;
     F.rec = S.frec.p;
     F.wrap = S.fwrap.p;
+    F.u32 = f32 ? S.fu32.p : nullptr;
+    F.v32 = f32 ? S.fv32.p : nullptr;
+    F.rec32 = f32 ? S.frec32.p : nullptr;
     return F;
 }
 
@@ -774,9 +780,10 @@ void destroy(mph_scene* S) {
                          &S->maxP, &S->runP, &S->fwrap})
         b->release();
     S->runStart.release();
-    for (DBuf<double>* b : {&S->falpha, &S->fdiag, &S->selfC, &S->pbeta, &S->frec}) b->release();
-    S->falpha32.release();
-    S->pbeta32.release();
+    for (DBuf<double>* b : {&S->falpha, &S->fdiag, &S->selfC, &S->fbeta, &S->frec}) b->release();
+    S->fu32.release();
+    S->fv32.release();
+    S->frec32.release();
     for (DBuf<double>* b : {&S->pos, &S->pos2, &S->vel, &S->vel2, &S->force, &S->mass, &S->kappa, &S->lambda,
                             &S->mu, &S->densA, &S->gcen, &S->pressA, &S->volStrainP, &S->divP, &S->pressP,
                             &S->virialP, &S->virialStress, &S->aval, &S->b, &S->cval, &S->dval, &S->vecP,
@@ -1206,14 +1213,18 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
     else if (n == "TunedFuse") *v = S->tuneFuse.best;
     else if (n == "TunedGalerkin") *v = S->tuneGal.best;
     else if (n == "FactoredBytes" || n == "FactoredBytesF32") {
-        // algorithmic bytes of one factored product: per A slot a column and a scalar,
-        // per P entry read a column and a scalar (first pass: the entries up to the
-        // last fluid one; second pass: all entries of fluid rows), the diagonal block,
-        // C_ii, lambda, the position and t once each, the counts, and the CR vectors
-        // as the assembled product (x read, y read and written: 72 B per fluid
-        // particle).  The gathers of neighbours' x, t and positions are cache reads.
+        // algorithmic bytes of one factored product, every array read or written once:
+        // per list entry a column and the stored value (FP64: alpha or beta, 8 B;
+        // FP32: u or v, 12 B) -- first pass: P entries up to the last fluid one,
+        // second pass: all entries of fluid rows; per particle the counts, lambda,
+        // the diagonal block, C_ii, the record written (FP64: x and t, 32 B, and the
+        // position read; FP32: 16 B), and the CR vectors as the assembled product (x
+        // read, y read and written: 72 B per fluid particle).  Gathers of neighbours'
+        // records are cache reads.
         if (!factored(*S)) return fail(MPH_E_STATE, "the scene applies the assembled operator");
-        const double vb = n == "FactoredBytes" ? 8.0 : 4.0;
+        const bool f64 = n == "FactoredBytes";
+        const double ab = f64 ? 8.0 : 12.0, vb = f64 ? 8.0 : 12.0, recb = f64 ? 32.0 : 16.0;
+        const double posb = f64 ? 24.0 : 0.0;
         std::vector<int> nfl, ncp, run;
         int rc;
         if ((rc = d2h(*S, nfl, S->nFluid.p, (size_t)P.nf)) || (rc = d2h(*S, ncp, S->nCountP.p, (size_t)P.np)) ||
@@ -1221,11 +1232,11 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
             return rc;
         double t = 0;
         for (int k = 0; k < P.np; ++k) {
-            t += 4.0 + 8.0;  // run count, t written
-            if (run[k] > 0) t += (4.0 + vb) * run[k] + 8.0 + 24.0;  // entries, lambda, position
+            t += 4.0 + recb;  // run count, record written
+            if (run[k] > 0) t += (4.0 + vb) * run[k] + 8.0 + posb;  // entries, lambda, position
         }
         for (int i = 0; i < P.nf; ++i)
-            t += (4.0 + vb) * nfl[i] + (4.0 + vb) * ncp[i] + 4.0 + 4.0 + 72.0 + 24.0 + 8.0 + 72.0;
+            t += (4.0 + ab) * nfl[i] + (4.0 + vb) * ncp[i] + 4.0 + 4.0 + 72.0 + 24.0 + 72.0;
         *v = t;
     }
     else if (n == "SpmvLaunchesF32") *v = (double)S->spmvLaunches32;
@@ -1292,8 +1303,8 @@ int mph_scene_device_bytes(mph_scene* S, unsigned long long* bytes) {
                          &S->nCount, &S->nFluid, &S->nCountP, &S->nInd, &S->nIndP, &S->width, &S->selfIdx, &S->acol,
                          &S->maxP, &S->runP, &S->fwrap})
         n += b->cap * sizeof(int);
-    for (DBuf<double>* b : {&S->falpha, &S->fdiag, &S->selfC, &S->pbeta, &S->frec}) n += b->cap * sizeof(double);
-    n += (S->falpha32.cap + S->pbeta32.cap) * sizeof(float);
+    for (DBuf<double>* b : {&S->falpha, &S->fdiag, &S->selfC, &S->fbeta, &S->frec}) n += b->cap * sizeof(double);
+    n += (S->fu32.cap + S->fv32.cap + S->frec32.cap) * sizeof(float);
     n += S->runStart.cap * sizeof(unsigned short);
     for (DBuf<double>* b : {&S->pos, &S->pos2, &S->vel, &S->vel2, &S->force, &S->mass, &S->kappa, &S->lambda,
                             &S->mu, &S->densA, &S->gcen, &S->pressA, &S->volStrainP, &S->divP, &S->pressP,

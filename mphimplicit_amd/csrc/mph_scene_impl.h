@@ -170,8 +170,8 @@ struct mph_scene {
     bool assembled = false;  // A (and the factored scalars) match the current particles
     // factored operator (MPH_OPERATOR_FACTORED): scalars beside the neighbour lists
     mph::host::DBuf<int> runP, fwrap;
-    mph::host::DBuf<double> falpha, fdiag, selfC, pbeta, frec;  // frec: FacOp::rec
-    mph::host::DBuf<float> falpha32, pbeta32;  // FP32 forms (mixed precision)
+    mph::host::DBuf<double> falpha, fdiag, selfC, fbeta, frec;  // FacOp::alpha, diag, selfC, beta, rec
+    mph::host::DBuf<float> fu32, fv32, frec32;  // FP32 form (mixed precision): FacOp::u32, v32, rec32
     // multigrid
     int hbbox[6] = {0, 0, 0, 0, 0, 0};
     int depth = 0;

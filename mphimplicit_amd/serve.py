@@ -20,7 +20,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .abi import PRECISION_MIXED, Params
+from .abi import OPERATOR_ASSEMBLED, PRECISION_MIXED, Params
 from .scene import Batch, Scene
 
 
@@ -188,7 +188,7 @@ class Server:
         self.batch.close()
 
 
-def make_requests(stream, precision=PRECISION_MIXED, tol=1e-16) -> list[Request]:
+def make_requests(stream, precision=PRECISION_MIXED, tol=1e-16, operator_form=OPERATOR_ASSEMBLED) -> list[Request]:
     """Requests from scenes.serve_stream tuples (arrival, steps, seed, dim, nf, mu);
     each scene is built when the server admits its request."""
     from . import scenes
@@ -199,6 +199,7 @@ def make_requests(stream, precision=PRECISION_MIXED, tol=1e-16) -> list[Request]
             p, g = scenes.box_scene(seed, dim, nf, mu, g_dir, steps=steps)
             p.precision = precision
             p.cr_tolerance = tol
+            p.operator_form = operator_form
             return p, g
         return make
 
