@@ -10,9 +10,8 @@ cell sort, neighbour search, explicit forces, assembly, multigrid build, solve).
 
 Prints ONE JSON line (see the fields below).  ``value`` is fluid-particle-steps/s
 with all state resident in HBM, with the particle matrix applied in the form that
---operator names (default auto: the factored form, the same matrix from one scalar
-per neighbour entry, for FP64 legs, and the assembled blocks for mixed-precision
-legs; ``fp64_assembled`` reports the bit-exact assembled FP64 form beside it).
+--operator names (default: factored, the same matrix from one scalar per neighbour
+entry; ``fp64_assembled`` reports the bit-exact assembled FP64 form beside it).
 ``roofline`` is measured live on the dominant kernel (the particle-matrix product
 of the CR loop) with HIP events on the scene's stream.  ``cpu_baseline`` is the CPU
 oracle (OpenMP) timed on a bounded sample of the same scene on this host.
@@ -71,13 +70,8 @@ def roofline(kernel, ms, launches, bytes_sum, ref_sum, traffic):
 
 
 def form_for(args, precision):
-    """Operator form of a leg: --operator, or with "auto" the faster form per
-    precision on this GPU (factored for FP64; assembled for mixed precision, whose
-    FP32 block copy halves the assembled bytes while the factored product stays bound
-    by its FP64 record gathers)."""
-    if args.operator != "auto":
-        return args.operator
-    return "factored" if precision == "fp64" else "assembled"
+    """Operator form of a leg (--operator; the same for every precision)."""
+    return args.operator
 
 
 def run_gpu(args, precision="fp64", form=None):
@@ -334,9 +328,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="3d_1m")
     ap.add_argument("--rel-residual", type=float, default=1.0e-8, help="CR stop: sqrt(rr/rr0) below this")
-    ap.add_argument("--operator", choices=["auto", "assembled", "factored"], default="auto",
-                    help="how the solve applies the particle matrix (mph_params.operator_form); auto: "
-                         "factored for FP64 legs, assembled for mixed-precision legs")
+    ap.add_argument("--operator", choices=["assembled", "factored"], default="factored",
+                    help="how the solve applies the particle matrix (mph_params.operator_form) in every leg")
     ap.add_argument("--assembled-leg", choices=["on", "off"], default="on",
                     help="when the FP64 leg is factored: also run plain FP64 with the assembled matrix (the "
                          "bit-exact form)")

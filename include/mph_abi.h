@@ -172,7 +172,11 @@ int mph_scene_run_phase(mph_scene* s, int phase);
  * scalars of mph_scene_set_timing, TunedSpmv64 TunedSpmv32 TunedCac TunedFuse
  * TunedGalerkin (the variants the run-time tuner kept, -1 while still exploring),
  * and for the factored operator FactoredBytes and FactoredBytesF32 (algorithmic
- * bytes of one product with FP64 / FP32 scalars). */
+ * bytes of one product with FP64 / FP32 scalars) and AssembledBlocks (1 when the
+ * step assembled the 3x3 blocks: always for the assembled form; for the factored
+ * form only when its multigrid level 0 spans fewer than 3 cells along an axis,
+ * otherwise level 0 and InvDiagA come from the factored scalars and CsrCofA is not
+ * available). */
 int mph_scene_field_size(mph_scene* s, const char* field, size_t* bytes, int* elem_bytes);
 int mph_scene_read(mph_scene* s, const char* field, void* host, size_t bytes);
 int mph_scene_write(mph_scene* s, const char* field, const void* host, size_t bytes);

@@ -49,6 +49,38 @@ __device__ inline double block_tree_sum(double t, double* sh) {
     return v;
 }
 
+// which of the visited neighbour cells (ascending CellId, as the neighbour search)
+// of cell g map to stencil cell c of level 0: bit q = run q of every particle's list
+__device__ inline unsigned galerkin_mask(const DevParams& P, const Level& L, const int* __restrict__ cellStart,
+                                         int g, int c) {
+    int gx, gy, gz;
+    grid_xyz(L, g, gx, gy, gz);
+    const int cx = gx + L.mn[0], cy = gy + L.mn[1], cz = gz + L.mn[2];
+    unsigned mask = 0;
+    const int* C = P.C;
+    const int rx = P.range[0], ry = P.range[1], rz = P.range[2];
+    int jx[3], jy[3], jz[3];
+    for (int k = 0; k < 2 * rx + 1; ++k) jx[k] = ((cx - rx + k) % C[0] + C[0]) % C[0];
+    for (int k = 0; k < 2 * ry + 1; ++k) jy[k] = ((cy - ry + k) % C[1] + C[1]) % C[1];
+    for (int k = 0; k < 2 * rz + 1; ++k) jz[k] = ((cz - rz + k) % C[2] + C[2]) % C[2];
+    const int bx = (2 * rx) - (cx + rx) % C[0], by = (2 * ry) - (cy + ry) % C[1], bz = (2 * rz) - (cz + rz) % C[2];
+    const int x0 = bx > 0 ? bx : 0, y0 = by > 0 ? by : 0, z0 = bz > 0 ? bz : 0;
+    int q = 0;
+    for (int a = x0; a < x0 + (2 * rx + 1); ++a)
+        for (int b = y0; b < y0 + (2 * ry + 1); ++b)
+            for (int cc = z0; cc < z0 + (2 * rz + 1); ++cc, ++q) {
+                const int ax = jx[a % (2 * rx + 1)], ay = jy[b % (2 * ry + 1)], az = jz[cc % (2 * rz + 1)];
+                const int jc = cell_id(P, ax, ay, az);
+                if (cellStart[jc] == cellStart[jc + 1]) continue;  // no fluid there
+                const int dx = (ax - L.mn[0]) - gx, dy = (ay - L.mn[1]) - gy, dz = (az - L.mn[2]) - gz;
+                const int ix = (dx + 1 + L.cnt[0]) % L.cnt[0];
+                const int iy = (dy + 1 + L.cnt[1]) % L.cnt[1];
+                const int iz = (dz + 1 + L.cnt[2]) % L.cnt[2];
+                if (L.cs[2] * L.cs[1] * ix + L.cs[2] * iy + iz == c) mask |= 1u << q;
+            }
+    return mask;
+}
+
 // ---------------------------------------------------------------------------
 // multigrid rows (preconditionWithMultiGrid)
 

@@ -113,6 +113,182 @@ __global__ void __launch_bounds__(256) k_fac_u(DevParams P, const int* __restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// Multigrid level 0 and the particle-level inverse diagonal from the factored form,
+// so a factored scene never assembles the 3x3 blocks.  Level 0 of
+// calculateMultiGridCsrA (main.cpp:3336-3410) is, for row cell g and stencil cell h,
+//   A0(g, h) = sum_{i in g} sum_{j in h} A_ij
+//            = sum_{i in g} [D_i if h = g] - sum_{i in g, j in h, j != i} alpha_ij r r^T
+//              + sum_k lambda_k G_gk G_hk^T,     G_gk = sum_{i in g, k in P(i)} C_ik,
+// k over every particle with fluid P neighbours in both cells.  A particle's P
+// neighbours lie in its own or the adjacent cells (cells are at least MaxRadius >=
+// 2 RadiusP wide), so G_k is a 27-slot table around k's cell.
+
+// slot of cell (tx, ty, tz) in the 3x3x3 block around (kx, ky, kz), -1 if outside
+__device__ inline int fac_slot(const DevParams& P, int tx, int ty, int tz, int kx, int ky, int kz) {
+    int d[3] = {tx - kx, ty - ky, tz - kz};
+    for (int a = 0; a < 3; ++a) {
+        if (d[a] > 1) d[a] -= P.C[a];
+        if (d[a] < -1) d[a] += P.C[a];
+        if (d[a] < -1 || d[a] > 1) return -1;
+    }
+    return 9 * (d[0] + 1) + 3 * (d[1] + 1) + (d[2] + 1);
+}
+
+__device__ inline void cell_xyz(const DevParams& P, int key, int& x, int& y, int& z) {
+    const int c = key % P.T;  // wall keys carry +T
+    x = c / (P.C[1] * P.C[2]);
+    y = (c / P.C[2]) % P.C[1];
+    z = c % P.C[2];
+}
+
+// G_k (27 slots x 3, zero where no fluid P neighbour lies) and the mask of used
+// slots, summed over k's P list in order; C_kk = C_ii of fluid k, C_jk = -beta r_kj
+__global__ void k_fac_g(DevParams P, const int* __restrict__ cellIndex, const int* __restrict__ runP,
+                        const long long* __restrict__ nPtrP, const int* __restrict__ nIndP,
+                        const double* __restrict__ beta, const double* __restrict__ pos,
+                        const double* __restrict__ selfC, double* __restrict__ gtab, int* __restrict__ gmask) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.np) return;
+    double* G = gtab + 81 * (long long)k;
+    for (int e = 0; e < 81; ++e) G[e] = 0.0;
+    int kx, ky, kz;
+    cell_xyz(P, cellIndex[k], kx, ky, kz);
+    const double pk0 = pos[3 * k], pk1 = pos[3 * k + 1], pk2 = pos[3 * k + 2];
+    const long long c0 = nPtrP[k];
+    const int run = runP[k];
+    unsigned mask = 0;
+    for (int jn = 0; jn < run; ++jn) {
+        const int j = nIndP[c0 + jn];
+        if (j >= P.nf) continue;
+        int jx, jy, jz;
+        cell_xyz(P, cellIndex[j], jx, jy, jz);
+        const int slot = fac_slot(P, jx, jy, jz, kx, ky, kz);
+        if (slot < 0) continue;  // cannot happen: P neighbours are in adjacent cells
+        double c[3];
+        if (j == k) {
+            c[0] = selfC[3 * k];
+            c[1] = selfC[3 * k + 1];
+            c[2] = selfC[3 * k + 2];
+        } else {
+            double r0, r1, r2;
+            body::fac_disp(P, pos, j, pk0, pk1, pk2, r0, r1, r2);
+            const double b = beta[c0 + jn];
+            c[0] = -(b * r0);
+            c[1] = -(b * r1);
+            c[2] = -(b * r2);
+        }
+        for (int a = 0; a < 3; ++a) G[3 * slot + a] += c[a];
+        mask |= 1u << slot;
+    }
+    gmask[k] = (int)mask;
+}
+
+// level-0 row cell g (workgroup), stencil cell c (thread): the 3x3 block A0(g, c)
+__global__ void __launch_bounds__(64) k_fac_gal0(DevParams P, Level L, FacOp F, const int* __restrict__ cellStart,
+                                                 const unsigned short* __restrict__ runStart,
+                                                 const double* __restrict__ gtab, const int* __restrict__ gmask) {
+    const int ncs = L.cs[0] * L.cs[1] * L.cs[2];
+    const int g = blockIdx.x, c = threadIdx.x;
+    if (c >= ncs) return;  // no barriers below
+    int gx, gy, gz;
+    body::grid_xyz(L, g, gx, gy, gz);
+    const int ax = gx + L.mn[0], ay = gy + L.mn[1], az = gz + L.mn[2];
+    const int ic = body::cell_id(P, ax, ay, az);
+    const int p0 = cellStart[ic], p1 = cellStart[ic + 1];
+    double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    // viscous part and the diagonal blocks: the runs of the row cell's particles
+    // that fall in stencil cell c
+    const unsigned vmask = body::galerkin_mask(P, L, cellStart, g, c);
+    if (vmask)
+        for (int ip = p0; ip < p1; ++ip) {
+            const unsigned short* rs = runStart + (long long)ip * kRuns;
+            const long long b0 = F.nPtr[ip];
+            const double pi0 = F.pos[3 * ip], pi1 = F.pos[3 * ip + 1], pi2 = F.pos[3 * ip + 2];
+            for (unsigned m = vmask; m; m &= m - 1) {
+                const int q = __builtin_ctz(m);
+                for (int jn = rs[q]; jn < rs[q + 1]; ++jn) {
+                    const int j = F.nInd[b0 + jn];
+                    if (j == ip) {
+                        for (int e = 0; e < 9; ++e) acc[e] += F.diag[9 * ip + e];
+                    } else {
+                        double r[3];
+                        body::fac_disp(P, F.pos, j, pi0, pi1, pi2, r[0], r[1], r[2]);
+                        const double a = ((const double*)F.alpha)[b0 + jn];
+                        for (int rr = 0; rr < 3; ++rr)
+                            for (int ss = 0; ss < 3; ++ss) acc[3 * rr + ss] -= a * r[rr] * r[ss];
+                    }
+                }
+            }
+        }
+    // C Lambda C^T part: lambda_k G_gk G_hk^T over the particles k around g whose G
+    // has both cells; h is stencil cell c (level-0 counts of at least 3 per axis:
+    // the caller checks)
+    const int iX = c / (L.cs[1] * L.cs[2]), iY = (c / L.cs[2]) % L.cs[1], iZ = c % L.cs[2];
+    const int hx = L.mn[0] + (gx + (L.cs[0] == 3 ? iX - 1 : 0) + L.cnt[0]) % L.cnt[0];
+    const int hy = L.mn[1] + (gy + (L.cs[1] == 3 ? iY - 1 : 0) + L.cnt[1]) % L.cnt[1];
+    const int hz = L.mn[2] + (gz + (L.cs[2] == 3 ? iZ - 1 : 0) + L.cnt[2]) % L.cnt[2];
+    const int* C = P.C;
+    for (int dx = -P.range[0]; dx <= P.range[0]; ++dx)
+        for (int dy = -P.range[1]; dy <= P.range[1]; ++dy)
+            for (int dz = -P.range[2]; dz <= P.range[2]; ++dz) {
+                const int kx = ((ax + dx) % C[0] + C[0]) % C[0];
+                const int ky = ((ay + dy) % C[1] + C[1]) % C[1];
+                const int kz = ((az + dz) % C[2] + C[2]) % C[2];
+                const int sg = fac_slot(P, ((ax % C[0]) + C[0]) % C[0], ((ay % C[1]) + C[1]) % C[1],
+                                        ((az % C[2]) + C[2]) % C[2], kx, ky, kz);
+                const int sh = fac_slot(P, ((hx % C[0]) + C[0]) % C[0], ((hy % C[1]) + C[1]) % C[1],
+                                        ((hz % C[2]) + C[2]) % C[2], kx, ky, kz);
+                if (sg < 0 || sh < 0) continue;
+                const int kc = body::cell_id(P, kx, ky, kz);
+                for (int half = 0; half < 2; ++half) {  // fluid, then wall particles of the cell
+                    const int q0 = cellStart[half * P.T + kc], q1 = cellStart[half * P.T + kc + 1];
+                    for (int k = q0; k < q1; ++k) {
+                        const unsigned m = (unsigned)gmask[k];
+                        if (!((m >> sg) & 1u) || !((m >> sh) & 1u)) continue;
+                        const double* G = gtab + 81 * (long long)k;
+                        const double lk = F.lambda[k];
+                        for (int rr = 0; rr < 3; ++rr) {
+                            const double lg = lk * G[3 * sg + rr];
+                            for (int ss = 0; ss < 3; ++ss) acc[3 * rr + ss] += lg * G[3 * sh + ss];
+                        }
+                    }
+                }
+            }
+    for (int rr = 0; rr < 3; ++rr)
+        for (int ss = 0; ss < 3; ++ss) L.A[(long long)(3 * c + ss) * L.nrow + 3 * g + rr] = acc[3 * rr + ss];
+}
+
+// InvDiagA (calculateInvDiagA, main.cpp:3045-3068): 1/A_ii[r][r] with
+// A_ii[r][r] = D_i[r][r] + sum_{k in P(i)} lambda_k C_ik[r]^2
+__global__ void k_fac_invdiag(DevParams P, FacOp F, double* __restrict__ invDiagA) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nf) return;
+    const double pi0 = F.pos[3 * i], pi1 = F.pos[3 * i + 1], pi2 = F.pos[3 * i + 2];
+    double d[3] = {F.diag[9 * i], F.diag[9 * i + 4], F.diag[9 * i + 8]};
+    const long long c0 = F.nPtrP[i];
+    const int n = F.nCountP[i];
+    for (int jn = 0; jn < n; ++jn) {
+        const int k = F.nIndP[c0 + jn];
+        double cv[3];
+        if (k == i) {
+            cv[0] = F.selfC[3 * i];
+            cv[1] = F.selfC[3 * i + 1];
+            cv[2] = F.selfC[3 * i + 2];
+        } else {
+            double r0, r1, r2;
+            body::fac_disp(P, F.pos, k, pi0, pi1, pi2, r0, r1, r2);
+            const double b = F.beta[c0 + jn];
+            cv[0] = b * r0;
+            cv[1] = b * r1;
+            cv[2] = b * r2;
+        }
+        const double lk = F.lambda[k];
+        for (int a = 0; a < 3; ++a) d[a] += lk * cv[a] * cv[a];
+    }
+    for (int a = 0; a < 3; ++a) invDiagA[3 * i + a] = d[a] != 0.0 ? 1.0 / d[a] : 0.0;
+}
+
 // the wrap flag is uniform, so the branch is taken by the whole launch; the FP32
 // form has no displacements to wrap
 template <class VT>
@@ -150,6 +326,17 @@ void launch_fac_p(hipStream_t st, const DevParams& P, const double* pos, const i
     if (P.np == 0) return;
     hipLaunchKernelGGL(k_fac_p, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, pos, nCountP, nPtrP, nIndP, beta, v32,
                        runP, selfC, rec);
+}
+
+void launch_fac_level0(hipStream_t st, const DevParams& P, const Level& L0, const FacOp& F, const int* cellIndex,
+                       const int* cellStart, const unsigned short* runStart, double* gtab, int* gmask,
+                       double* invDiagA) {
+    if (P.nf == 0) return;
+    hipLaunchKernelGGL(k_fac_g, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, cellIndex, F.runP, F.nPtrP, F.nIndP,
+                       F.beta, F.pos, F.selfC, gtab, gmask);
+    hipLaunchKernelGGL(k_fac_invdiag, dim3(nblk(P.nf, 128)), dim3(128), 0, st, P, F, invDiagA);
+    if (L0.cells > 0)
+        hipLaunchKernelGGL(k_fac_gal0, dim3(L0.cells), dim3(64), 0, st, P, L0, F, cellStart, runStart, gtab, gmask);
 }
 
 void launch_fac_u(hipStream_t st, const DevParams& P, const int* nFluid, const long long* nPtr, const int* nInd,

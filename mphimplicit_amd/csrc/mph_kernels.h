@@ -204,6 +204,12 @@ void launch_spmv(hipStream_t st, const DevParams& P, ASlices A, const int* nFlui
 // outputs as launch_spmv (two launches: t over every particle, then the fluid rows)
 void launch_fac_p(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP, const long long* nPtrP,
                   const int* nIndP, double* beta, float* v32, int* runP, double* selfC, double* rec);
+// multigrid level 0 (calculateMultiGridCsrA at level 0) and InvDiagA from the
+// factored form, with per-particle scratch gtab [np][27][3] and gmask [np] (level-0
+// counts of at least 3 per active axis: the caller checks)
+void launch_fac_level0(hipStream_t st, const DevParams& P, const Level& L0, const FacOp& F, const int* cellIndex,
+                       const int* cellStart, const unsigned short* runStart, double* gtab, int* gmask,
+                       double* invDiagA);
 // u32 of the FP32 form from alpha and the positions (after launch_fac_p)
 void launch_fac_u(hipStream_t st, const DevParams& P, const int* nFluid, const long long* nPtr, const int* nInd,
                   const double* alpha, const double* pos, float* u32);

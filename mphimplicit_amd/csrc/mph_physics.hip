@@ -204,7 +204,11 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
     auto store_alpha = [&](int k, double a) {
         if (falpha) __builtin_nontemporal_store(a, falpha + b0 + k);
     };
+    // the 3x3 blocks are stored unless aval is null (a factored scene that builds its
+    // multigrid level 0 from the factored form needs none)
+    const bool blocks = aval != nullptr;
     auto store1 = [&](int k, const double* v, int col) {
+        if (!blocks) return;
         acol[a_col_index(sb, k, lane)] = col;
         for (int e9 = 0; e9 < 9; ++e9) aval[a_val_index(sb, k, e9, lane)] = v[e9];
     };
@@ -213,7 +217,7 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
         const bool fluidj = jn < nfl;
         if (i == j) {
             iN = jn;
-            acol[a_col_index(sb, jn, lane)] = j;
+            if (blocks) acol[a_col_index(sb, jn, lane)] = j;
             if (pending) store1(jn - 1, pend, pendCol);  // the self block is stored last
             pending = false;
             if (jn < nfl) store_alpha(jn, 0.0);
@@ -258,8 +262,11 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
             } else if (pending) {
                 typedef double d2 __attribute__((ext_vector_type(2)));
                 typedef int i2 __attribute__((ext_vector_type(2)));
-                *(i2*)&acol[a_col_index(sb, jn - 1, lane)] = i2{pendCol, j};
-                for (int e9 = 0; e9 < 9; ++e9) *(d2*)&aval[a_val_index(sb, jn - 1, e9, lane)] = d2{pend[e9], v[e9]};
+                if (blocks) {
+                    *(i2*)&acol[a_col_index(sb, jn - 1, lane)] = i2{pendCol, j};
+                    for (int e9 = 0; e9 < 9; ++e9)
+                        *(d2*)&aval[a_val_index(sb, jn - 1, e9, lane)] = d2{pend[e9], v[e9]};
+                }
                 pending = false;
             } else {
                 store1(jn, v, j);
@@ -271,7 +278,8 @@ __global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const doub
     self[0] += coef;
     self[4] += coef;
     self[8] += coef;
-    for (int e9 = 0; e9 < 9; ++e9) aval[a_val_index(sb, iN, e9, lane)] = self[e9];
+    if (blocks)
+        for (int e9 = 0; e9 < 9; ++e9) aval[a_val_index(sb, iN, e9, lane)] = self[e9];
     if (fdiag)
         for (int e9 = 0; e9 < 9; ++e9) fdiag[9 * i + e9] = self[e9];
     selfIdx[i] = iN;

@@ -175,14 +175,23 @@ int init_constants(mph_scene& S, const mph_grid& g) {
 bool mixed(const mph_scene& S) { return S.prm.precision == MPH_PRECISION_MIXED; }
 bool factored(const mph_scene& S) { return S.prm.operator_form == MPH_OPERATOR_FACTORED; }
 
+bool blocks_wanted(const mph_scene& S) {
+    if (!factored(S)) return true;
+    if (!S.prm.multigrid) return false;  // nothing but the solve would read them
+    const int* bb = S.hbbox;           // max x,y,z then min x,y,z (cells of the fluid)
+    for (int d = 0; d < (S.P.dim == 2 ? 2 : 3); ++d)
+        if (bb[d] - bb[3 + d] + 1 < 3) return true;  // level-0 stencil narrower than 3 cells
+    return false;
+}
+
 ASlices aslices(mph_scene& S) {
     ASlices A;
     A.nslices = S.nslices;
     A.width = S.width.p;
     A.start = S.sliceStart.p;
-    A.col = S.acol.p;
-    A.val = S.aval.p;
-    A.val32 = mixed(S) && !factored(S) ? S.aval32.p : nullptr;
+    A.col = S.blocks ? S.acol.p : nullptr;
+    A.val = S.blocks ? S.aval.p : nullptr;
+    A.val32 = S.blocks && mixed(S) && !factored(S) ? S.aval32.p : nullptr;
     return A;
 }
 
@@ -241,6 +250,7 @@ int phase_neighbor_finish(mph_scene& S) {
     S.EP = h[1];
     S.slots = h[2] * kSlice;
     std::memcpy(S.hbbox, &h[3], 6 * sizeof(int));
+    S.blocks = blocks_wanted(S);
     int rc;
     if ((rc = S.nInd.ensure(S.E > 0 ? S.E : 1)) || (rc = S.nIndP.ensure(S.EP > 0 ? S.EP : 1))) return rc;
     launch_neighbor_fill(S.st, P, S.pos.p, S.cellIndex.p, S.cellStart.p, S.nCount.p, S.nPtr.p, S.nPtrP.p, S.nInd.p,
@@ -267,7 +277,8 @@ int phase_explicit(mph_scene& S) {
 
 int phase_matrix_a(mph_scene& S) {
     int rc;
-    if ((rc = S.acol.ensure(S.slots > 0 ? S.slots : 1)) || (rc = S.aval.ensure(S.slots > 0 ? 9 * S.slots : 1)))
+    if (S.blocks &&
+        ((rc = S.acol.ensure(S.slots > 0 ? S.slots : 1)) || (rc = S.aval.ensure(S.slots > 0 ? 9 * S.slots : 1))))
         return rc;
     if (factored(S)) {
         const long long ne = S.E > 0 ? S.E : 1;
@@ -307,6 +318,10 @@ int phase_multiply_c(mph_scene& S) {
     int rc;
     if (mixed(S) && !factored(S) && (rc = S.aval32.ensure(S.slots > 0 ? 9 * S.slots : 1))) return rc;
     launch_b_minus_cp(S.st, S.P, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.cval.p, S.vecP.p, S.b.p);
+    if (!S.blocks) {  // factored form: no blocks to add C Lambda C^T to
+        S.assembled = true;
+        return MPH_OK;
+    }
     // C Lambda C^T also writes the FP32 copy of A in mixed precision (aval32) unless
     // the solve applies the factored form
     if (S.hmaxP > kMaxP) {  // P lists too long for the LDS modes
@@ -382,8 +397,12 @@ int phase_multigrid(mph_scene& S) {
     for (int l = 0; l < depth; ++l) S.hlv[l] = S.lv[l].L;
     if ((rc = S.lvDev.ensure(kMaxLevels))) return rc;
     HIPCHK(hipMemcpyAsync(S.lvDev.p, S.hlv.data(), depth * sizeof(Level), hipMemcpyHostToDevice, S.st));
-    launch_inv_diag(S.st, S.P, S.selfIdx.p, aslices(S), S.invDiagA.p);
-    {
+    if (!S.blocks) {  // factored form: level 0 and InvDiagA from the factored scalars
+        if ((rc = S.fgtab.ensure(81 * (size_t)S.P.np)) || (rc = S.fgmask.ensure(S.P.np))) return rc;
+        launch_fac_level0(S.st, S.P, S.lv[0].L, fac_op(S, false), S.cellIndex.p, S.cellStart.p, S.runStart.p,
+                          S.fgtab.p, S.fgmask.p, S.invDiagA.p);
+    } else {
+        launch_inv_diag(S.st, S.P, S.selfIdx.p, aslices(S), S.invDiagA.p);
         // the LDS variant holds one particle's row: its slice width must fit
         int v = S.tuneGal.pick();
         if (v == 1 && S.hmaxW > kGalRowMax) {
@@ -777,8 +796,9 @@ void destroy(mph_scene* S) {
     if (S->st) (void)hipStreamSynchronize(S->st);
     for (DBuf<int>* b : {&S->type, &S->type2, &S->perm, &S->perm2, &S->cellIndex, &S->cellStart, &S->bbox,
                          &S->nCount, &S->nFluid, &S->nCountP, &S->nInd, &S->nIndP, &S->width, &S->selfIdx, &S->acol,
-                         &S->maxP, &S->runP, &S->fwrap})
+                         &S->maxP, &S->runP, &S->fwrap, &S->fgmask})
         b->release();
+    S->fgtab.release();
     S->runStart.release();
     for (DBuf<double>* b : {&S->falpha, &S->fdiag, &S->selfC, &S->fbeta, &S->frec}) b->release();
     S->fu32.release();
@@ -897,6 +917,7 @@ int host_field(mph_scene& S, const std::string& n, Blob& out, bool& found) {
         return MPH_OK;
     }
     if (n == "CsrPtrA" || n == "CsrIndA" || n == "CsrCofA" || n == "AVal" || n == "ACol") {
+        if (!S.blocks) return fail(MPH_E_STATE, "the factored form assembled no 3x3 blocks this step");
         std::vector<int> nfl, col;
         std::vector<long long> start;
         std::vector<double> val;
@@ -1202,6 +1223,7 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
     else if (n == "NeighborIndCount") *v = (double)S->E;
     else if (n == "NeighborIndCountP") *v = (double)S->EP;
     else if (n == "ASlots") *v = (double)S->slots;
+    else if (n == "AssembledBlocks") *v = S->blocks ? 1.0 : 0.0;
     else if (n == "MultiGridDepth") *v = S->depth;
     else if (n == "RefinementSteps") *v = S->irOuter;
     else if (n == "SpmvMs") *v = S->spmvMs;
@@ -1301,8 +1323,9 @@ int mph_scene_device_bytes(mph_scene* S, unsigned long long* bytes) {
     unsigned long long n = 0;
     for (DBuf<int>* b : {&S->type, &S->type2, &S->perm, &S->perm2, &S->cellIndex, &S->cellStart, &S->bbox,
                          &S->nCount, &S->nFluid, &S->nCountP, &S->nInd, &S->nIndP, &S->width, &S->selfIdx, &S->acol,
-                         &S->maxP, &S->runP, &S->fwrap})
+                         &S->maxP, &S->runP, &S->fwrap, &S->fgmask})
         n += b->cap * sizeof(int);
+    n += S->fgtab.cap * sizeof(double);
     for (DBuf<double>* b : {&S->falpha, &S->fdiag, &S->selfC, &S->fbeta, &S->frec}) n += b->cap * sizeof(double);
     n += (S->fu32.cap + S->fv32.cap + S->frec32.cap) * sizeof(float);
     n += S->runStart.cap * sizeof(unsigned short);

@@ -169,7 +169,9 @@ struct mph_scene {
     long long slots = 0;
     bool assembled = false;  // A (and the factored scalars) match the current particles
     // factored operator (MPH_OPERATOR_FACTORED): scalars beside the neighbour lists
-    mph::host::DBuf<int> runP, fwrap;
+    mph::host::DBuf<int> runP, fwrap, fgmask;
+    mph::host::DBuf<double> fgtab;  // level 0 from the factored form: per-particle G tables
+    bool blocks = true;  // this step assembles the 3x3 blocks (see blocks_wanted)
     mph::host::DBuf<double> falpha, fdiag, selfC, fbeta, frec;  // FacOp::alpha, diag, selfC, beta, rec
     mph::host::DBuf<float> fu32, fv32, frec32;  // FP32 form (mixed precision): FacOp::u32, v32, rec32
     // multigrid
@@ -210,6 +212,10 @@ namespace host {
 
 bool mixed(const mph_scene& S);
 bool factored(const mph_scene& S);
+// whether the step assembles the 3x3 blocks: always for the assembled form; for the
+// factored form only when the multigrid's level 0 cannot be built from the factored
+// scalars (a level-0 grid under 3 cells along an active axis)
+bool blocks_wanted(const mph_scene& S);
 mph::FacOp fac_op(mph_scene& S, bool f32);  // the factored form's pointers (FP64 or FP32 scalars)
 ASlices aslices(mph_scene& S);
 int run_phase(mph_scene& S, int ph);

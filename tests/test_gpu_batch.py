@@ -44,6 +44,8 @@ def alone(name, precision, steps, end_steps=None, form=abi.OPERATOR_ASSEMBLED):
 
 def same(a, b):
     for n in FIELDS:
+        if n == "CsrCofA" and not a.scalar("AssembledBlocks"):
+            n = "MGCofA_L0"  # factored scenes build no blocks; compare their level 0 instead
         x, y = a.read(n), b.read(n)
         assert x.shape == y.shape and np.array_equal(x.view(np.uint8), y.view(np.uint8)), n
 
