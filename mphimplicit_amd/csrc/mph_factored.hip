@@ -184,13 +184,27 @@ __global__ void k_fac_g(DevParams P, const int* __restrict__ cellIndex, const in
     gmask[k] = (int)mask;
 }
 
-// level-0 row cell g (workgroup), stencil cell c (thread): the 3x3 block A0(g, c)
+// level-0 row cell g (workgroup = one wavefront): the 3x3 blocks A0(g, c) of its
+// ncs stencil cells.  Lane l works on stencil cell c = l % ncs in partition
+// l / ncs; the 64 / ncs partitions split the row cell's particles (viscous part)
+// and the candidate list (C Lambda C^T part) round robin, and their sums are added
+// in partition order at the end.  The candidates are the particles of the cells
+// around g whose G table holds g, filtered 64 at a time into an ordered list in LDS
+// (flushed when full).  The list and the partition sums share one LDS buffer.
+constexpr int kGal0List = 384;
 __global__ void __launch_bounds__(64) k_fac_gal0(DevParams P, Level L, FacOp F, const int* __restrict__ cellStart,
                                                  const unsigned short* __restrict__ runStart,
                                                  const double* __restrict__ gtab, const int* __restrict__ gmask) {
+    __shared__ double sbuf[64 * 9];   // 4608 B: the list, then the partition sums
+    static_assert(3 * kGal0List * sizeof(int) <= sizeof(sbuf), "list exceeds the shared buffer");
+    int* lk = (int*)sbuf;             // candidate particle
+    int* lc = lk + kGal0List;         // its cell's offset from g, (dx+1) + 3 (dy+1) + 9 (dz+1)
+    int* lm = lc + kGal0List;         // its G mask
     const int ncs = L.cs[0] * L.cs[1] * L.cs[2];
-    const int g = blockIdx.x, c = threadIdx.x;
-    if (c >= ncs) return;  // no barriers below
+    const int nparts = 64 / ncs;
+    const int g = blockIdx.x, lane = threadIdx.x;
+    const int c = lane % ncs, part = lane / ncs;
+    const bool active = part < nparts;
     int gx, gy, gz;
     body::grid_xyz(L, g, gx, gy, gz);
     const int ax = gx + L.mn[0], ay = gy + L.mn[1], az = gz + L.mn[2];
@@ -198,65 +212,134 @@ __global__ void __launch_bounds__(64) k_fac_gal0(DevParams P, Level L, FacOp F, 
     const int p0 = cellStart[ic], p1 = cellStart[ic + 1];
     double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     // viscous part and the diagonal blocks: the runs of the row cell's particles
-    // that fall in stencil cell c
-    const unsigned vmask = body::galerkin_mask(P, L, cellStart, g, c);
+    // that fall in stencil cell c, four entries loaded at a time
+    const unsigned vmask = active ? body::galerkin_mask(P, L, cellStart, g, c) : 0u;
     if (vmask)
-        for (int ip = p0; ip < p1; ++ip) {
+        for (int ip = p0 + part; ip < p1; ip += nparts) {
             const unsigned short* rs = runStart + (long long)ip * kRuns;
             const long long b0 = F.nPtr[ip];
             const double pi0 = F.pos[3 * ip], pi1 = F.pos[3 * ip + 1], pi2 = F.pos[3 * ip + 2];
             for (unsigned m = vmask; m; m &= m - 1) {
                 const int q = __builtin_ctz(m);
-                for (int jn = rs[q]; jn < rs[q + 1]; ++jn) {
-                    const int j = F.nInd[b0 + jn];
-                    if (j == ip) {
-                        for (int e = 0; e < 9; ++e) acc[e] += F.diag[9 * ip + e];
-                    } else {
-                        double r[3];
-                        body::fac_disp(P, F.pos, j, pi0, pi1, pi2, r[0], r[1], r[2]);
-                        const double a = ((const double*)F.alpha)[b0 + jn];
-                        for (int rr = 0; rr < 3; ++rr)
-                            for (int ss = 0; ss < 3; ++ss) acc[3 * rr + ss] -= a * r[rr] * r[ss];
+                const int e1 = rs[q + 1];
+                for (int jn = rs[q]; jn < e1; jn += 4) {
+                    int j[4];
+                    double a[4], r[4][3];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const bool in = jn + u < e1;
+                        j[u] = in ? F.nInd[b0 + jn + u] : ip;
+                        a[u] = in ? ((const double*)F.alpha)[b0 + jn + u] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) body::fac_disp(P, F.pos, j[u], pi0, pi1, pi2, r[u][0], r[u][1], r[u][2]);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (jn + u >= e1) continue;
+                        if (j[u] == ip) {
+                            for (int e = 0; e < 9; ++e) acc[e] += F.diag[9 * ip + e];
+                        } else {
+                            for (int rr = 0; rr < 3; ++rr)
+                                for (int ss = 0; ss < 3; ++ss) acc[3 * rr + ss] -= a[u] * r[u][rr] * r[u][ss];
+                        }
                     }
                 }
             }
         }
-    // C Lambda C^T part: lambda_k G_gk G_hk^T over the particles k around g whose G
-    // has both cells; h is stencil cell c (level-0 counts of at least 3 per axis:
-    // the caller checks)
+    // C Lambda C^T part: lambda_k G_gk G_hk^T; h is stencil cell c (level-0 counts
+    // of at least 3 per axis: the caller checks)
     const int iX = c / (L.cs[1] * L.cs[2]), iY = (c / L.cs[2]) % L.cs[1], iZ = c % L.cs[2];
-    const int hx = L.mn[0] + (gx + (L.cs[0] == 3 ? iX - 1 : 0) + L.cnt[0]) % L.cnt[0];
-    const int hy = L.mn[1] + (gy + (L.cs[1] == 3 ? iY - 1 : 0) + L.cnt[1]) % L.cnt[1];
-    const int hz = L.mn[2] + (gz + (L.cs[2] == 3 ? iZ - 1 : 0) + L.cnt[2]) % L.cnt[2];
     const int* C = P.C;
+    const int hx = (((L.mn[0] + (gx + (L.cs[0] == 3 ? iX - 1 : 0) + L.cnt[0]) % L.cnt[0]) % C[0]) + C[0]) % C[0];
+    const int hy = (((L.mn[1] + (gy + (L.cs[1] == 3 ? iY - 1 : 0) + L.cnt[1]) % L.cnt[1]) % C[1]) + C[1]) % C[1];
+    const int hz = (((L.mn[2] + (gz + (L.cs[2] == 3 ? iZ - 1 : 0) + L.cnt[2]) % L.cnt[2]) % C[2]) + C[2]) % C[2];
+    const int wx = ((ax % C[0]) + C[0]) % C[0], wy = ((ay % C[1]) + C[1]) % C[1], wz = ((az % C[2]) + C[2]) % C[2];
+    int n = 0;  // entries in the list (uniform)
+    auto wrapc = [](int v, int cnt) { return v < 0 ? v + cnt : (v >= cnt ? v - cnt : v); };
+    auto consume = [&]() {
+        if (active)
+            for (int t0 = part; t0 < n; t0 += 4 * nparts) {  // four entries' loads in flight
+                int k[4], sg[4], sh[4];  // idle entries: particle 0, slot 0 (in bounds, not added)
+                bool on[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int t = t0 + u * nparts;
+                    k[u] = t < n ? lk[t] : 0;
+                    sg[u] = sh[u] = 0;
+                    on[u] = false;
+                    if (t < n) {
+                        const int o = lc[t];
+                        const int kx = wrapc(wx + o % 3 - 1, C[0]);
+                        const int ky = wrapc(wy + (o / 3) % 3 - 1, C[1]);
+                        const int kz = wrapc(wz + o / 9 - 1, C[2]);
+                        const int s = fac_slot(P, hx, hy, hz, kx, ky, kz);
+                        if (s >= 0 && (((unsigned)lm[t] >> s) & 1u)) {
+                            on[u] = true;
+                            sh[u] = s;
+                            sg[u] = fac_slot(P, wx, wy, wz, kx, ky, kz);
+                        }
+                    }
+                }
+                double gg[4][3], gh[4][3], lv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double* G = gtab + 81 * (long long)k[u];
+                    lv[u] = on[u] ? F.lambda[k[u]] : 0.0;
+                    for (int e = 0; e < 3; ++e) {
+                        gg[u][e] = G[3 * sg[u] + e];
+                        gh[u][e] = G[3 * sh[u] + e];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (on[u])
+                        for (int rr = 0; rr < 3; ++rr) {
+                            const double lg = lv[u] * gg[u][rr];
+                            for (int ss = 0; ss < 3; ++ss) acc[3 * rr + ss] += lg * gh[u][ss];
+                        }
+            }
+        __syncthreads();
+        n = 0;
+    };
     for (int dx = -P.range[0]; dx <= P.range[0]; ++dx)
         for (int dy = -P.range[1]; dy <= P.range[1]; ++dy)
             for (int dz = -P.range[2]; dz <= P.range[2]; ++dz) {
                 const int kx = ((ax + dx) % C[0] + C[0]) % C[0];
                 const int ky = ((ay + dy) % C[1] + C[1]) % C[1];
                 const int kz = ((az + dz) % C[2] + C[2]) % C[2];
-                const int sg = fac_slot(P, ((ax % C[0]) + C[0]) % C[0], ((ay % C[1]) + C[1]) % C[1],
-                                        ((az % C[2]) + C[2]) % C[2], kx, ky, kz);
-                const int sh = fac_slot(P, ((hx % C[0]) + C[0]) % C[0], ((hy % C[1]) + C[1]) % C[1],
-                                        ((hz % C[2]) + C[2]) % C[2], kx, ky, kz);
-                if (sg < 0 || sh < 0) continue;
+                const int sg = fac_slot(P, wx, wy, wz, kx, ky, kz);
+                if (sg < 0) continue;
                 const int kc = body::cell_id(P, kx, ky, kz);
                 for (int half = 0; half < 2; ++half) {  // fluid, then wall particles of the cell
                     const int q0 = cellStart[half * P.T + kc], q1 = cellStart[half * P.T + kc + 1];
-                    for (int k = q0; k < q1; ++k) {
-                        const unsigned m = (unsigned)gmask[k];
-                        if (!((m >> sg) & 1u) || !((m >> sh) & 1u)) continue;
-                        const double* G = gtab + 81 * (long long)k;
-                        const double lk = F.lambda[k];
-                        for (int rr = 0; rr < 3; ++rr) {
-                            const double lg = lk * G[3 * sg + rr];
-                            for (int ss = 0; ss < 3; ++ss) acc[3 * rr + ss] += lg * G[3 * sh + ss];
+                    for (int base = q0; base < q1; base += 64) {
+                        const int k = base + lane;
+                        const unsigned km = k < q1 ? (unsigned)gmask[k] : 0u;
+                        const bool hit = (km >> sg) & 1u;
+                        const unsigned long long bal = __ballot(hit);
+                        const int before = __popcll(bal & ((1ull << lane) - 1ull));
+                        const int cnt = __popcll(bal);
+                        if (n + cnt > kGal0List) consume();
+                        if (hit) {
+                            lk[n + before] = k;
+                            lc[n + before] = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                            lm[n + before] = (int)km;
                         }
+                        n += cnt;
+                        __syncthreads();
                     }
                 }
             }
-    for (int rr = 0; rr < 3; ++rr)
-        for (int ss = 0; ss < 3; ++ss) L.A[(long long)(3 * c + ss) * L.nrow + 3 * g + rr] = acc[3 * rr + ss];
+    consume();  // ends with a barrier: the list is dead, sbuf takes the partition sums
+    if (active)
+        for (int e = 0; e < 9; ++e) sbuf[9 * lane + e] = acc[e];
+    __syncthreads();
+    if (part == 0) {
+        for (int q = 1; q < nparts; ++q)
+            for (int e = 0; e < 9; ++e) acc[e] += sbuf[9 * (q * ncs + c) + e];
+        for (int rr = 0; rr < 3; ++rr)
+            for (int ss = 0; ss < 3; ++ss) L.A[(long long)(3 * c + ss) * L.nrow + 3 * g + rr] = acc[3 * rr + ss];
+    }
 }
 
 // InvDiagA (calculateInvDiagA, main.cpp:3045-3068): 1/A_ii[r][r] with
