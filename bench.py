@@ -143,8 +143,8 @@ def run_gpu(args, precision="fp64", form=None):
     rf64 = roofline(k64, scene.scalar("SpmvMs"), l64_prev, b64, r64, traffic if precision == "fp64" else None)
     if factored:
         rf64["note"] = ("the factored product moves ~2.8 GB of algorithmic HBM bytes where the assembled one moves "
-                        "13.2 GB; its limit is the cache traffic of gathering neighbours' 64-byte records (x, "
-                        "position, t) through L2 and the Infinity Cache, not HBM")
+                        "13.2 GB; its limit is the cache traffic of gathering neighbours' x, position and t "
+                        "(three 16-byte pairs per entry) through L2 and the Infinity Cache, not HBM")
     rf32 = roofline(k32, scene.scalar("SpmvMsF32"), l32_prev, b32, r32, None)
     tuned = {"spmv_fp64_variant": int(scene.scalar("TunedSpmv64")), "spmv_fp32_variant": int(scene.scalar("TunedSpmv32")),
              "cac_variant": int(scene.scalar("TunedCac")), "fuse_variant": int(scene.scalar("TunedFuse")),

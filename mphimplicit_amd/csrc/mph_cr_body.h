@@ -563,8 +563,20 @@ __device__ inline double group_sum(double v, int G) {
 }
 
 // lanes per particle in both passes, and the list entries each lane has in flight
-// per batch (viscous entries; P entries)
-constexpr int kFacLanes = 8, kFacBatch = 8, kFacBatchP = 4;
+// per batch (viscous entries FP64 and FP32; P entries)
+constexpr int kFacLanes = 8, kFacBatch = 4, kFacBatch32 = 8, kFacBatchP = 4;
+
+// the four planes of FacOp::rec
+struct FacPlanes {
+    double* a;  // [np][2] x0 x1
+    double* b;  // [np][2] x2 p0
+    double* c;  // [np][2] p1 p2
+    double* t;  // [np]
+};
+__device__ inline FacPlanes fac_planes(const DevParams& P, double* rec) {
+    const long long n = P.np;
+    return {rec, rec + 2 * n, rec + 4 * n, rec + 6 * n};
+}
 
 // first pass over the 256 particles of `chunk` (fluid or wall):
 //   t_k = lambda_k (C_kk . x_k [k fluid] - sum_{j fluid, j != k} beta_kj r_kj . x_j),
@@ -617,15 +629,15 @@ __device__ inline void fac_t_chunk(const DevParams& P, const FacOp& F, const dou
         }
         acc = group_sum(acc, G);
         if (g == 0 && k < P.np) {
-            double* rk = F.rec + 8 * (long long)k;
+            const FacPlanes R = fac_planes(P, F.rec);
             if (k < P.nf) {
                 const double x0 = x[3 * k], x1 = x[3 * k + 1], x2 = x[3 * k + 2];
                 acc += (F.selfC[3 * k] * x0 + F.selfC[3 * k + 1] * x1) + F.selfC[3 * k + 2] * x2;
-                rk[0] = x0;
-                rk[1] = x1;
-                rk[2] = x2;
+                R.a[2 * k] = x0;
+                R.a[2 * k + 1] = x1;
+                R.b[2 * k] = x2;
             }
-            rk[6] = run > 0 ? acc * F.lambda[k] : 0.0;
+            R.t[k] = run > 0 ? acc * F.lambda[k] : 0.0;
         }
     }
 }
@@ -643,7 +655,11 @@ __device__ inline void fac_y_chunk(const DevParams& P, const FacOp& F, const dou
     typedef double dv2 __attribute__((ext_vector_type(2)));
     constexpr int G = kFacLanes;
     const int g = threadIdx.x & (G - 1);
-    const dv2* __restrict__ rec = (const dv2*)F.rec;  // 4 x 16 B per particle
+    const FacPlanes R = fac_planes(P, F.rec);
+    const dv2* __restrict__ ra = (const dv2*)R.a;
+    const dv2* __restrict__ rb = (const dv2*)R.b;
+    const dv2* __restrict__ rc = (const dv2*)R.c;
+    const double* __restrict__ rt = R.t;
     const VT* __restrict__ alpha = (const VT*)F.alpha;
     const double W0 = P.W[0], W1 = P.W[1], W2 = P.W[2], h0 = P.hW[0], h1 = P.hW[1], h2 = P.hW[2];
     for (int r = threadIdx.x / G; r < 256; r += 256 / G) {
@@ -668,9 +684,9 @@ __device__ inline void fac_y_chunk(const DevParams& P, const FacOp& F, const dou
                 dv2 q0[U], q1[U], q2[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    q0[u] = rec[4 * (long long)j[u]];
-                    q1[u] = rec[4 * (long long)j[u] + 1];
-                    q2[u] = rec[4 * (long long)j[u] + 2];
+                    q0[u] = ra[j[u]];
+                    q1[u] = rb[j[u]];
+                    q2[u] = rc[j[u]];
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -696,19 +712,20 @@ __device__ inline void fac_y_chunk(const DevParams& P, const FacOp& F, const dou
                     k[u] = jn < n ? F.nIndP[c0 + jn] : i;
                     bb[u] = jn < n ? F.beta[c0 + jn] : 0.0;
                 }
-                dv2 q1[U], q2[U], q3[U];
+                dv2 q1[U], q2[U];
+                double q3[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    q1[u] = rec[4 * (long long)k[u] + 1];
-                    q2[u] = rec[4 * (long long)k[u] + 2];
-                    q3[u] = rec[4 * (long long)k[u] + 3];
+                    q1[u] = rb[k[u]];
+                    q2[u] = rc[k[u]];
+                    q3[u] = rt[k[u]];
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const double r0 = fac_wrap<kWrap>(q1[u][1] - pi0, W0, h0);
                     const double r1 = fac_wrap<kWrap>(q2[u][0] - pi1, W1, h1);
                     const double r2 = fac_wrap<kWrap>(q2[u][1] - pi2, W2, h2);
-                    const double f = bb[u] * q3[u][0];
+                    const double f = bb[u] * q3[u];
                     s0 += f * r0;
                     s1 += f * r1;
                     s2 += f * r2;
@@ -721,7 +738,7 @@ __device__ inline void fac_y_chunk(const DevParams& P, const FacOp& F, const dou
         if (g == 0 && i < P.nf) {
             const double* D = F.diag + 9 * i;
             const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
-            const double ti = F.rec[8 * (long long)i + 6];
+            const double ti = rt[i];
             double d0 = (D[0] * xi0 + D[1] * xi1) + D[2] * xi2;
             double d1 = (D[3] * xi0 + D[4] * xi1) + D[5] * xi2;
             double d2 = (D[6] * xi0 + D[7] * xi1) + D[8] * xi2;
@@ -813,7 +830,7 @@ __device__ inline void fac_y32_chunk(const DevParams& P, const FacOp& F, const d
                                      double* sh, double* rows) {
 #pragma clang fp contract(fast)
     typedef float fv4 __attribute__((ext_vector_type(4)));
-    constexpr int G = kFacLanes, U = kFacBatch;
+    constexpr int G = kFacLanes, U = kFacBatch32;
     const int g = threadIdx.x & (G - 1);
     const fv4* __restrict__ rec = (const fv4*)F.rec32;
     // the viscous loop is software-pipelined: the columns and u of the next batch

@@ -24,18 +24,18 @@
 // The product is the assembled one up to rounding: the same terms, associated
 // differently.  It reads about 12 bytes per block (a 4-byte column and an 8-byte
 // alpha) where the assembled product reads 76.  A neighbour's x, position and t are
-// gathered through the caches from its 64-byte record (FacOp::rec): three 16-byte
-// loads per entry, where separate arrays would take six or four 8-byte gathers.
-// Each row is walked by a group of 8 lanes, each with 8 entries in flight, so that
-// the gathers of one load instruction fall on consecutive particles of the
-// cell-ordered list and enough of them are outstanding.  Measured on a jittered
-// 100^3 lattice (172 entries per row), every arrangement of this pass (one lane per
-// row, groups of 8 to 64 lanes, with or without an XCD-aware block order) lands
-// between 0.94 and 1.3 ms: the cache traffic of the record gathers, not HBM,
-// bounds it.
+// gathered through the caches from planes of 16-byte pairs (FacOp::rec): three
+// 16-byte loads per entry, where separate arrays would take six or four 8-byte
+// gathers.  Each row is walked by a group of 8 lanes, each with 4 entries in flight,
+// so that the gathers of one load instruction fall on consecutive particles of the
+// cell-ordered list, and so on consecutive pairs of a plane.  Measured on a jittered
+// 100^3 lattice (172 entries per row), every arrangement of this pass with 64-byte
+// records per particle (one lane per row, groups of 8 to 64 lanes, with or without
+// an XCD-aware block order) landed between 0.94 and 1.3 ms; the planes take it to
+// 0.90 ms.  The cache traffic of the gathers, not HBM, bounds it.
 //
-// The assembled matrix is still built every step. The multigrid's Galerkin product
-// and the diagonal preconditioner read it.
+// Scenes whose multigrid levels allow it assemble no 3x3 blocks at all: level 0
+// and InvDiagA come from the same scalars (k_fac_g, k_fac_gal0, k_fac_invdiag).
 #include "mph_cr_body.h"
 
 namespace mph {
@@ -80,8 +80,14 @@ __global__ void k_fac_p(DevParams P, const double* __restrict__ pos, const int* 
         }
     }
     runP[k] = run;
-    double* rk = rec + 8 * (long long)k;
-    for (int e = 0; e < 8; ++e) rk[e] = e >= 3 && e < 6 ? pos[3 * k + e - 3] : 0.0;
+    const body::FacPlanes R = body::fac_planes(P, rec);
+    R.a[2 * k] = 0.0;
+    R.a[2 * k + 1] = 0.0;
+    R.b[2 * k] = 0.0;
+    R.b[2 * k + 1] = pk0;
+    R.c[2 * k] = pk1;
+    R.c[2 * k + 1] = pk2;
+    R.t[k] = 0.0;
     if (k < P.nf) {
         selfC[3 * k] = c0;
         selfC[3 * k + 1] = c1;

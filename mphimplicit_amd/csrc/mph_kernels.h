@@ -72,10 +72,12 @@ struct FacOp {
     const double* beta;       // -w_p'(d)/d per P entry
     const double* lambda;     // [np]
     const double* pos;        // [np][3]
-    // [np][8] per particle: x0 x1 x2 p0 p1 p2 t 0 -- the operand x (fluid), the
-    // position and t = lambda_k C_k^T x, so the second pass gathers a neighbour's
-    // operands with three aligned 16-byte loads.  Positions are written with the P
-    // slices; x and t by every first pass.
+    // [7 np] the operand x (fluid), the position and t = lambda_k C_k^T x of every
+    // particle in four planes: [np] pairs x0 x1, [np] pairs x2 p0, [np] pairs p1 p2,
+    // then [np] t (body::fac_planes).  The second pass gathers a neighbour's
+    // operands with three 16-byte loads, and the eight lanes of a row, which walk
+    // consecutive particles of the cell-ordered list, read consecutive 16-byte
+    // pairs.  Positions are written with the P slices; x and t by every first pass.
     double* rec;
     // 1 when some neighbour pair interacts through the periodic boundary (then the
     // displacements take the minimum image); set by the matrix-A launch

@@ -301,7 +301,7 @@ int phase_matrix_c(mph_scene& S) {
     if (factored(S)) {
         const long long ne = S.EP > 0 ? S.EP : 1;
         if ((rc = S.fbeta.ensure(ne)) || (rc = S.runP.ensure(S.P.np)) || (rc = S.selfC.ensure(3 * (size_t)S.P.nf)) ||
-            (rc = S.frec.ensure(8 * (size_t)S.P.np)))
+            (rc = S.frec.ensure(7 * (size_t)S.P.np)))
             return rc;
         if (mixed(S) && ((rc = S.fv32.ensure(3 * ne)) || (rc = S.fu32.ensure(3 * (S.E > 0 ? S.E : 1))) ||
                          (rc = S.frec32.ensure(4 * (size_t)S.P.np))))
