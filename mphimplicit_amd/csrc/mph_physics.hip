@@ -3,8 +3,11 @@
 // Replaces calculateConvection/Wall/PeriodicBoundary/resetForce, the explicit force
 // loops (source/main.cpp:1678-2128, 4741-4752), calculateMatrixA/C and
 // multiplyMatrixC (2137-2643), and the output-step diagnostics (1932-2063,
-// 4494-4738).  One thread owns one particle; each of its sums runs over the
-// neighbour list in list order, so every value is bit-identical to the reference.
+// 4494-4738).  Each of a particle's sums runs over the neighbour list in list
+// order, so every value is bit-identical to the reference.  The list walks of the
+// densities, forces and matrix A are entry-parallel: a group of kLanes lanes owns
+// the particle, lane g computes the terms of entries g, g + kLanes, ..., and the
+// group's first lane adds them, read back through LDS, in list order.
 #include "mph_kernels.h"
 
 namespace mph {
@@ -23,6 +26,17 @@ __device__ inline Rel rel(const DevParams& P, const double* __restrict__ pos, in
     for (int d = 0; d < 3; ++d) o.x[d] = min_image(pos[3 * j + d], pos[3 * i + d], P.W[d], P.hW[d]);
     o.r2 = (o.x[0] * o.x[0] + o.x[1] * o.x[1] + o.x[2] * o.x[2]);
     return o;
+}
+
+// lanes per particle of the entry-parallel list walks; a workgroup is 256 lanes
+constexpr int kLanes = 8;
+
+// orders the LDS writes and reads of the lanes of one wavefront (the lanes of a
+// group are in one wavefront and run the same iterations)
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // calculateConvection, calculateWall, calculatePeriodicBoundary, resetForce
@@ -58,43 +72,75 @@ __global__ void k_kinematics(DevParams P, WallKin wk, const int* __restrict__ ty
 }
 
 // calculateDensityA, calculateGravityCenter, calculateDensityP (three sums, one list walk)
-__global__ void k_densities(DevParams P, const int* __restrict__ type, const double* __restrict__ pos,
-                            const int* __restrict__ nCount, const long long* __restrict__ nPtr,
-                            const int* __restrict__ nInd, double* __restrict__ densA, double* __restrict__ gcen,
-                            double* __restrict__ volStrainP) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.np) return;
-    const int ti = type[i];
+__global__ void __launch_bounds__(256) k_densities(DevParams P, const int* __restrict__ type,
+                                                   const double* __restrict__ pos, const int* __restrict__ nCount,
+                                                   const long long* __restrict__ nPtr, const int* __restrict__ nInd,
+                                                   double* __restrict__ densA, double* __restrict__ gcen,
+                                                   double* __restrict__ volStrainP) {
+    constexpr int G = kLanes, K = 5;  // terms: DensityA, the three GravityCenter ones, DensityP
+    __shared__ double tb[256 * K];
+    __shared__ int tf[256];
+    const int g = threadIdx.x & (G - 1);
+    const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / G);
+    const bool own = i < P.np;
+    const int ti = own ? type[i] : 0;
     double sa = 0.0, sg0 = 0.0, sg1 = 0.0, sg2 = 0.0, sp = 0.0;
-    const long long b = nPtr[i], e = b + nCount[i];
-    for (long long k = b; k < e; ++k) {
-        const int j = nInd[k];
-        if (i == j) continue;
-        const double ratio = P.ratio[ti][type[j]];
-        const Rel q = rel(P, pos, i, j);
-        if (P.radA2 - q.r2 >= 0) {
-            const double r = sqrt(q.r2);
-            const double weight = ratio * wa_k(P.Ka, r, P.radA);
-            sa += weight;
+    const long long b = own ? nPtr[i] : 0;
+    const int n = own ? nCount[i] : 0;
+    for (int base = 0; base < n; base += G) {
+        const int jn = base + g;
+        int f = 0;
+        double t[K] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (jn < n) {
+            const int j = nInd[b + jn];
+            if (i != j) {
+                const double ratio = P.ratio[ti][type[j]];
+                const Rel q = rel(P, pos, i, j);
+                if (P.radA2 - q.r2 >= 0) {
+                    const double r = sqrt(q.r2);
+                    t[0] = ratio * wa_k(P.Ka, r, P.radA);
+                    f |= 1;
+                }
+                if (P.radG2 - q.r2 >= 0) {
+                    const double r = sqrt(q.r2);
+                    const double weight = ratio * wq_k(P.Kg, r, P.radG);
+                    t[1] = q.x[0] * weight / P.R2g * P.radG;
+                    t[2] = q.x[1] * weight / P.R2g * P.radG;
+                    t[3] = q.x[2] * weight / P.R2g * P.radG;
+                    f |= 2;
+                }
+                if (P.radP2 - q.r2 >= 0) {
+                    const double r = sqrt(q.r2);
+                    t[4] = wq_k(P.Kp, r, P.radP);
+                    f |= 4;
+                }
+            }
         }
-        if (P.radG2 - q.r2 >= 0) {
-            const double r = sqrt(q.r2);
-            const double weight = ratio * wq_k(P.Kg, r, P.radG);
-            sg0 += q.x[0] * weight / P.R2g * P.radG;
-            sg1 += q.x[1] * weight / P.R2g * P.radG;
-            sg2 += q.x[2] * weight / P.R2g * P.radG;
-        }
-        if (P.radP2 - q.r2 >= 0) {
-            const double r = sqrt(q.r2);
-            const double weight = wq_k(P.Kp, r, P.radP);
-            sp += weight;
-        }
+        for (int e = 0; e < K; ++e) tb[K * threadIdx.x + e] = t[e];
+        tf[threadIdx.x] = f;
+        wave_sync();
+        if (g == 0)
+            for (int u = 0; u < G && base + u < n; ++u) {
+                const int sl = threadIdx.x + u;
+                const int fu = tf[sl];
+                const double* tu = tb + K * sl;
+                if (fu & 1) sa += tu[0];
+                if (fu & 2) {
+                    sg0 += tu[1];
+                    sg1 += tu[2];
+                    sg2 += tu[3];
+                }
+                if (fu & 4) sp += tu[4];
+            }
+        wave_sync();
     }
-    densA[i] = sa;
-    gcen[3 * i + 0] = sg0;
-    gcen[3 * i + 1] = sg1;
-    gcen[3 * i + 2] = sg2;
-    volStrainP[i] = (sp - P.N0p);
+    if (g == 0 && own) {
+        densA[i] = sa;
+        gcen[3 * i + 0] = sg0;
+        gcen[3 * i + 1] = sg1;
+        gcen[3 * i + 2] = sg2;
+        volStrainP[i] = (sp - P.N0p);
+    }
 }
 
 // calculatePhysicalCoefficients (1704-1735) and the PressureA values (1807-1812)
@@ -116,49 +162,79 @@ __global__ void k_coefficients(DevParams P, const int* __restrict__ type, const 
 
 // calculatePressureA forces (1814-1848), calculateDiffuseInterface (1851-1901),
 // calculateGravity (2065-2074), calculateAcceleration (2076-2086)
-__global__ void k_forces(DevParams P, const int* __restrict__ type, const double* __restrict__ pos,
-                         const int* __restrict__ nCount, const long long* __restrict__ nPtr,
-                         const int* __restrict__ nInd, const double* __restrict__ pressA,
-                         const double* __restrict__ gcen, const double* __restrict__ mass, double* __restrict__ force,
-                         double* __restrict__ vel) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.np) return;
-    const int ti = type[i];
+__global__ void __launch_bounds__(256) k_forces(DevParams P, const int* __restrict__ type,
+                                                const double* __restrict__ pos, const int* __restrict__ nCount,
+                                                const long long* __restrict__ nPtr, const int* __restrict__ nInd,
+                                                const double* __restrict__ pressA, const double* __restrict__ gcen,
+                                                const double* __restrict__ mass, double* __restrict__ force,
+                                                double* __restrict__ vel) {
+    constexpr int G = kLanes, K = 9;  // terms: PressureA force, the two DiffuseInterface ones (3 each)
+    __shared__ double tb[256 * K];
+    __shared__ int tf[256];
+    const int g = threadIdx.x & (G - 1);
+    const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / G);
+    const bool own = i < P.np;
+    const int ti = own ? type[i] : 0;
     const double ai = P.ai[ti];
     const double aj = P.ai[ti];  // the reference uses i's type for aj (main.cpp:1863)
     const double VoL = P.V / P.l;
-    const double pai = pressA[i];
-    const double gi0 = gcen[3 * i], gi1 = gcen[3 * i + 1], gi2 = gcen[3 * i + 2];
+    const double pai = own ? pressA[i] : 0.0;
+    const double gi0 = own ? gcen[3 * i] : 0.0, gi1 = own ? gcen[3 * i + 1] : 0.0, gi2 = own ? gcen[3 * i + 2] : 0.0;
     double fa[3] = {0.0, 0.0, 0.0}, fd[3] = {0.0, 0.0, 0.0};
-    const long long b = nPtr[i], e = b + nCount[i];
-    for (long long k = b; k < e; ++k) {
-        const int j = nInd[k];
-        if (i == j) continue;
-        const int tj = type[j];
-        const double rij_ = P.ratio[ti][tj], rji_ = P.ratio[tj][ti];
-        const Rel q = rel(P, pos, i, j);
-        if (q.r2 == 0.0) continue;
-        if (P.radA2 - q.r2 > 0) {
-            const double r = sqrt(q.r2);
-            const double dwij = rij_ * dwadr_k(P.Ka, P.invA, r, P.radA);
-            const double dwji = rji_ * dwadr_k(P.Ka, P.invA, r, P.radA);
-            const double paj = pressA[j];
-            for (int d = 0; d < 3; ++d) fa[d] += (pai * dwij + paj * dwji) * (q.x[d] / r) * P.V;
+    const long long b = own ? nPtr[i] : 0;
+    const int n = own ? nCount[i] : 0;
+    for (int base = 0; base < n; base += G) {
+        const int jn = base + g;
+        int f = 0;
+        double t[K] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (jn < n) {
+            const int j = nInd[b + jn];
+            const Rel q = rel(P, pos, i, j);
+            if (i != j && q.r2 != 0.0) {
+                const int tj = type[j];
+                const double rij_ = P.ratio[ti][tj], rji_ = P.ratio[tj][ti];
+                if (P.radA2 - q.r2 > 0) {
+                    const double r = sqrt(q.r2);
+                    const double dwij = rij_ * dwadr_k(P.Ka, P.invA, r, P.radA);
+                    const double dwji = rji_ * dwadr_k(P.Ka, P.invA, r, P.radA);
+                    const double paj = pressA[j];
+                    for (int d = 0; d < 3; ++d) t[d] = (pai * dwij + paj * dwji) * (q.x[d] / r) * P.V;
+                    f |= 1;
+                }
+                if (P.radG2 - q.r2 > 0) {
+                    const double r = sqrt(q.r2);
+                    const double wij = rij_ * wq_k(P.Kg, r, P.radG);
+                    const double wji = rji_ * wq_k(P.Kg, r, P.radG);
+                    const double gj[3] = {gcen[3 * j], gcen[3 * j + 1], gcen[3 * j + 2]};
+                    const double gi[3] = {gi0, gi1, gi2};
+                    for (int d = 0; d < 3; ++d) t[3 + d] = (aj * gj[d] * wji - ai * gi[d] * wij) / P.R2g * P.radG * VoL;
+                    const double dwij = rij_ * dwq_k(P.Kg, P.m2G, r, P.radG);
+                    const double dwji = rji_ * dwq_k(P.Kg, P.m2G, r, P.radG);
+                    double gr = 0.0;
+                    for (int d = 0; d < 3; ++d) gr += (aj * gj[d] * dwji - ai * gi[d] * dwij) * q.x[d];
+                    for (int d = 0; d < 3; ++d) t[6 + d] = (gr) * (q.x[d] / r) / P.R2g * P.radG * VoL;
+                    f |= 2;
+                }
+            }
         }
-        if (P.radG2 - q.r2 > 0) {
-            const double r = sqrt(q.r2);
-            const double wij = rij_ * wq_k(P.Kg, r, P.radG);
-            const double wji = rji_ * wq_k(P.Kg, r, P.radG);
-            const double gj[3] = {gcen[3 * j], gcen[3 * j + 1], gcen[3 * j + 2]};
-            const double gi[3] = {gi0, gi1, gi2};
-            for (int d = 0; d < 3; ++d) fd[d] -= (aj * gj[d] * wji - ai * gi[d] * wij) / P.R2g * P.radG * VoL;
-            const double dwij = rij_ * dwq_k(P.Kg, P.m2G, r, P.radG);
-            const double dwji = rji_ * dwq_k(P.Kg, P.m2G, r, P.radG);
-            double gr = 0.0;
-            for (int d = 0; d < 3; ++d) gr += (aj * gj[d] * dwji - ai * gi[d] * dwij) * q.x[d];
-            for (int d = 0; d < 3; ++d) fd[d] -= (gr) * (q.x[d] / r) / P.R2g * P.radG * VoL;
-        }
+        for (int e = 0; e < K; ++e) tb[K * threadIdx.x + e] = t[e];
+        tf[threadIdx.x] = f;
+        wave_sync();
+        if (g == 0)
+            for (int u = 0; u < G && base + u < n; ++u) {
+                const int sl = threadIdx.x + u;
+                const int fu = tf[sl];
+                const double* tu = tb + K * sl;
+                if (fu & 1)
+                    for (int d = 0; d < 3; ++d) fa[d] += tu[d];
+                if (fu & 2) {
+                    for (int d = 0; d < 3; ++d) fd[d] -= tu[3 + d];
+                    for (int d = 0; d < 3; ++d) fd[d] -= tu[6 + d];
+                }
+            }
+        wave_sync();
     }
+    if (g != 0 || !own) return;
     double F[3];
     for (int d = 0; d < 3; ++d) {
         F[d] = force[3 * i + d];
@@ -176,104 +252,103 @@ __global__ void k_forces(DevParams P, const int* __restrict__ type, const double
 }
 
 // calculateMatrixA values (2251-2335): viscous blocks, inertia, wall terms of b
-__global__ void k_matrix_a(DevParams P, const int* __restrict__ type, const double* __restrict__ pos,
-                           const double* __restrict__ vel, const double* __restrict__ mass,
-                           const double* __restrict__ mu, const int* __restrict__ nCount,
-                           const int* __restrict__ nFluid, const long long* __restrict__ nPtr,
-                           const int* __restrict__ nInd, const long long* __restrict__ start, int* __restrict__ acol,
-                           double* __restrict__ aval, int* __restrict__ selfIdx, double* __restrict__ b,
-                           double* __restrict__ falpha, double* __restrict__ fdiag, int* __restrict__ fwrap) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.nf) return;
+__global__ void __launch_bounds__(256) k_matrix_a(DevParams P, const int* __restrict__ type,
+                                                  const double* __restrict__ pos, const double* __restrict__ vel,
+                                                  const double* __restrict__ mass, const double* __restrict__ mu,
+                                                  const int* __restrict__ nCount, const int* __restrict__ nFluid,
+                                                  const long long* __restrict__ nPtr, const int* __restrict__ nInd,
+                                                  const long long* __restrict__ start, int* __restrict__ acol,
+                                                  double* __restrict__ aval, int* __restrict__ selfIdx,
+                                                  double* __restrict__ b, double* __restrict__ falpha,
+                                                  double* __restrict__ fdiag, int* __restrict__ fwrap) {
+    constexpr int G = kLanes, K = 12;  // terms: the 3x3 coefficients, then a wall neighbour's velocity
+    __shared__ double tb[256 * K];
+    __shared__ int tf[256];  // bit 0: the self entry, bit 1: inside RadiusV, bit 2: and a wall neighbour
+    const int g = threadIdx.x & (G - 1);
+    const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / G);
+    const bool own = i < P.nf;
     int iN = 0;
     double self[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double sv0 = 0.0, sv1 = 0.0, sv2 = 0.0;
-    const double mui = mu[i];
-    const long long b0 = nPtr[i];
-    const int nfl = nFluid[i], nc = nCount[i];
-    const long long sb = start[i / kSlice];
+    const double mui = own ? mu[i] : 0.0;
+    const long long b0 = own ? nPtr[i] : 0;
+    const int nfl = own ? nFluid[i] : 0, nc = own ? nCount[i] : 0;
+    const long long sb = own ? start[i / kSlice] : 0;
     const int lane = i % kSlice;
-    // blocks 2m and 2m+1 are adjacent in memory: an even block waits in `pend` and
-    // is stored together with its odd partner as one 16-byte store per entry
-    double pend[9];
-    int pendCol = 0;
-    bool pending = false;
     // factored form: alpha_ij = visc mu_ij (-w_v'(d)) / d^3, the viscous block being
     // -alpha_ij r r^T (beside the neighbour list; 0 for self and beyond RadiusV)
-    // (streaming stores: each lane writes its own list, 64 lists per store)
     auto store_alpha = [&](int k, double a) {
         if (falpha) __builtin_nontemporal_store(a, falpha + b0 + k);
     };
     // the 3x3 blocks are stored unless aval is null (a factored scene that builds its
     // multigrid level 0 from the factored form needs none)
     const bool blocks = aval != nullptr;
-    auto store1 = [&](int k, const double* v, int col) {
-        if (!blocks) return;
-        acol[a_col_index(sb, k, lane)] = col;
-        for (int e9 = 0; e9 < 9; ++e9) aval[a_val_index(sb, k, e9, lane)] = v[e9];
-    };
-    for (int jn = 0; jn < nc; ++jn) {
-        const int j = nInd[b0 + jn];
-        const bool fluidj = jn < nfl;
-        if (i == j) {
-            iN = jn;
-            if (blocks) acol[a_col_index(sb, jn, lane)] = j;
-            if (pending) store1(jn - 1, pend, pendCol);  // the self block is stored last
-            pending = false;
-            if (jn < nfl) store_alpha(jn, 0.0);
-            continue;
-        }
-        double c9[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        double alpha = 0.0;
-        const Rel q = rel(P, pos, i, j);
-        // factored form: does any pair interact through the periodic boundary
-        if (fwrap)
-            for (int d = 0; d < 3; ++d)
-                if (fabs(pos[3 * j + d] - pos[3 * i + d]) > P.hW[d]) atomicOr(fwrap, 1);
-        const bool inside = q.r2 != 0.0 && P.radV2 - q.r2 > 0;
-        if (inside) {
-            const double dij = sqrt(q.r2);
-            const double wdrij = -dwq_k(P.Kv, P.m2V, dij, P.radV);
-            const double e[3] = {q.x[0] / dij, q.x[1] / dij, q.x[2] / dij};
-            const double muj = mu[j];
-            const double muij = 2.0 * (mui * muj) / (mui + muj);
-            if (falpha) alpha = P.visc * muij * wdrij / (dij * dij * dij);
-            for (int r = 0; r < 3; ++r)
-                for (int s = 0; s < 3; ++s) {
-                    const double coef = +P.visc * muij * e[s] * e[r] * wdrij / dij;
-                    self[3 * r + s] += coef;
-                    c9[3 * r + s] = coef;
-                }
-            if (is_wall(type[j])) {
-                const double v0 = vel[3 * j], v1 = vel[3 * j + 1], v2 = vel[3 * j + 2];
-                sv0 += c9[0] * v0; sv0 += c9[1] * v1; sv0 += c9[2] * v2;
-                sv1 += c9[3] * v0; sv1 += c9[4] * v1; sv1 += c9[5] * v2;
-                sv2 += c9[6] * v0; sv2 += c9[7] * v1; sv2 += c9[8] * v2;
-            }
-        }
-        if (fluidj) {
-            store_alpha(jn, alpha);
-            double v[9];
-            for (int e9 = 0; e9 < 9; ++e9) v[e9] = inside ? -c9[e9] : 0.0;
-            if ((jn & 1) == 0) {
-                for (int e9 = 0; e9 < 9; ++e9) pend[e9] = v[e9];
-                pendCol = j;
-                pending = true;
-            } else if (pending) {
-                typedef double d2 __attribute__((ext_vector_type(2)));
-                typedef int i2 __attribute__((ext_vector_type(2)));
-                if (blocks) {
-                    *(i2*)&acol[a_col_index(sb, jn - 1, lane)] = i2{pendCol, j};
-                    for (int e9 = 0; e9 < 9; ++e9)
-                        *(d2*)&aval[a_val_index(sb, jn - 1, e9, lane)] = d2{pend[e9], v[e9]};
-                }
-                pending = false;
+    for (int base = 0; base < nc; base += G) {
+        const int jn = base + g;
+        int f = 0;
+        double t[K] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (jn < nc) {
+            const int j = nInd[b0 + jn];
+            if (i == j) {
+                f = 1;
+                if (blocks) acol[a_col_index(sb, jn, lane)] = j;
+                if (jn < nfl) store_alpha(jn, 0.0);
             } else {
-                store1(jn, v, j);
+                double alpha = 0.0;
+                const Rel q = rel(P, pos, i, j);
+                // factored form: does any pair interact through the periodic boundary
+                if (fwrap)
+                    for (int d = 0; d < 3; ++d)
+                        if (fabs(pos[3 * j + d] - pos[3 * i + d]) > P.hW[d]) atomicOr(fwrap, 1);
+                const bool inside = q.r2 != 0.0 && P.radV2 - q.r2 > 0;
+                if (inside) {
+                    const double dij = sqrt(q.r2);
+                    const double wdrij = -dwq_k(P.Kv, P.m2V, dij, P.radV);
+                    const double e[3] = {q.x[0] / dij, q.x[1] / dij, q.x[2] / dij};
+                    const double muj = mu[j];
+                    const double muij = 2.0 * (mui * muj) / (mui + muj);
+                    if (falpha) alpha = P.visc * muij * wdrij / (dij * dij * dij);
+                    for (int r = 0; r < 3; ++r)
+                        for (int sdim = 0; sdim < 3; ++sdim)
+                            t[3 * r + sdim] = +P.visc * muij * e[sdim] * e[r] * wdrij / dij;
+                    f = 2;
+                    if (is_wall(type[j])) {
+                        t[9] = vel[3 * j];
+                        t[10] = vel[3 * j + 1];
+                        t[11] = vel[3 * j + 2];
+                        f |= 4;
+                    }
+                }
+                if (jn < nfl) {
+                    store_alpha(jn, alpha);
+                    if (blocks) {
+                        acol[a_col_index(sb, jn, lane)] = j;
+                        for (int e9 = 0; e9 < 9; ++e9) aval[a_val_index(sb, jn, e9, lane)] = inside ? -t[e9] : 0.0;
+                    }
+                }
             }
         }
+        for (int e = 0; e < K; ++e) tb[K * threadIdx.x + e] = t[e];
+        tf[threadIdx.x] = f;
+        wave_sync();
+        if (g == 0)
+            for (int u = 0; u < G && base + u < nc; ++u) {
+                const int sl = threadIdx.x + u;
+                const int fu = tf[sl];
+                const double* c9 = tb + K * sl;
+                if (fu & 1) iN = base + u;
+                if (fu & 2)
+                    for (int e9 = 0; e9 < 9; ++e9) self[e9] += c9[e9];
+                if (fu & 4) {
+                    const double v0 = c9[9], v1 = c9[10], v2 = c9[11];
+                    sv0 += c9[0] * v0; sv0 += c9[1] * v1; sv0 += c9[2] * v2;
+                    sv1 += c9[3] * v0; sv1 += c9[4] * v1; sv1 += c9[5] * v2;
+                    sv2 += c9[6] * v0; sv2 += c9[7] * v1; sv2 += c9[8] * v2;
+                }
+            }
+        wave_sync();
     }
-    if (pending) store1(nfl - 1, pend, pendCol);
+    if (g != 0 || !own) return;
     const double coef = mass[i] / P.V / P.dt;
     self[0] += coef;
     self[4] += coef;
@@ -501,8 +576,8 @@ void launch_kinematics(hipStream_t st, const DevParams& P, const WallKin& wk, co
 
 void launch_densities(hipStream_t st, const DevParams& P, const int* type, const double* pos, const int* nCount,
                       const long long* nPtr, const int* nInd, double* densA, double* gcen, double* volStrainP) {
-    hipLaunchKernelGGL(k_densities, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, type, pos, nCount, nPtr, nInd, densA,
-                       gcen, volStrainP);
+    hipLaunchKernelGGL(k_densities, dim3(nblk((long long)kLanes * P.np, 256)), dim3(256), 0, st, P, type, pos, nCount,
+                       nPtr, nInd, densA, gcen, volStrainP);
 }
 
 void launch_coefficients(hipStream_t st, const DevParams& P, const int* type, const double* densA,
@@ -515,16 +590,16 @@ void launch_coefficients(hipStream_t st, const DevParams& P, const int* type, co
 void launch_forces(hipStream_t st, const DevParams& P, const int* type, const double* pos, const int* nCount,
                    const long long* nPtr, const int* nInd, const double* pressA, const double* gcen,
                    const double* mass, double* force, double* vel) {
-    hipLaunchKernelGGL(k_forces, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, type, pos, nCount, nPtr, nInd, pressA,
-                       gcen, mass, force, vel);
+    hipLaunchKernelGGL(k_forces, dim3(nblk((long long)kLanes * P.np, 256)), dim3(256), 0, st, P, type, pos, nCount,
+                       nPtr, nInd, pressA, gcen, mass, force, vel);
 }
 
 void launch_matrix_a(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
                      const double* mass, const double* mu, const int* nCount, const int* nFluid,
                      const long long* nPtr, const int* nInd, ASlices A, int* selfIdx, double* b, double* alpha,
                      double* diag, int* wrap) {
-    hipLaunchKernelGGL(k_matrix_a, dim3(nblk(P.nf, kSlice)), dim3(kSlice), 0, st, P, type, pos, vel, mass, mu,
-                       nCount, nFluid, nPtr, nInd, A.start, A.col, A.val, selfIdx, b, alpha, diag, wrap);
+    hipLaunchKernelGGL(k_matrix_a, dim3(nblk((long long)kLanes * P.nf, 256)), dim3(256), 0, st, P, type, pos, vel,
+                       mass, mu, nCount, nFluid, nPtr, nInd, A.start, A.col, A.val, selfIdx, b, alpha, diag, wrap);
 }
 
 void launch_matrix_c(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP,
