@@ -197,15 +197,16 @@ __global__ void k_fac_g(DevParams P, const int* __restrict__ cellIndex, const in
 // in partition order at the end.  The candidates are the particles of the cells
 // around g whose G table holds g, filtered 64 at a time into an ordered list in LDS
 // (flushed when full).  The list and the partition sums share one LDS buffer.
-constexpr int kGal0List = 384;
+constexpr int kGal0List = 288;
 __global__ void __launch_bounds__(64) k_fac_gal0(DevParams P, Level L, FacOp F, const int* __restrict__ cellStart,
                                                  const unsigned short* __restrict__ runStart,
                                                  const double* __restrict__ gtab, const int* __restrict__ gmask) {
     __shared__ double sbuf[64 * 9];   // 4608 B: the list, then the partition sums
-    static_assert(3 * kGal0List * sizeof(int) <= sizeof(sbuf), "list exceeds the shared buffer");
+    static_assert(4 * kGal0List * sizeof(int) <= sizeof(sbuf), "list exceeds the shared buffer");
     int* lk = (int*)sbuf;             // candidate particle
     int* lc = lk + kGal0List;         // its cell's offset from g, (dx+1) + 3 (dy+1) + 9 (dz+1)
     int* lm = lc + kGal0List;         // its G mask
+    unsigned* lh = (unsigned*)(lm + kGal0List);  // the stencil cells its G mask reaches (a superset)
     const int ncs = L.cs[0] * L.cs[1] * L.cs[2];
     const int nparts = 64 / ncs;
     const int g = blockIdx.x, lane = threadIdx.x;
@@ -220,6 +221,8 @@ __global__ void __launch_bounds__(64) k_fac_gal0(DevParams P, Level L, FacOp F, 
     // viscous part and the diagonal blocks: the runs of the row cell's particles
     // that fall in stencil cell c, four entries loaded at a time
     const unsigned vmask = active ? body::galerkin_mask(P, L, cellStart, g, c) : 0u;
+    const bool wrapf = *F.wrap != 0;
+    const double W0 = P.W[0], W1 = P.W[1], W2 = P.W[2], h0 = P.hW[0], h1 = P.hW[1], h2 = P.hW[2];
     if (vmask)
         for (int ip = p0 + part; ip < p1; ip += nparts) {
             const unsigned short* rs = runStart + (long long)ip * kRuns;
@@ -238,7 +241,16 @@ __global__ void __launch_bounds__(64) k_fac_gal0(DevParams P, Level L, FacOp F, 
                         a[u] = in ? ((const double*)F.alpha)[b0 + jn + u] : 0.0;
                     }
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) body::fac_disp(P, F.pos, j[u], pi0, pi1, pi2, r[u][0], r[u][1], r[u][2]);
+                    for (int u = 0; u < 4; ++u) {
+                        r[u][0] = F.pos[3 * j[u]] - pi0;
+                        r[u][1] = F.pos[3 * j[u] + 1] - pi1;
+                        r[u][2] = F.pos[3 * j[u] + 2] - pi2;
+                        if (wrapf) {
+                            r[u][0] = body::fac_wrap<true>(r[u][0], W0, h0);
+                            r[u][1] = body::fac_wrap<true>(r[u][1], W1, h1);
+                            r[u][2] = body::fac_wrap<true>(r[u][2], W2, h2);
+                        }
+                    }
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         if (jn + u >= e1) continue;
@@ -263,46 +275,55 @@ __global__ void __launch_bounds__(64) k_fac_gal0(DevParams P, Level L, FacOp F, 
     int n = 0;  // entries in the list (uniform)
     auto wrapc = [](int v, int cnt) { return v < 0 ? v + cnt : (v >= cnt ? v - cnt : v); };
     auto consume = [&]() {
+        // stencil bits of each entry: for every slot s of k's mask, the stencil
+        // offset of cell(k) + s from g, wrapped as the stencil wraps the level grid
+        // (a superset of the stencil cells whose exact slot test below passes)
+        for (int t = lane; t < n; t += 64) {
+            const int o = lc[t];
+            const int ko[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
+            unsigned hb = 0;
+            for (unsigned m = (unsigned)lm[t]; m; m &= m - 1) {
+                const int sl = __builtin_ctz(m);
+                const int so[3] = {sl / 9 - 1, (sl / 3) % 3 - 1, sl % 3 - 1};
+                int idx = 0;
+                bool ok = true;
+                for (int a = 0; a < 3; ++a) {
+                    int v = ko[a] + so[a];
+                    if (L.cs[a] == 3) {
+                        if (v > 1) v -= L.cnt[a];
+                        else if (v < -1) v += L.cnt[a];
+                        ok = ok && v >= -1 && v <= 1;
+                        idx = 3 * idx + v + 1;
+                    } else {
+                        ok = ok && v == 0;
+                    }
+                }
+                if (ok) hb |= 1u << idx;
+            }
+            lh[t] = hb;
+        }
+        __syncthreads();
         if (active)
-            for (int t0 = part; t0 < n; t0 += 4 * nparts) {  // four entries' loads in flight
-                int k[4], sg[4], sh[4];  // idle entries: particle 0, slot 0 (in bounds, not added)
-                bool on[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int t = t0 + u * nparts;
-                    k[u] = t < n ? lk[t] : 0;
-                    sg[u] = sh[u] = 0;
-                    on[u] = false;
-                    if (t < n) {
-                        const int o = lc[t];
-                        const int kx = wrapc(wx + o % 3 - 1, C[0]);
-                        const int ky = wrapc(wy + (o / 3) % 3 - 1, C[1]);
-                        const int kz = wrapc(wz + o / 9 - 1, C[2]);
-                        const int s = fac_slot(P, hx, hy, hz, kx, ky, kz);
-                        if (s >= 0 && (((unsigned)lm[t] >> s) & 1u)) {
-                            on[u] = true;
-                            sh[u] = s;
-                            sg[u] = fac_slot(P, wx, wy, wz, kx, ky, kz);
-                        }
-                    }
+            for (int t = part; t < n; t += nparts) {
+                if (!((lh[t] >> c) & 1u)) continue;
+                const int o = lc[t];
+                const int kx = wrapc(wx + o % 3 - 1, C[0]);
+                const int ky = wrapc(wy + (o / 3) % 3 - 1, C[1]);
+                const int kz = wrapc(wz + o / 9 - 1, C[2]);
+                const int sh = fac_slot(P, hx, hy, hz, kx, ky, kz);
+                if (sh < 0 || !(((unsigned)lm[t] >> sh) & 1u)) continue;
+                const int sg = fac_slot(P, wx, wy, wz, kx, ky, kz);
+                const int k = lk[t];
+                const double* G = gtab + 81 * (long long)k;
+                const double lv = F.lambda[k];
+                const double gg0 = G[3 * sg], gg1 = G[3 * sg + 1], gg2 = G[3 * sg + 2];
+                const double gh0 = G[3 * sh], gh1 = G[3 * sh + 1], gh2 = G[3 * sh + 2];
+                const double lg[3] = {lv * gg0, lv * gg1, lv * gg2};
+                for (int rr = 0; rr < 3; ++rr) {
+                    acc[3 * rr] += lg[rr] * gh0;
+                    acc[3 * rr + 1] += lg[rr] * gh1;
+                    acc[3 * rr + 2] += lg[rr] * gh2;
                 }
-                double gg[4][3], gh[4][3], lv[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const double* G = gtab + 81 * (long long)k[u];
-                    lv[u] = on[u] ? F.lambda[k[u]] : 0.0;
-                    for (int e = 0; e < 3; ++e) {
-                        gg[u][e] = G[3 * sg[u] + e];
-                        gh[u][e] = G[3 * sh[u] + e];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (on[u])
-                        for (int rr = 0; rr < 3; ++rr) {
-                            const double lg = lv[u] * gg[u][rr];
-                            for (int ss = 0; ss < 3; ++ss) acc[3 * rr + ss] += lg * gh[u][ss];
-                        }
             }
         __syncthreads();
         n = 0;
