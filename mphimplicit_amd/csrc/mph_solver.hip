@@ -234,6 +234,58 @@ __global__ void k_mg_p2g_bu(DevParams P, Level L, const int* __restrict__ cellSt
     mg_begin_update_row(L, row);
 }
 
+// orders the LDS writes and reads of the lanes of one wavefront
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// k_mg_p2g_bu with kP2gLanes lanes per row: lane u loads q of the cell's particles
+// p0 + u, p0 + u + kP2gLanes, ... (kP2gK per pass), and the row's first lane adds
+// them in particle order from LDS, as mg_p2g_row does.  Level 0 has one row per
+// cell and component, too few to fill the GPU with one thread each.
+constexpr int kP2gLanes = 8, kP2gK = 8;
+__global__ void __launch_bounds__(256) k_mg_p2g_bu_lanes(DevParams P, Level L, const int* __restrict__ cellStart,
+                                                         const double* __restrict__ q) {
+    __shared__ double tb[256 * kP2gK];
+    const int u = threadIdx.x & (kP2gLanes - 1);
+    const int row = (int)(((long long)blockIdx.x * 256 + threadIdx.x) / kP2gLanes);
+    const bool on = row < L.nrow;
+    int p0 = 0, p1 = 0;
+    const int rd = row % 3;
+    if (on) {
+        int gx, gy, gz;
+        grid_xyz(L, row / 3, gx, gy, gz);
+        const int ic = cell_id(P, gx + L.mn[0], gy + L.mn[1], gz + L.mn[2]);
+        p0 = cellStart[ic];
+        p1 = cellStart[ic + 1];
+    }
+    double sum = 0.0;
+    for (int base = p0; base < p1; base += kP2gLanes * kP2gK) {  // the same trips in every lane of the row
+        double t[kP2gK];
+#pragma unroll
+        for (int k = 0; k < kP2gK; ++k) {
+            const int ip = base + u + kP2gLanes * k;
+            t[k] = ip < p1 ? q[3 * ip + rd] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kP2gK; ++k) tb[kP2gK * threadIdx.x + k] = t[k];
+        wave_sync();
+        if (u == 0)
+            for (int k = 0; k < kP2gK; ++k)
+                for (int v = 0; v < kP2gLanes; ++v)
+                    if (base + v + kP2gLanes * k < p1) sum += 1.0 * 1.0 * tb[kP2gK * (threadIdx.x + v) + k];
+        wave_sync();
+    }
+    if (u != 0 || !on) return;
+    double y = L.Q[row];
+    y *= 0.0;
+    y += sum;
+    L.Q[row] = y;
+    mg_begin_update_row(L, row);
+}
+
 // restriction into L, then begin + update of L on the same row
 __global__ void k_mg_restrict_bu(Level S, Level L) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
@@ -275,13 +327,6 @@ __global__ void k_mg_residual_update(Level L) {
 // reads it from the level (other shapes), at the price of runtime index arithmetic.
 constexpr int kRowLanes = 4, kRowBlock = 128, kRowQ = 7;  // kRowQ: stencil cells per lane (27 / 4, rounded up)
 constexpr int kLaneRowsMax = 32768;                     // levels with fewer rows take the lanes
-
-// orders the LDS writes and reads of the lanes of one wavefront
-__device__ inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 template <class VT, bool kUpdate, int C0, int C1, int C2>
 __global__ void __launch_bounds__(kRowBlock) k_mg_sweep_lanes(Level L) {
@@ -622,7 +667,8 @@ void launch_mg_p2g(hipStream_t st, const DevParams& P, Level& L, const int* cell
     hipLaunchKernelGGL(k_mg_p2g, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, P, L, cellStart, q);
 }
 void launch_mg_p2g_bu(hipStream_t st, const DevParams& P, Level& L, const int* cellStart, const double* q) {
-    hipLaunchKernelGGL(k_mg_p2g_bu, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, P, L, cellStart, q);
+    hipLaunchKernelGGL(k_mg_p2g_bu_lanes, dim3(nblk((long long)kP2gLanes * L.nrow, 256)), dim3(256), 0, st, P, L,
+                       cellStart, q);
 }
 void launch_mg_restrict_bu(hipStream_t st, const Level& S, Level& L) {
     hipLaunchKernelGGL(k_mg_restrict_bu, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, S, L);
