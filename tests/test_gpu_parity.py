@@ -63,6 +63,9 @@ SCENES = {
                            radius_ratio_v=4.6),
     # fluid filling the domain in x and y: neighbours through the periodic boundary
     "periodic3d": lambda: scenes.periodic_3d(),
+    # 39,330 level-0 rows: level 0 takes the one-thread-per-row multigrid sweeps, the
+    # coarser levels the lane-split ones (below 32768 rows), as on the 1M scene
+    "2d_160k": lambda: scenes.config_2d(nx=400, ny=400, box=(440, 420)),
 }
 
 
@@ -94,7 +97,7 @@ def test_phases_bitwise(name):
 
 
 @pytest.mark.parametrize("name,steps,tol", [("2d_40x40", 12, 1e-16), ("3d_4096", 8, 1e-16),
-                                            ("cube3d_sample", 3, 1e-12)])
+                                            ("cube3d_sample", 3, 1e-12), ("2d_160k", 2, 1e-16)])
 def test_trajectory_bitwise(name, steps, tol):
     """mph_scene_step over several steps (incl. the first step's VTK diagnostics)."""
     gpu, orc = _pair(name, tol)
