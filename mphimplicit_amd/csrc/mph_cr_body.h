@@ -562,6 +562,17 @@ __device__ inline double group_sum(double v, int G) {
     return v;
 }
 
+// one entry of FacOp::u32 / v32: three floats and the column
+struct FacEntry32 {
+    float a0, a1, a2;
+    int col;
+};
+__device__ inline FacEntry32 fac_entry32(const float* __restrict__ base, long long e) {
+    typedef float fv4 __attribute__((ext_vector_type(4)));
+    const fv4 v = ((const fv4*)base)[e];
+    return {v[0], v[1], v[2], __float_as_int(v[3])};
+}
+
 // lanes per particle in both passes, and the list entries each lane has in flight
 // per batch (viscous entries FP64 and FP32; P entries)
 constexpr int kFacLanes = 8, kFacBatch = 4, kFacBatch32 = 8, kFacBatchP = 4;
@@ -786,13 +797,13 @@ __device__ inline void fac_t32_chunk(const DevParams& P, const FacOp& F, const d
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int jn = jb + g + u * G;
-                    const int j = jn < run ? F.nIndP[c0 + jn] : k;
+                    const FacEntry32 en = fac_entry32(F.v32, c0 + (jn < run ? jn : 0));
+                    const int j = jn < run ? en.col : k;
                     const bool use = jn < run && j < P.nf && j != k;
                     jx[u] = use ? j : 0;  // unused entries read particle 0 with weight 0
-                    const long long e = 3 * (c0 + (use ? jn : 0));
-                    v[u][0] = use ? F.v32[e] : 0.0f;
-                    v[u][1] = use ? F.v32[e + 1] : 0.0f;
-                    v[u][2] = use ? F.v32[e + 2] : 0.0f;
+                    v[u][0] = use ? en.a0 : 0.0f;
+                    v[u][1] = use ? en.a1 : 0.0f;
+                    v[u][2] = use ? en.a2 : 0.0f;
                 }
                 double xv[U][3];
 #pragma unroll
@@ -845,11 +856,11 @@ __device__ inline void fac_y32_chunk(const DevParams& P, const FacOp& F, const d
         for (int u = 0; u < U; ++u) {
             const int jn = jb + g + u * G;
             const bool in = jn < w;
-            b.j[u] = in ? F.nInd[b0 + jn] : i;  // unused entries: i's own record, u = 0
-            const long long e = 3 * (b0 + (in ? jn : 0));
-            b.a[u][0] = in ? F.u32[e] : 0.0f;
-            b.a[u][1] = in ? F.u32[e + 1] : 0.0f;
-            b.a[u][2] = in ? F.u32[e + 2] : 0.0f;
+            const FacEntry32 en = fac_entry32(F.u32, b0 + (in ? jn : 0));
+            b.j[u] = in ? en.col : i;  // unused entries: i's own record, u = 0
+            b.a[u][0] = in ? en.a0 : 0.0f;
+            b.a[u][1] = in ? en.a1 : 0.0f;
+            b.a[u][2] = in ? en.a2 : 0.0f;
         }
     };
     Batch nb;
@@ -898,11 +909,11 @@ __device__ inline void fac_y32_chunk(const DevParams& P, const FacOp& F, const d
                 for (int u = 0; u < U; ++u) {
                     const int jn = jb + g + u * G;
                     const bool in = jn < n;
-                    k[u] = in ? F.nIndP[c0 + jn] : i;
-                    const long long e = 3 * (c0 + (in ? jn : 0));
-                    vv[u][0] = in ? F.v32[e] : 0.0f;
-                    vv[u][1] = in ? F.v32[e + 1] : 0.0f;
-                    vv[u][2] = in ? F.v32[e + 2] : 0.0f;
+                    const FacEntry32 en = fac_entry32(F.v32, c0 + (in ? jn : 0));
+                    k[u] = in ? en.col : i;
+                    vv[u][0] = in ? en.a0 : 0.0f;
+                    vv[u][1] = in ? en.a1 : 0.0f;
+                    vv[u][2] = in ? en.a2 : 0.0f;
                 }
                 fv4 q[U];
 #pragma unroll

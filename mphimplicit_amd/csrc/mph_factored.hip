@@ -74,9 +74,8 @@ __global__ void k_fac_p(DevParams P, const double* __restrict__ pos, const int* 
         if (j < P.nf) run = jn + 1;
         beta[b0 + jn] = b;
         if (v32) {
-            v32[3 * (b0 + jn)] = (float)(b * r0);
-            v32[3 * (b0 + jn) + 1] = (float)(b * r1);
-            v32[3 * (b0 + jn) + 2] = (float)(b * r2);
+            typedef float fv4 __attribute__((ext_vector_type(4)));
+            ((fv4*)v32)[b0 + jn] = fv4{(float)(b * r0), (float)(b * r1), (float)(b * r2), __int_as_float(j)};
         }
     }
     runP[k] = run;
@@ -113,9 +112,8 @@ __global__ void __launch_bounds__(256) k_fac_u(DevParams P, const int* __restric
         double r0, r1, r2;
         body::fac_disp(P, pos, j, pi0, pi1, pi2, r0, r1, r2);
         const double s = sqrt(alpha[b0 + jn]);  // alpha >= 0: visc, mu and -w' are not negative
-        u32[3 * (b0 + jn)] = (float)(s * r0);
-        u32[3 * (b0 + jn) + 1] = (float)(s * r1);
-        u32[3 * (b0 + jn) + 2] = (float)(s * r2);
+        typedef float fv4 __attribute__((ext_vector_type(4)));
+        ((fv4*)u32)[b0 + jn] = fv4{(float)(s * r0), (float)(s * r1), (float)(s * r2), __int_as_float(j)};
     }
 }
 

@@ -83,9 +83,10 @@ struct FacOp {
     // displacements take the minimum image); set by the matrix-A launch
     const int* wrap;
     // FP32 form (the inner solves of mixed precision), which needs no positions:
-    // u = sqrt(alpha) r per fluid-neighbour entry and v = beta r per P entry, three
-    // floats each beside the lists (u32[3*(nPtr[i]+jn)+c], v32[3*(nPtrP[k]+jn)+c]),
-    // and a 16-byte record x0 x1 x2 t per particle
+    // u = sqrt(alpha) r per fluid-neighbour entry and v = beta r per P entry, each a
+    // 16-byte entry beside the lists -- three floats and the column's bits in the
+    // fourth (u32[4*(nPtr[i]+jn)+c], v32[4*(nPtrP[k]+jn)+c]; body::fac_entry32), so an
+    // entry is one load -- and a 16-byte record x0 x1 x2 t per particle
     const float* u32;
     const float* v32;
     float* rec32;

@@ -303,7 +303,8 @@ int phase_matrix_c(mph_scene& S) {
         if ((rc = S.fbeta.ensure(ne)) || (rc = S.runP.ensure(S.P.np)) || (rc = S.selfC.ensure(3 * (size_t)S.P.nf)) ||
             (rc = S.frec.ensure(7 * (size_t)S.P.np)))
             return rc;
-        if (mixed(S) && ((rc = S.fv32.ensure(3 * ne)) || (rc = S.fu32.ensure(3 * (S.E > 0 ? S.E : 1))) ||
+        // one spare entry each: a batch's unused slots read entry 0 of a list
+        if (mixed(S) && ((rc = S.fv32.ensure(4 * (ne + 1))) || (rc = S.fu32.ensure(4 * (S.E + 1))) ||
                          (rc = S.frec32.ensure(4 * (size_t)S.P.np))))
             return rc;
         launch_fac_p(S.st, S.P, S.pos.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.fbeta.p, mixed(S) ? S.fv32.p : nullptr,

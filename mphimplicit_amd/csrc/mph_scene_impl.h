@@ -61,7 +61,9 @@ enum { kRR0 = 0, kRS0 = 1, kRR = 2, kRHO = 3, kRHOP = 4, kTMP = 5, kRS = 6 };
 // once every variant has `trials` timings the fastest mean is kept for good.
 struct Autotune {
     int nvar = 1, trials = 1, best = 0;
-    std::vector<double> sum;
+    std::vector<double> fastest;  // per variant, its fastest trial: a variant's first
+                                  // launches can carry one-off costs (code loading,
+                                  // cold caches) that a mean would charge to it
     std::vector<int> cnt;
     struct Pend {
         int v;
@@ -73,7 +75,7 @@ struct Autotune {
         nvar = n;
         trials = t;
         best = n == 1 ? 0 : -1;
-        sum.assign(n, 0.0);
+        fastest.assign(n, 0.0);
         cnt.assign(n, 0);
     }
     int pick() const {  // the variant for the next launch
@@ -109,7 +111,7 @@ struct Autotune {
         for (Pend& p : pend) {
             float ms = 0;
             if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess && p.v >= 0 && p.v < nvar) {
-                sum[p.v] += ms;
+                if (cnt[p.v] == 0 || ms < fastest[p.v]) fastest[p.v] = ms;
                 cnt[p.v]++;
             }
             pool.push_back(p.a);
@@ -121,7 +123,7 @@ struct Autotune {
             if (cnt[k] < trials) return;
         best = 0;
         for (int k = 1; k < nvar; ++k)
-            if (sum[k] / cnt[k] < sum[best] / cnt[best]) best = k;
+            if (fastest[k] < fastest[best]) best = k;
     }
     void release() {
         for (Pend& p : pend) pool.push_back(p.a), pool.push_back(p.b);
