@@ -226,14 +226,6 @@ __global__ void k_mg_p2g(DevParams P, Level L, const int* __restrict__ cellStart
     if (row < L.nrow) mg_p2g_row(P, L, cellStart, q, row);
 }
 
-// P2G, then begin + update of level 0 on the same row
-__global__ void k_mg_p2g_bu(DevParams P, Level L, const int* __restrict__ cellStart, const double* __restrict__ q) {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= L.nrow) return;
-    mg_p2g_row(P, L, cellStart, q, row);
-    mg_begin_update_row(L, row);
-}
-
 // orders the LDS writes and reads of the lanes of one wavefront
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -241,9 +233,10 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// k_mg_p2g_bu with kP2gLanes lanes per row: lane u loads q of the cell's particles
-// p0 + u, p0 + u + kP2gLanes, ... (kP2gK per pass), and the row's first lane adds
-// them in particle order from LDS, as mg_p2g_row does.  Level 0 has one row per
+// P2G, then begin + update of level 0 on the same row, with kP2gLanes lanes per
+// row: lane u loads q of the cell's particles p0 + u, p0 + u + kP2gLanes, ...
+// (kP2gK per pass), and the row's first lane adds them in particle order from LDS,
+// as mg_p2g_row does.  Level 0 has one row per
 // cell and component, too few to fill the GPU with one thread each.
 constexpr int kP2gLanes = 8, kP2gK = 8;
 __global__ void __launch_bounds__(256) k_mg_p2g_bu_lanes(DevParams P, Level L, const int* __restrict__ cellStart,
