@@ -146,8 +146,9 @@ __device__ inline void cell_xyz(const DevParams& P, int key, int& x, int& y, int
     z = c % P.C[2];
 }
 
-// G_k (27 slots x 3, zero where no fluid P neighbour lies) and the mask of used
-// slots, summed over k's P list in order; C_kk = C_ii of fluid k, C_jk = -beta r_kj
+// G_k (27 slots x 3; only the slots in the mask, those where a fluid P neighbour
+// lies, are written) and the mask, summed over k's P list in order; C_kk = C_ii of
+// fluid k, C_jk = -beta r_kj
 __global__ void k_fac_g(DevParams P, const int* __restrict__ cellIndex, const int* __restrict__ runP,
                         const long long* __restrict__ nPtrP, const int* __restrict__ nIndP,
                         const double* __restrict__ beta, const double* __restrict__ pos,
@@ -155,13 +156,17 @@ __global__ void k_fac_g(DevParams P, const int* __restrict__ cellIndex, const in
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P.np) return;
     double* G = gtab + 81 * (long long)k;
-    for (int e = 0; e < 81; ++e) G[e] = 0.0;
     int kx, ky, kz;
     cell_xyz(P, cellIndex[k], kx, ky, kz);
     const double pk0 = pos[3 * k], pk1 = pos[3 * k + 1], pk2 = pos[3 * k + 2];
     const long long c0 = nPtrP[k];
     const int run = runP[k];
     unsigned mask = 0;
+    // the list visits each neighbour cell in one run, so a slot's sum is kept in
+    // registers and stored when the run ends; slots outside the mask are never
+    // read and stay unwritten (a slot met again would continue from its stored sum)
+    int cur = -1;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int jn = 0; jn < run; ++jn) {
         const int j = nIndP[c0 + jn];
         if (j >= P.nf) continue;
@@ -169,6 +174,18 @@ __global__ void k_fac_g(DevParams P, const int* __restrict__ cellIndex, const in
         cell_xyz(P, cellIndex[j], jx, jy, jz);
         const int slot = fac_slot(P, jx, jy, jz, kx, ky, kz);
         if (slot < 0) continue;  // cannot happen: P neighbours are in adjacent cells
+        if (slot != cur) {
+            if (cur >= 0) {
+                G[3 * cur] = a0;
+                G[3 * cur + 1] = a1;
+                G[3 * cur + 2] = a2;
+            }
+            cur = slot;
+            const bool seen = (mask >> slot) & 1u;
+            a0 = seen ? G[3 * slot] : 0.0;
+            a1 = seen ? G[3 * slot + 1] : 0.0;
+            a2 = seen ? G[3 * slot + 2] : 0.0;
+        }
         double c[3];
         if (j == k) {
             c[0] = selfC[3 * k];
@@ -182,8 +199,15 @@ __global__ void k_fac_g(DevParams P, const int* __restrict__ cellIndex, const in
             c[1] = -(b * r1);
             c[2] = -(b * r2);
         }
-        for (int a = 0; a < 3; ++a) G[3 * slot + a] += c[a];
+        a0 += c[0];
+        a1 += c[1];
+        a2 += c[2];
         mask |= 1u << slot;
+    }
+    if (cur >= 0) {
+        G[3 * cur] = a0;
+        G[3 * cur + 1] = a1;
+        G[3 * cur + 2] = a2;
     }
     gmask[k] = (int)mask;
 }

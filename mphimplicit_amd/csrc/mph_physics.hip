@@ -365,39 +365,63 @@ __global__ void __launch_bounds__(256) k_matrix_a(DevParams P, const int* __rest
 // calculateMatrixC values (2463-2507): C[3i+r][k] = e_r * (-w'_p).  Also
 // D[3i+r][k] = C[3i+r][k] * Lambda[k], the left factor of every product of the
 // C Lambda C^T sum (main.cpp:2614 evaluates (C_ik * Lambda_k) * C_jk).
-__global__ void k_matrix_c(DevParams P, const double* __restrict__ pos, const int* __restrict__ nCountP,
-                           const long long* __restrict__ nPtrP, const int* __restrict__ nIndP,
-                           const double* __restrict__ lambda, double* __restrict__ cval,
-                           double* __restrict__ dval) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.nf) return;
+__global__ void __launch_bounds__(256) k_matrix_c(DevParams P, const double* __restrict__ pos,
+                                                  const int* __restrict__ nCountP, const long long* __restrict__ nPtrP,
+                                                  const int* __restrict__ nIndP, const double* __restrict__ lambda,
+                                                  double* __restrict__ cval, double* __restrict__ dval) {
+    constexpr int G = kLanes, K = 3;  // terms: the C block of an entry
+    __shared__ double tb[256 * K];
+    __shared__ int tf[256];  // bit 0: the self entry, bit 1: inside RadiusP
+    const int g = threadIdx.x & (G - 1);
+    const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / G);
+    const bool own = i < P.nf;
     int iN = 0;
     double self[3] = {0.0, 0.0, 0.0};
-    const long long b0 = nPtrP[i];
-    const int n = nCountP[i];
-    for (int jn = 0; jn < n; ++jn) {
-        const int j = nIndP[b0 + jn];
+    const long long b0 = own ? nPtrP[i] : 0;
+    const int n = own ? nCountP[i] : 0;
+    for (int base = 0; base < n; base += G) {
+        const int jn = base + g;
+        int f = 0;
         double c3[3] = {0.0, 0.0, 0.0};
-        if (i == j) {
-            iN = jn;
-        } else {
-            const Rel q = rel(P, pos, i, j);
-            if (q.r2 != 0.0 && P.radP2 - q.r2 > 0) {
-                const double dij = sqrt(q.r2);
-                const double e[3] = {q.x[0] / dij, q.x[1] / dij, q.x[2] / dij};
-                const double wpdrij = -dwq_k(P.Kp, P.m2P, dij, P.radP);
-                for (int r = 0; r < 3; ++r) {
-                    c3[r] = e[r] * wpdrij;
-                    self[r] += c3[r];
+        if (jn < n) {
+            const int j = nIndP[b0 + jn];
+            if (i == j) {
+                f = 1;
+            } else {
+                const Rel q = rel(P, pos, i, j);
+                if (q.r2 != 0.0 && P.radP2 - q.r2 > 0) {
+                    const double dij = sqrt(q.r2);
+                    const double e[3] = {q.x[0] / dij, q.x[1] / dij, q.x[2] / dij};
+                    const double wpdrij = -dwq_k(P.Kp, P.m2P, dij, P.radP);
+                    for (int r = 0; r < 3; ++r) c3[r] = e[r] * wpdrij;
+                    f = 2;
+                }
+                // the self entry's block is written once its sum is complete
+                for (int r = 0; r < 3; ++r) cval[3 * (b0 + jn) + r] = c3[r];
+                if (dval) {  // D = C Lambda, read only by C Lambda C^T (assembled form)
+                    const double lk = lambda[j];
+                    for (int r = 0; r < 3; ++r) dval[3 * (b0 + jn) + r] = c3[r] * lk;
                 }
             }
         }
-        for (int r = 0; r < 3; ++r) cval[3 * (b0 + jn) + r] = c3[r];
+        for (int e = 0; e < K; ++e) tb[K * threadIdx.x + e] = c3[e];
+        tf[threadIdx.x] = f;
+        wave_sync();
+        if (g == 0)
+            for (int u = 0; u < G && base + u < n; ++u) {
+                const int sl = threadIdx.x + u;
+                const int fu = tf[sl];
+                if (fu & 1) iN = base + u;
+                if (fu & 2)
+                    for (int r = 0; r < 3; ++r) self[r] += tb[K * sl + r];
+            }
+        wave_sync();
     }
+    if (g != 0 || !own) return;
     for (int r = 0; r < 3; ++r) cval[3 * (b0 + iN) + r] = self[r];
-    for (int jn = 0; jn < n; ++jn) {
-        const double lk = lambda[nIndP[b0 + jn]];
-        for (int r = 0; r < 3; ++r) dval[3 * (b0 + jn) + r] = cval[3 * (b0 + jn) + r] * lk;
+    if (dval) {
+        const double li = lambda[i];
+        for (int r = 0; r < 3; ++r) dval[3 * (b0 + iN) + r] = self[r] * li;
     }
 }
 
@@ -604,8 +628,8 @@ void launch_matrix_a(hipStream_t st, const DevParams& P, const int* type, const 
 
 void launch_matrix_c(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP,
                      const long long* nPtrP, const int* nIndP, const double* lambda, double* cval, double* dval) {
-    hipLaunchKernelGGL(k_matrix_c, dim3(nblk(P.nf, 128)), dim3(128), 0, st, P, pos, nCountP, nPtrP, nIndP, lambda,
-                       cval, dval);
+    hipLaunchKernelGGL(k_matrix_c, dim3(nblk((long long)kLanes * P.nf, 256)), dim3(256), 0, st, P, pos, nCountP,
+                       nPtrP, nIndP, lambda, cval, dval);
 }
 
 void launch_vector_p(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,

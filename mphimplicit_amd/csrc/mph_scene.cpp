@@ -294,8 +294,12 @@ int phase_matrix_a(mph_scene& S) {
 
 int phase_matrix_c(mph_scene& S) {
     int rc;
-    if ((rc = S.cval.ensure(3 * (S.EP > 0 ? S.EP : 1))) || (rc = S.dval.ensure(3 * (S.EP > 0 ? S.EP : 1)))) return rc;
-    launch_matrix_c(S.st, S.P, S.pos.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.lambda.p, S.cval.p, S.dval.p);
+    // D = C Lambda is read only by C Lambda C^T, which a scene without blocks skips
+    if ((rc = S.cval.ensure(3 * (S.EP > 0 ? S.EP : 1))) ||
+        (S.blocks && (rc = S.dval.ensure(3 * (S.EP > 0 ? S.EP : 1)))))
+        return rc;
+    launch_matrix_c(S.st, S.P, S.pos.p, S.nCountP.p, S.nPtrP.p, S.nIndP.p, S.lambda.p, S.cval.p,
+                    S.blocks ? S.dval.p : nullptr);
     launch_vector_p(S.st, S.P, S.type.p, S.pos.p, S.vel.p, S.kappa.p, S.lambda.p, S.volStrainP.p, S.nCountP.p,
                     S.nPtrP.p, S.nIndP.p, S.vecP.p);
     if (factored(S)) {
