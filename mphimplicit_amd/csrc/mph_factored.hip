@@ -46,38 +46,67 @@ using body::nblk;
 // beta beside the P lists (and v = beta r in FP32 for the FP32 form), the extent of
 // their fluid entries, C_ii of fluid particles, and the records (positions; x and t
 // zero until the first pass)
-__global__ void k_fac_p(DevParams P, const double* __restrict__ pos, const int* __restrict__ nCountP,
-                        const long long* __restrict__ nPtrP, const int* __restrict__ nIndP, double* __restrict__ beta,
-                        float* __restrict__ v32, int* __restrict__ runP, double* __restrict__ selfC,
-                        double* __restrict__ rec) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= P.np) return;
-    const long long b0 = nPtrP[k];
-    const int n = nCountP[k];
-    const double pk0 = pos[3 * k], pk1 = pos[3 * k + 1], pk2 = pos[3 * k + 2];
+__global__ void __launch_bounds__(256) k_fac_p(DevParams P, const double* __restrict__ pos,
+                                               const int* __restrict__ nCountP, const long long* __restrict__ nPtrP,
+                                               const int* __restrict__ nIndP, double* __restrict__ beta,
+                                               float* __restrict__ v32, int* __restrict__ runP,
+                                               double* __restrict__ selfC, double* __restrict__ rec) {
+    // entry-parallel: lane g handles entries g, g + 8, ... (beta, v); the group's
+    // first lane adds the C_ii terms in list order and finds the fluid extent
+    constexpr int G = 8, K = 3;
+    __shared__ double tb[256 * K];
+    __shared__ int tf[256];  // bit 0: inside RadiusP (a C_ii term), bit 1: fluid neighbour
+    const int g = threadIdx.x & (G - 1);
+    const int k = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / G);
+    const bool own = k < P.np;
+    const long long b0 = own ? nPtrP[k] : 0;
+    const int n = own ? nCountP[k] : 0;
+    const double pk0 = own ? pos[3 * k] : 0.0, pk1 = own ? pos[3 * k + 1] : 0.0, pk2 = own ? pos[3 * k + 2] : 0.0;
     double c0 = 0.0, c1 = 0.0, c2 = 0.0;
     int run = 0;
-    for (int jn = 0; jn < n; ++jn) {
-        const int j = nIndP[b0 + jn];
-        double b = 0.0, r0 = 0.0, r1 = 0.0, r2 = 0.0;
-        if (j != k) {
-            body::fac_disp(P, pos, j, pk0, pk1, pk2, r0, r1, r2);
-            const double r2s = (r0 * r0 + r1 * r1 + r2 * r2);
-            if (r2s != 0.0 && P.radP2 - r2s > 0) {
-                const double d = sqrt(r2s);
-                b = -dwq_k(P.Kp, P.m2P, d, P.radP) / d;
-                c0 += b * r0;
-                c1 += b * r1;
-                c2 += b * r2;
+    for (int base = 0; base < n; base += G) {
+        const int jn = base + g;
+        int f = 0;
+        double t[K] = {0.0, 0.0, 0.0};
+        if (jn < n) {
+            const int j = nIndP[b0 + jn];
+            double b = 0.0, r0 = 0.0, r1 = 0.0, r2 = 0.0;
+            if (j != k) {
+                body::fac_disp(P, pos, j, pk0, pk1, pk2, r0, r1, r2);
+                const double r2s = (r0 * r0 + r1 * r1 + r2 * r2);
+                if (r2s != 0.0 && P.radP2 - r2s > 0) {
+                    const double d = sqrt(r2s);
+                    b = -dwq_k(P.Kp, P.m2P, d, P.radP) / d;
+                    t[0] = b * r0;
+                    t[1] = b * r1;
+                    t[2] = b * r2;
+                    f = 1;
+                }
+            }
+            if (j < P.nf) f |= 2;
+            beta[b0 + jn] = b;
+            if (v32) {
+                typedef float fv4 __attribute__((ext_vector_type(4)));
+                ((fv4*)v32)[b0 + jn] = fv4{(float)(b * r0), (float)(b * r1), (float)(b * r2), __int_as_float(j)};
             }
         }
-        if (j < P.nf) run = jn + 1;
-        beta[b0 + jn] = b;
-        if (v32) {
-            typedef float fv4 __attribute__((ext_vector_type(4)));
-            ((fv4*)v32)[b0 + jn] = fv4{(float)(b * r0), (float)(b * r1), (float)(b * r2), __int_as_float(j)};
-        }
+        for (int e = 0; e < K; ++e) tb[K * threadIdx.x + e] = t[e];
+        tf[threadIdx.x] = f;
+        body::wave_sync();
+        if (g == 0)
+            for (int u = 0; u < G && base + u < n; ++u) {
+                const int sl = threadIdx.x + u;
+                const int fu = tf[sl];
+                if (fu & 1) {
+                    c0 += tb[K * sl];
+                    c1 += tb[K * sl + 1];
+                    c2 += tb[K * sl + 2];
+                }
+                if (fu & 2) run = base + u + 1;
+            }
+        body::wave_sync();
     }
+    if (g != 0 || !own) return;
     runP[k] = run;
     const body::FacPlanes R = body::fac_planes(P, rec);
     R.a[2 * k] = 0.0;
@@ -393,32 +422,53 @@ __global__ void __launch_bounds__(64) k_fac_gal0(DevParams P, Level L, FacOp F, 
 
 // InvDiagA (calculateInvDiagA, main.cpp:3045-3068): 1/A_ii[r][r] with
 // A_ii[r][r] = D_i[r][r] + sum_{k in P(i)} lambda_k C_ik[r]^2
-__global__ void k_fac_invdiag(DevParams P, FacOp F, double* __restrict__ invDiagA) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.nf) return;
-    const double pi0 = F.pos[3 * i], pi1 = F.pos[3 * i + 1], pi2 = F.pos[3 * i + 2];
-    double d[3] = {F.diag[9 * i], F.diag[9 * i + 4], F.diag[9 * i + 8]};
-    const long long c0 = F.nPtrP[i];
-    const int n = F.nCountP[i];
-    for (int jn = 0; jn < n; ++jn) {
-        const int k = F.nIndP[c0 + jn];
-        double cv[3];
-        if (k == i) {
-            cv[0] = F.selfC[3 * i];
-            cv[1] = F.selfC[3 * i + 1];
-            cv[2] = F.selfC[3 * i + 2];
-        } else {
-            double r0, r1, r2;
-            body::fac_disp(P, F.pos, k, pi0, pi1, pi2, r0, r1, r2);
-            const double b = F.beta[c0 + jn];
-            cv[0] = b * r0;
-            cv[1] = b * r1;
-            cv[2] = b * r2;
-        }
-        const double lk = F.lambda[k];
-        for (int a = 0; a < 3; ++a) d[a] += lk * cv[a] * cv[a];
+__global__ void __launch_bounds__(256) k_fac_invdiag(DevParams P, FacOp F, double* __restrict__ invDiagA) {
+    // entry-parallel: lane g forms the terms of entries g, g + 8, ...; the group's
+    // first lane adds them in list order
+    constexpr int G = 8, K = 3;
+    __shared__ double tb[256 * K];
+    const int g = threadIdx.x & (G - 1);
+    const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / G);
+    const bool own = i < P.nf;
+    const double pi0 = own ? F.pos[3 * i] : 0.0, pi1 = own ? F.pos[3 * i + 1] : 0.0, pi2 = own ? F.pos[3 * i + 2] : 0.0;
+    double d[3] = {0.0, 0.0, 0.0};
+    if (own) {
+        d[0] = F.diag[9 * i];
+        d[1] = F.diag[9 * i + 4];
+        d[2] = F.diag[9 * i + 8];
     }
-    for (int a = 0; a < 3; ++a) invDiagA[3 * i + a] = d[a] != 0.0 ? 1.0 / d[a] : 0.0;
+    const long long c0 = own ? F.nPtrP[i] : 0;
+    const int n = own ? F.nCountP[i] : 0;
+    for (int base = 0; base < n; base += G) {
+        const int jn = base + g;
+        double t[K] = {0.0, 0.0, 0.0};
+        if (jn < n) {
+            const int k = F.nIndP[c0 + jn];
+            double cv[3];
+            if (k == i) {
+                cv[0] = F.selfC[3 * i];
+                cv[1] = F.selfC[3 * i + 1];
+                cv[2] = F.selfC[3 * i + 2];
+            } else {
+                double r0, r1, r2;
+                body::fac_disp(P, F.pos, k, pi0, pi1, pi2, r0, r1, r2);
+                const double b = F.beta[c0 + jn];
+                cv[0] = b * r0;
+                cv[1] = b * r1;
+                cv[2] = b * r2;
+            }
+            const double lk = F.lambda[k];
+            for (int a = 0; a < 3; ++a) t[a] = lk * cv[a] * cv[a];
+        }
+        for (int e = 0; e < K; ++e) tb[K * threadIdx.x + e] = t[e];
+        body::wave_sync();
+        if (g == 0)
+            for (int u = 0; u < G && base + u < n; ++u)
+                for (int a = 0; a < 3; ++a) d[a] += tb[K * (threadIdx.x + u) + a];
+        body::wave_sync();
+    }
+    if (g == 0 && own)
+        for (int a = 0; a < 3; ++a) invDiagA[3 * i + a] = d[a] != 0.0 ? 1.0 / d[a] : 0.0;
 }
 
 // the wrap flag is uniform, so the branch is taken by the whole launch; the FP32
@@ -456,8 +506,8 @@ void fac_launch(hipStream_t st, const DevParams& P, const FacOp& F, const double
 void launch_fac_p(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP, const long long* nPtrP,
                   const int* nIndP, double* beta, float* v32, int* runP, double* selfC, double* rec) {
     if (P.np == 0) return;
-    hipLaunchKernelGGL(k_fac_p, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, pos, nCountP, nPtrP, nIndP, beta, v32,
-                       runP, selfC, rec);
+    hipLaunchKernelGGL(k_fac_p, dim3(nblk(8LL * P.np, 256)), dim3(256), 0, st, P, pos, nCountP, nPtrP, nIndP, beta,
+                       v32, runP, selfC, rec);
 }
 
 void launch_fac_level0(hipStream_t st, const DevParams& P, const Level& L0, const FacOp& F, const int* cellIndex,
@@ -466,7 +516,7 @@ void launch_fac_level0(hipStream_t st, const DevParams& P, const Level& L0, cons
     if (P.nf == 0) return;
     hipLaunchKernelGGL(k_fac_g, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, cellIndex, F.runP, F.nPtrP, F.nIndP,
                        F.beta, F.pos, F.selfC, gtab, gmask);
-    hipLaunchKernelGGL(k_fac_invdiag, dim3(nblk(P.nf, 128)), dim3(128), 0, st, P, F, invDiagA);
+    hipLaunchKernelGGL(k_fac_invdiag, dim3(nblk(8LL * P.nf, 256)), dim3(256), 0, st, P, F, invDiagA);
     if (L0.cells > 0)
         hipLaunchKernelGGL(k_fac_gal0, dim3(L0.cells), dim3(64), 0, st, P, L0, F, cellStart, runStart, gtab, gmask);
 }

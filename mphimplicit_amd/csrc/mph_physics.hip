@@ -466,21 +466,38 @@ __global__ void k_vector_p(DevParams P, const int* __restrict__ type, const doub
 }
 
 // b -= C P (multiplyMatrixC, 2575-2584), accumulated directly into b
-__global__ void k_b_minus_cp(DevParams P, const int* __restrict__ nCountP, const long long* __restrict__ nPtrP,
-                             const int* __restrict__ nIndP, const double* __restrict__ cval,
-                             const double* __restrict__ vecP, double* __restrict__ b) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.nf) return;
-    double b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
-    const long long p0 = nPtrP[i];
-    const int n = nCountP[i];
-    for (int jn = 0; jn < n; ++jn) {
-        const double pk = vecP[nIndP[p0 + jn]];
-        const double* c = &cval[3 * (p0 + jn)];
-        b0 -= c[0] * pk;
-        b1 -= c[1] * pk;
-        b2 -= c[2] * pk;
+__global__ void __launch_bounds__(256) k_b_minus_cp(DevParams P, const int* __restrict__ nCountP,
+                                                    const long long* __restrict__ nPtrP,
+                                                    const int* __restrict__ nIndP, const double* __restrict__ cval,
+                                                    const double* __restrict__ vecP, double* __restrict__ b) {
+    constexpr int G = kLanes, K = 3;  // terms: c_r * P_k
+    __shared__ double tb[256 * K];
+    const int g = threadIdx.x & (G - 1);
+    const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / G);
+    const bool own = i < P.nf;
+    double b0 = own ? b[3 * i] : 0.0, b1 = own ? b[3 * i + 1] : 0.0, b2 = own ? b[3 * i + 2] : 0.0;
+    const long long p0 = own ? nPtrP[i] : 0;
+    const int n = own ? nCountP[i] : 0;
+    for (int base = 0; base < n; base += G) {
+        const int jn = base + g;
+        double t[K] = {0.0, 0.0, 0.0};
+        if (jn < n) {
+            const double pk = vecP[nIndP[p0 + jn]];
+            const double* c = &cval[3 * (p0 + jn)];
+            for (int r = 0; r < 3; ++r) t[r] = c[r] * pk;
+        }
+        for (int e = 0; e < K; ++e) tb[K * threadIdx.x + e] = t[e];
+        wave_sync();
+        if (g == 0)
+            for (int u = 0; u < G && base + u < n; ++u) {
+                const double* tu = tb + K * (threadIdx.x + u);
+                b0 -= tu[0];
+                b1 -= tu[1];
+                b2 -= tu[2];
+            }
+        wave_sync();
     }
+    if (g != 0 || !own) return;
     b[3 * i] = b0;
     b[3 * i + 1] = b1;
     b[3 * i + 2] = b2;
@@ -641,8 +658,8 @@ void launch_vector_p(hipStream_t st, const DevParams& P, const int* type, const 
 
 void launch_b_minus_cp(hipStream_t st, const DevParams& P, const int* nCountP, const long long* nPtrP,
                        const int* nIndP, const double* cval, const double* vecP, double* b) {
-    hipLaunchKernelGGL(k_b_minus_cp, dim3(nblk(P.nf, 128)), dim3(128), 0, st, P, nCountP, nPtrP, nIndP, cval, vecP,
-                       b);
+    hipLaunchKernelGGL(k_b_minus_cp, dim3(nblk((long long)kLanes * P.nf, 256)), dim3(256), 0, st, P, nCountP, nPtrP,
+                       nIndP, cval, vecP, b);
 }
 
 void launch_cac(hipStream_t st, const DevParams& P, const int* nFluid, const int* nCountP, const long long* nPtrP,

@@ -226,13 +226,6 @@ __global__ void k_mg_p2g(DevParams P, Level L, const int* __restrict__ cellStart
     if (row < L.nrow) mg_p2g_row(P, L, cellStart, q, row);
 }
 
-// orders the LDS writes and reads of the lanes of one wavefront
-__device__ inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // P2G, then begin + update of level 0 on the same row, with kP2gLanes lanes per
 // row: lane u loads q of the cell's particles p0 + u, p0 + u + kP2gLanes, ...
 // (kP2gK per pass), and the row's first lane adds them in particle order from LDS,
