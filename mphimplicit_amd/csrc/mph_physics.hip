@@ -28,8 +28,11 @@ __device__ inline Rel rel(const DevParams& P, const double* __restrict__ pos, in
     return o;
 }
 
-// lanes per particle of the entry-parallel list walks; a workgroup is 256 lanes
-constexpr int kLanes = 8;
+// lanes per particle of the entry-parallel list walks (a workgroup is 256 lanes):
+// kLanes for the MaxRadius lists (about 180 entries in 3D), kLanesP for the RadiusP
+// lists (about 20).  Measured on the 1M scene: 4 lanes beat 2, 8 and 16 on the long
+// lists (k_forces 3.50 against 3.71 / 4.04 / 5.40 ms), 8 beat 4 on the short ones.
+constexpr int kLanes = 4, kLanesP = 8;
 
 // orders the LDS writes and reads of the lanes of one wavefront (the lanes of a
 // group are in one wavefront and run the same iterations)
@@ -369,7 +372,7 @@ __global__ void __launch_bounds__(256) k_matrix_c(DevParams P, const double* __r
                                                   const int* __restrict__ nCountP, const long long* __restrict__ nPtrP,
                                                   const int* __restrict__ nIndP, const double* __restrict__ lambda,
                                                   double* __restrict__ cval, double* __restrict__ dval) {
-    constexpr int G = kLanes, K = 3;  // terms: the C block of an entry
+    constexpr int G = kLanesP, K = 3;  // terms: the C block of an entry
     __shared__ double tb[256 * K];
     __shared__ int tf[256];  // bit 0: the self entry, bit 1: inside RadiusP
     const int g = threadIdx.x & (G - 1);
@@ -470,7 +473,7 @@ __global__ void __launch_bounds__(256) k_b_minus_cp(DevParams P, const int* __re
                                                     const long long* __restrict__ nPtrP,
                                                     const int* __restrict__ nIndP, const double* __restrict__ cval,
                                                     const double* __restrict__ vecP, double* __restrict__ b) {
-    constexpr int G = kLanes, K = 3;  // terms: c_r * P_k
+    constexpr int G = kLanesP, K = 3;  // terms: c_r * P_k
     __shared__ double tb[256 * K];
     const int g = threadIdx.x & (G - 1);
     const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / G);
@@ -645,7 +648,7 @@ void launch_matrix_a(hipStream_t st, const DevParams& P, const int* type, const 
 
 void launch_matrix_c(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP,
                      const long long* nPtrP, const int* nIndP, const double* lambda, double* cval, double* dval) {
-    hipLaunchKernelGGL(k_matrix_c, dim3(nblk((long long)kLanes * P.nf, 256)), dim3(256), 0, st, P, pos, nCountP,
+    hipLaunchKernelGGL(k_matrix_c, dim3(nblk((long long)kLanesP * P.nf, 256)), dim3(256), 0, st, P, pos, nCountP,
                        nPtrP, nIndP, lambda, cval, dval);
 }
 
@@ -658,7 +661,7 @@ void launch_vector_p(hipStream_t st, const DevParams& P, const int* type, const 
 
 void launch_b_minus_cp(hipStream_t st, const DevParams& P, const int* nCountP, const long long* nPtrP,
                        const int* nIndP, const double* cval, const double* vecP, double* b) {
-    hipLaunchKernelGGL(k_b_minus_cp, dim3(nblk((long long)kLanes * P.nf, 256)), dim3(256), 0, st, P, nCountP, nPtrP,
+    hipLaunchKernelGGL(k_b_minus_cp, dim3(nblk((long long)kLanesP * P.nf, 256)), dim3(256), 0, st, P, nCountP, nPtrP,
                        nIndP, cval, vecP, b);
 }
 
