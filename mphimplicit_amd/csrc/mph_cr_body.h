@@ -982,8 +982,17 @@ __device__ inline void dot_partial_chunk(int nf, const double* __restrict__ x, c
 // scal[shift_slot] receives the previous scal[slot] first (rho -> rho_prev).
 __device__ inline void dot_final_body(const double* __restrict__ part, int nchunks, double* scal, int slot,
                                       int shift_slot, double* sh) {
+    // lane l adds chunks l, l + 256, ... in that order; eight loads are issued
+    // before their adds so the lane does not wait on each in turn
     double acc = 0.0;
-    for (int c = threadIdx.x; c < nchunks; c += 256) acc = acc + part[c];
+    for (int c0 = threadIdx.x; c0 < nchunks; c0 += 256 * 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = c0 + 256 * u < nchunks ? part[c0 + 256 * u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (c0 + 256 * u < nchunks) acc = acc + v[u];
+    }
     const double v = block_tree_sum(acc, sh);
     if (threadIdx.x == 0) {
         if (shift_slot >= 0) scal[shift_slot] = scal[slot];
