@@ -495,10 +495,11 @@ __global__ void __launch_bounds__(256) k_fac_y(DevParams P, FacOp F, const doubl
 
 template <bool kNeg, class VT>
 void fac_launch(hipStream_t st, const DevParams& P, const FacOp& F, const double* x, double beta, const double* yin,
-                double* y, double* y2, double* p_copy, const double* dotx, double* part) {
-    hipLaunchKernelGGL(k_fac_t<VT>, dim3(nblk(P.np, 256)), dim3(256), 0, st, P, F, x);
-    hipLaunchKernelGGL((k_fac_y<kNeg, VT>), dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P, F, x, beta, yin, y, y2,
-                       p_copy, dotx, part);
+                double* y, double* y2, double* p_copy, const double* dotx, double* part, int passes) {
+    if (passes & 1) hipLaunchKernelGGL(k_fac_t<VT>, dim3(nblk(P.np, 256)), dim3(256), 0, st, P, F, x);
+    if (passes & 2)
+        hipLaunchKernelGGL((k_fac_y<kNeg, VT>), dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P, F, x, beta, yin, y,
+                           y2, p_copy, dotx, part);
 }
 
 }  // namespace
@@ -530,15 +531,15 @@ void launch_fac_u(hipStream_t st, const DevParams& P, const int* nFluid, const l
 
 void launch_fac_spmv(hipStream_t st, const DevParams& P, const FacOp& F, double alpha, const double* x, double beta,
                      const double* yin, double* y, double* y2, double* p_copy, const double* dotx, double* part,
-                     bool f32) {
+                     bool f32, int passes) {
     // alpha is +1 or -1 at every call site (main.cpp:4337, 4353, 4357)
     if (P.nf == 0) return;
     if (f32) {
-        if (alpha < 0) fac_launch<true, float>(st, P, F, x, beta, yin, y, y2, p_copy, dotx, part);
-        else fac_launch<false, float>(st, P, F, x, beta, yin, y, y2, p_copy, dotx, part);
+        if (alpha < 0) fac_launch<true, float>(st, P, F, x, beta, yin, y, y2, p_copy, dotx, part, passes);
+        else fac_launch<false, float>(st, P, F, x, beta, yin, y, y2, p_copy, dotx, part, passes);
     } else {
-        if (alpha < 0) fac_launch<true, double>(st, P, F, x, beta, yin, y, y2, p_copy, dotx, part);
-        else fac_launch<false, double>(st, P, F, x, beta, yin, y, y2, p_copy, dotx, part);
+        if (alpha < 0) fac_launch<true, double>(st, P, F, x, beta, yin, y, y2, p_copy, dotx, part, passes);
+        else fac_launch<false, double>(st, P, F, x, beta, yin, y, y2, p_copy, dotx, part, passes);
     }
 }
 

@@ -216,9 +216,11 @@ void launch_fac_level0(hipStream_t st, const DevParams& P, const Level& L0, cons
 // u32 of the FP32 form from alpha and the positions (after launch_fac_p)
 void launch_fac_u(hipStream_t st, const DevParams& P, const int* nFluid, const long long* nPtr, const int* nInd,
                   const double* alpha, const double* pos, float* u32);
+// passes: 1 the first pass (t from x) only, 2 the second only (t already in F.rec),
+// 3 both
 void launch_fac_spmv(hipStream_t st, const DevParams& P, const FacOp& F, double alpha, const double* x, double beta,
                      const double* yin, double* y, double* y2, double* p_copy, const double* dotx, double* part,
-                     bool f32);
+                     bool f32, int passes = 3);
 
 void launch_convert_f32(hipStream_t st, const double* in, float* out, long long n);
 void launch_axpy(hipStream_t st, long long n, const double* x, double* y);  // y += 1.0*x

@@ -492,6 +492,24 @@ int read_scalar(mph_scene& S, int slot, double* v) {
     return MPH_OK;
 }
 
+void prefetch_fac_t(mph_scene& S, const double* x, bool f32);
+
+// read_scalar for the CR loop of the factored form: the first pass of the next
+// product on x is enqueued behind the copy and runs while the host waits for the
+// copy alone (the 8-byte round trip would otherwise leave the GPU idle every
+// iteration).  If the loop then stops, that pass only wrote the product's scratch t.
+int read_scalar_prefetch(mph_scene& S, int slot, double* v, const double* x, bool f32) {
+    if (!factored(S)) return read_scalar(S, slot, v);
+    if (!S.rrEv) HIPCHK(hipEventCreateWithFlags(&S.rrEv, hipEventDisableTiming));
+    HIPCHK(hipMemcpyAsync(S.hpin + 8, S.scal.p + slot, 8, hipMemcpyDeviceToHost, S.st));
+    HIPCHK(hipEventRecord(S.rrEv, S.st));
+    prefetch_fac_t(S, x, f32);
+    HIPCHK(hipEventSynchronize(S.rrEv));
+    *v = S.hpin[8];
+    collect_tuning(S);
+    return MPH_OK;
+}
+
 FacOp fac_op(mph_scene& S, bool f32) {
     FacOp F;
     F.nFluid = S.nFluid.p;
@@ -516,24 +534,52 @@ FacOp fac_op(mph_scene& S, bool f32) {
 }
 
 // the particle-matrix SpMV, bracketed by events when per-launch timing is on
-void spmv(mph_scene& S, double alpha, const double* x, double beta, const double* yin, double* y, double* y2,
-          double* p_copy, const double* dotx, double* part, bool f32 = false) {
+// room for one more SpMV event pair at S.spmvUsed
+void spmv_event_room(mph_scene& S) {
+    if ((int)S.spmvEv.size() < S.spmvUsed + 2) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipEventCreateWithFlags(&e0, hipEventDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&e1, hipEventDefault);
+        if (e != hipSuccess) S.spmvTimingErr = (int)e;
+        S.spmvEv.push_back(e0);
+        S.spmvEv.push_back(e1);
+        S.spmvEvF32.push_back(0);
+    }
+}
+
+// the first pass of the next product on x, enqueued ahead (factored form)
+void prefetch_fac_t(mph_scene& S, const double* x, bool f32) {
+    if (!factored(S) || S.P.nf == 0) return;
     if (S.spmvTiming) {
-        if ((int)S.spmvEv.size() < S.spmvUsed + 2) {
-            hipEvent_t e0 = nullptr, e1 = nullptr;
-            hipError_t e = hipEventCreateWithFlags(&e0, hipEventDefault);
-            if (e == hipSuccess) e = hipEventCreateWithFlags(&e1, hipEventDefault);
-            if (e != hipSuccess) S.spmvTimingErr = (int)e;
-            S.spmvEv.push_back(e0);
-            S.spmvEv.push_back(e1);
-            S.spmvEvF32.push_back(0);
-        }
+        spmv_event_room(S);
         S.spmvEvF32[S.spmvUsed / 2] = f32 ? 1 : 0;
         hipError_t e = hipEventRecord(S.spmvEv[S.spmvUsed], S.st);
         if (e != hipSuccess) S.spmvTimingErr = (int)e;
+        S.facPrefEv = S.spmvUsed;
     }
+    launch_fac_spmv(S.st, S.P, fac_op(S, f32), 1.0, x, 0.0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, f32,
+                    1);
+    S.facPref = x;
+    S.facPrefF32 = f32;
+}
+
+void spmv(mph_scene& S, double alpha, const double* x, double beta, const double* yin, double* y, double* y2,
+          double* p_copy, const double* dotx, double* part, bool f32 = false) {
+    // a first pass enqueued ahead for this operand: only the second pass is left
+    const bool pref = factored(S) && x != nullptr && S.facPref == x && S.facPrefF32 == f32;
+    if (S.spmvTiming) {
+        spmv_event_room(S);
+        S.spmvEvF32[S.spmvUsed / 2] = f32 ? 1 : 0;
+        if (!pref || S.facPrefEv != S.spmvUsed) {  // else the pair starts at the first pass
+            hipError_t e = hipEventRecord(S.spmvEv[S.spmvUsed], S.st);
+            if (e != hipSuccess) S.spmvTimingErr = (int)e;
+        }
+    }
+    S.facPref = nullptr;
+    S.facPrefEv = -1;
     if (factored(S)) {
-        launch_fac_spmv(S.st, S.P, fac_op(S, f32), alpha, x, beta, yin, y, y2, p_copy, dotx, part, f32);
+        launch_fac_spmv(S.st, S.P, fac_op(S, f32), alpha, x, beta, yin, y, y2, p_copy, dotx, part, f32,
+                        pref ? 2 : 3);
     } else {
         Autotune& T = f32 ? S.tuneSpmv32 : S.tuneSpmv64;
         const int v = T.pick();
@@ -566,6 +612,12 @@ void collect_spmv_timing(mph_scene& S) {
             S.spmvTimingErr = (int)e;
         }
     }
+    // a pending first pass keeps its start event: it becomes pair 0
+    if (S.facPrefEv > 0) {
+        std::swap(S.spmvEv[0], S.spmvEv[S.facPrefEv]);
+        S.spmvEvF32[0] = S.spmvEvF32[S.facPrefEv / 2];
+    }
+    if (S.facPrefEv >= 0) S.facPrefEv = 0;
     S.spmvUsed = 0;
 }
 
@@ -597,7 +649,7 @@ int phase_solve(mph_scene& S) {
     const long long maxit = S.prm.max_iterations > 0 ? S.prm.max_iterations : n;
     long long iter;
     for (iter = 0; iter < maxit; ++iter) {
-        if ((rc = read_scalar(S, kRR, &S.rr))) return rc;
+        if ((rc = read_scalar_prefetch(S, kRR, &S.rr, S.s.p, false))) return rc;  // both products below are on s
         if (S.spmvTiming) collect_spmv_timing(S);
         if (S.rr / S.rr0 < S.prm.cr_tolerance) break;
         if (iter == 0) {  // y = A s; p = s; q = A p (same sum); rho = s.y
@@ -613,6 +665,8 @@ int phase_solve(mph_scene& S) {
         launch_xrs_update(st, P, S.scal.p, S.p.p, S.q.p, S.u.p, S.x.p, S.r.p, S.s.p, S.part.p);
         launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1);
     }
+    S.facPref = nullptr;  // a first pass enqueued ahead of the stop is not used
+    S.facPrefEv = -1;
     launch_dot_partial(st, P, S.r.p, S.s.p, S.part.p);
     launch_dot_final(st, P, S.part.p, S.scal.p, kRS, -1);
     launch_store_velocity(st, P, S.x.p, S.vel.p);
@@ -676,7 +730,7 @@ int phase_solve_mixed(mph_scene& S) {
         double rri = rrOut;
         for (long long it = 0; total < maxit; ++it, ++total) {
             if (it > 0) {
-                if ((rc = read_scalar(S, kRR, &rri))) return rc;
+                if ((rc = read_scalar_prefetch(S, kRR, &rri, S.s.p, true))) return rc;
                 if (S.spmvTiming) collect_spmv_timing(S);
             }
             if (rri / rri0 < target) break;
@@ -693,6 +747,8 @@ int phase_solve_mixed(mph_scene& S) {
             launch_xrs_update(st, P, S.scal.p, S.p.p, S.q.p, S.u.p, S.dvec.p, S.r.p, S.s.p, S.part.p);
             launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1);
         }
+        S.facPref = nullptr;  // a first pass enqueued ahead of the inner stop is not used
+        S.facPrefEv = -1;
         set_levels_f32(S, 0);
         // x += d; rOut = b - A64 x
         launch_axpy(st, n, S.dvec.p, S.x.p);
@@ -832,6 +888,7 @@ void destroy(mph_scene* S) {
     for (hipEvent_t& e : S->ev)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t& e : S->spmvEv) (void)hipEventDestroy(e);
+    if (S->rrEv) (void)hipEventDestroy(S->rrEv);
     if (S->st) (void)hipStreamDestroy(S->st);
     delete S;
 }

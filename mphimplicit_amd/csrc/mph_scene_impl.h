@@ -199,6 +199,14 @@ struct mph_scene {
     std::vector<hipEvent_t> spmvEv;
     std::vector<char> spmvEvF32;  // per event pair: FP32-matrix launch
     int spmvUsed = 0;
+    // the factored solve enqueues the next product's first pass before it waits for
+    // the convergence scalar: facPref is that pass's operand (null: none pending),
+    // facPrefEv the index of its timing start event in spmvEv (-1: none), rrEv the
+    // event the host waits on
+    const double* facPref = nullptr;
+    bool facPrefF32 = false;  // the pass was the FP32 form's
+    int facPrefEv = -1;
+    hipEvent_t rrEv = nullptr;
     double spmvMs = 0, spmvMs32 = 0;
     long long spmvLaunches = 0, spmvLaunches32 = 0;
     int spmvTimingErr = 0;
