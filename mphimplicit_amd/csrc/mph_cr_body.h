@@ -981,7 +981,7 @@ __device__ inline void dot_partial_chunk(int nf, const double* __restrict__ x, c
 // fold the chunk partials: lane l sums chunks l, l+256, ... from 0.0, then the tree.
 // scal[shift_slot] receives the previous scal[slot] first (rho -> rho_prev).
 __device__ inline void dot_final_body(const double* __restrict__ part, int nchunks, double* scal, int slot,
-                                      int shift_slot, double* sh) {
+                                      int shift_slot, double* sh, double* host = nullptr) {
     // lane l adds chunks l, l + 256, ... in that order; eight loads are issued
     // before their adds so the lane does not wait on each in turn
     double acc = 0.0;
@@ -997,6 +997,7 @@ __device__ inline void dot_final_body(const double* __restrict__ part, int nchun
     if (threadIdx.x == 0) {
         if (shift_slot >= 0) scal[shift_slot] = scal[slot];
         scal[slot] = v;
+        if (host) *host = v;  // coherent pinned host memory: the CR stop test reads it
     }
 }
 

@@ -225,7 +225,9 @@ void launch_fac_spmv(hipStream_t st, const DevParams& P, const FacOp& F, double 
 void launch_convert_f32(hipStream_t st, const double* in, float* out, long long n);
 void launch_axpy(hipStream_t st, long long n, const double* x, double* y);  // y += 1.0*x
 void launch_dot_partial(hipStream_t st, const DevParams& P, const double* x, const double* y, double* part);
-void launch_dot_final(hipStream_t st, const DevParams& P, const double* part, double* scal, int slot, int shift_slot);
+// host: optional coherent pinned host memory that also receives the value
+void launch_dot_final(hipStream_t st, const DevParams& P, const double* part, double* scal, int slot, int shift_slot,
+                      double* host = nullptr);
 void launch_pq_update(hipStream_t st, const DevParams& P, const double* scal, const double* s, const double* y,
                       double* p, double* q);
 void launch_xrs_update(hipStream_t st, const DevParams& P, const double* scal, const double* p, const double* q,

@@ -498,17 +498,21 @@ void prefetch_fac_t(mph_scene& S, const double* x, bool f32);
 // product on x is enqueued behind the copy and runs while the host waits for the
 // copy alone (the 8-byte round trip would otherwise leave the GPU idle every
 // iteration).  If the loop then stops, that pass only wrote the product's scratch t.
+// The kRR reductions that precede it wrote rr straight into S.hrr (rr_host), so the
+// host waits on an event behind that kernel and no copy is enqueued.
 int read_scalar_prefetch(mph_scene& S, int slot, double* v, const double* x, bool f32) {
-    if (!factored(S)) return read_scalar(S, slot, v);
+    if (!factored(S) || !S.hrr || slot != kRR) return read_scalar(S, slot, v);
     if (!S.rrEv) HIPCHK(hipEventCreateWithFlags(&S.rrEv, hipEventDisableTiming));
-    HIPCHK(hipMemcpyAsync(S.hpin + 8, S.scal.p + slot, 8, hipMemcpyDeviceToHost, S.st));
     HIPCHK(hipEventRecord(S.rrEv, S.st));
     prefetch_fac_t(S, x, f32);
     HIPCHK(hipEventSynchronize(S.rrEv));
-    *v = S.hpin[8];
+    *v = S.hrr[0];
     collect_tuning(S);
     return MPH_OK;
 }
+
+// where the kRR reductions of a factored CR loop also put rr (null: copy it back)
+double* rr_host(mph_scene& S) { return factored(S) ? S.hrr : nullptr; }
 
 FacOp fac_op(mph_scene& S, bool f32) {
     FacOp F;
@@ -644,7 +648,7 @@ int phase_solve(mph_scene& S) {
     spmv(S, -1.0, S.x.p, 1.0, S.b.p, S.r.p, nullptr, nullptr, nullptr, nullptr);
     if ((rc = apply_m(S, S.r.p, S.s.p, nullptr))) return rc;
     launch_dot_partial(st, P, S.r.p, S.r.p, S.part.p);
-    launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1);
+    launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1, rr_host(S));
     if ((rc = read_scalar(S, kRR0, &S.rr0))) return rc;
     const long long maxit = S.prm.max_iterations > 0 ? S.prm.max_iterations : n;
     long long iter;
@@ -663,7 +667,7 @@ int phase_solve(mph_scene& S) {
         if ((rc = apply_m(S, S.q.p, S.u.p, S.part.p))) return rc;  // u = M^-1 q; tmp = q.u
         launch_dot_final(st, P, S.part.p, S.scal.p, kTMP, -1);
         launch_xrs_update(st, P, S.scal.p, S.p.p, S.q.p, S.u.p, S.x.p, S.r.p, S.s.p, S.part.p);
-        launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1);
+        launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1, rr_host(S));
     }
     S.facPref = nullptr;  // a first pass enqueued ahead of the stop is not used
     S.facPrefEv = -1;
@@ -745,7 +749,7 @@ int phase_solve_mixed(mph_scene& S) {
             if ((rc = apply_m(S, S.q.p, S.u.p, S.part.p))) return rc;
             launch_dot_final(st, P, S.part.p, S.scal.p, kTMP, -1);
             launch_xrs_update(st, P, S.scal.p, S.p.p, S.q.p, S.u.p, S.dvec.p, S.r.p, S.s.p, S.part.p);
-            launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1);
+            launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1, rr_host(S));
         }
         S.facPref = nullptr;  // a first pass enqueued ahead of the inner stop is not used
         S.facPrefEv = -1;
@@ -845,6 +849,7 @@ int allocate(mph_scene& S) {
     ENS(S.u, 3 * nf); ENS(S.part, nf / kChunk + 2); ENS(S.scal, 16);
 #undef ENS
     HIPCHK(hipHostMalloc((void**)&S.hpin, 64 * sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&S.hrr, 8 * sizeof(double), hipHostMallocCoherent));
     // zero-fill the per-particle state the reference allocates uninitialised
     for (DBuf<double>* d : {&S.force, &S.mass, &S.kappa, &S.lambda, &S.mu, &S.densA, &S.gcen, &S.pressA,
                             &S.volStrainP, &S.divP, &S.pressP, &S.virialP, &S.virialStress, &S.b, &S.vecP})
@@ -885,6 +890,7 @@ void destroy(mph_scene* S) {
     S->tuneFuse.release();
     S->tuneGal.release();
     if (S->hpin) (void)hipHostFree(S->hpin);
+    if (S->hrr) (void)hipHostFree(S->hrr);
     for (hipEvent_t& e : S->ev)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t& e : S->spmvEv) (void)hipEventDestroy(e);

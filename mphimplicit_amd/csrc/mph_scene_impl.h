@@ -207,6 +207,7 @@ struct mph_scene {
     bool facPrefF32 = false;  // the pass was the FP32 form's
     int facPrefEv = -1;
     hipEvent_t rrEv = nullptr;
+    double* hrr = nullptr;  // coherent pinned host memory: rr as the last kRR reduction wrote it
     double spmvMs = 0, spmvMs32 = 0;
     long long spmvLaunches = 0, spmvLaunches32 = 0;
     int spmvTimingErr = 0;

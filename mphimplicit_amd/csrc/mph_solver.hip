@@ -438,9 +438,9 @@ __global__ void __launch_bounds__(256) k_dot_partial(int nf, const double* __res
 }
 
 __global__ void __launch_bounds__(256) k_dot_final(const double* __restrict__ part, int nchunks, double* scal,
-                                                   int slot, int shift_slot) {
+                                                   int slot, int shift_slot, double* host) {
     __shared__ double sh[256];
-    dot_final_body(part, nchunks, scal, slot, shift_slot, sh);
+    dot_final_body(part, nchunks, scal, slot, shift_slot, sh, host);
 }
 
 __global__ void k_pq_update(int nf, const double* __restrict__ scal, const double* __restrict__ s,
@@ -746,8 +746,9 @@ void launch_dot_partial(hipStream_t st, const DevParams& P, const double* x, con
     hipLaunchKernelGGL(k_dot_partial, dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P.nf, x, y, part);
 }
 void launch_dot_final(hipStream_t st, const DevParams& P, const double* part, double* scal, int slot,
-                      int shift_slot) {
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, st, part, nblk(P.nf, kChunk), scal, slot, shift_slot);
+                      int shift_slot, double* host) {
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, st, part, nblk(P.nf, kChunk), scal, slot, shift_slot,
+                       host);
 }
 void launch_pq_update(hipStream_t st, const DevParams& P, const double* scal, const double* s, const double* y,
                       double* p, double* q) {
