@@ -584,6 +584,10 @@ __device__ inline FacEntry32 fac_entry32(const float* __restrict__ base, long lo
 // lanes per particle in both passes, and the list entries each lane has in flight
 // per batch (viscous entries FP64 and FP32; P entries)
 constexpr int kFacLanes = 8, kFacBatch = 4, kFacBatch32 = 8, kFacBatchP = 4;
+// the first pass (k's P list up to its last fluid entry, about 20 entries): lanes per
+// particle and entries in flight per lane (198 us against 200 us with 8 x 4 on the 1M
+// scene)
+constexpr int kFacLanesT = 4, kFacBatchT = 8;
 
 // the four planes of FacOp::rec
 struct FacPlanes {
@@ -605,7 +609,7 @@ __device__ inline FacPlanes fac_planes(const DevParams& P, double* rec) {
 template <bool kWrap>
 __device__ inline void fac_t_chunk(const DevParams& P, const FacOp& F, const double* __restrict__ x, int chunk) {
 #pragma clang fp contract(fast)
-    constexpr int G = kFacLanes, U = kFacBatchP;
+    constexpr int G = kFacLanesT, U = kFacBatchT;
     const int g = threadIdx.x & (G - 1);
     const double W0 = P.W[0], W1 = P.W[1], W2 = P.W[2], h0 = P.hW[0], h1 = P.hW[1], h2 = P.hW[2];
     for (int r = threadIdx.x / G; r < 256; r += 256 / G) {
