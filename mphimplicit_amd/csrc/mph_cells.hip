@@ -172,25 +172,42 @@ __device__ inline double dist2(const DevParams& P, const double* __restrict__ po
 }
 
 // count pass, main.cpp:1406-1511; also the largest P-neighbour count of a fluid
-// particle (it sizes the LDS staging of the C Lambda C^T kernel)
+// particle (it sizes the LDS staging of the C Lambda C^T kernel), the largest counts
+// of any particle (they size the scratch lists of the next search), and, when scrM is
+// set, the lists themselves: the hits go in visiting order into per-particle scratch
+// of capM / capP entries (runStart as the fill pass writes it), and a particle with
+// more raises *overflow, which sends the search to the fill pass.
 __global__ void k_neighbor_count(DevParams P, const double* __restrict__ pos, const int* __restrict__ cellIndex,
                                  const int* __restrict__ cellStart, int* __restrict__ nCount,
-                                 int* __restrict__ nFluid, int* __restrict__ nCountP, int* __restrict__ maxFluidP) {
+                                 int* __restrict__ nFluid, int* __restrict__ nCountP, int* __restrict__ maxFluidP,
+                                 int* __restrict__ scrM, int capM, int* __restrict__ scrP, int capP,
+                                 unsigned short* __restrict__ runStart, int* __restrict__ overflow) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.np) return;
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
     const int key = cellIndex[i];
     int n = 0, nfl = 0, npp = 0;
+    int* sm = scrM ? scrM + (long long)i * capM : nullptr;
+    int* sp = scrM ? scrP + (long long)i * capP : nullptr;
+    unsigned short* rs = scrM && i < P.nf ? runStart + (long long)i * kRuns : nullptr;
+    int q = 0;
     for_cells<true>(P, cellStart, key, [&](int lo, int hi) {
+        if (rs) rs[q] = (unsigned short)n;
+        q++;
         for (int j = lo; j < hi; ++j) {
             const double q2 = dist2(P, pos, xi, yi, zi, j);
             if (q2 <= P.maxR2) {
+                if (sm && n < capM) sm[n] = j;
                 n++;
                 nfl++;
-                if (q2 <= P.radP2) npp++;
+                if (q2 <= P.radP2) {
+                    if (sp && npp < capP) sp[npp] = j;
+                    npp++;
+                }
             }
         }
     });
+    if (rs) rs[q] = (unsigned short)n;
     // walls without fluid neighbours skip the wall scan (main.cpp:1474); walls
     // occupy [nf, np) after every sort
     if (!(i >= P.nf && n == 0))
@@ -198,8 +215,12 @@ __global__ void k_neighbor_count(DevParams P, const double* __restrict__ pos, co
             for (int j = lo; j < hi; ++j) {
                 const double q2 = dist2(P, pos, xi, yi, zi, j);
                 if (q2 <= P.maxR2) {
+                    if (sm && n < capM) sm[n] = j;
                     n++;
-                    if (q2 <= P.radP2) npp++;
+                    if (q2 <= P.radP2) {
+                        if (sp && npp < capP) sp[npp] = j;
+                        npp++;
+                    }
                 }
             }
         });
@@ -210,6 +231,29 @@ __global__ void k_neighbor_count(DevParams P, const double* __restrict__ pos, co
         atomicMax(maxFluidP, npp);      // [0]: largest P-neighbour count of a fluid particle
         atomicMax(maxFluidP + 1, nfl);  // [1]: largest fluid-neighbour count (A row length)
     }
+    atomicMax(maxFluidP + 2, n);    // [2]: largest neighbour count
+    atomicMax(maxFluidP + 3, npp);  // [3]: largest P-neighbour count
+    if (sm && (n > capM || npp > capP)) atomicOr(overflow, 1);
+}
+
+// the lists from the scratch of the count pass, 32 lanes per particle (coalesced)
+__global__ void __launch_bounds__(256) k_neighbor_compact(DevParams P, const int* __restrict__ nCount,
+                                                          const int* __restrict__ nCountP,
+                                                          const long long* __restrict__ nPtr,
+                                                          const long long* __restrict__ nPtrP,
+                                                          const int* __restrict__ scrM, int capM,
+                                                          const int* __restrict__ scrP, int capP,
+                                                          int* __restrict__ nInd, int* __restrict__ nIndP) {
+    constexpr int G = 32;
+    const int g = threadIdx.x & (G - 1);
+    const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / G);
+    if (i >= P.np) return;
+    const int n = nCount[i], npp = nCountP[i];
+    const long long b = nPtr[i], bp = nPtrP[i];
+    const int* sm = scrM + (long long)i * capM;
+    const int* sp = scrP + (long long)i * capP;
+    for (int k = g; k < n; k += G) nInd[b + k] = sm[k];
+    for (int k = g; k < npp; k += G) nIndP[bp + k] = sp[k];
 }
 
 // fill pass, main.cpp:1567-1666.  For fluid particles it also records where the
@@ -361,9 +405,17 @@ void launch_reorder(hipStream_t st, const DevParams& P, const uint64_t* keys, co
 }
 
 void launch_neighbor_count(hipStream_t st, const DevParams& P, const double* pos, const int* cellIndex,
-                           const int* cellStart, int* nCount, int* nFluid, int* nCountP, int* maxFluidP) {
+                           const int* cellStart, int* nCount, int* nFluid, int* nCountP, int* maxFluidP, int* scrM,
+                           int capM, int* scrP, int capP, unsigned short* runStart, int* overflow) {
     hipLaunchKernelGGL(k_neighbor_count, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, pos, cellIndex, cellStart,
-                       nCount, nFluid, nCountP, maxFluidP);
+                       nCount, nFluid, nCountP, maxFluidP, scrM, capM, scrP, capP, runStart, overflow);
+}
+
+void launch_neighbor_compact(hipStream_t st, const DevParams& P, const int* nCount, const int* nCountP,
+                             const long long* nPtr, const long long* nPtrP, const int* scrM, int capM,
+                             const int* scrP, int capP, int* nInd, int* nIndP) {
+    hipLaunchKernelGGL(k_neighbor_compact, dim3(nblk(32LL * P.np, 256)), dim3(256), 0, st, P, nCount, nCountP, nPtr,
+                       nPtrP, scrM, capM, scrP, capP, nInd, nIndP);
 }
 
 void launch_neighbor_fill(hipStream_t st, const DevParams& P, const double* pos, const int* cellIndex,

@@ -123,8 +123,17 @@ void launch_reorder(hipStream_t st, const DevParams& P, const uint64_t* keys, co
                     const double* pos, const double* vel, int* type2, int* perm2, double* pos2, double* vel2,
                     int* cellIndex);
 // ---- neighbours (calculateNeighbor) ----
+// maxFluidP[4]: largest P count and fluid count of a fluid particle, largest count and
+// P count of any particle.  With scrM set, the lists also go into per-particle
+// scratch (capM / capP entries) and runStart is written; *overflow is raised if a
+// list does not fit (then launch_neighbor_fill builds the lists instead).
 void launch_neighbor_count(hipStream_t st, const DevParams& P, const double* pos, const int* cellIndex,
-                           const int* cellStart, int* nCount, int* nFluid, int* nCountP, int* maxFluidP);
+                           const int* cellStart, int* nCount, int* nFluid, int* nCountP, int* maxFluidP, int* scrM,
+                           int capM, int* scrP, int capP, unsigned short* runStart, int* overflow);
+// the lists from the count pass's scratch
+void launch_neighbor_compact(hipStream_t st, const DevParams& P, const int* nCount, const int* nCountP,
+                             const long long* nPtr, const long long* nPtrP, const int* scrM, int capM,
+                             const int* scrP, int capP, int* nInd, int* nIndP);
 void launch_scan_i64(hipStream_t st, const int* in, long long* out, int n, long long* scratch, int scratch_len);
 int scan_scratch_len(int n);
 void launch_neighbor_fill(hipStream_t st, const DevParams& P, const double* pos, const int* cellIndex,

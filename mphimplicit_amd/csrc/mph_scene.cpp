@@ -222,11 +222,30 @@ int phase_cells(mph_scene& S) {
     return MPH_OK;
 }
 
+// scratch of the one-scan neighbour search, per scene (a scene in a batch keeps the
+// two passes: the serving loop budgets device memory per scene without it)
+constexpr size_t kSearchScratchMax = size_t(8) << 30;
+
 int phase_neighbor_launch(mph_scene& S) {
     const DevParams& P = S.P;
-    launch_fill_i32(S.st, S.maxP.p, 2, 0);
+    launch_fill_i32(S.st, S.maxP.p, 4, 0);
+    // one scan writes the lists into scratch when the last search's largest counts
+    // (plus a margin) bound them; else the count pass is followed by the fill pass
+    S.searchCapM = S.searchCapP = 0;
+    if (S.batches == 0 && S.hmaxN > 0) {
+        const int capM = (S.hmaxN + 16 + 3) & ~3, capP = (S.hmaxNP + 8 + 3) & ~3;
+        if ((size_t)P.np * (size_t)(capM + capP) * sizeof(int) <= kSearchScratchMax &&
+            S.scrM.ensure((size_t)P.np * capM) == MPH_OK && S.scrP.ensure((size_t)P.np * capP) == MPH_OK &&
+            S.searchOvf.ensure(1) == MPH_OK) {
+            S.searchCapM = capM;
+            S.searchCapP = capP;
+            launch_fill_i32(S.st, S.searchOvf.p, 1, 0);
+        }
+    }
+    const bool scr = S.searchCapM > 0;
     launch_neighbor_count(S.st, P, S.pos.p, S.cellIndex.p, S.cellStart.p, S.nCount.p, S.nFluid.p, S.nCountP.p,
-                          S.maxP.p);
+                          S.maxP.p, scr ? S.scrM.p : nullptr, S.searchCapM, scr ? S.scrP.p : nullptr,
+                          S.searchCapP, S.runStart.p, scr ? S.searchOvf.p : nullptr);
     launch_scan_i64(S.st, S.nCount.p, S.nPtr.p, P.np, S.scanScratch.p, (int)S.scanScratch.cap);
     launch_scan_i64(S.st, S.nCountP.p, S.nPtrP.p, P.np, S.scanScratch.p, (int)S.scanScratch.cap);
     launch_slice_width(S.st, P.nf, S.nslices, S.nFluid.p, S.width.p);
@@ -236,7 +255,8 @@ int phase_neighbor_launch(mph_scene& S) {
     HIPCHK(hipMemcpyAsync(&h[1], S.nPtrP.p + P.np, 8, hipMemcpyDeviceToHost, S.st));
     HIPCHK(hipMemcpyAsync(&h[2], S.sliceStart.p + S.nslices, 8, hipMemcpyDeviceToHost, S.st));
     HIPCHK(hipMemcpyAsync(&h[3], S.bbox.p, 6 * sizeof(int), hipMemcpyDeviceToHost, S.st));
-    HIPCHK(hipMemcpyAsync(&h[6], S.maxP.p, 2 * sizeof(int), hipMemcpyDeviceToHost, S.st));
+    HIPCHK(hipMemcpyAsync(&h[6], S.maxP.p, 4 * sizeof(int), hipMemcpyDeviceToHost, S.st));
+    if (scr) HIPCHK(hipMemcpyAsync(&h[10], S.searchOvf.p, sizeof(int), hipMemcpyDeviceToHost, S.st));
     return MPH_OK;
 }
 
@@ -246,6 +266,9 @@ int phase_neighbor_finish(mph_scene& S) {
     const long long* h = (const long long*)S.hpin;
     S.hmaxP = ((const int*)&h[6])[0];
     S.hmaxW = ((const int*)&h[6])[1];
+    S.hmaxN = ((const int*)&h[6])[2];
+    S.hmaxNP = ((const int*)&h[6])[3];
+    const bool scanned = S.searchCapM > 0 && *(const int*)&h[10] == 0;  // every list fit its scratch
     S.E = h[0];
     S.EP = h[1];
     S.slots = h[2] * kSlice;
@@ -253,8 +276,12 @@ int phase_neighbor_finish(mph_scene& S) {
     S.blocks = blocks_wanted(S);
     int rc;
     if ((rc = S.nInd.ensure(S.E > 0 ? S.E : 1)) || (rc = S.nIndP.ensure(S.EP > 0 ? S.EP : 1))) return rc;
-    launch_neighbor_fill(S.st, P, S.pos.p, S.cellIndex.p, S.cellStart.p, S.nCount.p, S.nPtr.p, S.nPtrP.p, S.nInd.p,
-                         S.nIndP.p, S.runStart.p);
+    if (scanned)
+        launch_neighbor_compact(S.st, P, S.nCount.p, S.nCountP.p, S.nPtr.p, S.nPtrP.p, S.scrM.p, S.searchCapM,
+                                S.scrP.p, S.searchCapP, S.nInd.p, S.nIndP.p);
+    else
+        launch_neighbor_fill(S.st, P, S.pos.p, S.cellIndex.p, S.cellStart.p, S.nCount.p, S.nPtr.p, S.nPtrP.p,
+                             S.nInd.p, S.nIndP.p, S.runStart.p);
     return MPH_OK;
 }
 
@@ -862,7 +889,7 @@ void destroy(mph_scene* S) {
     if (S->st) (void)hipStreamSynchronize(S->st);
     for (DBuf<int>* b : {&S->type, &S->type2, &S->perm, &S->perm2, &S->cellIndex, &S->cellStart, &S->bbox,
                          &S->nCount, &S->nFluid, &S->nCountP, &S->nInd, &S->nIndP, &S->width, &S->selfIdx, &S->acol,
-                         &S->maxP, &S->runP, &S->fwrap, &S->fgmask})
+                         &S->maxP, &S->runP, &S->fwrap, &S->fgmask, &S->scrM, &S->scrP, &S->searchOvf})
         b->release();
     S->fgtab.release();
     S->runStart.release();
@@ -1391,7 +1418,7 @@ int mph_scene_device_bytes(mph_scene* S, unsigned long long* bytes) {
     unsigned long long n = 0;
     for (DBuf<int>* b : {&S->type, &S->type2, &S->perm, &S->perm2, &S->cellIndex, &S->cellStart, &S->bbox,
                          &S->nCount, &S->nFluid, &S->nCountP, &S->nInd, &S->nIndP, &S->width, &S->selfIdx, &S->acol,
-                         &S->maxP, &S->runP, &S->fwrap, &S->fgmask})
+                         &S->maxP, &S->runP, &S->fwrap, &S->fgmask, &S->scrM, &S->scrP, &S->searchOvf})
         n += b->cap * sizeof(int);
     n += S->fgtab.cap * sizeof(double);
     for (DBuf<double>* b : {&S->falpha, &S->fdiag, &S->selfC, &S->fbeta, &S->frec}) n += b->cap * sizeof(double);

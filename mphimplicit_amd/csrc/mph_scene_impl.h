@@ -162,6 +162,11 @@ struct mph_scene {
     long long E = 0, EP = 0;
     int hmaxP = 0;
     int hmaxW = 0;  // largest fluid-neighbour count (A row length in blocks)
+    // the neighbour search in one scan: per-particle scratch lists sized by the last
+    // search's largest counts (hmaxN, hmaxNP) plus a margin; searchCapM = 0 when the
+    // current search uses the count-then-fill passes instead
+    mph::host::DBuf<int> scrM, scrP, searchOvf;
+    int hmaxN = 0, hmaxNP = 0, searchCapM = 0, searchCapP = 0;
     // A and C
     int nslices = 0;
     mph::host::DBuf<int> width, selfIdx, acol;
