@@ -176,7 +176,10 @@ int mph_scene_run_phase(mph_scene* s, int phase);
  * step assembled the 3x3 blocks: always for the assembled form; for the factored
  * form only when its multigrid level 0 spans fewer than 3 cells along an axis,
  * otherwise level 0 and InvDiagA come from the factored scalars and CsrCofA is not
- * available). */
+ * available), and NeighborOneScan (1 when the last neighbour search built the lists
+ * in one scan, 0 when it ran the count and fill passes: a scene's first search, a
+ * scene in a batch, or a list longer than the last search's largest plus a margin;
+ * the lists are the same either way). */
 int mph_scene_field_size(mph_scene* s, const char* field, size_t* bytes, int* elem_bytes);
 int mph_scene_read(mph_scene* s, const char* field, void* host, size_t bytes);
 int mph_scene_write(mph_scene* s, const char* field, const void* host, size_t bytes);

@@ -276,6 +276,7 @@ int phase_neighbor_finish(mph_scene& S) {
     S.blocks = blocks_wanted(S);
     int rc;
     if ((rc = S.nInd.ensure(S.E > 0 ? S.E : 1)) || (rc = S.nIndP.ensure(S.EP > 0 ? S.EP : 1))) return rc;
+    S.oneScan = scanned;
     if (scanned)
         launch_neighbor_compact(S.st, P, S.nCount.p, S.nCountP.p, S.nPtr.p, S.nPtrP.p, S.scrM.p, S.searchCapM,
                                 S.scrP.p, S.searchCapP, S.nInd.p, S.nIndP.p);
@@ -1319,6 +1320,7 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
     else if (n == "NeighborIndCountP") *v = (double)S->EP;
     else if (n == "ASlots") *v = (double)S->slots;
     else if (n == "AssembledBlocks") *v = S->blocks ? 1.0 : 0.0;
+    else if (n == "NeighborOneScan") *v = S->oneScan ? 1.0 : 0.0;
     else if (n == "MultiGridDepth") *v = S->depth;
     else if (n == "RefinementSteps") *v = S->irOuter;
     else if (n == "SpmvMs") *v = S->spmvMs;

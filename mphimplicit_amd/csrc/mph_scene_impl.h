@@ -167,6 +167,7 @@ struct mph_scene {
     // current search uses the count-then-fill passes instead
     mph::host::DBuf<int> scrM, scrP, searchOvf;
     int hmaxN = 0, hmaxNP = 0, searchCapM = 0, searchCapP = 0;
+    bool oneScan = false;  // the last search built its lists from the scratch
     // A and C
     int nslices = 0;
     mph::host::DBuf<int> width, selfIdx, acol;

@@ -111,6 +111,33 @@ def test_trajectory_bitwise(name, steps, tol):
         assert m is None, m
 
 
+def test_neighbor_search_paths():
+    """The one-scan neighbour search sizes its scratch by the previous search's
+    largest counts. Compressing the particles by 0.85 about the centre makes the
+    lists longer than that, so the search falls back to the count and fill passes;
+    the search after it fits again. Lists and runs stay bit-identical to the
+    oracle's on every path."""
+    gpu, orc = _pair("3d_4096")
+    assert gpu.scalar("NeighborOneScan") == 0.0  # the first search has no counts to size by
+    gpu.step(1), orc.step(1)
+    assert gpu.scalar("NeighborOneScan") == 1.0
+    pos = gpu.read("Position")
+    assert pos.tobytes() == orc.read("Position").tobytes()
+    xyz = pos.reshape(-1, 3)
+    c = 0.5 * (xyz.min(axis=0) + xyz.max(axis=0))
+    packed = (c + 0.85 * (xyz - c)).reshape(-1)
+    for sc in (gpu, orc):
+        sc.write("Position", packed)
+    for expect in (0.0, 1.0):  # overflow: fill pass; then scratch sized for the denser lists
+        for sc in (gpu, orc):
+            sc.run_phase(abi.PHASE_CELLS)
+            sc.run_phase(abi.PHASE_NEIGHBOR)
+        assert gpu.scalar("NeighborOneScan") == expect
+        for n in ["CellParticle", "NeighborInd", "NeighborIndP", "Position"]:
+            m = diff(n, gpu.read(n), orc.read(n))
+            assert m is None, m
+
+
 @pytest.mark.parametrize("name,steps", [("2d_40x40", 20), ("3d_4096", 10), ("bridge2d_sample", 3)])
 def test_against_reference_dot_order(name, steps):
     """GPU vs the reference's serial dot order: positions and velocities agree to
