@@ -25,6 +25,10 @@ __device__ inline int cell_id(const DevParams& P, int cx, int cy, int cz) {
            ((cz) % C[2] + C[2]) % C[2];
 }
 
+// a level's S buffer by ping-pong parity, as a select: indexing the two-pointer array
+// with a run-time value would keep a Level copied into a local in private memory
+__device__ inline double* lv_s(const Level& L, unsigned parity) { return parity ? L.S[1] : L.S[0]; }
+
 __device__ inline void grid_xyz(const Level& L, int g, int& gx, int& gy, int& gz) {
     gx = (g / (L.cnt[1] * L.cnt[2])) % L.cnt[0];
     gy = (g / L.cnt[2]) % L.cnt[1];
@@ -100,13 +104,13 @@ __device__ inline void mg_p2g_row(const DevParams& P, const Level& L, const int*
 }
 
 __device__ inline void mg_begin_row(const Level& L, int row) {  // myDscal(0.0, S); myDcopy(Q, R)
-    double* S = L.S[L.cur];
+    double* S = lv_s(L, L.cur);
     S[row] *= 0.0;
     L.R[row] = L.Q[row];
 }
 
 __device__ inline void mg_update_row(const Level& L, int row) {  // myDdmv(1.0, invD, R, 1.0, S)
-    double* S = L.S[L.cur];
+    double* S = lv_s(L, L.cur);
     double y = S[row];
     y *= 1.0;
     y += 1.0 * L.invD[row] * L.R[row];
@@ -115,7 +119,7 @@ __device__ inline void mg_update_row(const Level& L, int row) {  // myDdmv(1.0, 
 
 // mg_begin_row then mg_update_row on the same row (both touch that row only)
 __device__ inline void mg_begin_update_row(const Level& L, int row) {
-    double* S = L.S[L.cur];
+    double* S = lv_s(L, L.cur);
     double y = S[row];
     y *= 0.0;
     const double r = L.Q[row];
@@ -207,7 +211,7 @@ __device__ inline double mg_row_sum(const Level& L, const VT* __restrict__ A, co
 template <class VT>
 __device__ inline void mg_residual_row(const Level& L, int row) {
     const VT* A = sizeof(VT) == 8 ? (const VT*)L.A : (const VT*)L.A32;
-    const double sum = mg_row_sum(L, A, L.S[L.cur], row);
+    const double sum = mg_row_sum(L, A, lv_s(L, L.cur), row);
     double y = L.Q[row];
     y *= 1.0;
     y += sum;
@@ -218,7 +222,7 @@ __device__ inline void mg_residual_row(const Level& L, int row) {
 // S buffer so every thread still reads the old one
 template <class VT>
 __device__ inline void mg_residual_update_row(const Level& L, int row) {
-    const double* So = L.S[L.cur];
+    const double* So = lv_s(L, L.cur);
     const VT* A = sizeof(VT) == 8 ? (const VT*)L.A : (const VT*)L.A32;
     const double sum = mg_row_sum(L, A, So, row);
     double y = L.Q[row];
@@ -228,7 +232,7 @@ __device__ inline void mg_residual_update_row(const Level& L, int row) {
     double s = So[row];
     s *= 1.0;
     s += 1.0 * L.invD[row] * y;
-    L.S[L.cur ^ 1][row] = s;
+    lv_s(L, L.cur ^ 1)[row] = s;
 }
 
 // restriction (R of 3115-3175): Q_L = 0*Q_L + sum of the clipped 2x2x2 children of R_S
@@ -261,8 +265,8 @@ __device__ inline void mg_prolong_row(const Level& L, const Level& S, int row) {
     const int lg = grid_id(L, ((sx + S.mn[0]) / 2) - L.mn[0], ((sy + S.mn[1]) / 2) - L.mn[1],
                            ((sz + S.mn[2]) / 2) - L.mn[2]);
     double sum = 0.0;
-    sum += 1.0 * 1.0 * L.S[L.cur][3 * lg + rd];
-    double* Sv = S.S[S.cur];
+    sum += 1.0 * 1.0 * lv_s(L, L.cur)[3 * lg + rd];
+    double* Sv = lv_s(S, S.cur);
     double y = Sv[row];
     y *= 1.0;
     y += sum;
@@ -279,18 +283,76 @@ __device__ inline void mg_prolong_row(const Level& L, const Level& S, int row) {
 // on entry, lc-1 included; it advances as the separate launches would advance it
 // (afterwards every fused level but the coarsest has flipped).  f32: use a level's FP32
 // stencil where it has one.
+// mg_row_sum with three stencil cells loaded at a time: the same terms in the same
+// order in fewer registers, for the one-workgroup coarse cycle (1024 threads, so at
+// most 128 VGPRs: the 27-cell batches of mg_row_sum spill there)
+template <class VT>
+__device__ inline double mg_row_sum_narrow(const Level& L, const VT* __restrict__ A, const double* __restrict__ x,
+                                           int row) {
+    int gx, gy, gz;
+    grid_xyz(L, row / 3, gx, gy, gz);
+    const int cs1 = L.cs[1], cs2 = L.cs[2], N = L.cs[0] * cs1 * cs2;
+    const int c0 = L.cnt[0], c1 = L.cnt[1], c2 = L.cnt[2];
+    const long long nrow = L.nrow;
+    double sum = 0.0;
+    int ex = 0, ey = 0, ez = 0;
+    for (int q0 = 0; q0 < N; q0 += 3) {
+        VT av[9];
+        double xv[9];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const int q = q0 + t;
+            if (q < N) {
+                const int jx = wrap3(gx + ex - 1 + c0, c0), jy = wrap3(gy + ey - 1 + c1, c1);
+                const int jz = wrap3(gz + ez - 1 + c2, c2);
+                const double* xc = x + 3 * ((c1 * c2) * jx + c2 * jy + jz);
+#pragma unroll
+                for (int sd = 0; sd < 3; ++sd) {
+                    av[3 * t + sd] = A[(3 * q + sd) * nrow + row];
+                    xv[3 * t + sd] = xc[sd];
+                }
+                if (++ez == cs2) {
+                    ez = 0;
+                    if (++ey == cs1) {
+                        ey = 0;
+                        ++ex;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+            if (q0 + t < N)
+#pragma unroll
+                for (int sd = 0; sd < 3; ++sd) sum += -1.0 * (double)av[3 * t + sd] * xv[3 * t + sd];
+    }
+    return sum;
+}
+
 __device__ inline void mg_coarse_cycle(const Level* lvs, int lc, int D, unsigned cur, bool f32) {
     auto lev = [&](int l) {
         Level L = lvs[l];
         L.cur = (cur >> l) & 1u;
         return L;
     };
+    auto sweep = [&](const Level& L, int row, bool update) {  // mg_residual(_update)_row, narrow sum
+        const double* So = lv_s(L, L.cur);
+        const double sum = f32 && L.A32 ? mg_row_sum_narrow<float>(L, L.A32, So, row)
+                                        : mg_row_sum_narrow<double>(L, L.A, So, row);
+        double y = L.Q[row];
+        y *= 1.0;
+        y += sum;
+        L.R[row] = y;
+        if (update) {
+            double s = So[row];
+            s *= 1.0;
+            s += 1.0 * L.invD[row] * y;
+            lv_s(L, L.cur ^ 1)[row] = s;
+        }
+    };
     auto ru = [&](int l) {
         const Level L = lev(l);
-        for (int row = threadIdx.x; row < L.nrow; row += blockDim.x) {
-            if (f32 && L.A32) mg_residual_update_row<float>(L, row);
-            else mg_residual_update_row<double>(L, row);
-        }
+        for (int row = threadIdx.x; row < L.nrow; row += blockDim.x) sweep(L, row, true);
         __syncthreads();
         cur ^= 1u << l;
     };
@@ -315,10 +377,7 @@ __device__ inline void mg_coarse_cycle(const Level* lvs, int lc, int D, unsigned
         ru(l);
         ru(l);
         const Level L = lev(l);
-        for (int row = threadIdx.x; row < L.nrow; row += blockDim.x) {
-            if (f32 && L.A32) mg_residual_row<float>(L, row);
-            else mg_residual_row<double>(L, row);
-        }
+        for (int row = threadIdx.x; row < L.nrow; row += blockDim.x) sweep(L, row, false);
         __syncthreads();
     }
     {
@@ -344,7 +403,7 @@ __device__ inline void mg_g2p_chunk(const DevParams& P, const Level& L, const in
         const int key = cellIndex[i];
         const int cx = key / (P.C[1] * P.C[2]) % P.C[0], cy = key / (P.C[2]) % P.C[1], cz = key % P.C[2];
         const int g = grid_id(L, cx - L.mn[0], cy - L.mn[1], cz - L.mn[2]);
-        const double* S0 = L.S[L.cur];
+        const double* S0 = lv_s(L, L.cur);
         double uu[3], qq[3];
         for (int r = 0; r < 3; ++r) {
             qq[r] = q[3 * i + r];
