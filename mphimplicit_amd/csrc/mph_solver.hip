@@ -479,30 +479,54 @@ __device__ inline Rel rel(const DevParams& P, const double* __restrict__ pos, in
     return o;
 }
 
-__global__ void k_divergence(DevParams P, const double* __restrict__ pos, const double* __restrict__ vel,
-                             const double* __restrict__ lambda, const double* __restrict__ kappa,
-                             const double* __restrict__ volStrainP, const int* __restrict__ nCount,
-                             const long long* __restrict__ nPtr, const int* __restrict__ nInd,
-                             double* __restrict__ divP, double* __restrict__ pressP) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.np) return;
+// entry-parallel: a group of four lanes owns particle i, lane g forms the terms of
+// entries g, g + 4, ...; the group's first lane subtracts them in list order
+__global__ void __launch_bounds__(256) k_divergence(DevParams P, const double* __restrict__ pos,
+                                                    const double* __restrict__ vel, const double* __restrict__ lambda,
+                                                    const double* __restrict__ kappa,
+                                                    const double* __restrict__ volStrainP,
+                                                    const int* __restrict__ nCount, const long long* __restrict__ nPtr,
+                                                    const int* __restrict__ nInd, double* __restrict__ divP,
+                                                    double* __restrict__ pressP) {
+    constexpr int G = 4, K = 3;  // terms: u_d e_d dw for d = 0, 1, 2
+    __shared__ double tb[256 * K];
+    __shared__ int tf[256];
+    const int g = threadIdx.x & (G - 1);
+    const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / G);
+    const bool own = i < P.np;
     double sum = 0.0;
-    const long long b = nPtr[i], e = b + nCount[i];
-    for (long long k = b; k < e; ++k) {
-        const int j = nInd[k];
-        if (i == j) continue;
-        const Rel q = rel(P, pos, i, j);
-        if (q.r2 == 0.0) continue;
-        if (P.radP2 - q.r2 >= 0) {
-            const double r = sqrt(q.r2);
-            const double dw = dwq_k(P.Kp, P.m2P, r, P.radP);
-            const double ev[3] = {q.x[0] / r, q.x[1] / r, q.x[2] / r};
-            for (int d = 0; d < 3; ++d) {
-                const double u = vel[3 * j + d] - vel[3 * i + d];
-                sum -= u * ev[d] * dw;
+    const long long b = own ? nPtr[i] : 0;
+    const int n = own ? nCount[i] : 0;
+    for (int base = 0; base < n; base += G) {
+        const int jn = base + g;
+        int f = 0;
+        double t[K] = {0.0, 0.0, 0.0};
+        if (jn < n) {
+            const int j = nInd[b + jn];
+            if (i != j) {
+                const Rel q = rel(P, pos, i, j);
+                if (q.r2 != 0.0 && P.radP2 - q.r2 >= 0) {
+                    const double r = sqrt(q.r2);
+                    const double dw = dwq_k(P.Kp, P.m2P, r, P.radP);
+                    const double ev[3] = {q.x[0] / r, q.x[1] / r, q.x[2] / r};
+                    for (int d = 0; d < 3; ++d) {
+                        const double u = vel[3 * j + d] - vel[3 * i + d];
+                        t[d] = u * ev[d] * dw;
+                    }
+                    f = 1;
+                }
             }
         }
+        for (int e = 0; e < K; ++e) tb[K * threadIdx.x + e] = t[e];
+        tf[threadIdx.x] = f;
+        wave_sync();
+        if (g == 0)
+            for (int u = 0; u < G && base + u < n; ++u)
+                if (tf[threadIdx.x + u])
+                    for (int d = 0; d < 3; ++d) sum -= tb[K * (threadIdx.x + u) + d];
+        wave_sync();
     }
+    if (g != 0 || !own) return;
     divP[i] = sum;
     double pp = -lambda[i] * sum;
     if (volStrainP[i] > 0.0) pp += kappa[i] * volStrainP[i];
@@ -767,7 +791,7 @@ void launch_diagnostics(hipStream_t st, const DevParams& P, const int* type, con
                         const double* pressA, const double* gcen, const int* nCount, const long long* nPtr,
                         const int* nInd, const int* nCountP, const long long* nPtrP, const int* nIndP, double* divP,
                         double* pressP, double* force, double* virialStress, double* virialP) {
-    hipLaunchKernelGGL(k_divergence, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, pos, vel, lambda, kappa,
+    hipLaunchKernelGGL(k_divergence, dim3(nblk(4LL * P.np, 256)), dim3(256), 0, st, P, pos, vel, lambda, kappa,
                        volStrainP, nCount, nPtr, nInd, divP, pressP);
     hipLaunchKernelGGL(k_diag_forces, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, type, pos, vel, mu, pressA, gcen,
                        pressP, nCount, nPtr, nInd, nCountP, nPtrP, nIndP, force, virialStress, virialP);
