@@ -44,13 +44,16 @@ extern "C" {
 #define MPH_PRECISION_FP64 0  /* plain FP64, bit-compatible with the reference   */
 #define MPH_PRECISION_MIXED 1 /* FP32 SpMV + multigrid inside FP64 refinement    */
 
-/* How the solve applies the particle matrix (mph_params.operator_form).  Both build
- * the assembled matrix, which the multigrid preconditioner reads.
- * ASSEMBLED multiplies by the stored 3x3 blocks, the reference's CSR product
- * (main.cpp:4033-4050), bit for bit.
+/* How the solve applies the particle matrix (mph_params.operator_form).
+ * ASSEMBLED stores the 3x3 blocks of A and multiplies by them, the reference's CSR
+ * product (main.cpp:4033-4050), bit for bit.
  * FACTORED multiplies by A = viscous part + C Lambda C^T, rebuilt from one scalar per
- * block and per RadiusP pair plus the positions.  It reads about a sixth of the bytes
- * per product and equals the assembled product up to rounding, not bit for bit.
+ * fluid-neighbour pair and per RadiusP pair plus the positions.  A product moves about
+ * a fifth of the assembled form's bytes (0.21 on a 1M-particle 3D scene) and equals
+ * the assembled product up to rounding, not bit for bit.  Such a scene assembles no
+ * 3x3 blocks: the multigrid level 0 and InvDiagA come from the same scalars, except
+ * when the level-0 grid spans fewer than 3 cells along an axis (scalar
+ * "AssembledBlocks" = 1 then); reading CsrCofA otherwise fails with MPH_E_STATE.
  * A batch may hold scenes of both forms. */
 #define MPH_OPERATOR_ASSEMBLED 0
 #define MPH_OPERATOR_FACTORED 1
@@ -110,6 +113,7 @@ typedef struct mph_step_stats {
     int vtk_steps;           /* steps on which the VTK diagnostics were computed    */
     double ms_phase[12];     /* summed device time per MPH_PHASE_* (if timing on)   */
     double ms_total;
+    int status;              /* MPH_OK, or why this batch member failed (batches)  */
 } mph_step_stats;
 
 /* Phases of one time step, in the order of source/main.cpp:386-474. */
@@ -147,7 +151,9 @@ typedef struct mph_scene mph_scene;
 int mph_scene_create(const mph_params* p, const mph_grid* g, int device, mph_scene** out);
 
 /* Advance nsteps iterations of the reference's time loop (source/main.cpp:386-474),
- * including the output-step diagnostics on VTK steps.  `st` may be NULL. */
+ * including the output-step diagnostics on VTK steps.  `st` may be NULL.
+ * MPH_E_STATE when the CR residual is not finite (NaN or infinity in the system):
+ * the reference would iterate to its cap of 3*Nf (main.cpp:4344-4346) instead. */
 int mph_scene_step(mph_scene* s, int nsteps, mph_step_stats* st);
 
 /* Run one MPH_PHASE_* of the time step on the current state (parity tests). */
@@ -209,6 +215,14 @@ int mph_write_vtk_arrays(const char* path, int n, const int* type, const double*
  * "SpmvMsF32" and "SpmvLaunchesF32" (FP32 matrix, mixed precision only). */
 int mph_scene_set_timing(mph_scene* s, int on);
 
+/* Schedule options of a scene (no counterpart in the reference); they never change a
+ * result, only which of two bit-identical kernels runs.
+ *   "LaneRowsMax"  multigrid levels with fewer rows split each row of a smoothing
+ *                  sweep over 4 lanes, larger ones take one thread per row
+ *                  (default 32768).  Applies from the next multigrid build.
+ * MPH_E_FIELD for an unknown name, MPH_E_ARG for a value out of range. */
+int mph_scene_set_option(mph_scene* s, const char* name, double value);
+
 /* y = A x with the particle matrix of the last assembly, in the scene's operator
  * form (myDcsrmvForA, source/main.cpp:4033-4050, as called with alpha 1, beta 0).
  * x and y are host arrays of 3*FluidCount doubles in the current particle order.
@@ -231,8 +245,12 @@ int mph_scene_destroy(mph_scene* s);
  * one host synchronisation per iteration), then diagnostics and the advance.
  * Each scene's results are bit-identical to mph_scene_step on that scene.
  * `stats` (may be NULL) receives one record per member, in mph_batch_add order;
- * ms_total is the wall time of the whole batch step.  A failing step leaves the
- * members in an unspecified state. */
+ * ms_total is the wall time of the whole batch step.
+ * A member whose CR residual is not finite leaves the lockstep solve at once and
+ * does not advance; its record has status MPH_E_STATE, the other members finish the
+ * step, and the call returns MPH_E_STATE (remove the failed member before the next
+ * step).  Any other failure returns its code and leaves the members in an
+ * unspecified state. */
 typedef struct mph_batch mph_batch;
 int mph_batch_create(int device, mph_batch** out);
 int mph_batch_add(mph_batch* b, mph_scene* s);

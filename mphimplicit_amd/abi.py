@@ -117,6 +117,7 @@ class StepStats(C.Structure):
         ("vtk_steps", C.c_int),
         ("ms_phase", C.c_double * 12),
         ("ms_total", C.c_double),
+        ("status", C.c_int),
     ]
 
 

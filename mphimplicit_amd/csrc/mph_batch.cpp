@@ -16,6 +16,7 @@
 // Every scene runs the same arithmetic as mph_scene_step, so its results are
 // bit-identical alone and in any batch.
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstring>
 #include <string>
@@ -274,8 +275,14 @@ void finish_levels(const VCycle& V) {
         for (int l = 0; l < V.sc[s]->depth && l < kMaxLevels; ++l) V.sc[s]->lv[l].L.cur = V.cur[s][l];
 }
 
+// A member whose residual is not finite (NaN or infinity in its system) would
+// never meet the stop rule (rr/rr0 < tol is false for NaN) and would keep the whole
+// batch iterating to its cap of 3*Nf (main.cpp:4344-4346): it leaves the solve at
+// once and is flagged in `failed`.
+bool nonfinite(double v) { return !std::isfinite(v); }
+
 // solveWithConjugatedGradient (4270-4421) for every scene of `sc` in lockstep
-int solve_fp64(mph_batch& B, const std::vector<mph_scene*>& sc) {
+int solve_fp64(mph_batch& B, const std::vector<mph_scene*>& sc, std::vector<char>& failed) {
     const int ns = (int)sc.size();
     Tables T{};
     int rc;
@@ -295,6 +302,7 @@ int solve_fp64(mph_batch& B, const std::vector<mph_scene*>& sc) {
     if ((rc = read_scalars(B, ns))) return rc;
     std::vector<double> rr0(ns);
     std::vector<long long> iter(ns, 0), maxit(ns);
+    failed.assign(ns, 0);
     for (int s = 0; s < ns; ++s) {
         rr0[s] = B.hscal[8 * s + kRR0];
         const long long n = 3LL * sc[s]->P.nf;
@@ -305,6 +313,10 @@ int solve_fp64(mph_batch& B, const std::vector<mph_scene*>& sc) {
         Set now, first, rest;
         for (int s = 0; s < ns; ++s) {
             if (!active.has(s)) continue;
+            if (nonfinite(rr0[s]) || nonfinite(B.hscal[8 * s + kRR])) {
+                failed[s] = 1;
+                continue;
+            }
             if (iter[s] >= maxit[s]) continue;
             if (B.hscal[8 * s + kRR] / rr0[s] < sc[s]->prm.cr_tolerance) continue;
             now.add(s);
@@ -324,9 +336,14 @@ int solve_fp64(mph_batch& B, const std::vector<mph_scene*>& sc) {
             if (active.has(s)) iter[s]++;
         if ((rc = read_scalars(B, ns))) return rc;
     }
-    dot_partial(T, all.m, kR, kS);
-    dot_final(T, all.m, kRS, -1);
-    store_velocity(T, all.m);
+    Set ok;  // the members that did not fail
+    for (int s = 0; s < ns; ++s)
+        if (!failed[s]) ok.add(s);
+    if (!ok.empty()) {
+        dot_partial(T, ok.m, kR, kS);
+        dot_final(T, ok.m, kRS, -1);
+        store_velocity(T, ok.m);
+    }
     if ((rc = read_scalars(B, ns))) return rc;
     for (int s = 0; s < ns; ++s) {
         mph_scene& S = *sc[s];
@@ -343,7 +360,7 @@ int solve_fp64(mph_batch& B, const std::vector<mph_scene*>& sc) {
 // the mixed-precision solve of mph_scene.cpp (FP64 iterative refinement around the
 // CR with FP32 matrices) for every scene of `sc`, each at its own point of the
 // refinement: per tick a scene runs one inner iteration, or its outer update
-int solve_mixed(mph_batch& B, const std::vector<mph_scene*>& sc) {
+int solve_mixed(mph_batch& B, const std::vector<mph_scene*>& sc, std::vector<char>& failed) {
     const int ns = (int)sc.size();
     Tables T{};
     int rc;
@@ -366,6 +383,7 @@ int solve_mixed(mph_batch& B, const std::vector<mph_scene*>& sc) {
         double rr0 = 0, rrOut = 0, rri0 = 0, target = 0;
     };
     std::vector<St> z(ns);
+    failed.assign(ns, 0);
     for (int s = 0; s < ns; ++s) {
         z[s].rr0 = B.hscal[8 * s + kRR0];
         const long long n = 3LL * sc[s]->P.nf;
@@ -386,6 +404,11 @@ int solve_mixed(mph_batch& B, const std::vector<mph_scene*>& sc) {
                     continue;
                 }
                 q.rrOut = B.hscal[8 * s + kRR];
+                if (nonfinite(q.rr0) || nonfinite(q.rrOut)) {
+                    failed[s] = 1;
+                    q.state = kDone;
+                    continue;
+                }
                 if (q.rrOut / q.rr0 < tol || q.total >= q.maxit) {
                     q.state = kDone;
                     continue;
@@ -401,6 +424,11 @@ int solve_mixed(mph_batch& B, const std::vector<mph_scene*>& sc) {
             }
             if (q.state == kInner) {
                 const double rri = B.hscal[8 * s + kRR];
+                if (nonfinite(rri)) {
+                    failed[s] = 1;
+                    q.state = kDone;
+                    continue;
+                }
                 if (rri / q.rri0 < q.target || q.total >= q.maxit) {
                     q.state = kUpdate;
                     update.add(s);
@@ -435,7 +463,10 @@ int solve_mixed(mph_batch& B, const std::vector<mph_scene*>& sc) {
         }
         if ((rc = read_scalars(B, ns))) return rc;
     }
-    store_velocity(T, all.m);
+    Set ok;  // the members that did not fail
+    for (int s = 0; s < ns; ++s)
+        if (!failed[s]) ok.add(s);
+    if (!ok.empty()) store_velocity(T, ok.m);
     HIPCHK(hipStreamSynchronize(B.st));
     for (int s = 0; s < ns; ++s) {
         mph_scene& S = *sc[s];
@@ -538,12 +569,25 @@ int mph_batch_step(mph_batch* B, mph_step_stats* stats) {
     if ((rc = sync_all(act))) return rc;
     std::vector<mph_scene*> plain, mix;
     for (mph_scene* S : act) (mixed(*S) ? mix : plain).push_back(S);
-    if (!plain.empty() && (rc = solve_fp64(*B, plain))) return rc;
-    if (!mix.empty() && (rc = solve_mixed(*B, mix))) return rc;
+    std::vector<char> badPlain, badMix;
+    if (!plain.empty() && (rc = solve_fp64(*B, plain, badPlain))) return rc;
+    if (!mix.empty() && (rc = solve_mixed(*B, mix, badMix))) return rc;
     if ((rc = check_launch())) return rc;
     HIPCHK(hipStreamSynchronize(B->st));
+    std::vector<char> bad(act.size(), 0);  // per active member: its solve failed
+    std::string badNames;
+    for (size_t k = 0; k < act.size(); ++k) {
+        const std::vector<mph_scene*>& grp = mixed(*act[k]) ? mix : plain;
+        const std::vector<char>& flags = mixed(*act[k]) ? badMix : badPlain;
+        const size_t at = std::find(grp.begin(), grp.end(), act[k]) - grp.begin();
+        if (at < flags.size() && flags[at]) {
+            bad[k] = 1;
+            badNames += (badNames.empty() ? "" : ", ") + std::to_string(idx[k]);
+        }
+    }
     for (size_t k = 0; k < act.size(); ++k) {
         mph_scene& S = *act[k];
+        if (bad[k]) continue;  // not advanced
         if (step_vtk_due(S)) {
             if ((rc = run_phase(S, MPH_PHASE_DIAGNOSTICS))) return rc;
             step_vtk_done(S);
@@ -558,6 +602,10 @@ int mph_batch_step(mph_batch* B, mph_step_stats* stats) {
         for (size_t k = 0; k < act.size(); ++k) {
             mph_step_stats& st = stats[idx[k]];
             const mph_scene& S = *act[k];
+            if (bad[k]) {
+                st.status = MPH_E_STATE;
+                continue;
+            }
             st.steps_done = 1;
             st.cr_iterations = S.crIter;
             st.cr_iterations_total = S.crIter;
@@ -569,6 +617,8 @@ int mph_batch_step(mph_batch* B, mph_step_stats* stats) {
             st.ms_total = ms;
         }
     }
+    if (!badNames.empty())
+        return fail(MPH_E_STATE, "non-finite CR residual in batch member(s) " + badNames + " (not advanced)");
     return MPH_OK;
 }
 

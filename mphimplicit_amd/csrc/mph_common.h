@@ -46,6 +46,15 @@ struct DevParams {
     double ratio[4][4];  // InteractionRatio
 };
 
+// orders the LDS writes and reads of the lanes of one wavefront: the entry-parallel
+// walks hand terms to a group's first lane, which adds them in order (the lanes of a
+// group are in one wavefront and run the same iterations)
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 MPH_HD bool is_fluid(int t) { return 0 <= t && t < 2; }
 MPH_HD bool is_wall(int t) { return 2 <= t && t < 4; }
 

@@ -621,14 +621,6 @@ __device__ inline double group_sum(double v, int G) {
     return v;
 }
 
-// orders the LDS writes and reads of the lanes of one wavefront (the entry-
-// parallel walks hand terms to a group's first lane, which adds them in order)
-__device__ inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // one entry of FacOp::u32 / v32: three floats and the column
 struct FacEntry32 {
     float a0, a1, a2;

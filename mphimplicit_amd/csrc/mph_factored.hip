@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(256) k_fac_p(DevParams P, const double* __rest
         }
         for (int e = 0; e < K; ++e) tb[K * threadIdx.x + e] = t[e];
         tf[threadIdx.x] = f;
-        body::wave_sync();
+        wave_sync();
         if (g == 0)
             for (int u = 0; u < G && base + u < n; ++u) {
                 const int sl = threadIdx.x + u;
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(256) k_fac_p(DevParams P, const double* __rest
                 }
                 if (fu & 2) run = base + u + 1;
             }
-        body::wave_sync();
+        wave_sync();
     }
     if (g != 0 || !own) return;
     runP[k] = run;
@@ -461,11 +461,11 @@ __global__ void __launch_bounds__(256) k_fac_invdiag(DevParams P, FacOp F, doubl
             for (int a = 0; a < 3; ++a) t[a] = lk * cv[a] * cv[a];
         }
         for (int e = 0; e < K; ++e) tb[K * threadIdx.x + e] = t[e];
-        body::wave_sync();
+        wave_sync();
         if (g == 0)
             for (int u = 0; u < G && base + u < n; ++u)
                 for (int a = 0; a < 3; ++a) d[a] += tb[K * (threadIdx.x + u) + a];
-        body::wave_sync();
+        wave_sync();
     }
     if (g == 0 && own)
         for (int a = 0; a < 3; ++a) invDiagA[3 * i + a] = d[a] != 0.0 ? 1.0 / d[a] : 0.0;

@@ -42,7 +42,11 @@ struct Level {  // geometry + storage of one multigrid level
     int cur;       // which S buffer holds the current iterate
     float* A32;    // FP32 copy of A for the mixed-precision inner solve (or null)
     int f32;       // smoothers read A32 instead of A
+    int lanes;     // sweeps split each row over lanes (small levels; set by the host)
 };
+
+// levels with fewer rows than this split their sweeps' rows over lanes by default
+constexpr int kLaneRowsDefault = 32768;
 
 struct ASlices {
     int nslices;

@@ -34,14 +34,6 @@ __device__ inline Rel rel(const DevParams& P, const double* __restrict__ pos, in
 // lists (k_forces 3.50 against 3.71 / 4.04 / 5.40 ms), 8 beat 4 on the short ones.
 constexpr int kLanes = 4, kLanesP = 8;
 
-// orders the LDS writes and reads of the lanes of one wavefront (the lanes of a
-// group are in one wavefront and run the same iterations)
-__device__ inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // calculateConvection, calculateWall, calculatePeriodicBoundary, resetForce
 __global__ void k_kinematics(DevParams P, WallKin wk, const int* __restrict__ type, double* __restrict__ pos,
                              double* __restrict__ vel, double* __restrict__ force) {

@@ -234,12 +234,22 @@ int phase_neighbor_launch(mph_scene& S) {
     S.searchCapM = S.searchCapP = 0;
     if (S.batches == 0 && S.hmaxN > 0) {
         const int capM = (S.hmaxN + 16 + 3) & ~3, capP = (S.hmaxNP + 8 + 3) & ~3;
-        if ((size_t)P.np * (size_t)(capM + capP) * sizeof(int) <= kSearchScratchMax &&
-            S.scrM.ensure((size_t)P.np * capM) == MPH_OK && S.scrP.ensure((size_t)P.np * capP) == MPH_OK &&
-            S.searchOvf.ensure(1) == MPH_OK) {
-            S.searchCapM = capM;
-            S.searchCapP = capP;
-            launch_fill_i32(S.st, S.searchOvf.p, 1, 0);
+        if ((size_t)P.np * (size_t)(capM + capP) * sizeof(int) <= kSearchScratchMax) {
+            if (S.scrM.ensure((size_t)P.np * capM) == MPH_OK && S.scrP.ensure((size_t)P.np * capP) == MPH_OK &&
+                S.searchOvf.ensure(1) == MPH_OK) {
+                S.searchCapM = capM;
+                S.searchCapP = capP;
+                launch_fill_i32(S.st, S.searchOvf.p, 1, 0);
+            } else {
+                // the scratch is optional: without it the fill pass builds the lists.  A
+                // failed hipMalloc leaves its error as the thread's last error, which the
+                // launch check after the phase would report; clear it, and give back
+                // what was allocated so that the rest of the step has the memory
+                (void)hipGetLastError();
+                S.scrM.release();
+                S.scrP.release();
+                g_err.clear();
+            }
         }
     }
     const bool scr = S.searchCapM > 0;
@@ -410,6 +420,7 @@ int phase_multigrid(mph_scene& S) {
         L.Q = H.Q.p;
         L.cur = 0;
         L.f32 = 0;
+        L.lanes = L.nrow < S.laneRowsMax ? 1 : 0;
         L.A32 = nullptr;
         if (mixed(S) && l + 1 < depth) {  // the coarsest level stays FP64
             if ((rc = H.A32.ensure((size_t)L.nrow * L.nshift))) return rc;
@@ -653,6 +664,21 @@ void collect_spmv_timing(mph_scene& S) {
     S.spmvUsed = 0;
 }
 
+// A residual that is not finite (NaN or infinity in b or in the iterates) never
+// meets the stop rule: rr/rr0 < tol is false for NaN, so the reference iterates to
+// its cap of 3*Nf (main.cpp:4344-4346).  The solve fails at once instead; the scene
+// state is then unspecified.
+int solve_nonfinite(mph_scene& S, const char* what, double v) {
+    S.facPref = nullptr;  // a first pass enqueued ahead is never used
+    S.facPrefEv = -1;
+    (void)hipStreamSynchronize(S.st);
+    if (S.spmvTiming) collect_spmv_timing(S);
+    collect_tuning(S);
+    char msg[160];
+    std::snprintf(msg, sizeof msg, "non-finite CR residual (%s = %g) at step %d", what, v, S.iStep);
+    return fail(MPH_E_STATE, msg);
+}
+
 // solveWithConjugatedGradient (4270-4421)
 int phase_solve(mph_scene& S) {
     const DevParams& P = S.P;
@@ -678,11 +704,13 @@ int phase_solve(mph_scene& S) {
     launch_dot_partial(st, P, S.r.p, S.r.p, S.part.p);
     launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1, rr_host(S));
     if ((rc = read_scalar(S, kRR0, &S.rr0))) return rc;
+    if (!std::isfinite(S.rr0)) return solve_nonfinite(S, "b.b", S.rr0);
     const long long maxit = S.prm.max_iterations > 0 ? S.prm.max_iterations : n;
     long long iter;
     for (iter = 0; iter < maxit; ++iter) {
         if ((rc = read_scalar_prefetch(S, kRR, &S.rr, S.s.p, false))) return rc;  // both products below are on s
         if (S.spmvTiming) collect_spmv_timing(S);
+        if (!std::isfinite(S.rr)) return solve_nonfinite(S, "r.r", S.rr);
         if (S.rr / S.rr0 < S.prm.cr_tolerance) break;
         if (iter == 0) {  // y = A s; p = s; q = A p (same sum); rho = s.y
             spmv(S, 1.0, S.s.p, 0.0, S.y.p, S.y.p, S.q.p, S.p.p, S.s.p, S.part.p);
@@ -743,6 +771,7 @@ int phase_solve_mixed(mph_scene& S) {
     launch_dot_partial(st, P, S.rOut.p, S.rOut.p, S.part.p);
     launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1);
     if ((rc = read_scalar(S, kRR0, &S.rr0))) return rc;
+    if (!std::isfinite(S.rr0)) return solve_nonfinite(S, "b.b", S.rr0);
     const long long maxit = S.prm.max_iterations > 0 ? S.prm.max_iterations : n;
     long long total = 0;
     int outer = 0;
@@ -750,6 +779,7 @@ int phase_solve_mixed(mph_scene& S) {
     for (; outer < 50; ++outer) {
         if ((rc = read_scalar(S, kRR, &rrOut))) return rc;
         if (S.spmvTiming) collect_spmv_timing(S);
+        if (!std::isfinite(rrOut)) return solve_nonfinite(S, "outer r.r", rrOut);
         if (rrOut / S.rr0 < S.prm.cr_tolerance || total >= maxit) break;
         // inner solve A32 d = rOut from d = 0 down to the remaining reduction
         double target = S.prm.cr_tolerance * S.rr0 / rrOut;
@@ -764,6 +794,10 @@ int phase_solve_mixed(mph_scene& S) {
             if (it > 0) {
                 if ((rc = read_scalar_prefetch(S, kRR, &rri, S.s.p, true))) return rc;
                 if (S.spmvTiming) collect_spmv_timing(S);
+                if (!std::isfinite(rri)) {
+                    set_levels_f32(S, 0);
+                    return solve_nonfinite(S, "inner r.r", rri);
+                }
             }
             if (rri / rri0 < target) break;
             if (it == 0) {
@@ -1394,6 +1428,17 @@ int mph_scene_set_timing(mph_scene* S, int on) {
     S->spmvMs = 0;
     S->spmvLaunches = 0;
     return MPH_OK;
+}
+
+int mph_scene_set_option(mph_scene* S, const char* name, double value) {
+    if (!S || !name) return fail(MPH_E_ARG, "null argument");
+    const std::string n = name;
+    if (n == "LaneRowsMax") {
+        if (!(value >= 0.0 && value <= 2147483647.0)) return fail(MPH_E_ARG, "LaneRowsMax out of range");
+        S->laneRowsMax = (int)value;
+        return MPH_OK;
+    }
+    return fail(MPH_E_FIELD, "unknown option " + n);
 }
 
 int mph_scene_apply_matrix(mph_scene* S, int precision, const double* x, double* y) {

@@ -58,7 +58,8 @@ enum { kRR0 = 0, kRS0 = 1, kRR = 2, kRHO = 3, kRHOP = 4, kTMP = 5, kRS = 6 };
 
 // Run-time choice among kernel variants that compute identical bits: while
 // exploring, launches cycle through the variants with HIP events around them;
-// once every variant has `trials` timings the fastest mean is kept for good.
+// once every variant has `trials` timings, the variant with the fastest single
+// trial is kept for good.
 struct Autotune {
     int nvar = 1, trials = 1, best = 0;
     std::vector<double> fastest;  // per variant, its fastest trial: a variant's first
@@ -185,6 +186,7 @@ struct mph_scene {
     // multigrid
     int hbbox[6] = {0, 0, 0, 0, 0, 0};
     int depth = 0;
+    int laneRowsMax = mph::kLaneRowsDefault;  // levels with fewer rows take the lane-split sweeps
     int fuseOf[mph::kFuseVariants] = {0, 0, 0};  // first level of the one-workgroup coarse cycle
     std::vector<mph::host::HostLevel> lv;
     std::vector<mph::Level> hlv;           // level records as uploaded to lvDev

@@ -312,7 +312,6 @@ __global__ void k_mg_residual_update(Level L) {
 // take 20-30% off.  C0 C1 C2 give the stencil at compile time (3x3x3, 3x3x1); 0 0 0
 // reads it from the level (other shapes), at the price of runtime index arithmetic.
 constexpr int kRowLanes = 4, kRowBlock = 128, kRowQ = 7;  // kRowQ: stencil cells per lane (27 / 4, rounded up)
-constexpr int kLaneRowsMax = 32768;                     // levels with fewer rows take the lanes
 
 template <class VT, bool kUpdate, int C0, int C1, int C2>
 __global__ void __launch_bounds__(kRowBlock) k_mg_sweep_lanes(Level L) {
@@ -371,7 +370,7 @@ __global__ void __launch_bounds__(kRowBlock) k_mg_sweep_lanes(Level L) {
 // launches the lane-split sweep on a small level; false for a large one
 template <bool kUpdate>
 bool launch_sweep_lanes(hipStream_t st, const Level& L) {
-    if (L.nrow >= kLaneRowsMax || L.cs[0] * L.cs[1] * L.cs[2] > kRowLanes * kRowQ) return false;
+    if (!L.lanes || L.cs[0] * L.cs[1] * L.cs[2] > kRowLanes * kRowQ) return false;
     const int nb = nblk((long long)kRowLanes * L.nrow, kRowBlock);
     const bool s3 = L.cs[0] == 3 && L.cs[1] == 3 && L.cs[2] == 3, s2 = L.cs[0] == 3 && L.cs[1] == 3 && L.cs[2] == 1;
 #define MPH_SWEEP(VT, A, B, C) hipLaunchKernelGGL((k_mg_sweep_lanes<VT, kUpdate, A, B, C>), dim3(nb), dim3(kRowBlock), 0, st, L)

@@ -53,6 +53,7 @@ def lib():
         L.mph_scene_write_prof.argtypes = [vp, cp]
         L.mph_scene_write_vtk.argtypes = [vp, cp]
         L.mph_scene_set_timing.argtypes = [vp, i]
+        L.mph_scene_set_option.argtypes = [vp, cp, d]
         L.mph_scene_destroy.argtypes = [vp]
         L.mph_scene_device_bytes.argtypes = [vp, C.POINTER(C.c_ulonglong)]
         L.mph_scene_apply_matrix.argtypes = [vp, i, C.POINTER(d), C.POINTER(d)]
@@ -157,6 +158,10 @@ class Scene:
     def set_timing(self, on: int) -> None:
         _check(lib().mph_scene_set_timing(self.h, int(on)))
 
+    def set_option(self, name: str, value: float) -> None:
+        """mph_scene_set_option: a schedule choice that never changes a result."""
+        _check(lib().mph_scene_set_option(self.h, name.encode(), float(value)))
+
     def read(self, name: str) -> np.ndarray:
         nb = C.c_size_t()
         es = C.c_int()
@@ -230,11 +235,20 @@ class Batch:
         return int(lib().mph_batch_size(self.h))
 
     def step(self) -> list:
-        """One time step of every member; returns one StepStats per member."""
+        """One time step of every member; returns one StepStats per member.
+
+        A member whose solve failed on its own (a non-finite CR residual) has a
+        non-zero ``status`` and did not advance; the others did, and no exception is
+        raised for it.  Remove such a member before the next step.  Any other failure
+        raises MphError."""
         n = len(self.members)
         arr = (StepStats * max(n, 1))()
-        _check(lib().mph_batch_step(self.h, C.cast(arr, C.c_void_p)))
-        return [arr[k] for k in range(n)]
+        rc = lib().mph_batch_step(self.h, C.cast(arr, C.c_void_p))
+        stats = [arr[k] for k in range(n)]
+        if rc == abi.E_STATE and any(st.status != 0 for st in stats):
+            return stats
+        _check(rc)
+        return stats
 
     def close(self) -> None:
         if getattr(self, "h", None):

@@ -169,6 +169,47 @@ def tiny_3d(seed=5, steps=5, lift=2):
     return config_3d_drop(n=6, box=14, lift=lift, seed=seed, steps=steps)
 
 
+def moving_walls(p: Params, g: GridData, dim: int) -> tuple[Params, GridData]:
+    """Give the walls rigid motion (test scenes of the moving-wall branch): walls on
+    the low-x side become type 3, the others stay type 2, and both types get a
+    translation velocity and an angular velocity about their own centre
+    (the Wall2/Wall3 lines of the reference's *.data file, main.cpp:554-555).
+    In 2D the rotation axis is z, so walls stay in the plane."""
+    typ = g.type.copy()
+    pos = g.pos
+    wall = typ >= 2
+    xmin_fluid = pos[typ < 2, 0].min()
+    side = wall & (pos[:, 0] < xmin_fluid - 2.0 * g.spacing)
+    typ[side] = 3
+    for w, sel in ((2, wall & ~side), (3, side)):
+        c = pos[sel].mean(axis=0)
+        for d in range(3):
+            p.wall_center[w][d] = float(c[d])
+    if dim == 2:
+        vel = {2: (0.02, -0.01, 0.0), 3: (0.0, 0.03, 0.0)}
+        omg = {2: (0.0, 0.0, 1.5), 3: (0.0, 0.0, -0.8)}
+    else:
+        vel = {2: (0.02, -0.01, 0.005), 3: (0.0, 0.03, -0.01)}
+        omg = {2: (0.0, 0.0, 2.0), 3: (0.5, -1.0, 0.7)}
+    for w in (2, 3):
+        for d in range(3):
+            p.wall_velocity[w][d] = vel[w][d]
+            p.wall_omega[w][d] = omg[w][d]
+    return p, GridData(g.time, g.spacing, g.domain_min, g.domain_max, typ, g.pos, g.vel)
+
+
+def tiny_2d_moving(steps=3):
+    """tiny_2d with translating and rotating walls of both wall types (test scene)."""
+    p, g = tiny_2d(steps=steps)
+    return moving_walls(p, g, 2)
+
+
+def tiny_3d_moving(steps=3):
+    """tiny_3d with translating and rotating walls of both wall types (test scene)."""
+    p, g = tiny_3d(steps=steps)
+    return moving_walls(p, g, 3)
+
+
 def box_scene(seed: int, dim: int, nf_target: int, mu: float, gravity, dt=5.0e-4, steps=20):
     """A fluid square (2D) or cube (3D) of about nf_target particles resting on the
     floor, centred, in an open-top 4-layer box whose interior is 2.5x the fluid
@@ -203,26 +244,28 @@ def box_scene_counts(dim: int, nf_target: int) -> tuple[int, int]:
     return n ** dim, n ** dim + walls
 
 
+def batch_sweep_member(s: int, seed0=1000, steps=20, nf_range=(2000, 200000), mu_range=(0.01, 10.0)):
+    """Scene s of the batched workload: seed seed0+s, 2D or 3D with probability 1/2,
+    a log-uniform fluid count in nf_range and viscosity in mu_range, and gravity 9.8
+    along a random unit vector (in-plane in 2D)."""
+    u = splitmix64_uniform(seed0 + s + 0x5EED, 6)
+    dim = 2 if u[0] < 0.5 else 3
+    nf = math.exp(math.log(nf_range[0]) + u[1] * (math.log(nf_range[1]) - math.log(nf_range[0])))
+    mu = math.exp(math.log(mu_range[0]) + u[2] * (math.log(mu_range[1]) - math.log(mu_range[0])))
+    if dim == 2:
+        a = 2.0 * math.pi * u[3]
+        g = (9.8 * math.cos(a), 9.8 * math.sin(a), 0.0)
+    else:
+        z = 2.0 * u[3] - 1.0
+        a = 2.0 * math.pi * u[4]
+        r = math.sqrt(max(0.0, 1.0 - z * z))
+        g = (9.8 * r * math.cos(a), 9.8 * r * math.sin(a), 9.8 * z)
+    return box_scene(seed0 + s, dim, int(nf), mu, g, steps=steps)
+
+
 def batch_sweep(count=64, seed0=1000, steps=20, nf_range=(2000, 200000), mu_range=(0.01, 10.0)):
-    """The batched workload: scene s has seed seed0+s, is 2D or 3D with
-    probability 1/2, has a log-uniform fluid count in nf_range and viscosity in
-    mu_range, and gravity 9.8 along a random unit vector (in-plane in 2D)."""
-    out = []
-    for s in range(count):
-        u = splitmix64_uniform(seed0 + s + 0x5EED, 6)
-        dim = 2 if u[0] < 0.5 else 3
-        nf = math.exp(math.log(nf_range[0]) + u[1] * (math.log(nf_range[1]) - math.log(nf_range[0])))
-        mu = math.exp(math.log(mu_range[0]) + u[2] * (math.log(mu_range[1]) - math.log(mu_range[0])))
-        if dim == 2:
-            a = 2.0 * math.pi * u[3]
-            g = (9.8 * math.cos(a), 9.8 * math.sin(a), 0.0)
-        else:
-            z = 2.0 * u[3] - 1.0
-            a = 2.0 * math.pi * u[4]
-            r = math.sqrt(max(0.0, 1.0 - z * z))
-            g = (9.8 * r * math.cos(a), 9.8 * r * math.sin(a), 9.8 * z)
-        out.append(box_scene(seed0 + s, dim, int(nf), mu, g, steps=steps))
-    return out
+    """The batched workload: scenes 0..count-1 of batch_sweep_member."""
+    return [batch_sweep_member(s, seed0, steps, nf_range, mu_range) for s in range(count)]
 
 
 def serve_stream(count: int, rate=200.0, seed=7, median=50000, sigma=1.0, clip=(5000, 500000),

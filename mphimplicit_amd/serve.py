@@ -10,18 +10,27 @@ budget; finished requests leave the batch and free their memory at once.  A
 scene's arithmetic does not depend on its batch-mates, so each result is the same
 as running that request alone.
 
+Admitted requests are built on a worker thread (the scene's particles in Python,
+then mph_scene_create: upload, first cell sort, neighbour search) while the batch
+keeps stepping; a scene joins the batch at the first step after it is ready.  A
+request whose build fails, or whose solve fails on its own (a non-finite residual:
+the batch reports it per member), is retired as failed with its error and does not
+disturb the others.
+
 The reference has no serving layer (one scene per process, source/main.cpp:313-521);
 this module is the host-side scheduler around the C ABI.
 """
 from __future__ import annotations
 
+import queue
+import threading
 import time
 from dataclasses import dataclass, field
 
 import numpy as np
 
 from .abi import OPERATOR_ASSEMBLED, PRECISION_MIXED, Params
-from .scene import Batch, Scene
+from .scene import Batch, MphError, Scene
 
 
 @dataclass
@@ -42,6 +51,8 @@ class Request:
     scene: Scene | None = None
     est_bytes: int = 0
     result: dict = field(default_factory=dict)
+    error: str = ""         # why the request failed ("" if it did not)
+    built: float = -1.0     # when its scene was ready to join the batch
 
     @property
     def latency(self) -> float:
@@ -72,42 +83,97 @@ class Server:
     """Continuous-batching scheduler of scene-step requests on one device."""
 
     def __init__(self, device: int = 0, budget_bytes: int = 200 << 30, max_active: int = 256,
-                 keep_fields: tuple = (), lazy_mixed: bool = True):
+                 keep_fields: tuple = (), lazy_mixed: bool = True, build_ahead: bool = True):
         self.device = device
         self.budget = int(budget_bytes)
         self.max_active = max_active
         self.keep_fields = keep_fields
         self.lazy_mixed = lazy_mixed  # precision assumed for requests not yet built
+        self.build_ahead = build_ahead  # build admitted scenes on the worker thread
         self.batch = Batch(device)
         self.model = FootprintModel()
         self.pending: list[Request] = []
         self.active: list[Request] = []
         self.finished: list[Request] = []
+        self.building: list[Request] = []  # admitted, scene being built on the worker
+        self.failed: list[Request] = []
         self.particle_steps = 0
         self.fluid_steps = 0
         self.peak_bytes = 0
         self.batch_steps = 0
+        self._todo: queue.Queue = queue.Queue()
+        self._ready: queue.Queue = queue.Queue()
+        self._worker = threading.Thread(target=self._build_loop, name="mph-scene-builder", daemon=True)
+        self._worker.start()
 
     # -- accounting ---------------------------------------------------------
     def used_bytes(self) -> int:
-        return sum(r.est_bytes for r in self.active)
+        return sum(r.est_bytes for r in self.active) + sum(r.est_bytes for r in self.building)
+
+    def _build_loop(self) -> None:
+        """Worker thread: build admitted requests' scenes in admission order."""
+        while True:
+            r = self._todo.get()
+            if r is None:
+                return
+            self._build(r)
+            self._ready.put(r)
+
+    def _build(self, r: Request) -> None:
+        try:
+            if r.params is None:
+                r.params, r.grid = r.make()
+            r.scene = Scene(r.params, r.grid, self.device)
+        except Exception as e:  # a bad request fails alone
+            r.error = "build: %s" % e
+            r.scene = None
 
     def _admit(self, now: float) -> None:
-        while self.pending and len(self.active) < self.max_active:
+        while self.pending and len(self.active) + len(self.building) < self.max_active:
             r = self.pending[0]
             np_ = r.grid.np if r.grid is not None else r.np
             mixed = r.params.precision == PRECISION_MIXED if r.params is not None else self.lazy_mixed
             est = self.model.estimate(r.dim, r.nf, np_, mixed)
-            if self.active and self.used_bytes() + est > self.budget:
+            if (self.active or self.building) and self.used_bytes() + est > self.budget:
                 break  # wait for memory (an empty server admits anything)
             self.pending.pop(0)
-            if r.params is None:
-                r.params, r.grid = r.make()
-            r.scene = Scene(r.params, r.grid, self.device)
             r.est_bytes = est
             r.admitted = now
+            self.building.append(r)
+            if self.build_ahead:
+                self._todo.put(r)
+            else:  # build here; the scene joins before the next step
+                self._build(r)
+                self._ready.put(r)
+
+    def _join_ready(self, now: float, wait: bool) -> None:
+        """Move built scenes into the batch (waiting for one if `wait`)."""
+        while self.building:
+            try:
+                r = self._ready.get(block=wait, timeout=None)
+            except queue.Empty:
+                return
+            wait = False
+            self.building.remove(r)
+            r.built = now
+            if r.scene is None:
+                self._fail(r, now, r.error)
+                continue
             self.batch.add(r.scene)
             self.active.append(r)
+
+    def _fail(self, r: Request, now: float, why: str) -> None:
+        r.error = why or "failed"
+        if r.scene is not None:
+            if r in self.active:
+                self.batch.remove(r.scene)
+                self.active.remove(r)
+            r.scene.close()
+            r.scene = None
+        if r.make is not None:
+            r.params = r.grid = None
+        r.done = now
+        self.failed.append(r)
 
     def _retire(self, r: Request, now: float) -> None:
         self.batch.remove(r.scene)
@@ -127,6 +193,9 @@ class Server:
         self.batch_steps += 1
         now = clock()
         for r, st in zip(list(self.active), stats):
+            if st.status != 0:
+                self._fail(r, now, "solve: %s" % MphError(st.status, "non-finite CR residual"))
+                continue
             r.steps_done += st.steps_done
             self.particle_steps += st.steps_done * r.grid.np
             self.fluid_steps += st.steps_done * r.nf
@@ -153,11 +222,16 @@ class Server:
                 self.pending.append(reqs[nxt])
                 nxt += 1
             self._admit(now)
+            # join what the worker has built; with nothing to step, wait for it
+            self._join_ready(now, wait=not self.active and bool(self.building))
             if self.active:
                 self.step(clock)
                 if log and self.batch_steps % 10 == 0:
-                    log("[serve] t=%.1fs active %d pending %d done %d" %
-                        (clock(), len(self.active), len(self.pending), len(self.finished)))
+                    log("[serve] t=%.1fs active %d building %d pending %d done %d failed %d" %
+                        (clock(), len(self.active), len(self.building), len(self.pending), len(self.finished),
+                         len(self.failed)))
+            elif self.building:
+                continue
             elif nxt < len(reqs):
                 time.sleep(max(0.0, min(0.05, reqs[nxt].arrival - clock())))
             else:
@@ -166,12 +240,16 @@ class Server:
                 break
         wall = clock()
         lat = np.array([r.latency for r in self.finished]) if self.finished else np.zeros(1)
+        # offered load: the fluid-particle-steps of the requests that arrived in the run
+        offered = sum(r.nf * r.steps for r in reqs[:nxt])
         return {
             "wall_s": wall,
             "requests_arrived": nxt,
             "requests_done": len(self.finished),
-            "requests_active_at_end": len(self.active),
+            "requests_failed": len(self.failed),
+            "requests_active_at_end": len(self.active) + len(self.building),
             "requests_pending_at_end": len(self.pending),
+            "offered_fluid_particle_steps_per_s": offered / wall if wall > 0 else 0.0,
             "fluid_particle_steps_per_s": self.fluid_steps / wall if wall > 0 else 0.0,
             "particle_steps_per_s": self.particle_steps / wall if wall > 0 else 0.0,
             "p50_latency_s": float(np.percentile(lat, 50)),
@@ -181,6 +259,13 @@ class Server:
         }
 
     def close(self) -> None:
+        self._todo.put(None)
+        self._worker.join()
+        while self.building:  # built but never joined
+            r = self._ready.get()
+            self.building.remove(r)
+            if r.scene is not None:
+                r.scene.close()
         for r in list(self.active):
             self.batch.remove(r.scene)
             r.scene.close()

@@ -108,7 +108,11 @@ def main():
                            stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
             run_case(name, os.path.join(d, "box.data"), os.path.join(d, "box.grid"), 2, steps, ref_bin, args.out)
         # seeded synthetic scenes (inputs written with %.17e, VTK/PROF files kept)
-        for name, fn, dim, steps in [("tiny2d", scenes.tiny_2d, 2, 3), ("tiny3d", scenes.tiny_3d, 3, 3)]:
+        # tiny2d_moving / tiny3d_moving: the same scenes with translating and rotating
+        # walls of both wall types (initializeWall, calculateWall)
+        for name, fn, dim, steps in [("tiny2d", scenes.tiny_2d, 2, 3), ("tiny3d", scenes.tiny_3d, 3, 3),
+                                     ("tiny2d_moving", scenes.tiny_2d_moving, 2, 3),
+                                     ("tiny3d_moving", scenes.tiny_3d_moving, 3, 3)]:
             p, g = fn(steps=steps)
             p.vtk_output_interval = p.dt  # every step is a VTK step
             p.output_interval = p.dt

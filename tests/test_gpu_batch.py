@@ -117,3 +117,31 @@ def test_member_cannot_be_destroyed():
     b.remove(s)
     s.close()
     b.close()
+
+
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64, abi.PRECISION_MIXED])
+def test_nonfinite_member_fails_alone(precision):
+    """A member whose system is not finite leaves the lockstep solve at once with
+    status MPH_E_STATE and does not advance; the other members step as alone."""
+    from test_gpu_parity import _diverging
+    names = ["tiny2d", "tiny3d", "2d_40x40"]
+    p, g = _diverging("tiny3d")
+    p.precision = precision
+    p.end_time = 1e30
+    bad = Scene(p, g)
+    members, alone_runs = [], []
+    for n in names:
+        mp, mg = make(n, precision)
+        members.append(Scene(mp, mg))
+        alone_runs.append(alone(n, precision, 2)[0])
+    with Batch() as b:
+        for s in members[:1] + [bad] + members[1:]:
+            b.add(s)
+        st = b.step()
+        assert st[1].status == abi.E_STATE and st[1].steps_done == 0
+        assert all(x.status == 0 and x.steps_done == 1 for k, x in enumerate(st) if k != 1)
+        b.remove(bad)
+        st = b.step()
+        assert all(x.status == 0 and x.steps_done == 1 for x in st)
+    for s, a in zip(members, alone_runs):
+        same(s, a)

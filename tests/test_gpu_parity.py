@@ -18,7 +18,7 @@ import pytest
 
 import pyoracle
 from mphimplicit_amd import abi, files, scenes
-from mphimplicit_amd.scene import Scene
+from mphimplicit_amd.scene import MphError, Scene
 from parity import diff, step_by_phase
 
 pytestmark = pytest.mark.gpu
@@ -66,6 +66,9 @@ SCENES = {
     # 39,330 level-0 rows: level 0 takes the one-thread-per-row multigrid sweeps, the
     # coarser levels the lane-split ones (below 32768 rows), as on the 1M scene
     "2d_160k": lambda: scenes.config_2d(nx=400, ny=400, box=(440, 420)),
+    # translating and rotating walls of both wall types (initializeWall, calculateWall)
+    "tiny2d_moving": lambda: scenes.tiny_2d_moving(steps=5),
+    "tiny3d_moving": lambda: scenes.tiny_3d_moving(steps=5),
 }
 
 
@@ -97,7 +100,8 @@ def test_phases_bitwise(name):
 
 
 @pytest.mark.parametrize("name,steps,tol", [("2d_40x40", 12, 1e-16), ("3d_4096", 8, 1e-16),
-                                            ("cube3d_sample", 3, 1e-12), ("2d_160k", 2, 1e-16)])
+                                            ("cube3d_sample", 3, 1e-12), ("2d_160k", 2, 1e-16),
+                                            ("tiny3d_moving", 5, 1e-16)])
 def test_trajectory_bitwise(name, steps, tol):
     """mph_scene_step over several steps (incl. the first step's VTK diagnostics)."""
     gpu, orc = _pair(name, tol)
@@ -195,3 +199,39 @@ def test_mixed_precision_against_fp64(name, steps):
         print("[mixed] %s %s: mixed err %.3e, fp64 err %.3e" % (name, n, em, ef), flush=True)
         assert em <= 2.0 * ef + 1e-12, (n, em, ef)
         assert em <= bound, (n, em)
+
+
+@pytest.mark.parametrize("name,lane_rows", [("3d_4096", 0), ("3d_4096", 1 << 30), ("2d_160k", 1 << 30)])
+def test_sweep_paths_bitwise(name, lane_rows):
+    """The multigrid smoothing sweeps exist as one thread per row and as rows split
+    over lanes (the default below 32768 rows).  The option LaneRowsMax moves every
+    level of a scene to one form (0: one thread per row everywhere, as level 0 of
+    the 1M 3D scene; 2^30: lanes everywhere): the step stays bit-identical to the
+    oracle either way."""
+    gpu, orc = _pair(name)
+    gpu.set_option("LaneRowsMax", lane_rows)
+    bad = step_by_phase(gpu, orc, steps=1, diagnostics=False)
+    assert not bad, "\n".join(bad[:10])
+
+
+def _diverging(name):
+    """A scene whose implicit system is not finite: an infinite shear viscosity makes
+    the harmonic-mean viscosities inf/inf = NaN, so A and b hold NaN while positions,
+    cells and lists stay finite (the viscosity enters the implicit terms only)."""
+    p, g = SCENES[name]()
+    for t in range(4):
+        p.shear_viscosity[t] = float("inf")
+    return p, g
+
+
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64, abi.PRECISION_MIXED])
+def test_nonfinite_residual_fails_fast(precision):
+    """rr/rr0 < tol is false for NaN, so the reference would iterate to its cap of
+    3*Nf; the solve reports MPH_E_STATE at the first check instead."""
+    p, g = _diverging("tiny3d")
+    p.precision = precision
+    s = Scene(p, g)
+    with pytest.raises(MphError) as e:
+        s.step(1)
+    assert e.value.code == abi.E_STATE and "non-finite" in str(e.value)
+    s.close()

@@ -19,7 +19,7 @@ import pyoracle
 from mphimplicit_amd import abi, files
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-CASES = ["tiny2d", "tiny3d", "squaredrop2d", "bridge2d", "glued2d", "cube3d"]
+CASES = ["tiny2d", "tiny3d", "squaredrop2d", "bridge2d", "glued2d", "cube3d", "tiny2d_moving", "tiny3d_moving"]
 
 # probe array -> (phase after which it is read, oracle field)
 AFTER = {

@@ -31,8 +31,9 @@ class FakeScene:
 
 
 class FakeStats:
-    def __init__(self, done):
+    def __init__(self, done, status=0):
         self.steps_done = done
+        self.status = status
 
 
 class FakeBatch:
@@ -51,7 +52,9 @@ class FakeBatch:
         out = []
         for s in self.members:
             p = s.params
-            if s.time < p.end_time + 1e-5 * p.dt:
+            if getattr(p, "diverges", False):  # its solve fails: not advanced
+                out.append(FakeStats(0, -5))
+            elif s.time < p.end_time + 1e-5 * p.dt:
                 s.time += p.dt
                 out.append(FakeStats(1))
             else:
@@ -82,7 +85,7 @@ def small_requests(n=8, arrival_gap=0.0):
 
 def test_all_requests_served_once(fake):
     reqs = small_requests()
-    srv = serve.Server(keep_fields=("Velocity",))
+    srv = serve.Server(keep_fields=("Velocity",), build_ahead=False)
     res = srv.run(reqs, duration_s=30)
     assert res["requests_done"] == len(reqs)
     assert sorted(r.rid for r in srv.finished) == list(range(len(reqs)))
@@ -100,14 +103,15 @@ def test_budget_limits_concurrency(fake):
     est = serve.FootprintModel().estimate
     one = max(est(r.dim, r.nf, r.grid.np, True) for r in reqs)
     budget = int(1.5 * one)
-    srv = serve.Server(budget_bytes=budget)
+    srv = serve.Server(budget_bytes=budget, build_ahead=False)
     seen = []
     admit = srv._admit
 
     def checked(now):
         admit(now)
-        seen.append((len(srv.active), srv.used_bytes()))
-        assert srv.used_bytes() <= budget or len(srv.active) == 1
+        n = len(srv.active) + len(srv.building)
+        seen.append((n, srv.used_bytes()))
+        assert srv.used_bytes() <= budget or n == 1
 
     srv._admit = checked
     res = srv.run(reqs, duration_s=30)
@@ -118,7 +122,7 @@ def test_budget_limits_concurrency(fake):
 
 def test_tiny_budget_serialises(fake):
     reqs = small_requests(5)
-    srv = serve.Server(budget_bytes=1)
+    srv = serve.Server(budget_bytes=1, build_ahead=False)
     res = srv.run(reqs, duration_s=30)
     assert res["requests_done"] == 5
     assert srv.batch.max_members == 1
@@ -183,3 +187,40 @@ def test_lazy_requests_built_at_admission(fake):
     res = srv.run(reqs, duration_s=30)
     assert res["requests_done"] == 6
     assert all(r.params is None for r in srv.finished)  # released again after retiring
+
+
+def test_build_ahead_serves_all(fake):
+    """Scenes built on the worker thread join the batch as they become ready:
+    every request is served exactly once and every scene is released."""
+    reqs = small_requests(12, arrival_gap=0.001)
+    srv = serve.Server(keep_fields=("Velocity",))
+    res = srv.run(reqs, duration_s=30)
+    srv.close()
+    assert res["requests_done"] == 12 and res["requests_failed"] == 0
+    assert sorted(r.rid for r in srv.finished) == list(range(12))
+    for r in srv.finished:
+        assert r.steps_done == r.steps and r.done >= r.built >= r.admitted >= 0
+    assert FakeScene.live == 0
+
+
+@pytest.mark.parametrize("ahead", [False, True])
+def test_failing_requests_fail_alone(fake, ahead):
+    """A request whose build raises and one whose solve fails (the batch reports a
+    member status) are retired as failed; the others are served as usual."""
+    reqs = small_requests(6)
+
+    def broken():
+        raise ValueError("bad scene")
+
+    reqs[2].params = reqs[2].grid = None
+    reqs[2].make = broken
+    reqs[4].params.diverges = True
+    srv = serve.Server(build_ahead=ahead)
+    res = srv.run(reqs, duration_s=30)
+    srv.close()
+    assert res["requests_failed"] == 2 and res["requests_done"] == 4
+    failed = {r.rid: r.error for r in srv.failed}
+    assert set(failed) == {2, 4}
+    assert "bad scene" in failed[2] and "non-finite" in failed[4]
+    assert sorted(r.rid for r in srv.finished) == [0, 1, 3, 5]
+    assert FakeScene.live == 0
