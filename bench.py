@@ -69,6 +69,80 @@ def roofline(kernel, ms, launches, bytes_sum, ref_sum, traffic):
     }
 
 
+def step_bytes(scene, factored, iters):
+    """Algorithmic HBM bytes of one time step, each array counted once per logical
+    pass, in the executed format and in the reference's CSR format (FP64 value, int32
+    column, int64 row pointer), for `iters` CR iterations.  Terms:
+      cells + neighbour search   36 Np (hash) + 16 Np (sort) + 104 Np (reorder)
+                                 + 48 Np + 4 (E + E_P) (lists written)
+      explicit forces            24 E + 100 Np
+      assembly                   factored: lists, alpha (8 B per fluid pair), D, b,
+                                 beta and C (8 + 24 B per P pair), C_ii, the records'
+                                 positions, P; assembled: 3x3 blocks written (76 B) and
+                                 read-modified-written by C Lambda C^T (144 B);
+                                 reference: 12 nnzA + 12 nnzC + 4 E + 16 n + 8 Np
+      multigrid build            factored: alpha and beta lists re-read, the G tables
+                                 written and read, level 0 written; assembled and
+                                 reference: A read once; coarser levels read and written
+      per CR iteration           one product (factored: FactoredBytes; assembled: blocks
+                                 + vectors; reference: 12 nnzA + 8 (n+1) + 16 n), one
+                                 preconditioner (48 n + per sweep 8 B per stencil value +
+                                 32 B per row: 4 sweeps on level 0, 6 on inner levels, 5
+                                 on the coarsest; reference: 12 B per stencil value), and
+                                 160 n of dot products and updates
+      (I + 1) products and (I + 2) preconditioner applications per step.
+    Returns (executed bytes, reference-format bytes, breakdown dict)."""
+    Np, Nf = scene.scalar("ParticleCount"), scene.scalar("FluidCount")
+    n = 3 * Nf
+    E, EP = scene.scalar("NeighborIndCount"), scene.scalar("NeighborIndCountP")
+    EA, EC = scene.scalar("ABlocks"), scene.scalar("CBlocks")
+    depth = int(scene.scalar("MultiGridDepth"))
+    levels = []
+    for l in range(depth):
+        rows = scene.field_bytes("MGInvD_L%d" % l) // 8
+        vals = scene.field_bytes("MGCofA_L%d" % l) // 8
+        levels.append((rows, vals))
+    dim = 3 if levels and levels[0][0] and levels[0][1] // levels[0][0] > 27 else 2
+    cells = 36 * Np + 16 * Np + 104 * Np + 48 * Np + 4 * (E + EP)
+    explicit = 24 * E + 100 * Np
+    c_common = 4 * EP + 24 * EP + 8 * Np + (4 + 24) * EC + 24 * Nf  # C, P; b -= C P
+    if factored:
+        assembly = (4 + 8) * EA + 72 * Nf + 24 * Nf + c_common + 8 * EP + 24 * Nf + 56 * Np
+        gslots = 27 if dim == 3 else 9
+        mg0 = 12 * EA + 12 * EP + 2 * 8 * 3 * gslots * Np + 24 * Nf
+        product = scene.scalar("FactoredBytes")
+    else:
+        assembly = (4 + 76) * EA + 24 * Nf + c_common + 144 * EA
+        mg0 = 76 * EA + 24 * Nf
+        product = (9 * 8 + 4) * EA + 4 * Nf + 24 * n
+    ref_assembly = 12 * 9 * EA + 12 * 3 * EC + 4 * E + 16 * n + 8 * Np
+    ref_mg0 = 12 * 9 * EA + 24 * n
+    mg_rest = sum(2 * 8 * vals for rows, vals in levels[1:])
+    ref_mg_rest = sum(2 * 12 * vals for rows, vals in levels[1:])
+    sweeps = [4] + [6] * max(0, depth - 2) + ([5] if depth >= 2 else [])
+    precond = 48 * n + sum(s * (8 * vals + 32 * rows) for s, (rows, vals) in zip(sweeps, levels))
+    ref_precond = 48 * n + sum(s * (12 * vals + 32 * rows) for s, (rows, vals) in zip(sweeps, levels))
+    ref_product = 12 * 9 * EA + 8 * (n + 1) + 16 * n
+    it = iters
+    parts = {
+        "cells_and_neighbours": cells,
+        "explicit": explicit,
+        "assembly": assembly,
+        "multigrid_build": mg0 + mg_rest,
+        "products": (it + 1) * product,
+        "preconditioner": (it + 2) * precond,
+        "vector_ops": it * 160 * n,
+    }
+    mine = float(sum(parts.values()))
+    ref = float(cells + explicit + ref_assembly + ref_mg0 + ref_mg_rest + (it + 1) * ref_product
+                + (it + 2) * ref_precond + it * 160 * n)
+    return mine, ref, {k: round(v / 1e9, 3) for k, v in parts.items()}
+
+
+PHASES = ["kinematics", "cells", "neighbor", "explicit", "matrix_a", "matrix_c", "multiply_c", "multigrid", "solve",
+          "diagnostics", "advance"]
+
+
 def form_for(args, precision):
     """Operator form of a leg (--operator; the same for every precision)."""
     return args.operator
@@ -146,6 +220,28 @@ def run_gpu(args, precision="fp64", form=None):
                         "13.2 GB; its limit is the cache traffic of gathering neighbours' x, position and t "
                         "(three 16-byte pairs per entry) through L2 and the Infinity Cache, not HBM")
     rf32 = roofline(k32, scene.scalar("SpmvMsF32"), l32_prev, b32, r32, None)
+    step = None
+    if precision == "fp64":
+        # the whole step against the roofline: algorithmic bytes at the mean iteration
+        # count over the mean step time, and the device time per phase of one more
+        # step (phase timing adds a synchronisation per phase: outside the timed steps)
+        sb, sb_ref, parts = step_bytes(scene, factored, float(np.mean(iters)))
+        ms = 1e3 * float(np.mean(lat))
+        scene.set_timing(1)
+        st = scene.step(1)
+        step = {
+            "bytes": round(sb),
+            "achieved": round(sb / (ms / 1e3) / 1e9, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(sb / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "ms_per_step": round(ms, 3),
+            "bytes_by_part_gb": parts,
+            "reference_csr_bytes": round(sb_ref),
+            "achieved_reference_csr_equivalent": round(sb_ref / (ms / 1e3) / 1e9, 1),
+            "phase_ms": {PHASES[k - 1]: round(st.ms_phase[k], 3) for k in range(1, 12) if st.ms_phase[k] > 0},
+            "phase_ms_step_total": round(st.ms_total, 3),
+        }
     tuned = {"spmv_fp64_variant": int(scene.scalar("TunedSpmv64")), "spmv_fp32_variant": int(scene.scalar("TunedSpmv32")),
              "cac_variant": int(scene.scalar("TunedCac")), "fuse_variant": int(scene.scalar("TunedFuse")),
              "galerkin_variant": int(scene.scalar("TunedGalerkin"))}
@@ -165,6 +261,7 @@ def run_gpu(args, precision="fp64", form=None):
         "first": first,
         "last": last,
         "tuned": tuned,
+        "step": step,
     }
 
 
@@ -179,12 +276,82 @@ def field_error(a, b):
     return out
 
 
+def cpu_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "0")) or (
+        len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
+
+
+def calibrate_port(threads, steps=3):
+    """Wall time of the port (the oracle, OpenMP) against the reference's own OpenMP
+    executable (oracle/_ref/Mph_gcc3d, compiled from the reference's sources) on the
+    same 3d_64k scene at the reference's hard-coded stop rule (rr/rr0 < 1e-12), over
+    time steps 2..steps+1 (after the first step, which carries the output-step
+    diagnostics).  The reference prints no per-step wall time, so its figure is the
+    difference of two whole runs (1 step and steps+1 steps, identical reading,
+    initialisation and file output).  None when the reference binary is absent."""
+    import subprocess
+    import tempfile
+    import shutil
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from mphimplicit_amd import files, scenes
+
+    exe = os.path.join(ROOT, "oracle", "_ref", "Mph_gcc3d")
+    if not os.path.exists(exe):
+        return None
+    params, grid = scenes.config_3d_drop()
+    params.cr_tolerance = 1.0e-12
+    work = tempfile.mkdtemp(prefix="mph_calib_")
+    try:
+        files.write_grid_file(os.path.join(work, "box.grid"), grid)
+
+        def ref_run(n):
+            q = params.copy()
+            q.end_time = (n - 1) * q.dt
+            files.write_data_file(os.path.join(work, "box.data"), q)
+            t = time.perf_counter()
+            subprocess.run([exe, "box.data", "box.grid", "box%03d.prof", "box%03d.vtk", "box.log", str(threads)],
+                           cwd=work, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            dt = time.perf_counter() - t
+            its = [int(line.split("iter,=")[1].split(",")[0]) for line in open(os.path.join(work, "box.log"))
+                   if "iter,=" in line]
+            return dt, its
+
+        t1, _ = ref_run(1)
+        tn, ref_its = ref_run(steps + 1)
+        q = params.copy()
+        q.end_time = 1.0e30
+        orc = pyoracle.OracleScene(q, grid, pyoracle.DOT_OMP)
+        port_its = [orc.step(1).cr_iterations]
+        t = time.perf_counter()
+        for _ in range(steps):
+            port_its.append(orc.step(1).cr_iterations)
+        tp = time.perf_counter() - t
+        orc.close()
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    tr = tn - t1
+    print("[bench] cpu calibration on 3d_64k: reference %.2f s, port %.2f s for %d steps" % (tr, tp, steps),
+          file=sys.stderr, flush=True)
+    return {
+        "port_vs_reference": round(tp / tr, 4),
+        "reference_s": round(tr, 3),
+        "port_s": round(tp, 3),
+        "cr_iterations_reference": ref_its,
+        "cr_iterations_port": port_its,
+        "sample": "3d_64k (64,000 fluid + 118,016 wall particles), time steps 2-%d at rr/rr0 < 1e-12, %d threads; "
+                  "reference = oracle/_ref/Mph_gcc3d (source/main.cpp built -O3 -fopenmp), its time the difference "
+                  "of a %d-step and a 1-step run" % (steps + 1, threads, steps + 1),
+    }
+
+
 def run_cpu_baseline(args):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from mphimplicit_amd import scenes
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
+    threads = cpu_threads()
     pyoracle.set_threads(threads)
     params, grid = scenes.CONFIGS[args.config]()
     params.cr_tolerance = args.rel_residual ** 2
@@ -194,21 +361,42 @@ def run_cpu_baseline(args):
     orc = pyoracle.OracleScene(params, grid, pyoracle.DOT_OMP)
     init_s = time.perf_counter() - t
     t = time.perf_counter()
-    st = orc.step(args.cpu_steps)
-    dt = time.perf_counter() - t
+    st = orc.step(1)  # warm-up: the first step, with the output-step diagnostics
+    warm_s = time.perf_counter() - t
+    warm_its = st.cr_iterations
     fields = {"Velocity": orc.read("Velocity"), "PressureP": orc.read("PressureP")}
+    lat, its = [], []
+    for _ in range(args.cpu_steps):
+        t = time.perf_counter()
+        st = orc.step(1)
+        lat.append(time.perf_counter() - t)
+        its.append(st.cr_iterations)
     orc.close()
-    print("[bench] cpu baseline: %d step(s) in %.1f s (%d CR iterations, %d threads, init %.1f s)"
-          % (args.cpu_steps, dt, st.cr_iterations, threads, init_s), file=sys.stderr, flush=True)
-    return {
+    dt = float(np.sum(lat))
+    print("[bench] cpu baseline: warm-up %.1f s, %d step(s) in %.1f s (CR iterations %s, %d threads, init %.1f s)"
+          % (warm_s, args.cpu_steps, dt, its, threads, init_s), file=sys.stderr, flush=True)
+    out = {
         "value": round(nf * args.cpu_steps / dt, 1),
         "unit": "fluid-particle-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": "first %d time step(s) of the same %s scene after its initial sort and neighbour search, "
-                  "oracle/mph_oracle.cpp built -O3 -fopenmp (no -march), OpenMP dot products, %d threads; "
-                  "%d CR iterations, %.1f s wall" % (args.cpu_steps, args.config, threads, st.cr_iterations, dt),
-    }, fields
+        "sample": "time steps 2-%d of the same %s scene (step 1, with the output-step diagnostics, is the warm-up), "
+                  "oracle/mph_oracle.cpp built -O3 -fopenmp (no -march), OpenMP dot products, %d threads, "
+                  "stop rule rr/rr0 < %g; %.1f s wall"
+                  % (args.cpu_steps + 1, args.config, threads, args.rel_residual ** 2, dt),
+        "cr_iterations": its,
+        "cr_iterations_warmup": warm_its,
+        "p50_step_s": round(float(np.percentile(lat, 50)), 3),
+        "p99_step_s": round(float(np.percentile(lat, 99)), 3),
+    }
+    if args.cpu_calibrate == "on":
+        cal = calibrate_port(threads)
+        out["calibration"] = cal
+        if cal:
+            out["port_vs_reference"] = cal["port_vs_reference"]
+            # what the reference's own executable would take on the same sample
+            out["reference_equivalent_value"] = round(out["value"] * cal["port_vs_reference"], 1)
+    return out, fields
 
 
 def run_batch(args):
@@ -285,26 +473,30 @@ def run_batch(args):
     return out
 
 
-def run_serve(args, rate=None):
+def run_serve(args, rate=None, seconds=None):
     """Serving stream: Poisson arrivals of scene-step requests against the wall
-    clock for a bounded time, continuous batching over one resident batch."""
+    clock for a bounded time (or until serve_requests requests), continuous batching
+    over one resident batch.  Reports the offered load (fluid-particle-steps/s of the
+    requests that arrived) beside the served throughput and latencies."""
     from mphimplicit_amd import abi, scenes
     from mphimplicit_amd.serve import Server, make_requests
 
     rate = args.serve_rate if rate is None else rate
-    count = min(args.serve_requests, int(rate * args.serve_seconds * 1.5) + 10)  # enough to outlast the run
+    seconds = args.serve_seconds if seconds is None else seconds
+    count = min(args.serve_requests, int(rate * seconds * 1.5) + 10)  # enough to outlast the run
     stream = scenes.serve_stream(count, rate=rate)
     form = abi.OPERATOR_FACTORED if form_for(args, "mixed") == "factored" else abi.OPERATOR_ASSEMBLED
     reqs = make_requests(stream, tol=args.rel_residual ** 2, operator_form=form)
     srv = Server(lazy_mixed=True)
     try:
-        r = srv.run(reqs, duration_s=args.serve_seconds,
-                    log=lambda m: print(m, file=sys.stderr, flush=True))
+        r = srv.run(reqs, duration_s=seconds, log=lambda m: print(m, file=sys.stderr, flush=True))
     finally:
         srv.close()
     r["workload"] = ("Poisson arrivals at %g req/s, 1-20 steps, lognormal fluid counts (median 50k, sigma 1, "
-                     "clipped 5k-500k), 2D/3D, mixed precision, %s particle matrix, %g s run"
-                     % (rate, form_for(args, "mixed"), args.serve_seconds))
+                     "clipped 5k-500k), 2D/3D, mixed precision, %s particle matrix, %g s run or %d requests"
+                     % (rate, form_for(args, "mixed"), seconds, count))
+    if r["fluid_particle_steps_per_s"] > 0:
+        r["offered_over_served"] = r["offered_fluid_particle_steps_per_s"] / r["fluid_particle_steps_per_s"]
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
 
 
@@ -334,18 +526,21 @@ def main():
                     help="when the FP64 leg is factored: also run plain FP64 with the assembled matrix (the "
                          "bit-exact form)")
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="timed CPU steps after one warm-up step")
+    ap.add_argument("--cpu-calibrate", choices=["on", "off"], default="on",
+                    help="time the CPU port against the reference's own OpenMP executable on 3d_64k")
     ap.add_argument("--mixed", choices=["on", "off"], default="on",
                     help="also run the mixed-precision form (FP32 matrices inside FP64 refinement)")
     ap.add_argument("--batch", choices=["on", "off"], default="on", help="batched-sweep leg")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--serve", choices=["on", "off"], default="on", help="serving leg")
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-light-rate", type=float, default=5.0,
                     help="a second serving run at this arrival rate, below the GPU's capacity (0: skip)")
+    ap.add_argument("--serve-light-seconds", type=float, default=20.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the known-size kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -388,7 +583,7 @@ def main():
         "p99_step_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "cr_iterations": g["iters"],
         "autotuned": g["tuned"],
-        "roofline": g["roofline"],
+        "roofline": dict(g["roofline"], step=g["step"]),
         "hbm_ceiling": {"read_gbs": round(read_gbs, 1), "copy_gbs": round(copy_gbs, 1),
                         "spmv_frac_of_read": round(g["roofline"]["achieved"] / read_gbs, 4)
                         if g["roofline"]["achieved"] else None,
@@ -441,13 +636,12 @@ def main():
     if args.serve == "on":
         out["serving"] = run_serve(args)
         if args.serve_light_rate > 0:
-            out["serving_below_capacity"] = run_serve(args, args.serve_light_rate)
+            out["serving_below_capacity"] = run_serve(args, args.serve_light_rate, args.serve_light_seconds)
     if args.cpu_baseline == "on":
         cb, cfields = run_cpu_baseline(args)
-        if args.cpu_steps == 1:
-            cb["rel_error_fp64_gpu_vs_cpu_after_first_step"] = field_error(g["first"], cfields)
-            if m is not None:
-                cb["rel_error_mixed_gpu_vs_cpu_after_first_step"] = field_error(m["first"], cfields)
+        cb["rel_error_fp64_gpu_vs_cpu_after_first_step"] = field_error(g["first"], cfields)
+        if m is not None:
+            cb["rel_error_mixed_gpu_vs_cpu_after_first_step"] = field_error(m["first"], cfields)
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 

@@ -162,6 +162,13 @@ class Scene:
         """mph_scene_set_option: a schedule choice that never changes a result."""
         _check(lib().mph_scene_set_option(self.h, name.encode(), float(value)))
 
+    def field_bytes(self, name: str) -> int:
+        """Current size of a named field in bytes (mph_scene_field_size)."""
+        nb = C.c_size_t()
+        es = C.c_int()
+        _check(lib().mph_scene_field_size(self.h, name.encode(), C.byref(nb), C.byref(es)))
+        return int(nb.value)
+
     def read(self, name: str) -> np.ndarray:
         nb = C.c_size_t()
         es = C.c_int()

@@ -30,6 +30,44 @@ namespace {
 
 thread_local std::string g_err;
 
+// Large arrays (neighbour lists, the CSR matrices) are allocated without the serial
+// value-initialisation of std::vector and then filled by all threads, as the
+// reference's malloc + parallel first writes do: the pages land on the threads'
+// memory and no single thread touches gigabytes.  Values are the same as assign().
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U>&) {}
+    template <class U>
+    void construct(U* p) noexcept {
+        ::new ((void*)p) U;
+    }
+    template <class U, class... A>
+    void construct(U* p, A&&... a) {
+        ::new ((void*)p) U(std::forward<A>(a)...);
+    }
+};
+template <class T>
+using bvec = std::vector<T, NoInitAlloc<T>>;
+
+template <class V, class T>
+void par_assign(V& v, size_t n, T val) {
+    v.resize(n);
+    T* d = v.data();
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < (long)n; ++k) d[k] = val;
+}
+
+inline void par_copy(const double* from, long n, double* to) {
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < n; ++k) to[k] = from[k];
+}
+
 constexpr int kFluidBegin = 0, kFluidEnd = 2, kWallBegin = 2, kWallEnd = 4;  // main.cpp:52-56
 inline bool is_fluid(int t) { return kFluidBegin <= t && t < kFluidEnd; }
 inline bool is_wall(int t) { return kWallBegin <= t && t < kWallEnd; }
@@ -77,11 +115,12 @@ struct Scene {
     // neighbours
     std::vector<int> nCount, nFluid, nCountP;
     std::vector<long> nPtr, nPtrP;
-    std::vector<int> nInd, nIndP;
+    bvec<int> nInd, nIndP;
     // implicit system
-    std::vector<long> aPtr, cPtr;
-    std::vector<int> aInd, cInd;
-    std::vector<double> aVal, b, cVal, vecP;
+    bvec<long> aPtr, cPtr;
+    bvec<int> aInd, cInd;
+    bvec<double> aVal, cVal;
+    std::vector<double> b, vecP;
     // multigrid
     int depth = 0;
     std::vector<Level> lv;
@@ -404,8 +443,8 @@ int phase_neighbor(Scene& s) {
         s.nPtr[i + 1] = s.nPtr[i] + s.nCount[i];
         s.nPtrP[i + 1] = s.nPtrP[i] + s.nCountP[i];
     }
-    s.nInd.assign(s.nPtr[s.np], 0);
-    s.nIndP.assign(s.nPtrP[s.np], 0);
+    par_assign(s.nInd, s.nPtr[s.np], 0);
+    par_assign(s.nIndP, s.nPtrP[s.np], 0);
 #pragma omp parallel for
     for (int i = 0; i < s.np; ++i) {
         long k = s.nPtr[i], kp = s.nPtrP[i];
@@ -536,13 +575,13 @@ void phase_explicit(Scene& s) {
 void phase_matrix_a(Scene& s) {
     const int nf = s.fe - s.fb;
     const long n = 3L * nf;
-    s.aPtr.assign(n + 1, 0);
+    par_assign(s.aPtr, n + 1, 0L);
     for (int i = 0; i < nf; ++i)
         for (int r = 0; r < 3; ++r) s.aPtr[3 * i + r + 1] = 3L * s.nFluid[i];
     for (long k = 0; k < n; ++k) s.aPtr[k + 1] += s.aPtr[k];
     const long nnz = s.aPtr[n];
-    s.aInd.assign(nnz, 0);
-    s.aVal.assign(nnz, 0.0);
+    par_assign(s.aInd, nnz, 0);
+    par_assign(s.aVal, nnz, 0.0);
     s.b.assign(n, 0.0);
 #pragma omp parallel for
     for (int i = 0; i < nf; ++i)
@@ -594,13 +633,13 @@ void phase_matrix_a(Scene& s) {
 void phase_matrix_c(Scene& s) {
     const int nf = s.fe - s.fb;
     const long n = 3L * nf;
-    s.cPtr.assign(n + 1, 0);
+    par_assign(s.cPtr, n + 1, 0L);
     for (int i = 0; i < nf; ++i)
         for (int r = 0; r < 3; ++r) s.cPtr[3 * i + r + 1] = s.nCountP[i];
     for (long k = 0; k < n; ++k) s.cPtr[k + 1] += s.cPtr[k];
     const long nnz = s.cPtr[n];
-    s.cInd.assign(nnz, 0);
-    s.cVal.assign(nnz, 0.0);
+    par_assign(s.cInd, nnz, 0);
+    par_assign(s.cVal, nnz, 0.0);
     s.vecP.assign(s.np, 0.0);
 #pragma omp parallel for
     for (int i = 0; i < nf; ++i)
@@ -919,9 +958,9 @@ inline int grid_of_particle(const Scene& s, const Level& L0, int ip) {
 void precondition(Scene& s, const double* q, double* out, double* buf) {
     const int nf = s.fe - s.fb;
     const long n = 3L * nf;
-    std::copy(q, q + n, buf);
+    par_copy(q, n, buf);
     ddmv(n, 1.0, s.invDiagA.data(), buf, 0.0, out);
-    std::copy(q, q + n, buf);
+    par_copy(q, n, buf);
     const int D = s.depth;
     Level& L0 = s.lv[0];
     const int* C = s.cellCount;
@@ -1057,7 +1096,9 @@ void spmv_a(const Scene& s, double alpha, const double* x, double beta, double* 
 void phase_solve(Scene& s) {
     const int nf = s.fe - s.fb;
     const long n = 3L * nf;
-    std::vector<double> x(n), r(n, -1.0), sv(n, -1.0), y(n, -1.0), p(n, -1.0), q(n, -1.0), u(n, -1.0), buf(n, -1.0);
+    bvec<double> x, r, sv, y, p, q, u, buf;
+    par_assign(x, n, 0.0);
+    for (bvec<double>* v : {&r, &sv, &y, &p, &q, &u, &buf}) par_assign(*v, n, -1.0);
     const double dt = s.p.dt;
     for (int i = 0; i < nf; ++i)
         for (int d = 0; d < 3; ++d) x[3 * i + d] = s.vel[3 * i + d] - s.force[3 * i + d] / s.mass[i] * dt;
@@ -1069,7 +1110,7 @@ void phase_solve(Scene& s) {
         }
     auto M = [&](const double* in, double* out) {
         if (s.p.multigrid) precondition(s, in, out, buf.data());
-        else std::copy(in, in + n, out);
+        else par_copy(in, n, out);
     };
     auto dot = [&](const double* a, const double* b) { return mpho_dot(s.dot_mode, n, a, b); };
     const double* b = s.b.data();
@@ -1077,7 +1118,7 @@ void phase_solve(Scene& s) {
     M(b, sv.data());
     s.rs0 = dot(b, sv.data());
     s.rr0 = dot(b, b);
-    std::copy(b, b + n, r.begin());
+    par_copy(b, n, r.data());
     spmv_a(s, -1.0, x.data(), 1.0, r.data());
     M(r.data(), sv.data());
     const long maxit = s.p.max_iterations > 0 ? s.p.max_iterations : n;
@@ -1088,21 +1129,27 @@ void phase_solve(Scene& s) {
         spmv_a(s, 1.0, sv.data(), 0.0, y.data());
         rho = dot(sv.data(), y.data());
         if (iter == 0) {
-            p = sv;
+            par_copy(sv.data(), n, p.data());
             spmv_a(s, 1.0, p.data(), 0.0, q.data());
-        } else {
+        } else {  // myDscal + myDaxpy of p and q, element by element
             beta = rho / rhop;
-            for (long k = 0; k < n; ++k) p[k] *= beta;
-            for (long k = 0; k < n; ++k) p[k] += 1.0 * sv[k];
-            for (long k = 0; k < n; ++k) q[k] *= beta;
-            for (long k = 0; k < n; ++k) q[k] += 1.0 * y[k];
+#pragma omp parallel for schedule(static)
+            for (long k = 0; k < n; ++k) {
+                p[k] *= beta;
+                p[k] += 1.0 * sv[k];
+                q[k] *= beta;
+                q[k] += 1.0 * y[k];
+            }
         }
         M(q.data(), u.data());
         tmp = dot(q.data(), u.data());
         alpha = rho / tmp;
-        for (long k = 0; k < n; ++k) x[k] += alpha * p[k];
-        for (long k = 0; k < n; ++k) r[k] += -alpha * q[k];
-        for (long k = 0; k < n; ++k) sv[k] += -alpha * u[k];
+#pragma omp parallel for schedule(static)
+        for (long k = 0; k < n; ++k) {  // the three myDaxpy of x, r, s
+            x[k] += alpha * p[k];
+            r[k] += -alpha * q[k];
+            sv[k] += -alpha * u[k];
+        }
         rhop = rho;
     }
     s.rr = dot(r.data(), r.data());

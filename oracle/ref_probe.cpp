@@ -24,6 +24,7 @@
 #include <cstdarg>
 #include <string>
 #include <sys/stat.h>
+#include <time.h>
 
 static int    probe_cr_iter = -1;
 static double probe_cr_rr0 = 0, probe_cr_rr = 0, probe_cr_rs0 = 0, probe_cr_rs = 0;
@@ -61,19 +62,41 @@ int probe_log(const char* format, ...)
 }
 
 static std::string g_out;
+static bool g_time = false;  // REF_PROBE_TIME: phase timing, no dumps
 static std::string g_dir;
 static FILE* g_manifest = nullptr;
 
 static void open_step(const char* tag)
 {
+    if (g_time) return;
     if (g_manifest) fclose(g_manifest);
     g_dir = g_out + "/" + tag;
     mkdir(g_dir.c_str(), 0755);
     g_manifest = ::fopen((g_dir + "/manifest.txt").c_str(), "w");
 }
 
+// REF_PROBE_TIME=1: no dumps; the wall time of every phase of every step is printed
+// to stdout instead ("phase <step> <name> <seconds>"), for timing the reference
+// phase by phase (built with -fopenmp as ref_probe_3d_omp)
+static double probe_now()
+{
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+static double g_t0 = 0;
+static void phase_done(int step, const char* name)
+{
+    if (!g_time) return;
+    const double t = probe_now();
+    printf("phase %d %s %.6f\n", step, name, t - g_t0);
+    fflush(stdout);
+    g_t0 = probe_now();
+}
+
 static void dump(const char* name, const void* p, const char* dtype, long count, size_t esize)
 {
+    if (g_time) return;
     std::string path = g_dir + "/" + name + ".bin";
     FILE* f = ::fopen(path.c_str(), "wb");
     if (!f) { perror(path.c_str()); exit(2); }
@@ -136,6 +159,7 @@ int main(int argc, char** argv)
     mkdir(g_out.c_str(), 0755);
     const int nsteps = atoi(argv[4]);
     const int mask = argc > 5 ? atoi(argv[5]) : 0xffff;
+    g_time = getenv("REF_PROBE_TIME") != nullptr;
 
     log_open((g_out + "/probe.log").c_str());
     readDataFile(argv[1]);
@@ -198,13 +222,16 @@ int main(int argc, char** argv)
             OutputNext += OutputInterval;
         }
 
+        g_t0 = probe_now();
         calculateConvection();
         calculateWall();
         calculatePeriodicBoundary();
         resetForce();
+        phase_done(done, "kinematics");
         dump_particles("_kin");
 
         calculateCellParticle();
+        phase_done(done, "cells");
         if (mask & D_CELLS) {
             DUMP_I32("ParticleIndex", ParticleIndex, ParticleCount);
             DUMP_I32("CellIndex", CellIndex, PowerParticleCount);
@@ -218,6 +245,7 @@ int main(int argc, char** argv)
             dump_particles("_sort");
         }
         calculateNeighbor();
+        phase_done(done, "neighbor");
         if (mask & D_NEIGH) {
             DUMP_I32("NeighborCount", NeighborCount, ParticleCount);
             DUMP_I32("NeighborFluidCount", NeighborFluidCount, ParticleCount);
@@ -236,6 +264,7 @@ int main(int argc, char** argv)
         calculateDiffuseInterface();
         calculateGravity();
         calculateAcceleration();
+        phase_done(done, "explicit");
         if (mask & D_EXPL) {
             DUMP_F64("DensityA", DensityA, ParticleCount);
             DUMP_F64("GravityCenter", GravityCenter, 3L * ParticleCount);
@@ -250,6 +279,7 @@ int main(int argc, char** argv)
         }
 
         calculateMatrixA();
+        phase_done(done, "matrix_a");
         const int fluid = FluidParticleEnd - FluidParticleBegin;
         const long n = 3L * fluid;
         if (mask & D_MATA) {
@@ -259,6 +289,7 @@ int main(int argc, char** argv)
             DUMP_F64("VectorB_visc", VectorB, n);
         }
         calculateMatrixC();
+        phase_done(done, "matrix_c");
         if (mask & D_MATC) {
             DUMP_I64("CsrPtrC", CsrPtrC, n + 1);
             DUMP_I32("CsrIndC", CsrIndC, NonzeroCountC);
@@ -267,11 +298,13 @@ int main(int argc, char** argv)
         }
         multiplyMatrixC();
         freeMatrixC();
+        phase_done(done, "multiply_c");
         if (mask & D_MULT) {
             DUMP_F64("CsrCofA", CsrCofA, NonzeroCountA);
             DUMP_F64("VectorB", VectorB, n);
         }
         calculateMultiGridMatrix();
+        phase_done(done, "multigrid");
         if (mask & D_MG) dump_mg();
         if (mask & D_PRECOND) {
             // M^-1 applied to a fixed vector e_k = 1 + ((k*7919) % 13) / 8
@@ -292,9 +325,11 @@ int main(int argc, char** argv)
             std::free(q); std::free(s); std::free(buf);
         }
         probe_cr_iter = -1;
+        g_t0 = probe_now();
         solveWithConjugatedGradient();
         freeMultiGridMatrix();
         freeMatrixA();
+        phase_done(done, "solve");
         if (mask & D_SOLVE) {
             DUMP_F64("VelocitySolved", Velocity, 3L * ParticleCount);
             double cr[5] = {(double)probe_cr_iter, probe_cr_rr0, probe_cr_rr, probe_cr_rs0, probe_cr_rs};
