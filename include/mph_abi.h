@@ -231,6 +231,19 @@ int mph_scene_set_option(mph_scene* s, const char* name, double value);
  * particles last moved, or FP32 values asked of an FP64 scene. */
 int mph_scene_apply_matrix(mph_scene* s, int precision, const double* x, double* y);
 
+/* mph_scene_apply_matrix for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc); it has
+ * no counterpart in the reference.  x and y are DEVICE arrays of 3*FluidCount
+ * doubles on the scene's device, owned by the caller; the result is bit-identical
+ * to mph_scene_apply_matrix.  The call allocates nothing: the factored product's
+ * records (x, t and the positions) go to a fixed 96 MiB scratch array that the
+ * library holds on every device it runs on, so a scene of the factored form may
+ * have at most 1,797,558 particles (FP64) or 6,291,456 (FP32 values); larger ones
+ * are rejected with MPH_E_ARG.  The scratch is shared by every call on a device:
+ * calls that overlap on one device race.  Runs on the scene's stream and returns
+ * after it. */
+int mph_scene_apply_matrix_noalloc(mph_scene* s, int precision, const double* x, double* y);
+
 int mph_scene_destroy(mph_scene* s);
 
 /* ---- Batched executor -------------------------------------------------------

@@ -529,6 +529,21 @@ void launch_fac_u(hipStream_t st, const DevParams& P, const int* nFluid, const l
                        u32);
 }
 
+__global__ void __launch_bounds__(256) k_fac_rec_positions(DevParams P, const double* __restrict__ from,
+                                                           double* __restrict__ to) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.np) return;
+    const body::FacPlanes F = body::fac_planes(P, const_cast<double*>(from)), T = body::fac_planes(P, to);
+    T.b[2 * k + 1] = F.b[2 * k + 1];  // p0
+    T.c[2 * k] = F.c[2 * k];          // p1
+    T.c[2 * k + 1] = F.c[2 * k + 1];  // p2
+}
+
+void launch_fac_rec_positions(hipStream_t st, const DevParams& P, const double* from, double* to) {
+    if (P.np == 0) return;
+    hipLaunchKernelGGL(k_fac_rec_positions, dim3(nblk(P.np, 256)), dim3(256), 0, st, P, from, to);
+}
+
 void launch_fac_spmv(hipStream_t st, const DevParams& P, const FacOp& F, double alpha, const double* x, double beta,
                      const double* yin, double* y, double* y2, double* p_copy, const double* dotx, double* part,
                      bool f32, int passes) {

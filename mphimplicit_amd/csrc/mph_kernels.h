@@ -234,6 +234,9 @@ void launch_fac_u(hipStream_t st, const DevParams& P, const int* nFluid, const l
 void launch_fac_spmv(hipStream_t st, const DevParams& P, const FacOp& F, double alpha, const double* x, double beta,
                      const double* yin, double* y, double* y2, double* p_copy, const double* dotx, double* part,
                      bool f32, int passes = 3);
+// the positions of the FP64 records (planes b odd and c of FacOp::rec, 7*np doubles)
+// copied from one record buffer into another; x and t are written by each product
+void launch_fac_rec_positions(hipStream_t st, const DevParams& P, const double* from, double* to);
 
 void launch_convert_f32(hipStream_t st, const double* in, float* out, long long n);
 void launch_axpy(hipStream_t st, long long n, const double* x, double* y);  // y += 1.0*x

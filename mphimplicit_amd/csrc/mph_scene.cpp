@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1134,6 +1135,45 @@ int host_field(mph_scene& S, const std::string& n, Blob& out, bool& found) {
 // lets the host-only I/O translation unit report its failures through mph_last_error
 void mph_internal_set_error(const char* msg) { g_err = msg ? msg : ""; }
 
+// ---------------------------------------------------------------------------
+// scratch of the allocation-free entry point (mph_scene_apply_matrix_noalloc): the
+// records of the factored product (x and t, written by the product, and the
+// positions, copied in from the scene) live in this file-scope device array
+// instead of the scene's record buffer.  Each device holds its own copy from the
+// library's load until the process ends.  In host code `scratch` only names the
+// symbol: its device address comes from hipGetSymbolAddress, once per device with
+// that device current, and only that address is passed to kernels.  Every call on
+// one device shares the array: calls that overlap on a device race.
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(96) << 20;  // 56 B per particle in FP64: up to 1,797,558
+constexpr int kNoAllocMaxDevices = 64;
+__device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+std::mutex g_scratchMu;
+double* g_scratchPtr[kNoAllocMaxDevices] = {};
+
+// the device address of `scratch` on `device` (current); looked up on first use
+int noalloc_scratch(int device, double** out) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return fail(MPH_E_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (!g_scratchPtr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MPH_E_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        g_scratchPtr[device] = (double*)p;
+    }
+    *out = g_scratchPtr[device];
+    return MPH_OK;
+}
+
+// bytes of scratch one product of the scene needs: the FP64 records (7 doubles per
+// particle) or the FP32 ones (4 floats); the assembled product needs none
+size_t noalloc_scratch_bytes(const mph_scene& S, bool f32) {
+    if (!factored(S)) return 0;
+    return (size_t)S.P.np * (f32 ? 4 * sizeof(float) : 7 * sizeof(double));
+}
+}  // namespace
+
 extern "C" {
 
 const char* mph_last_error(void) { return g_err.c_str(); }
@@ -1457,6 +1497,47 @@ int mph_scene_apply_matrix(mph_scene* S, int precision, const double* x, double*
     HIPCHK(hipStreamSynchronize(S->st));
     if (S->spmvTiming) collect_spmv_timing(*S);
     collect_tuning(*S);
+    return MPH_OK;
+}
+
+int mph_scene_apply_matrix_noalloc(mph_scene* S, int precision, const double* x, double* y) {
+    if (!S || !x || !y) return fail(MPH_E_ARG, "null argument");
+    if (precision != MPH_PRECISION_FP64 && precision != MPH_PRECISION_MIXED)
+        return fail(MPH_E_ARG, "unknown precision mode");
+    const bool f32 = precision == MPH_PRECISION_MIXED;
+    if (f32 && !mixed(*S)) return fail(MPH_E_STATE, "FP32 values exist in mixed-precision scenes only");
+    const size_t need = noalloc_scratch_bytes(*S, f32);
+    if (need > kNoAllocScratchBytes)
+        return fail(MPH_E_ARG, "the product's records need " + std::to_string(need) + " bytes, more than the " +
+                                   std::to_string(kNoAllocScratchBytes) + "-byte scratch of the allocation-free entry "
+                                   "point (at most " + std::to_string(kNoAllocScratchBytes / (f32 ? 16 : 56)) +
+                                   " particles)");
+    if (!S->assembled) return fail(MPH_E_STATE, "no particle matrix for the current particles");
+    HIPCHK(hipSetDevice(S->device));
+    double* scr = nullptr;
+    int rc;
+    if ((rc = noalloc_scratch(S->device, &scr))) return rc;
+    S->facPref = nullptr;  // a first pass enqueued ahead by the solve is not this product's
+    S->facPrefEv = -1;
+    const DevParams& P = S->P;
+    launch_fill(S->st, y, 3LL * P.nf, 0.0);  // y = 0*y + A x reads y
+    if (factored(*S)) {
+        FacOp F = fac_op(*S, f32);
+        if (f32) {
+            F.rec32 = (float*)scr;
+        } else {
+            F.rec = scr;
+            launch_fac_rec_positions(S->st, P, S->frec.p, scr);
+        }
+        launch_fac_spmv(S->st, P, F, 1.0, x, 0.0, y, y, nullptr, nullptr, nullptr, nullptr, f32, 3);
+    } else {
+        const Autotune& T = f32 ? S->tuneSpmv32 : S->tuneSpmv64;
+        launch_spmv(S->st, P, aslices(*S), S->nFluid.p, 1.0, x, 0.0, y, y, nullptr, nullptr, nullptr, nullptr, f32,
+                    T.best >= 0 ? T.best : 0);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(MPH_E_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+    HIPCHK(hipStreamSynchronize(S->st));
     return MPH_OK;
 }
 

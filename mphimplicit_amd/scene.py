@@ -57,6 +57,7 @@ def lib():
         L.mph_scene_destroy.argtypes = [vp]
         L.mph_scene_device_bytes.argtypes = [vp, C.POINTER(C.c_ulonglong)]
         L.mph_scene_apply_matrix.argtypes = [vp, i, C.POINTER(d), C.POINTER(d)]
+        L.mph_scene_apply_matrix_noalloc.argtypes = [vp, i, vp, vp]  # device pointers
         L.mph_batch_create.argtypes = [i, C.POINTER(vp)]
         L.mph_batch_add.argtypes = [vp, vp]
         L.mph_batch_remove.argtypes = [vp, vp]
