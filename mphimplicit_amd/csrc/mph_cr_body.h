@@ -675,11 +675,16 @@ __device__ inline void fac_t_chunk(const DevParams& P, const FacOp& F, const dou
                 double bb[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
+                    // entries past the end load the run's last entry (run >= 1), so the
+                    // list loads need no branches and issue together
                     const int jn = jb + g + u * G;
-                    const int j = jn < run ? F.nIndP[c0 + jn] : k;
+                    const long long e = c0 + (jn < run ? jn : run - 1);
+                    const int je = F.nIndP[e];
+                    const double be = F.beta[e];
+                    const int j = jn < run ? je : k;
                     const bool use = jn < run && j < P.nf && j != k;
                     jx[u] = use ? j : 0;  // unused entries read particle 0 with weight 0
-                    bb[u] = use ? F.beta[c0 + jn] : 0.0;
+                    bb[u] = use ? be : 0.0;
                 }
                 double v[U][6];
 #pragma unroll

@@ -493,13 +493,329 @@ __global__ void __launch_bounds__(256) k_fac_y(DevParams P, FacOp F, const doubl
     else body::fac_y_chunk<kNeg, VT, false>(P, F, x, beta, yin, y, y2, p_copy, dotx, part, blockIdx.x, sh, rows);
 }
 
+// ---------------------------------------------------------------------------
+// The FP64 second pass with the viscous gathers from LDS.
+//
+// Every row of one cell visits the same 27 (9 in 2D) cells, so one workgroup per row
+// cell stages the records (x and position) of the fluid particles of those cells in
+// LDS once, and its rows read their fluid neighbours' records from there instead of
+// gathering them through L1 and L2 per entry (each gather instruction of the chunk
+// kernel touched about 21 cache lines).  The P part (about a tenth of the entries,
+// walls included) keeps the global gathers.  A row's entries, lanes and order of
+// summation are those of body::fac_y_chunk, so the results are the chunk kernel's.
+// The rows go straight to y; the dot partials of the fused epilogue come from a
+// dot-partial launch afterwards (the chunks of 256 rows do not follow cells).
+constexpr int kCellThreads = 512;  // 64 rows of kFacLanes lanes per pass
+constexpr int kCellSlots = 1536;   // staged records of 48 B: 72 KiB, two workgroups per CU
+constexpr int kCellTable = 32;     // visited cells (27 in 3D) padded for the binary search
+
+// y (and y2, p) of row i from its sum, as spmv_store (no contraction: the separate
+// roundings of y = beta*yin + sum)
+__device__ __noinline__ void fac_cell_store(int i, double s0, double s1, double s2, const double* __restrict__ x,
+                                            double beta, const double* yin, double* y, double* y2, double* p_copy) {
+    const double sm[3] = {s0, s1, s2};
+    double out[3];
+    for (int r = 0; r < 3; ++r) {
+        double yy = yin[3 * i + r];
+        yy *= beta;
+        yy += sm[r];
+        out[r] = yy;
+    }
+    if (y2)
+        for (int r = 0; r < 3; ++r) {
+            double yy = y2[3 * i + r];
+            yy *= beta;
+            yy += sm[r];
+            y2[3 * i + r] = yy;
+        }
+    for (int r = 0; r < 3; ++r) y[3 * i + r] = out[r];
+    if (p_copy)
+        for (int r = 0; r < 3; ++r) p_copy[3 * i + r] = x[3 * i + r];
+}
+
+typedef double dv2c __attribute__((ext_vector_type(2)));
+
+// the visited cells of a row cell: fluid ranges in visiting order and LDS offsets
+struct FacCellTable {
+    int vlo[kCellTable], voff[kCellTable], vcnt[kCellTable];
+    int total;  // staged records of the visited cells
+};
+struct FacCellShared {  // the product's LDS: the table and the records
+    FacCellTable tab;
+    dv2c rec[3 * kCellSlots];
+};
+
+// Workgroup b's row cell (cell (x, y, z) of the fluid bounding box, z fastest): its
+// fluid rows [rs, rs + nrows) and, in T, the visited cells in the neighbour search's
+// order (ascending CellId, so their fluid ranges ascend too) with their LDS offsets,
+// filled by wavefront 0.  False (for the whole workgroup) if the cell has no fluid.
+// Ends with a workgroup barrier when it returns true.
+__device__ inline bool fac_cell_table(const DevParams& P, const FacOp& F, FacCellTable& T, int& rs, int& nrows,
+                                      int& nv) {
+    const int t = threadIdx.x;
+    const int b = blockIdx.x;
+    const int* C = P.C;
+    const int cz = F.cellLo[2] + b % F.cellCnt[2];
+    const int cy = F.cellLo[1] + (b / F.cellCnt[2]) % F.cellCnt[1];
+    const int cx = F.cellLo[0] + b / (F.cellCnt[2] * F.cellCnt[1]);
+    const int c = (cx * C[1] + cy) * C[2] + cz;
+    rs = F.cellStart[c];
+    nrows = F.cellStart[c + 1] - rs;
+    if (nrows <= 0) return false;
+    const int nx = 2 * P.range[0] + 1, ny = 2 * P.range[1] + 1, nz = 2 * P.range[2] + 1;
+    nv = nx * ny * nz;
+    if (t < 64) {
+        int lo = 0x7fffffff, cnt = 0;
+        if (t < nv) {
+            const int bx = 2 * P.range[0] - (cx + P.range[0]) % C[0];
+            const int by = 2 * P.range[1] - (cy + P.range[1]) % C[1];
+            const int bz = 2 * P.range[2] - (cz + P.range[2]) % C[2];
+            const int a = (bx > 0 ? bx : 0) + t / (ny * nz), bb = (by > 0 ? by : 0) + (t / nz) % ny;
+            const int cc = (bz > 0 ? bz : 0) + t % nz;
+            const int jx = ((cx - P.range[0] + a % nx) % C[0] + C[0]) % C[0];
+            const int jy = ((cy - P.range[1] + bb % ny) % C[1] + C[1]) % C[1];
+            const int jz = ((cz - P.range[2] + cc % nz) % C[2] + C[2]) % C[2];
+            const int jc = (jx * C[1] + jy) * C[2] + jz;
+            lo = F.cellStart[jc];
+            cnt = F.cellStart[jc + 1] - lo;
+        }
+        int incl = cnt;  // inclusive scan of the counts over the wavefront
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(incl, d, 64);
+            if (t >= d) incl += u;
+        }
+        if (t < kCellTable) {
+            T.vlo[t] = lo;
+            T.vcnt[t] = cnt;
+            T.voff[t] = (incl - cnt) - (t < nv ? lo : 0);
+        }
+        if (t == 63) T.total = incl;
+    }
+    __syncthreads();
+    return true;
+}
+
+// the staged place of fluid particle j: j + voff of the last visited range starting at
+// or before j
+__device__ inline int fac_cell_slot(const FacCellTable& T, int j) {
+    int v = 0;
+#pragma unroll
+    for (int step = kCellTable / 2; step >= 1; step >>= 1)
+        if (T.vlo[v + step] <= j) v += step;
+    return j + T.voff[v];
+}
+
+// FacOp::slot of every fluid-neighbour entry of the fluid rows of cells whose visited
+// cells fit the staging (the product gathers the others' records from global memory)
+__global__ void __launch_bounds__(kCellThreads) k_fac_slots(DevParams P, FacOp F) {
+    __shared__ FacCellTable T;
+    int rs, nrows, nv;
+    if (!fac_cell_table(P, F, T, rs, nrows, nv)) return;
+    if (T.total > kCellSlots) return;
+    constexpr int G = body::kFacLanes;
+    for (int r = threadIdx.x / G; r < nrows; r += kCellThreads / G) {
+        const int i = rs + r;
+        const long long b0 = F.nPtr[i];
+        const int w = F.nFluid[i];
+        for (int jn = threadIdx.x & (G - 1); jn < w; jn += G)
+            F.slot[b0 + jn] = (unsigned short)fac_cell_slot(T, F.nInd[b0 + jn]);
+    }
+}
+
+// the viscous list entries of a batch: lane g takes entries jb + g + u*G (u < U).
+// kSlot: the staged places (FacOp::slot), else the columns.  Entries past the row's
+// end take i's own record (own) with weight 0; their loads read the row's last entry
+// (w >= 1), so that the loads need no branches and issue together
+template <int U, bool kSlot>
+__device__ inline void fac_cell_list(const FacOp& F, const double* __restrict__ alpha, long long b0, int w, int own,
+                                     int jb, int g, int* j, double* a) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int jn = jb + g + u * body::kFacLanes;
+        const long long e = b0 + (jn < w ? jn : w - 1);
+        const int jj = kSlot ? (int)F.slot[e] : F.nInd[e];
+        const double aa = alpha[e];
+        j[u] = jn < w ? jj : own;
+        a[u] = jn < w ? aa : 0.0;
+    }
+}
+
+// The rows of one cell.  kStaged: the fluid records of the visited cells are in LDS
+// (L.rec), read through the precomputed places (FacOp::slot) if kSlot, else through
+// a search of the table; otherwise every record is gathered from global memory.
+// None of this changes which lane takes an entry or the order in which a lane adds
+// its terms.  (Loading the next batch's list entries before the current batch's
+// records, or 8 entries per batch instead of 4, measured slower: 151.7 and 169.8
+// against 125.9 ms per step on the 1M scene.)
+template <bool kNeg, bool kWrap, bool kStaged, bool kSlot>
+__device__ inline void fac_y_cell_rows(const DevParams& P, const FacOp& F, const double* __restrict__ x, double beta,
+                                       const double* yin, double* y, double* y2, double* p_copy,
+                                       const FacCellShared& L, int rs, int nrows) {
+#pragma clang fp contract(fast)
+    typedef double dv2 __attribute__((ext_vector_type(2)));
+    constexpr int G = body::kFacLanes, U = body::kFacBatch;
+    const int t = threadIdx.x;
+    const body::FacPlanes R = body::fac_planes(P, F.rec);
+    const dv2* __restrict__ ra = (const dv2*)R.a;
+    const dv2* __restrict__ rb = (const dv2*)R.b;
+    const dv2* __restrict__ rc = (const dv2*)R.c;
+    const double* __restrict__ rt = R.t;
+    const double* __restrict__ alpha = (const double*)F.alpha;
+    const double W0 = P.W[0], W1 = P.W[1], W2 = P.W[2], h0 = P.hW[0], h1 = P.hW[1], h2 = P.hW[2];
+    const int g = t & (G - 1);
+    const int passes = (nrows + kCellThreads / G - 1) / (kCellThreads / G);
+    for (int ps = 0; ps < passes; ++ps) {  // uniform trip count: every lane reaches the group sums
+        const int r = ps * (kCellThreads / G) + t / G;
+        const bool on = r < nrows;
+        const int i = rs + (on ? r : 0);
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        if (on) {
+            const double pi0 = F.pos[3 * i], pi1 = F.pos[3 * i + 1], pi2 = F.pos[3 * i + 2];
+            // viscous part over the fluid neighbours (alpha 0: self, beyond RadiusV)
+            const long long b0 = F.nPtr[i];
+            const int w = F.nFluid[i];
+            const int own = kStaged && kSlot ? fac_cell_slot(L.tab, i) : i;
+            int j[U];
+            double a[U];
+            fac_cell_list<U, kStaged && kSlot>(F, alpha, b0, w, own, 0, g, j, a);
+            for (int jb = 0; jb < w; jb += G * U) {
+                dv2 q0[U], q1[U], q2[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (kStaged) {
+                        const int sl = kSlot ? j[u] : fac_cell_slot(L.tab, j[u]);
+                        q0[u] = L.rec[3 * sl];
+                        q1[u] = L.rec[3 * sl + 1];
+                        q2[u] = L.rec[3 * sl + 2];
+                    } else {
+                        q0[u] = ra[j[u]];
+                        q1[u] = rb[j[u]];
+                        q2[u] = rc[j[u]];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const double r0 = body::fac_wrap<kWrap>(q1[u][1] - pi0, W0, h0);
+                    const double r1 = body::fac_wrap<kWrap>(q2[u][0] - pi1, W1, h1);
+                    const double r2 = body::fac_wrap<kWrap>(q2[u][1] - pi2, W2, h2);
+                    const double f = a[u] * ((r0 * q0[u][0] + r1 * q0[u][1]) + r2 * q1[u][0]);
+                    s0 -= f * r0;
+                    s1 -= f * r1;
+                    s2 -= f * r2;
+                }
+                if (jb + G * U < w) fac_cell_list<U, kStaged && kSlot>(F, alpha, b0, w, own, jb + G * U, g, j, a);
+            }
+            // C Lambda C^T part over the P neighbours (beta 0: self)
+            const long long c0 = F.nPtrP[i];
+            const int n = F.nCountP[i];
+            for (int jb = 0; jb < n; jb += G * body::kFacBatchP) {
+                constexpr int UP = body::kFacBatchP;
+                int k[UP];
+                double bb[UP];
+#pragma unroll
+                for (int u = 0; u < UP; ++u) {  // branch-free loads, as fac_cell_list (n >= 1: self)
+                    const int jn = jb + g + u * G;
+                    const long long e = c0 + (jn < n ? jn : n - 1);
+                    const int kk = F.nIndP[e];
+                    const double be = F.beta[e];
+                    k[u] = jn < n ? kk : i;
+                    bb[u] = jn < n ? be : 0.0;
+                }
+                dv2 q1[UP], q2[UP];
+                double q3[UP];
+#pragma unroll
+                for (int u = 0; u < UP; ++u) {
+                    q1[u] = rb[k[u]];
+                    q2[u] = rc[k[u]];
+                    q3[u] = rt[k[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < UP; ++u) {
+                    const double r0 = body::fac_wrap<kWrap>(q1[u][1] - pi0, W0, h0);
+                    const double r1 = body::fac_wrap<kWrap>(q2[u][0] - pi1, W1, h1);
+                    const double r2 = body::fac_wrap<kWrap>(q2[u][1] - pi2, W2, h2);
+                    const double f = bb[u] * q3[u];
+                    s0 += f * r0;
+                    s1 += f * r1;
+                    s2 += f * r2;
+                }
+            }
+        }
+        s0 = body::group_sum(s0, G);
+        s1 = body::group_sum(s1, G);
+        s2 = body::group_sum(s2, G);
+        if (g == 0 && on) {
+            const double* D = F.diag + 9 * i;
+            const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
+            const double ti = rt[i];
+            double d0 = (D[0] * xi0 + D[1] * xi1) + D[2] * xi2;
+            double d1 = (D[3] * xi0 + D[4] * xi1) + D[5] * xi2;
+            double d2 = (D[6] * xi0 + D[7] * xi1) + D[8] * xi2;
+            d0 += F.selfC[3 * i] * ti;
+            d1 += F.selfC[3 * i + 1] * ti;
+            d2 += F.selfC[3 * i + 2] * ti;
+            s0 += d0;
+            s1 += d1;
+            s2 += d2;
+            fac_cell_store(i, kNeg ? -s0 : s0, kNeg ? -s1 : s1, kNeg ? -s2 : s2, x, beta, yin, y, y2, p_copy);
+        }
+    }
+}
+
+template <bool kNeg, bool kWrap>
+__device__ inline void fac_y_cell_body(const DevParams& P, const FacOp& F, const double* __restrict__ x, double beta,
+                                       const double* yin, double* y, double* y2, double* p_copy, FacCellShared& L) {
+    int rs, nrows, nv;
+    if (!fac_cell_table(P, F, L.tab, rs, nrows, nv)) return;  // the whole workgroup: no fluid in this cell
+    if (L.tab.total > kCellSlots) {  // too many records to stage: gather them from global memory
+        fac_y_cell_rows<kNeg, kWrap, false, false>(P, F, x, beta, yin, y, y2, p_copy, L, rs, nrows);
+        return;
+    }
+    {
+        // wavefront w copies visited cells w, w + 8, ...; lane l particles lo + l, ...
+        const body::FacPlanes R = body::fac_planes(P, F.rec);
+        const dv2c* __restrict__ ra = (const dv2c*)R.a;
+        const dv2c* __restrict__ rb = (const dv2c*)R.b;
+        const dv2c* __restrict__ rc = (const dv2c*)R.c;
+        const int t = threadIdx.x;
+        for (int v = t >> 6; v < nv; v += kCellThreads / 64) {
+            const int lo = L.tab.vlo[v], cnt = L.tab.vcnt[v], base = L.tab.voff[v] + lo;
+            for (int k = t & 63; k < cnt; k += 64) {
+                const dv2c qa = ra[lo + k], qb = rb[lo + k], qc = rc[lo + k];
+                L.rec[3 * (base + k)] = qa;
+                L.rec[3 * (base + k) + 1] = qb;
+                L.rec[3 * (base + k) + 2] = qc;
+            }
+        }
+    }
+    __syncthreads();
+    if (F.slot) fac_y_cell_rows<kNeg, kWrap, true, true>(P, F, x, beta, yin, y, y2, p_copy, L, rs, nrows);
+    else fac_y_cell_rows<kNeg, kWrap, true, false>(P, F, x, beta, yin, y, y2, p_copy, L, rs, nrows);
+}
+
+template <bool kNeg>
+__global__ void __launch_bounds__(kCellThreads) k_fac_y_cell(DevParams P, FacOp F, const double* __restrict__ x,
+                                                             double beta, const double* yin, double* y, double* y2,
+                                                             double* p_copy) {
+    __shared__ FacCellShared L;
+    if (*F.wrap) fac_y_cell_body<kNeg, true>(P, F, x, beta, yin, y, y2, p_copy, L);
+    else fac_y_cell_body<kNeg, false>(P, F, x, beta, yin, y, y2, p_copy, L);
+}
+
 template <bool kNeg, class VT>
 void fac_launch(hipStream_t st, const DevParams& P, const FacOp& F, const double* x, double beta, const double* yin,
                 double* y, double* y2, double* p_copy, const double* dotx, double* part, int passes) {
     if (passes & 1) hipLaunchKernelGGL(k_fac_t<VT>, dim3(nblk(P.np, 256)), dim3(256), 0, st, P, F, x);
-    if (passes & 2)
-        hipLaunchKernelGGL((k_fac_y<kNeg, VT>), dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P, F, x, beta, yin, y,
-                           y2, p_copy, dotx, part);
+    if (!(passes & 2)) return;
+    const long long cells = (long long)F.cellCnt[0] * F.cellCnt[1] * F.cellCnt[2];
+    if (sizeof(VT) == 8 && F.cellStart && cells > 0) {
+        hipLaunchKernelGGL((k_fac_y_cell<kNeg>), dim3((unsigned)cells), dim3(kCellThreads), 0, st, P, F, x, beta, yin,
+                           y, y2, p_copy);
+        if (part) launch_dot_partial(st, P, dotx, y, part);
+        return;
+    }
+    hipLaunchKernelGGL((k_fac_y<kNeg, VT>), dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P, F, x, beta, yin, y, y2,
+                       p_copy, dotx, part);
 }
 
 }  // namespace
@@ -542,6 +858,12 @@ __global__ void __launch_bounds__(256) k_fac_rec_positions(DevParams P, const do
 void launch_fac_rec_positions(hipStream_t st, const DevParams& P, const double* from, double* to) {
     if (P.np == 0) return;
     hipLaunchKernelGGL(k_fac_rec_positions, dim3(nblk(P.np, 256)), dim3(256), 0, st, P, from, to);
+}
+
+void launch_fac_slots(hipStream_t st, const DevParams& P, const FacOp& F) {
+    const long long cells = (long long)F.cellCnt[0] * F.cellCnt[1] * F.cellCnt[2];
+    if (P.nf == 0 || !F.cellStart || !F.slot || cells <= 0) return;
+    hipLaunchKernelGGL(k_fac_slots, dim3((unsigned)cells), dim3(kCellThreads), 0, st, P, F);
 }
 
 void launch_fac_spmv(hipStream_t st, const DevParams& P, const FacOp& F, double alpha, const double* x, double beta,

@@ -94,6 +94,14 @@ struct FacOp {
     const float* u32;
     const float* v32;
     float* rec32;
+    // cell-staged FP64 second pass (null cellStart: the chunk kernel): one workgroup
+    // per cell of the fluid bounding box cellLo + [0, cellCnt), which stages the
+    // records of the row cell's visited cells in LDS for the viscous gathers
+    const int* cellStart;
+    int cellLo[3], cellCnt[3];
+    // [E] per fluid-neighbour entry of a fluid row: the neighbour's place in the row
+    // cell's staged records (launch_fac_slots), read instead of the column (or null)
+    unsigned short* slot;
 };
 
 constexpr int kSlice = 64;    // particles per A slice (one wavefront)
@@ -237,6 +245,8 @@ void launch_fac_spmv(hipStream_t st, const DevParams& P, const FacOp& F, double 
 // the positions of the FP64 records (planes b odd and c of FacOp::rec, 7*np doubles)
 // copied from one record buffer into another; x and t are written by each product
 void launch_fac_rec_positions(hipStream_t st, const DevParams& P, const double* from, double* to);
+// FacOp::slot of the fluid-neighbour entries (after the lists; the cell-staged pass)
+void launch_fac_slots(hipStream_t st, const DevParams& P, const FacOp& F);
 
 void launch_convert_f32(hipStream_t st, const double* in, float* out, long long n);
 void launch_axpy(hipStream_t st, long long n, const double* x, double* y);  // y += 1.0*x

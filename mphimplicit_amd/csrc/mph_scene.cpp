@@ -285,6 +285,7 @@ int phase_neighbor_finish(mph_scene& S) {
     S.slots = h[2] * kSlice;
     std::memcpy(S.hbbox, &h[3], 6 * sizeof(int));
     S.blocks = blocks_wanted(S);
+    S.fslotValid = false;  // the staged places follow the new lists (matrix-C phase)
     int rc;
     if ((rc = S.nInd.ensure(S.E > 0 ? S.E : 1)) || (rc = S.nIndP.ensure(S.EP > 0 ? S.EP : 1))) return rc;
     S.oneScan = scanned;
@@ -354,6 +355,12 @@ int phase_matrix_c(mph_scene& S) {
                      S.runP.p, S.selfC.p, S.frec.p);
         if (mixed(S))  // alpha was written by the matrix-A phase
             launch_fac_u(S.st, S.P, S.nFluid.p, S.nPtr.p, S.nInd.p, S.falpha.p, S.pos.p, S.fu32.p);
+        // the staged places of the viscous entries for the cell-staged FP64 pass
+        if (S.facCells && S.batches == 0) {
+            if ((rc = S.fslot.ensure(S.E > 0 ? S.E : 1))) return rc;
+            S.fslotValid = true;
+            launch_fac_slots(S.st, S.P, fac_op(S, false));
+        }
     }
     return MPH_OK;
 }
@@ -574,6 +581,13 @@ FacOp fac_op(mph_scene& S, bool f32) {
     F.u32 = f32 ? S.fu32.p : nullptr;
     F.v32 = f32 ? S.fv32.p : nullptr;
     F.rec32 = f32 ? S.frec32.p : nullptr;
+    // the cell-staged second pass over the fluid bounding box (max x,y,z then min)
+    F.cellStart = S.facCells ? S.cellStart.p : nullptr;
+    F.slot = S.facCells && S.fslot.p && S.fslotValid ? S.fslot.p : nullptr;
+    for (int d = 0; d < 3; ++d) {
+        F.cellLo[d] = S.hbbox[3 + d];
+        F.cellCnt[d] = S.facCells ? S.hbbox[d] - S.hbbox[3 + d] + 1 : 0;
+    }
     return F;
 }
 
@@ -931,6 +945,7 @@ void destroy(mph_scene* S) {
     S->runStart.release();
     for (DBuf<double>* b : {&S->falpha, &S->fdiag, &S->selfC, &S->fbeta, &S->frec}) b->release();
     S->fu32.release();
+    S->fslot.release();
     S->fv32.release();
     S->frec32.release();
     for (DBuf<double>* b : {&S->pos, &S->pos2, &S->vel, &S->vel2, &S->force, &S->mass, &S->kappa, &S->lambda,
@@ -1413,10 +1428,13 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
         // the diagonal block, C_ii, the record written (FP64: x and t, 32 B, and the
         // position read; FP32: 16 B), and the CR vectors as the assembled product (x
         // read, y read and written: 72 B per fluid particle).  Gathers of neighbours'
-        // records are cache reads.
+        // records are cache reads (in the cell-staged FP64 pass, LDS reads of records
+        // staged per row cell).  That pass reads a 2-byte staged place per viscous entry
+        // in place of the 4-byte column.
         if (!factored(*S)) return fail(MPH_E_STATE, "the scene applies the assembled operator");
         const bool f64 = n == "FactoredBytes";
         const double ab = f64 ? 8.0 : 12.0, vb = f64 ? 8.0 : 12.0, recb = f64 ? 32.0 : 16.0;
+        const double colb = f64 && S->facCells && S->fslotValid ? 2.0 : 4.0;
         const double posb = f64 ? 24.0 : 0.0;
         std::vector<int> nfl, ncp, run;
         int rc;
@@ -1429,7 +1447,7 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
             if (run[k] > 0) t += (4.0 + vb) * run[k] + 8.0 + posb;  // entries, lambda, position
         }
         for (int i = 0; i < P.nf; ++i)
-            t += (4.0 + ab) * nfl[i] + (4.0 + vb) * ncp[i] + 4.0 + 4.0 + 72.0 + 24.0 + 72.0;
+            t += (colb + ab) * nfl[i] + (4.0 + vb) * ncp[i] + 4.0 + 4.0 + 72.0 + 24.0 + 72.0;
         *v = t;
     }
     else if (n == "SpmvLaunchesF32") *v = (double)S->spmvLaunches32;
@@ -1476,6 +1494,11 @@ int mph_scene_set_option(mph_scene* S, const char* name, double value) {
     if (n == "LaneRowsMax") {
         if (!(value >= 0.0 && value <= 2147483647.0)) return fail(MPH_E_ARG, "LaneRowsMax out of range");
         S->laneRowsMax = (int)value;
+        return MPH_OK;
+    }
+    if (n == "FactoredCells") {
+        if (value != 0.0 && value != 1.0) return fail(MPH_E_ARG, "FactoredCells is 0 or 1");
+        S->facCells = value != 0.0;
         return MPH_OK;
     }
     return fail(MPH_E_FIELD, "unknown option " + n);
@@ -1551,6 +1574,7 @@ int mph_scene_device_bytes(mph_scene* S, unsigned long long* bytes) {
     n += S->fgtab.cap * sizeof(double);
     for (DBuf<double>* b : {&S->falpha, &S->fdiag, &S->selfC, &S->fbeta, &S->frec}) n += b->cap * sizeof(double);
     n += (S->fu32.cap + S->fv32.cap + S->frec32.cap) * sizeof(float);
+    n += S->fslot.cap * sizeof(unsigned short);
     n += S->runStart.cap * sizeof(unsigned short);
     for (DBuf<double>* b : {&S->pos, &S->pos2, &S->vel, &S->vel2, &S->force, &S->mass, &S->kappa, &S->lambda,
                             &S->mu, &S->densA, &S->gcen, &S->pressA, &S->volStrainP, &S->divP, &S->pressP,

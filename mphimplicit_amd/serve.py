@@ -242,11 +242,15 @@ class Server:
         lat = np.array([r.latency for r in self.finished]) if self.finished else np.zeros(1)
         # offered load: the fluid-particle-steps of the requests that arrived in the run
         offered = sum(r.nf * r.steps for r in reqs[:nxt])
+        errors: dict[str, int] = {}
+        for r in self.failed:
+            errors[r.error] = errors.get(r.error, 0) + 1
         return {
             "wall_s": wall,
             "requests_arrived": nxt,
             "requests_done": len(self.finished),
             "requests_failed": len(self.failed),
+            "failures": errors,
             "requests_active_at_end": len(self.active) + len(self.building),
             "requests_pending_at_end": len(self.pending),
             "offered_fluid_particle_steps_per_s": offered / wall if wall > 0 else 0.0,

@@ -92,6 +92,36 @@ def test_factored_product(name, precision):
     assert np.abs(a.apply_matrix(x) - yo).max() <= 1e-12 * scale
 
 
+@pytest.mark.parametrize("name", PRODUCT_SCENES + ["2d_160k"])
+def test_cell_staged_pass_bitwise(name):
+    """The FP64 second pass exists as the chunk kernel (every gather through the
+    caches) and as the cell-staged kernel (the row cell's visited cells in LDS, the
+    default).  Same entries, lanes and order of summation: the products are
+    bit-identical, and so are whole steps (the CR scalars and velocities) with
+    either kernel; the periodic scene takes the wrapped displacements."""
+    outs = []
+    for cells in (0, 1):  # chunk kernel, then the cell-staged one
+        p, g = SCENES[name]()
+        p.operator_form = abi.OPERATOR_FACTORED
+        p.cr_tolerance = 1e-16
+        s = Scene(p, g)
+        s.set_option("FactoredCells", cells)
+        st = s.step(1)
+        nf = int(s.scalar("FluidCount"))
+        x = np.random.default_rng(5).standard_normal(3 * nf)
+        outs.append((s.apply_matrix(x), st.cr_iterations, st.rr, s.read("Velocity")))
+        if cells == 0:  # the cell kernel without the staged places of this step: the table search
+            s.set_option("FactoredCells", 1)
+            ys = s.apply_matrix(x)
+        s.close()
+    y0, it0, rr0, v0 = outs[0]
+    assert np.array_equal(y0.view(np.int64), ys.view(np.int64)), np.abs(y0 - ys).max()
+    for k, (y1, it1, rr1, v1) in enumerate(outs[1:]):
+        assert np.array_equal(y0.view(np.int64), y1.view(np.int64)), (k, np.abs(y0 - y1).max())
+        assert it0 == it1 and rr0 == rr1, k
+        assert np.array_equal(v0.view(np.int64), v1.view(np.int64)), k
+
+
 # scenes whose solves converge to rr/rr0 < 1e-24 (glued2d_sample stops at the
 # reference's iteration cap 3*nf near 2e-9, where even two dot orders of the oracle
 # differ by 4e-7: its operator is checked by the product test only)
