@@ -167,6 +167,8 @@ def run_gpu(args, precision="fp64", form=None):
     scene = Scene(params, grid, device=0)
     if factored and args.factored_cells == "off":
         scene.set_option("FactoredCells", 0)  # the chunk kernel for the FP64 second pass
+    if factored and args.factored_cells_xcd == "off":
+        scene.set_option("FactoredCellsXcd", 0)
     setup_s = time.perf_counter() - t
     print("[bench] %s %s scene %s: %d particles (%d fluid), created in %.1f s"
           % (precision, form, args.config, grid.np, nf, setup_s), file=sys.stderr, flush=True)
@@ -524,6 +526,8 @@ def main():
     ap.add_argument("--rel-residual", type=float, default=1.0e-8, help="CR stop: sqrt(rr/rr0) below this")
     ap.add_argument("--factored-cells", choices=["on", "off"], default="on",
                     help="FP64 second pass of the factored product staged per cell in LDS (off: the chunk kernel)")
+    ap.add_argument("--factored-cells-xcd", choices=["on", "off"], default="on",
+                    help="cell-staged pass: each XCD takes a contiguous range of cells (off: round robin)")
     ap.add_argument("--operator", choices=["assembled", "factored"], default="factored",
                     help="how the solve applies the particle matrix (mph_params.operator_form) in every leg")
     ap.add_argument("--assembled-leg", choices=["on", "off"], default="on",

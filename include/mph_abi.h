@@ -220,6 +220,12 @@ int mph_scene_set_timing(mph_scene* s, int on);
  *   "LaneRowsMax"  multigrid levels with fewer rows split each row of a smoothing
  *                  sweep over 4 lanes, larger ones take one thread per row
  *                  (default 32768).  Applies from the next multigrid build.
+ *   "FactoredCells"     1 (default): the FP64 factored product's second pass runs
+ *                  one workgroup per cell with the visited cells' records staged in
+ *                  LDS; 0: the chunk kernel, every neighbour gathered through the
+ *                  caches.
+ *   "FactoredCellsXcd"  1 (default): the cells of that pass are dealt to the XCDs in
+ *                  contiguous ranges; 0: round robin.
  * MPH_E_FIELD for an unknown name, MPH_E_ARG for a value out of range. */
 int mph_scene_set_option(mph_scene* s, const char* name, double value);
 

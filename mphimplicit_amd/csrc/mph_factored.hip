@@ -553,7 +553,14 @@ struct FacCellShared {  // the product's LDS: the table and the records
 __device__ inline bool fac_cell_table(const DevParams& P, const FacOp& F, FacCellTable& T, int& rs, int& nrows,
                                       int& nv) {
     const int t = threadIdx.x;
-    const int b = blockIdx.x;
+    int b = blockIdx.x;
+    if (F.cellXcd) {
+        // workgroups are dealt to the 8 XCDs in turn: XCD x = b mod 8 takes a contiguous
+        // range of cells instead, so that the cells running on one XCD at a time are
+        // neighbours and share their visited cells' records in that XCD's L2
+        const int nb = F.cellCnt[0] * F.cellCnt[1] * F.cellCnt[2], per = nb / 8, rem = nb % 8, x = b % 8;
+        b = x * per + (x < rem ? x : rem) + b / 8;
+    }
     const int* C = P.C;
     const int cz = F.cellLo[2] + b % F.cellCnt[2];
     const int cy = F.cellLo[1] + (b / F.cellCnt[2]) % F.cellCnt[1];

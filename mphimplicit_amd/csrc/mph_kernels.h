@@ -99,6 +99,7 @@ struct FacOp {
     // records of the row cell's visited cells in LDS for the viscous gathers
     const int* cellStart;
     int cellLo[3], cellCnt[3];
+    int cellXcd;  // 1: each XCD takes a contiguous range of cells (else round robin)
     // [E] per fluid-neighbour entry of a fluid row: the neighbour's place in the row
     // cell's staged records (launch_fac_slots), read instead of the column (or null)
     unsigned short* slot;

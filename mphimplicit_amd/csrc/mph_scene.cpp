@@ -584,6 +584,7 @@ FacOp fac_op(mph_scene& S, bool f32) {
     // the cell-staged second pass over the fluid bounding box (max x,y,z then min)
     F.cellStart = S.facCells ? S.cellStart.p : nullptr;
     F.slot = S.facCells && S.fslot.p && S.fslotValid ? S.fslot.p : nullptr;
+    F.cellXcd = S.facCellXcd ? 1 : 0;
     for (int d = 0; d < 3; ++d) {
         F.cellLo[d] = S.hbbox[3 + d];
         F.cellCnt[d] = S.facCells ? S.hbbox[d] - S.hbbox[3 + d] + 1 : 0;
@@ -1434,6 +1435,7 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
         if (!factored(*S)) return fail(MPH_E_STATE, "the scene applies the assembled operator");
         const bool f64 = n == "FactoredBytes";
         const double ab = f64 ? 8.0 : 12.0, vb = f64 ? 8.0 : 12.0, recb = f64 ? 32.0 : 16.0;
+        // the cell-staged FP64 pass reads the 2-byte place instead of the column
         const double colb = f64 && S->facCells && S->fslotValid ? 2.0 : 4.0;
         const double posb = f64 ? 24.0 : 0.0;
         std::vector<int> nfl, ncp, run;
@@ -1499,6 +1501,11 @@ int mph_scene_set_option(mph_scene* S, const char* name, double value) {
     if (n == "FactoredCells") {
         if (value != 0.0 && value != 1.0) return fail(MPH_E_ARG, "FactoredCells is 0 or 1");
         S->facCells = value != 0.0;
+        return MPH_OK;
+    }
+    if (n == "FactoredCellsXcd") {
+        if (value != 0.0 && value != 1.0) return fail(MPH_E_ARG, "FactoredCellsXcd is 0 or 1");
+        S->facCellXcd = value != 0.0;
         return MPH_OK;
     }
     return fail(MPH_E_FIELD, "unknown option " + n);

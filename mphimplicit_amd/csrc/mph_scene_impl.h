@@ -184,6 +184,7 @@ struct mph_scene {
     mph::host::DBuf<double> falpha, fdiag, selfC, fbeta, frec;  // FacOp::alpha, diag, selfC, beta, rec
     mph::host::DBuf<float> fu32, fv32, frec32;  // FP32 form (mixed precision): FacOp::u32, v32, rec32
     bool facCells = true;  // FP64 second pass staged per cell in LDS (else the chunk kernel)
+    bool facCellXcd = true;  // FacOp::cellXcd
     mph::host::DBuf<unsigned short> fslot;  // FacOp::slot
     bool fslotValid = false;                // fslot matches the current lists
     // multigrid

@@ -228,6 +228,16 @@ def box_scene(seed: int, dim: int, nf_target: int, mu: float, gravity, dt=5.0e-4
     dmax[dim - 1] = (inner + n + pad) * l0  # headroom above the open top
     if dim == 2:
         dmin[2], dmax[2] = -0.05, 0.05
+    # A domain width that is a whole multiple of MaxRadius gives cells of MaxRadius,
+    # where rounding can make ceil(MaxRadius / CellWidth) = 2 (initializeDomain,
+    # main.cpp:1078-1096): the reference's neighbour search then refuses the domain
+    # (main.cpp:1402).  Widen such a domain by half a spacing.
+    max_r = max(3.5 * l0, 2.0 * (1.7 * l0), 3.5 * l0)
+    for d in range(dim):
+        w = dmax[d] - dmin[d]
+        cells = math.floor(w / max_r)
+        if math.ceil(max_r / (w / cells)) > 1:
+            dmax[d] += 0.5 * l0
     g = assemble(fluid, walls, 0.0, l0, dmin, dmax, seed, dim)
     p = base_params(dim, mu, gravity, dt, steps)
     return p, g

@@ -92,32 +92,38 @@ def test_factored_product(name, precision):
     assert np.abs(a.apply_matrix(x) - yo).max() <= 1e-12 * scale
 
 
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64, abi.PRECISION_MIXED])
 @pytest.mark.parametrize("name", PRODUCT_SCENES + ["2d_160k"])
-def test_cell_staged_pass_bitwise(name):
+def test_cell_staged_pass_bitwise(name, precision):
     """The FP64 second pass exists as the chunk kernel (every gather through the
     caches) and as the cell-staged kernel (the row cell's visited cells in LDS, the
     default).  Same entries, lanes and order of summation: the products are
-    bit-identical, and so are whole steps (the CR scalars and velocities) with
-    either kernel; the periodic scene takes the wrapped displacements."""
+    bit-identical, and so are whole steps (the CR scalars and velocities) with either
+    kernel, in plain FP64 and in mixed precision (whose outer residuals take the FP64
+    pass; its FP32 products are the chunk kernel's either way); the periodic scene
+    takes the wrapped displacements."""
     outs = []
+    precs = (abi.PRECISION_FP64, abi.PRECISION_MIXED) if precision == abi.PRECISION_MIXED else (abi.PRECISION_FP64,)
     for cells in (0, 1):  # chunk kernel, then the cell-staged one
         p, g = SCENES[name]()
         p.operator_form = abi.OPERATOR_FACTORED
+        p.precision = precision
         p.cr_tolerance = 1e-16
         s = Scene(p, g)
         s.set_option("FactoredCells", cells)
         st = s.step(1)
         nf = int(s.scalar("FluidCount"))
         x = np.random.default_rng(5).standard_normal(3 * nf)
-        outs.append((s.apply_matrix(x), st.cr_iterations, st.rr, s.read("Velocity")))
+        outs.append(([s.apply_matrix(x, q) for q in precs], st.cr_iterations, st.rr, s.read("Velocity")))
         if cells == 0:  # the cell kernel without the staged places of this step: the table search
             s.set_option("FactoredCells", 1)
             ys = s.apply_matrix(x)
         s.close()
     y0, it0, rr0, v0 = outs[0]
-    assert np.array_equal(y0.view(np.int64), ys.view(np.int64)), np.abs(y0 - ys).max()
+    assert np.array_equal(y0[0].view(np.int64), ys.view(np.int64)), np.abs(y0[0] - ys).max()
     for k, (y1, it1, rr1, v1) in enumerate(outs[1:]):
-        assert np.array_equal(y0.view(np.int64), y1.view(np.int64)), (k, np.abs(y0 - y1).max())
+        for a, b in zip(y0, y1):
+            assert np.array_equal(a.view(np.int64), b.view(np.int64)), (k, np.abs(a - b).max())
         assert it0 == it1 and rr0 == rr1, k
         assert np.array_equal(v0.view(np.int64), v1.view(np.int64)), k
 
