@@ -33,6 +33,10 @@ METRIC = "particle-steps/s, 1M 3D to 1e-8 res; mixed/batched vs plain FP64; p99 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = {"assembled": os.path.join(ROOT, "profiles", "spmv_traffic_3d_1m.json"),
                 "factored": os.path.join(ROOT, "profiles", "fac_traffic_3d_1m.json")}
+# the scene's counters of the FP64 CR loop's captured iterations
+GRAPH_SCALARS = ("CrGraphLaunches", "CrCaptures", "CrInstantiations", "CrCaptureMs")
+# a batch step's wall time per group of phases (mph_step_stats::ms_phase slots)
+BATCH_GROUPS = {"cells_and_neighbours": 3, "explicit_to_multigrid_build": 8, "solve": 9, "advance": 11}
 
 
 def spmv_bytes(blocks: float, nf: int, value_bytes: int = 8) -> tuple[float, float]:
@@ -169,6 +173,7 @@ def run_gpu(args, precision="fp64", form=None):
         scene.set_option("FactoredCells", 0)  # the chunk kernel for the FP64 second pass
     if factored and args.factored_cells_xcd == "off":
         scene.set_option("FactoredCellsXcd", 0)
+    scene.set_option("CrStop", args.cr_stop)
     setup_s = time.perf_counter() - t
     print("[bench] %s %s scene %s: %d particles (%d fluid), created in %.1f s"
           % (precision, form, args.config, grid.np, nf, setup_s), file=sys.stderr, flush=True)
@@ -178,16 +183,28 @@ def run_gpu(args, precision="fp64", form=None):
         if first is None:
             first = {"Velocity": scene.read("Velocity"), "PressureP": scene.read("PressureP")}
         print("[bench] warmup step %d: %d CR iterations" % (k, st.cr_iterations), file=sys.stderr, flush=True)
-    scene.set_timing(2)  # bracket every SpMV launch with HIP events
     lat, iters, refine = [], [], []
-    b64 = r64 = b32 = r32 = 0.0
-    l64_prev = l32_prev = 0.0
+    g0 = {n: scene.scalar(n) for n in GRAPH_SCALARS}
     for k in range(args.steps):
         t0 = time.perf_counter()
         st = scene.step(1)
         lat.append(time.perf_counter() - t0)
         iters.append(st.cr_iterations)
         refine.append(int(scene.scalar("RefinementSteps")))
+        if first is None:
+            first = {"Velocity": scene.read("Velocity"), "PressureP": scene.read("PressureP")}
+        print("[bench] step %d: %.1f ms, %d CR iterations" % (k, 1e3 * lat[-1], st.cr_iterations), file=sys.stderr,
+              flush=True)
+    graph = {n: scene.scalar(n) - g0[n] for n in GRAPH_SCALARS}
+    graph["CrGraphError"] = scene.scalar("CrGraphError")
+    # The products' launch durations: further steps with every product bracketed by HIP
+    # events on the scene stream (the CR iterations are then launched one by one rather
+    # than replayed from graphs; the kernels are the same)
+    scene.set_timing(2)
+    b64 = r64 = b32 = r32 = 0.0
+    l64_prev = l32_prev = 0.0
+    for k in range(args.spmv_timing_steps):
+        st = scene.step(1)
         l64, l32 = scene.scalar("SpmvLaunches"), scene.scalar("SpmvLaunchesF32")
         blocks = scene.scalar("ABlocks")
         mine, ref = spmv_bytes(blocks, nf, 8)
@@ -201,10 +218,7 @@ def run_gpu(args, precision="fp64", form=None):
         b32 += mine * (l32 - l32_prev)
         r32 += ref * (l32 - l32_prev)
         l64_prev, l32_prev = l64, l32
-        if first is None:
-            first = {"Velocity": scene.read("Velocity"), "PressureP": scene.read("PressureP")}
-        print("[bench] step %d: %.1f ms, %d CR iterations" % (k, 1e3 * lat[-1], st.cr_iterations), file=sys.stderr,
-              flush=True)
+        print("[bench] SpMV-timed step %d: %d CR iterations" % (k, st.cr_iterations), file=sys.stderr, flush=True)
     print("[bench] SpMV timing: FP64 %.3f ms over %d launches, FP32 %.3f ms over %d launches (error code %d)"
           % (scene.scalar("SpmvMs"), l64_prev, scene.scalar("SpmvMsF32"), l32_prev,
              scene.scalar("SpmvTimingError")), file=sys.stderr, flush=True)
@@ -224,6 +238,10 @@ def run_gpu(args, precision="fp64", form=None):
                         "13.2 GB; its limit is the cache traffic of gathering neighbours' x, position and t "
                         "(three 16-byte pairs per entry) through L2 and the Infinity Cache, not HBM")
     rf32 = roofline(k32, scene.scalar("SpmvMsF32"), l32_prev, b32, r32, None)
+    for rf in (rf64, rf32):
+        rf["measured_over"] = ("%d steps after the timed ones, every product bracketed by HIP events on the scene "
+                               "stream" % args.spmv_timing_steps)
+    scene.set_timing(0)
     step = None
     if precision == "fp64":
         # the whole step against the roofline: algorithmic bytes at the mean iteration
@@ -265,6 +283,7 @@ def run_gpu(args, precision="fp64", form=None):
         "first": first,
         "last": last,
         "tuned": tuned,
+        "graph": graph,
         "step": step,
     }
 
@@ -443,21 +462,25 @@ def run_batch(args):
         t_seq += time.perf_counter() - t
         s.close()
     print("[bench] batch sweep: sequential FP64 %.2f s" % t_seq, file=sys.stderr, flush=True)
-    res = {}
+    res, splits = {}, {}
     for name, prec in (("batched_mixed", abi.PRECISION_MIXED), ("batched_fp64", abi.PRECISION_FP64)):
         members = mk(prec)
         with Batch() as b:
             for s in members:
                 b.add(s)
             lat = []
+            split = np.zeros(4)
             for _ in range(args.batch_steps):
                 t = time.perf_counter()
-                b.step()
+                st = b.step()
                 lat.append(time.perf_counter() - t)
+                split += [st[0].ms_phase[k] for k in BATCH_GROUPS.values()]
         for s in members:
             s.close()
         res[name] = lat
-        print("[bench] batch sweep: %s %.2f s" % (name, sum(lat)), file=sys.stderr, flush=True)
+        splits[name] = {g: round(float(v) / args.batch_steps, 2) for g, v in zip(BATCH_GROUPS, split)}
+        print("[bench] batch sweep: %s %.2f s, mean ms per step %s" % (name, sum(lat), splits[name]),
+              file=sys.stderr, flush=True)
     work = sum(nf) * args.batch_steps
     v_seq = work / t_seq
     out = {
@@ -474,6 +497,7 @@ def run_batch(args):
         out[name + "_speedup_vs_sequential_fp64"] = round(v / v_seq, 3)
         out[name + "_p50_step_ms"] = round(1e3 * float(np.percentile(lat, 50)), 2)
         out[name + "_p99_step_ms"] = round(1e3 * float(np.percentile(lat, 99)), 2)
+        out[name + "_wall_ms_per_step"] = splits[name]
     return out
 
 
@@ -524,10 +548,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="3d_1m")
     ap.add_argument("--rel-residual", type=float, default=1.0e-8, help="CR stop: sqrt(rr/rr0) below this")
-    ap.add_argument("--factored-cells", choices=["on", "off"], default="on",
+    ap.add_argument("--factored-cells", choices=["on", "off"], default="on",  # on: the scene's default rule
                     help="FP64 second pass of the factored product staged per cell in LDS (off: the chunk kernel)")
     ap.add_argument("--factored-cells-xcd", choices=["on", "off"], default="on",
                     help="cell-staged pass: each XCD takes a contiguous range of cells (off: round robin)")
+    ap.add_argument("--cr-stop", type=int, choices=[0, 1, 2], default=1,
+                    help="the FP64 CR stop test: 0 on the host per iteration, 1 on the device, 2 on the device with "
+                         "the iterations replayed from captured HIP graphs")
+    ap.add_argument("--spmv-timing-steps", type=int, default=2,
+                    help="steps after the timed ones with every product bracketed by HIP events (its roofline)")
     ap.add_argument("--operator", choices=["assembled", "factored"], default="factored",
                     help="how the solve applies the particle matrix (mph_params.operator_form) in every leg")
     ap.add_argument("--assembled-leg", choices=["on", "off"], default="on",
@@ -591,6 +620,11 @@ def main():
         "p99_step_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "cr_iterations": g["iters"],
         "autotuned": g["tuned"],
+        "cr_stop": {"mode": args.cr_stop, "graph_launches": int(g["graph"]["CrGraphLaunches"]),
+                    "captures": int(g["graph"]["CrCaptures"]),
+                    "instantiations": int(g["graph"]["CrInstantiations"]),
+                    "capture_ms_per_step": round(g["graph"]["CrCaptureMs"] / args.steps, 3),
+                    "graph_error": int(g["graph"]["CrGraphError"])},
         "roofline": dict(g["roofline"], step=g["step"]),
         "hbm_ceiling": {"read_gbs": round(read_gbs, 1), "copy_gbs": round(copy_gbs, 1),
                         "spmv_frac_of_read": round(g["roofline"]["achieved"] / read_gbs, 4)

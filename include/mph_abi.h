@@ -220,12 +220,23 @@ int mph_scene_set_timing(mph_scene* s, int on);
  *   "LaneRowsMax"  multigrid levels with fewer rows split each row of a smoothing
  *                  sweep over 4 lanes, larger ones take one thread per row
  *                  (default 32768).  Applies from the next multigrid build.
- *   "FactoredCells"     1 (default): the FP64 factored product's second pass runs
- *                  one workgroup per cell with the visited cells' records staged in
+ *   "FactoredCells"     1: the FP64 factored product's second pass runs one
+ *                  workgroup per cell with the visited cells' records staged in
  *                  LDS; 0: the chunk kernel, every neighbour gathered through the
- *                  caches.
+ *                  caches; 2 (default): the cells in 3D scenes and in scenes of at
+ *                  most 16384 fluid particles, the chunk kernel otherwise (larger 2D
+ *                  scenes, whose cells hold too few rows for a workgroup).  Alone
+ *                  and in a batch; the same bits either way.
  *   "FactoredCellsXcd"  1 (default): the cells of that pass are dealt to the XCDs in
  *                  contiguous ranges; 0: round robin.
+ *   "CrStop"       where the FP64 CR loop tests rr/rr0 < tol (main.cpp:4344-4346):
+ *                  0 on the host, reading rr back every iteration; 1 (default) on
+ *                  the device, in the r.r reduction, with the host one iteration
+ *                  ahead (the kernels of an iteration enqueued after the stop return
+ *                  at once); 2 as 1, with iterations 1, 2, ... replayed from captured
+ *                  HIP graphs.  The same iterations and bits in every mode.  Scalars
+ *                  "CrGraphLaunches", "CrCaptures", "CrInstantiations", "CrCaptureMs"
+ *                  and "CrGraphError" count the graph work of mode 2.
  * MPH_E_FIELD for an unknown name, MPH_E_ARG for a value out of range. */
 int mph_scene_set_option(mph_scene* s, const char* name, double value);
 
@@ -264,7 +275,11 @@ int mph_scene_destroy(mph_scene* s);
  * one host synchronisation per iteration), then diagnostics and the advance.
  * Each scene's results are bit-identical to mph_scene_step on that scene.
  * `stats` (may be NULL) receives one record per member, in mph_batch_add order;
- * ms_total is the wall time of the whole batch step.
+ * ms_total is the wall time of the whole batch step, and ms_phase the batch's wall
+ * time per group of phases, the same in every record: [MPH_PHASE_NEIGHBOR]
+ * kinematics, cells and neighbour lists; [MPH_PHASE_MULTIGRID] explicit forces
+ * through the multigrid build; [MPH_PHASE_SOLVE] the lockstep solves;
+ * [MPH_PHASE_ADVANCE] diagnostics and advance.
  * A member whose CR residual is not finite leaves the lockstep solve at once and
  * does not advance; its record has status MPH_E_STATE, the other members finish the
  * step, and the call returns MPH_E_STATE (remove the failed member before the next

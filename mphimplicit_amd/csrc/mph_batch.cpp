@@ -7,9 +7,11 @@
 //     scenes so small scenes overlap on the GPU, with ONE host synchronisation for
 //     all neighbour-list totals of the batch;
 //   * the CR solve of all members in lockstep: one launch per kernel covers every
-//     scene (scene-table kernels of mph_batch.hip) and one host synchronisation
-//     per iteration reads the stop-rule scalars of the whole batch.  A scene that
-//     has converged drops out of the launch sets; multigrid levels of scenes with
+//     scene (scene-table kernels of mph_batch.hip).  In plain FP64 each scene's stop
+//     test runs on the device and the host, one iteration ahead, reads the stop flags
+//     of the whole batch once per iteration; the mixed-precision solve reads the
+//     residuals back instead.  A scene that has converged drops out of the launch
+//     sets (its kernels of the one iteration queued after its stop return at once); multigrid levels of scenes with
 //     different depths are scheduled level by level so every scene sees its own
 //     V-cycle (preconditionWithMultiGrid, main.cpp:4122-4266) in its own order;
 //   * diagnostics and the time advance on the scene's stream.
@@ -42,7 +44,11 @@ struct mph_batch {
     std::vector<int> hoff;
     Autotune tune64, tune32;  // SpMV load-group depth of the scene-table launches
     std::vector<char> fac;      // per scene of the current solve: factored operator
+    std::vector<char> cell;     // ... and its FP64 second pass cell-staged
     Autotune tuneFuse;        // V-cycle levels fused into one workgroup per scene
+    int* hctl = nullptr;      // [ns][kNCtl] coherent pinned mirrors of the scenes' CR control blocks
+    size_t hctlCap = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};  // the FP64 loop's last two iterations
 };
 
 namespace {
@@ -68,15 +74,28 @@ int ensure_pinned(mph_batch& B, size_t n) {
     return MPH_OK;
 }
 
-// scene tables and block offsets of one solve over `sc` (after the multigrid build)
-int build_tables(mph_batch& B, const std::vector<mph_scene*>& sc, Tables& T) {
+int ensure_ctl(mph_batch& B, size_t n) {
+    if (B.hctl && B.hctlCap >= n) return MPH_OK;
+    if (B.hctl) (void)hipHostFree(B.hctl);
+    B.hctl = nullptr;
+    B.hctlCap = 0;
+    HIPCHK(hipHostMalloc((void**)&B.hctl, n * sizeof(int), hipHostMallocCoherent));
+    B.hctlCap = n;
+    return MPH_OK;
+}
+
+// scene tables and block offsets of one solve over `sc` (after the multigrid build);
+// stop: the FP64 solve with its stop test on the device (BScene::ctl)
+int build_tables(mph_batch& B, const std::vector<mph_scene*>& sc, Tables& T, bool stop) {
     const int ns = (int)sc.size();
     int rc;
     if ((rc = B.tab.ensure(ns)) || (rc = B.off.ensure((size_t)kOffRows * (ns + 1))) ||
         (rc = B.scal.ensure((size_t)ns * 8)) || (rc = ensure_pinned(B, (size_t)ns * 8)))
         return rc;
+    if (stop && (rc = ensure_ctl(B, (size_t)ns * kNCtl))) return rc;
     B.htab.assign(ns, BScene{});
     B.fac.assign(ns, 0);
+    B.cell.assign(ns, 0);
     B.hoff.assign((size_t)kOffRows * (ns + 1), 0);
     for (int s = 0; s < ns; ++s) {
         mph_scene& S = *sc[s];
@@ -113,7 +132,19 @@ int build_tables(mph_batch& B, const std::vector<mph_scene*>& sc, Tables& T) {
         b.depth = b.mg ? S.depth : 0;
         for (int v = 0; v < kFuseVariants; ++v) b.fuseOf[v] = b.mg ? S.fuseOf[v] : 0;
         for (int l = 0; l < b.depth; ++l) b.lv[l] = S.lv[l].L;
+        if (stop) {
+            if ((rc = S.ctl.ensure(kNCtl))) return rc;
+            b.ctl = S.ctl.p;
+            b.hctl = B.hctl + (size_t)kNCtl * s;
+            b.tol = S.prm.cr_tolerance;
+            b.maxit = S.prm.max_iterations > 0 ? S.prm.max_iterations : 3LL * S.P.nf;
+        }
         auto put = [&](int row, int blocks) { B.hoff[(size_t)row * (ns + 1) + s + 1] = blocks; };
+        // the cell-staged FP64 pass: one workgroup per cell of the fluid bounding box
+        const FacOp& F = b.fac[0];
+        const long long cells = (long long)F.cellCnt[0] * F.cellCnt[1] * F.cellCnt[2];
+        B.cell[s] = b.factored && F.cellStart && cells > 0 ? 1 : 0;
+        put(kOffCell, B.cell[s] ? (int)cells : 0);
         put(kOffChunk, (S.P.nf + 255) / 256);
         put(kOffElem, (int)((3LL * S.P.nf + 255) / 256));
         put(kOffPart, (S.P.np + 255) / 256);
@@ -143,13 +174,14 @@ int read_scalars(mph_batch& B, int ns) {
 // the scene-table SpMV with the batch's tuned load-group depth
 void bspmv(mph_batch& B, const Tables& T, const SceneSet& on, const SceneSet& first, bool f32, double alpha, int x,
            double beta, int yin, int y, int y2, int p_copy, int dot) {
-    SceneSet onA = on, onF{};  // assembled and factored members
+    SceneSet onA = on, onF{}, cells{};  // assembled and factored members
     bool anyA = false, anyF = false;
     for (int s = 0; s < T.ns; ++s) {
         if (!((on.w[s >> 6] >> (s & 63)) & 1ull)) continue;
         if (B.fac[s]) {
             onA.w[s >> 6] &= ~(1ull << (s & 63));
             onF.w[s >> 6] |= 1ull << (s & 63);
+            if (B.cell[s]) cells.w[s >> 6] |= 1ull << (s & 63);
             anyF = true;
         } else {
             anyA = true;
@@ -162,7 +194,7 @@ void bspmv(mph_batch& B, const Tables& T, const SceneSet& on, const SceneSet& fi
         spmv(T, onA, first, f32, alpha, x, beta, yin, y, y2, p_copy, dot, v);
         A.end(B.st);
     }
-    if (anyF) spmv_fac(T, onF, first, f32, alpha, x, beta, yin, y, y2, p_copy, dot);
+    if (anyF) spmv_fac(T, onF, first, cells, f32, alpha, x, beta, yin, y, y2, p_copy, dot);
 }
 
 // the V-cycle of every scene in `on`, level by level; cur[s][l] is the ping-pong
@@ -281,15 +313,37 @@ void finish_levels(const VCycle& V) {
 // once and is flagged in `failed`.
 bool nonfinite(double v) { return !std::isfinite(v); }
 
-// solveWithConjugatedGradient (4270-4421) for every scene of `sc` in lockstep
+// one iteration of the CR loop body (4352-4378) for the scenes of `on`: y = A s
+// (the scenes of `first` also q = y, p = s; the others update p and q), rho = s.y,
+// u = M^-1 q, tmp = q.u, x, r and s updated, then r.r and the stop test on the device
+void fp64_iteration(mph_batch& B, const Tables& T, VCycle& V, const Set& on, const Set& first) {
+    Set rest;
+    for (int s = 0; s < T.ns; ++s)
+        if (on.has(s) && !first.has(s)) rest.add(s);
+    bspmv(B, T, on.m, first.m, false, 1.0, kS, 0.0, kY, kY, kQ, kP, kS);
+    dot_final(T, on.m, kRHO, kRHOP);
+    pq_update(T, rest.m);
+    apply_m(V, on, kQ, kU, true, false);
+    dot_final(T, on.m, kTMP, -1);
+    xrs_update(T, on.m, kX);
+    dot_final_stop(T, on.m, 1);
+}
+
+// solveWithConjugatedGradient (4270-4421) for every scene of `sc` in lockstep, with
+// each scene's stop test (main.cpp:4344-4346) on the device: the host enqueues
+// iteration k+1 for the scenes it has not yet seen stop, then waits for iteration k
+// and drops the scenes that stopped; their kernels in iteration k+1 return at once
 int solve_fp64(mph_batch& B, const std::vector<mph_scene*>& sc, std::vector<char>& failed) {
     const int ns = (int)sc.size();
     Tables T{};
     int rc;
-    if ((rc = build_tables(B, sc, T))) return rc;
+    if ((rc = build_tables(B, sc, T, true))) return rc;
+    for (hipEvent_t& e : B.ev)
+        if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     Set all;
     for (int s = 0; s < ns; ++s) all.add(s);
     VCycle V(B, T, sc);
+    ctl_reset(T, all.m);
     solve_prologue(T, sc, all);
     apply_m(V, all, kB, kS, true, false);  // s = M^-1 b; rs0 = b.s; rr0 = b.b
     dot_final(T, all.m, kRS0, -1);
@@ -298,44 +352,48 @@ int solve_fp64(mph_batch& B, const std::vector<mph_scene*>& sc, std::vector<char
     bspmv(B, T, all.m, SceneSet{}, false, -1.0, kX, 1.0, kB, kR, -1, -1, -1);  // r = b - A x; s = M^-1 r
     apply_m(V, all, kR, kS, false, false);
     dot_partial(T, all.m, kR, kR);
-    dot_final(T, all.m, kRR, -1);
+    dot_final_stop(T, all.m, 0);
     if ((rc = read_scalars(B, ns))) return rc;
-    std::vector<double> rr0(ns);
-    std::vector<long long> iter(ns, 0), maxit(ns);
     failed.assign(ns, 0);
+    const volatile int* hc = B.hctl;
+    Set active;
     for (int s = 0; s < ns; ++s) {
-        rr0[s] = B.hscal[8 * s + kRR0];
-        const long long n = 3LL * sc[s]->P.nf;
-        maxit[s] = sc[s]->prm.max_iterations > 0 ? sc[s]->prm.max_iterations : n;
+        if (nonfinite(B.hscal[8 * s + kRR0]) || hc[kNCtl * s + kCtlBad]) failed[s] = 1;
+        else if (!hc[kNCtl * s + kCtlHalt]) active.add(s);
     }
-    Set active = all;
-    for (;;) {
-        Set now, first, rest;
-        for (int s = 0; s < ns; ++s) {
-            if (!active.has(s)) continue;
-            if (nonfinite(rr0[s]) || nonfinite(B.hscal[8 * s + kRR])) {
-                failed[s] = 1;
-                continue;
+    if (!active.empty()) {
+        // the trials of the autotuners are read as their iterations complete (mark:
+        // how many each had recorded at an iteration's event)
+        Autotune* tn[] = {&B.tune64, &B.tune32, &B.tuneFuse};
+        constexpr int kTn = 3;
+        size_t mark[2][kTn];
+        fp64_iteration(B, T, V, active, active);
+        HIPCHK(hipEventRecord(B.ev[0], B.st));
+        for (int t = 0; t < kTn; ++t) mark[0][t] = tn[t]->pend.size();
+        for (long long k = 1;; ++k) {
+            fp64_iteration(B, T, V, active, Set{});
+            HIPCHK(hipEventRecord(B.ev[k & 1], B.st));
+            for (int t = 0; t < kTn; ++t) mark[k & 1][t] = tn[t]->pend.size();
+            HIPCHK(hipEventSynchronize(B.ev[(k - 1) & 1]));
+            for (int t = 0; t < kTn; ++t) {
+                const size_t n = mark[(k - 1) & 1][t];
+                tn[t]->collect(n);
+                mark[k & 1][t] -= n;
             }
-            if (iter[s] >= maxit[s]) continue;
-            if (B.hscal[8 * s + kRR] / rr0[s] < sc[s]->prm.cr_tolerance) continue;
-            now.add(s);
-            (iter[s] == 0 ? first : rest).add(s);
+            Set now;
+            for (int s = 0; s < ns; ++s)
+                if (active.has(s) && !hc[kNCtl * s + kCtlHalt]) now.add(s);
+            active = now;
+            if (active.empty()) break;
         }
-        active = now;
-        if (active.empty()) break;
-        // y = A s (first iteration also q = y, p = s); rho = s.y
-        bspmv(B, T, active.m, first.m, false, 1.0, kS, 0.0, kY, kY, kQ, kP, kS);
-        dot_final(T, active.m, kRHO, kRHOP);
-        pq_update(T, rest.m);
-        apply_m(V, active, kQ, kU, true, false);  // u = M^-1 q; tmp = q.u
-        dot_final(T, active.m, kTMP, -1);
-        xrs_update(T, active.m, kX);
-        dot_final(T, active.m, kRR, -1);
-        for (int s = 0; s < ns; ++s)
-            if (active.has(s)) iter[s]++;
-        if ((rc = read_scalars(B, ns))) return rc;
     }
+    HIPCHK(hipStreamSynchronize(B.st));  // the mirrors are final
+    std::vector<long long> iter(ns, 0);
+    for (int s = 0; s < ns; ++s) {
+        iter[s] = hc[kNCtl * s + kCtlIter];
+        if (hc[kNCtl * s + kCtlBad]) failed[s] = 1;
+    }
+    ctl_release(T, all.m);
     Set ok;  // the members that did not fail
     for (int s = 0; s < ns; ++s)
         if (!failed[s]) ok.add(s);
@@ -364,7 +422,7 @@ int solve_mixed(mph_batch& B, const std::vector<mph_scene*>& sc, std::vector<cha
     const int ns = (int)sc.size();
     Tables T{};
     int rc;
-    if ((rc = build_tables(B, sc, T))) return rc;
+    if ((rc = build_tables(B, sc, T, false))) return rc;
     Set all;
     for (int s = 0; s < ns; ++s) all.add(s);
     VCycle V(B, T, sc);
@@ -563,10 +621,13 @@ int mph_batch_step(mph_batch* B, mph_step_stats* stats) {
     for (mph_scene* S : act)
         if ((rc = phase_neighbor_finish(*S))) return rc;
     if ((rc = check_launch())) return rc;
+    if ((rc = sync_all(act))) return rc;
+    const auto t1 = std::chrono::steady_clock::now();
     for (int ph = MPH_PHASE_EXPLICIT; ph <= MPH_PHASE_MULTIGRID; ++ph)
         for (mph_scene* S : act)
             if ((rc = run_phase(*S, ph))) return rc;
     if ((rc = sync_all(act))) return rc;
+    const auto t2 = std::chrono::steady_clock::now();
     std::vector<mph_scene*> plain, mix;
     for (mph_scene* S : act) (mixed(*S) ? mix : plain).push_back(S);
     std::vector<char> badPlain, badMix;
@@ -574,6 +635,7 @@ int mph_batch_step(mph_batch* B, mph_step_stats* stats) {
     if (!mix.empty() && (rc = solve_mixed(*B, mix, badMix))) return rc;
     if ((rc = check_launch())) return rc;
     HIPCHK(hipStreamSynchronize(B->st));
+    const auto t3 = std::chrono::steady_clock::now();
     std::vector<char> bad(act.size(), 0);  // per active member: its solve failed
     std::string badNames;
     for (size_t k = 0; k < act.size(); ++k) {
@@ -596,9 +658,19 @@ int mph_batch_step(mph_batch* B, mph_step_stats* stats) {
         if ((rc = run_phase(S, MPH_PHASE_ADVANCE))) return rc;
     }
     if ((rc = sync_all(act))) return rc;
-    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    const auto t4 = std::chrono::steady_clock::now();
+    auto dms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    const double ms = dms(t0, t4);
     if (stats) {
-        for (int k = 0; k < n; ++k) stats[k] = mph_step_stats{};
+        for (int k = 0; k < n; ++k) {
+            stats[k] = mph_step_stats{};
+            stats[k].ms_phase[MPH_PHASE_NEIGHBOR] = dms(t0, t1);
+            stats[k].ms_phase[MPH_PHASE_MULTIGRID] = dms(t1, t2);
+            stats[k].ms_phase[MPH_PHASE_SOLVE] = dms(t2, t3);
+            stats[k].ms_phase[MPH_PHASE_ADVANCE] = dms(t3, t4);
+        }
         for (size_t k = 0; k < act.size(); ++k) {
             mph_step_stats& st = stats[idx[k]];
             const mph_scene& S = *act[k];
@@ -631,6 +703,9 @@ int mph_batch_destroy(mph_batch* B) {
     B->off.release();
     B->scal.release();
     if (B->hscal) (void)hipHostFree(B->hscal);
+    if (B->hctl) (void)hipHostFree(B->hctl);
+    for (hipEvent_t& e : B->ev)
+        if (e) (void)hipEventDestroy(e);
     B->tune64.release();
     B->tune32.release();
     B->tuneFuse.release();

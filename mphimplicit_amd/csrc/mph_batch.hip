@@ -12,25 +12,13 @@ namespace {
 
 using namespace body;
 
-__device__ inline bool in_set(const SceneSet& m, int s) { return (m.w[s >> 6] >> (s & 63)) & 1ull; }
-
-// scene of workgroup b: the last s with o[s] <= b (o[0] = 0, o[ns] = total)
-__device__ inline int find_scene(const int* __restrict__ o, int ns, int b) {
-    int lo = 0, hi = ns;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (o[mid] <= b) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 #define MPH_SCENE(row)                                                     \
     const int* o_ = off + (long long)(row) * (ns + 1);                     \
     const int s = find_scene(o_, ns, blockIdx.x);                          \
     if (!in_set(on, s)) return;                                            \
     const int lb = blockIdx.x - o_[s];                                     \
-    const BScene& B = tab[s];
+    const BScene& B = tab[s];                                              \
+    if (halted(B)) return;
 
 __global__ void kb_fill(const BScene* tab, const int* off, int ns, SceneSet on, int role, double v) {
     MPH_SCENE(kOffElem)
@@ -124,7 +112,42 @@ __global__ void __launch_bounds__(256) kb_dot_final(const BScene* tab, int ns, S
     const int s = blockIdx.x;
     if (s >= ns || !in_set(on, s)) return;
     const BScene& B = tab[s];
+    if (halted(B)) return;
     dot_final_body(B.part, nblk(B.P.nf, 256), B.scal, slot, shift_slot, sh);
+}
+
+// r.r and the CR stop test of each scene (k_dot_final_stop of mph_solver.hip)
+__global__ void __launch_bounds__(256) kb_dot_final_stop(const BScene* tab, int ns, SceneSet on, int step) {
+    __shared__ double sh[256];
+    const int s = blockIdx.x;
+    if (s >= ns || !in_set(on, s)) return;
+    const BScene& B = tab[s];
+    if (halted(B)) return;
+    dot_final_body(B.part, nblk(B.P.nf, 256), B.scal, kRR, -1, sh);
+    if (threadIdx.x != 0) return;
+    const double rr = B.scal[kRR];
+    const int it = B.ctl[kCtlIter] + step;
+    int halt = 0, bad = 0;
+    if (it >= B.maxit) halt = 1;
+    else if (!isfinite(rr)) halt = bad = 1;
+    else if (rr / B.scal[kRR0] < B.tol) halt = 1;
+    B.ctl[kCtlIter] = it;
+    B.ctl[kCtlBad] = bad;
+    B.ctl[kCtlHalt] = halt;
+    B.hctl[kCtlIter] = it;
+    B.hctl[kCtlBad] = bad;
+    B.hctl[kCtlHalt] = halt;
+}
+
+// the control block of each scene: cleared before the solve, its stop flag lowered
+// after the loop (release)
+__global__ void kb_ctl(const BScene* tab, int ns, SceneSet on, int release) {
+    const int s = blockIdx.x;
+    if (s >= ns || !in_set(on, s) || threadIdx.x >= kNCtl) return;
+    const BScene& B = tab[s];
+    if (!B.ctl) return;
+    if (!release || threadIdx.x == kCtlHalt) B.ctl[threadIdx.x] = 0;
+    if (!release) B.hctl[threadIdx.x] = 0;
 }
 
 __global__ void kb_pq_update(const BScene* tab, const int* off, int ns, SceneSet on) {
@@ -246,6 +269,7 @@ __global__ void __launch_bounds__(1024) kb_mg_coarse(const BScene* tab, int ns, 
     const int s = blockIdx.x;
     if (s >= ns || !in_set(on, s)) return;
     const BScene& B = tab[s];
+    if (halted(B)) return;
     unsigned cur = 0;
 #pragma unroll
     for (int l = 0; l < kMaxLevels; ++l) cur |= (in_set(cb.lv[l], s) ? 1u : 0u) << l;
@@ -298,10 +322,19 @@ void spmv(const Tables& T, const SceneSet& on, const SceneSet& first, bool f32, 
     }
 #undef MPH_SPMV
 }
-void spmv_fac(const Tables& T, const SceneSet& on, const SceneSet& first, bool f32, double alpha, int x, double beta,
-              int yin, int y, int y2, int p_copy, int dot) {
+void spmv_fac(const Tables& T, const SceneSet& all, const SceneSet& first, const SceneSet& cells, bool f32,
+              double alpha, int x, double beta, int yin, int y, int y2, int p_copy, int dot) {
+    SceneSet on = all, staged{};
+    for (int k = 0; k < kMaxScenes / 64; ++k) staged.w[k] = f32 ? 0 : all.w[k] & cells.w[k];
     if (f32) MPH_LAUNCH(kb_fac_t<float>, kOffPart, x);
     else MPH_LAUNCH(kb_fac_t<double>, kOffPart, x);
+    if (any(staged)) {  // the cell-staged pass, then its dot partials
+        spmv_fac_cells(T, staged, first, alpha, x, beta, yin, y, y2, p_copy);
+        on = staged;
+        if (dot >= 0) MPH_LAUNCH(kb_dot_partial, kOffChunk, dot, y);
+        on = all;
+        for (int k = 0; k < kMaxScenes / 64; ++k) on.w[k] &= ~staged.w[k];
+    }
 #define MPH_FAC(NEG, VT) MPH_LAUNCH((kb_fac_y<NEG, VT>), kOffChunk, first, x, beta, yin, y, y2, p_copy, dot)
     if (f32) {
         if (alpha < 0) MPH_FAC(true, float);
@@ -316,6 +349,16 @@ void dot_partial(const Tables& T, const SceneSet& on, int a, int b) { MPH_LAUNCH
 void dot_final(const Tables& T, const SceneSet& on, int slot, int shift_slot) {
     if (T.ns > 0 && any(on))
         hipLaunchKernelGGL(kb_dot_final, dim3(T.ns), dim3(256), 0, T.st, T.tab, T.ns, on, slot, shift_slot);
+}
+void dot_final_stop(const Tables& T, const SceneSet& on, int step) {
+    if (T.ns > 0 && any(on))
+        hipLaunchKernelGGL(kb_dot_final_stop, dim3(T.ns), dim3(256), 0, T.st, T.tab, T.ns, on, step);
+}
+void ctl_reset(const Tables& T, const SceneSet& on) {
+    if (T.ns > 0 && any(on)) hipLaunchKernelGGL(kb_ctl, dim3(T.ns), dim3(64), 0, T.st, T.tab, T.ns, on, 0);
+}
+void ctl_release(const Tables& T, const SceneSet& on) {
+    if (T.ns > 0 && any(on)) hipLaunchKernelGGL(kb_ctl, dim3(T.ns), dim3(64), 0, T.st, T.tab, T.ns, on, 1);
 }
 void pq_update(const Tables& T, const SceneSet& on) {
     const int g = T.blocks(kOffElem);

@@ -44,7 +44,13 @@ struct DevParams {
     double gravity[3];
     double density[4], bulk[4], bulkVisc[4], shear[4];
     double ratio[4][4];  // InteractionRatio
+    // the CR stop flag: the kernels of a CR iteration return at once while *halt is
+    // nonzero (null outside the CR loop, and in the solve with the host stop test)
+    const int* halt;
 };
+
+// a CR iteration kernel's entry test (uniform: the whole grid returns or none of it)
+__device__ inline bool cr_halted(const int* halt) { return halt && *halt; }
 
 // orders the LDS writes and reads of the lanes of one wavefront: the entry-parallel
 // walks hand terms to a group's first lane, which adds them in order (the lanes of a

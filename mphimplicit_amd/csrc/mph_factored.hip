@@ -36,6 +36,7 @@
 //
 // Scenes whose multigrid levels allow it assemble no 3x3 blocks at all: level 0
 // and InvDiagA come from the same scalars (k_fac_g, k_fac_gal0, k_fac_invdiag).
+#include "mph_batch.h"
 #include "mph_cr_body.h"
 
 namespace mph {
@@ -475,6 +476,7 @@ __global__ void __launch_bounds__(256) k_fac_invdiag(DevParams P, FacOp F, doubl
 // form has no displacements to wrap
 template <class VT>
 __global__ void __launch_bounds__(256) k_fac_t(DevParams P, FacOp F, const double* __restrict__ x) {
+    if (cr_halted(P.halt)) return;
     if constexpr (sizeof(VT) == 4) body::fac_t32_chunk(P, F, x, blockIdx.x);
     else if (*F.wrap) body::fac_t_chunk<true>(P, F, x, blockIdx.x);
     else body::fac_t_chunk<false>(P, F, x, blockIdx.x);
@@ -484,6 +486,7 @@ template <bool kNeg, class VT>
 __global__ void __launch_bounds__(256) k_fac_y(DevParams P, FacOp F, const double* __restrict__ x, double beta,
                                                const double* yin, double* y, double* y2, double* p_copy,
                                                const double* __restrict__ dotx, double* __restrict__ part) {
+    if (cr_halted(P.halt)) return;
     __shared__ double sh[256];
     __shared__ double rows[3 * 256];
     if constexpr (sizeof(VT) == 4)
@@ -545,15 +548,15 @@ struct FacCellShared {  // the product's LDS: the table and the records
     dv2c rec[3 * kCellSlots];
 };
 
-// Workgroup b's row cell (cell (x, y, z) of the fluid bounding box, z fastest): its
+// Workgroup b's row cell (b counts the scene's workgroups of the launch; cell
+// (x, y, z) of the fluid bounding box, z fastest): its
 // fluid rows [rs, rs + nrows) and, in T, the visited cells in the neighbour search's
 // order (ascending CellId, so their fluid ranges ascend too) with their LDS offsets,
 // filled by wavefront 0.  False (for the whole workgroup) if the cell has no fluid.
 // Ends with a workgroup barrier when it returns true.
 __device__ inline bool fac_cell_table(const DevParams& P, const FacOp& F, FacCellTable& T, int& rs, int& nrows,
-                                      int& nv) {
+                                      int& nv, int b) {
     const int t = threadIdx.x;
-    int b = blockIdx.x;
     if (F.cellXcd) {
         // workgroups are dealt to the 8 XCDs in turn: XCD x = b mod 8 takes a contiguous
         // range of cells instead, so that the cells running on one XCD at a time are
@@ -617,7 +620,7 @@ __device__ inline int fac_cell_slot(const FacCellTable& T, int j) {
 __global__ void __launch_bounds__(kCellThreads) k_fac_slots(DevParams P, FacOp F) {
     __shared__ FacCellTable T;
     int rs, nrows, nv;
-    if (!fac_cell_table(P, F, T, rs, nrows, nv)) return;
+    if (!fac_cell_table(P, F, T, rs, nrows, nv, blockIdx.x)) return;
     if (T.total > kCellSlots) return;
     constexpr int G = body::kFacLanes;
     for (int r = threadIdx.x / G; r < nrows; r += kCellThreads / G) {
@@ -771,9 +774,10 @@ __device__ inline void fac_y_cell_rows(const DevParams& P, const FacOp& F, const
 
 template <bool kNeg, bool kWrap>
 __device__ inline void fac_y_cell_body(const DevParams& P, const FacOp& F, const double* __restrict__ x, double beta,
-                                       const double* yin, double* y, double* y2, double* p_copy, FacCellShared& L) {
+                                       const double* yin, double* y, double* y2, double* p_copy, FacCellShared& L,
+                                       int b) {
     int rs, nrows, nv;
-    if (!fac_cell_table(P, F, L.tab, rs, nrows, nv)) return;  // the whole workgroup: no fluid in this cell
+    if (!fac_cell_table(P, F, L.tab, rs, nrows, nv, b)) return;  // the whole workgroup: no fluid in this cell
     if (L.tab.total > kCellSlots) {  // too many records to stage: gather them from global memory
         fac_y_cell_rows<kNeg, kWrap, false, false>(P, F, x, beta, yin, y, y2, p_copy, L, rs, nrows);
         return;
@@ -804,9 +808,40 @@ template <bool kNeg>
 __global__ void __launch_bounds__(kCellThreads) k_fac_y_cell(DevParams P, FacOp F, const double* __restrict__ x,
                                                              double beta, const double* yin, double* y, double* y2,
                                                              double* p_copy) {
+    if (cr_halted(P.halt)) return;
     __shared__ FacCellShared L;
-    if (*F.wrap) fac_y_cell_body<kNeg, true>(P, F, x, beta, yin, y, y2, p_copy, L);
-    else fac_y_cell_body<kNeg, false>(P, F, x, beta, yin, y, y2, p_copy, L);
+    if (*F.wrap) fac_y_cell_body<kNeg, true>(P, F, x, beta, yin, y, y2, p_copy, L, blockIdx.x);
+    else fac_y_cell_body<kNeg, false>(P, F, x, beta, yin, y, y2, p_copy, L, blockIdx.x);
+}
+
+// The same pass for the scenes of a batch (the scene-table launches of mph_batch.hip):
+// workgroup b belongs to the scene s with R.pre[s] <= b < R.pre[s+1] and runs that
+// scene's cell b - R.pre[s] with its FP64 factored operator.  The grid covers the
+// scenes of the launch's set only (converged scenes have empty ranges).  The scenes
+// of `first` also write y2 and p_copy.
+template <bool kNeg>
+__global__ void __launch_bounds__(kCellThreads) kb_fac_y_cell(const batch::BScene* __restrict__ tab, batch::CellRanges R, int ns,
+                                                              batch::SceneSet first, int x, double beta, int yin, int y,
+                                                              int y2, int p_copy) {
+    int lo = 0, hi = ns;  // the last s with R.pre[s] <= b
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (R.pre[mid] <= (int)blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const int s = lo, lb = (int)blockIdx.x - R.pre[lo];
+    const batch::BScene& B = tab[s];
+    if (batch::halted(B)) return;
+    __shared__ FacCellShared L;
+    // the scene's parameters as values: uniform loads into scalar registers (read
+    // through the table, the compiler keeps them in vector registers and spills)
+    const DevParams P = B.P;
+    const FacOp F = B.fac[0];
+    const bool f = batch::in_set(first, s);
+    double* y2p = f && y2 >= 0 ? B.vec[y2] : nullptr;
+    double* pc = f && p_copy >= 0 ? B.vec[p_copy] : nullptr;
+    if (*F.wrap) fac_y_cell_body<kNeg, true>(P, F, B.vec[x], beta, B.vec[yin], B.vec[y], y2p, pc, L, lb);
+    else fac_y_cell_body<kNeg, false>(P, F, B.vec[x], beta, B.vec[yin], B.vec[y], y2p, pc, L, lb);
 }
 
 template <bool kNeg, class VT>
@@ -872,6 +907,27 @@ void launch_fac_slots(hipStream_t st, const DevParams& P, const FacOp& F) {
     if (P.nf == 0 || !F.cellStart || !F.slot || cells <= 0) return;
     hipLaunchKernelGGL(k_fac_slots, dim3((unsigned)cells), dim3(kCellThreads), 0, st, P, F);
 }
+
+namespace batch {
+void spmv_fac_cells(const Tables& T, const SceneSet& on, const SceneSet& first, double alpha, int x, double beta,
+                    int yin, int y, int y2, int p_copy) {
+    CellRanges R;
+    const int* o = T.hoff + (size_t)kOffCell * (T.ns + 1);
+    int g = 0;
+    for (int s = 0; s < T.ns; ++s) {
+        R.pre[s] = g;
+        if ((on.w[s >> 6] >> (s & 63)) & 1ull) g += o[s + 1] - o[s];
+    }
+    R.pre[T.ns] = g;
+    if (g <= 0) return;
+    if (alpha < 0)
+        hipLaunchKernelGGL((kb_fac_y_cell<true>), dim3(g), dim3(kCellThreads), 0, T.st, T.tab, R, T.ns, first, x, beta,
+                           yin, y, y2, p_copy);
+    else
+        hipLaunchKernelGGL((kb_fac_y_cell<false>), dim3(g), dim3(kCellThreads), 0, T.st, T.tab, R, T.ns, first, x,
+                           beta, yin, y, y2, p_copy);
+}
+}  // namespace batch
 
 void launch_fac_spmv(hipStream_t st, const DevParams& P, const FacOp& F, double alpha, const double* x, double beta,
                      const double* yin, double* y, double* y2, double* p_copy, const double* dotx, double* part,

@@ -43,6 +43,7 @@ struct Level {  // geometry + storage of one multigrid level
     float* A32;    // FP32 copy of A for the mixed-precision inner solve (or null)
     int f32;       // smoothers read A32 instead of A
     int lanes;     // sweeps split each row over lanes (small levels; set by the host)
+    const int* halt;  // the CR stop flag, as DevParams::halt
 };
 
 // levels with fewer rows than this split their sweeps' rows over lanes by default
@@ -211,8 +212,9 @@ void launch_mg_restrict(hipStream_t st, const Level& S, Level& L);
 void launch_mg_prolong(hipStream_t st, const Level& L, Level& S);
 // levels lc..D-1 of the V-cycle in one workgroup; lvs = device copy of the levels,
 // cur = their ping-pong bits (bit l, lc-1 included).  Afterwards every level in
-// lc..D-2 has flipped (5 sweeps) and the coarsest has not (4 sweeps).
-void launch_mg_coarse(hipStream_t st, const Level* lvs, int lc, int D, unsigned cur, bool f32);
+// lc..D-2 has flipped (5 sweeps) and the coarsest has not (4 sweeps).  halt: the CR
+// stop flag (DevParams::halt), or null.
+void launch_mg_coarse(hipStream_t st, const Level* lvs, int lc, int D, unsigned cur, bool f32, const int* halt);
 void launch_mg_g2p(hipStream_t st, const DevParams& P, const Level& L0, const int* cellIndex,
                    const double* invDiagA, const double* q, double* u, double* part);
 // ---- conjugate residual ----
@@ -255,6 +257,18 @@ void launch_dot_partial(hipStream_t st, const DevParams& P, const double* x, con
 // host: optional coherent pinned host memory that also receives the value
 void launch_dot_final(hipStream_t st, const DevParams& P, const double* part, double* scal, int slot, int shift_slot,
                       double* host = nullptr);
+// The CR loop's control block int[kNCtl] in device memory: the stop flag the
+// iteration kernels test (DevParams::halt points at ctl[kCtlHalt]), the iterations
+// done and a non-finite flag.
+enum { kCtlHalt = 0, kCtlIter = 1, kCtlBad = 2, kNCtl = 4 };
+// the r.r reduction of the CR loop (scal[2]) followed by the loop's stop test
+// (main.cpp:4344-4346) on the device: ctl[kCtlIter] += step; then the loop stops
+// (ctl[kCtlHalt] = 1) once ctl[kCtlIter] >= maxit, or on a residual that is not
+// finite (also ctl[kCtlBad] = 1), or when rr/rr0 < tol, tested in that order.  A
+// no-op once halted.  hctl and hrr: optional coherent pinned host memory that also
+// receives ctl[0..2] and rr.
+void launch_dot_final_stop(hipStream_t st, const DevParams& P, const double* part, double* scal, int* ctl, int step,
+                           double tol, long long maxit, int* hctl, double* hrr);
 void launch_pq_update(hipStream_t st, const DevParams& P, const double* scal, const double* s, const double* y,
                       double* p, double* q);
 void launch_xrs_update(hipStream_t st, const DevParams& P, const double* scal, const double* p, const double* q,

@@ -8,6 +8,7 @@
 // stop rule rr/rr0 < tol of source/main.cpp:4346).
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -176,6 +177,10 @@ int init_constants(mph_scene& S, const mph_grid& g) {
 bool mixed(const mph_scene& S) { return S.prm.precision == MPH_PRECISION_MIXED; }
 bool factored(const mph_scene& S) { return S.prm.operator_form == MPH_OPERATOR_FACTORED; }
 
+bool fac_cells(const mph_scene& S) {
+    return S.facCells == 1 || (S.facCells == 2 && (S.P.dim == 3 || S.P.nf <= kFacCellsSmall));
+}
+
 bool blocks_wanted(const mph_scene& S) {
     if (!factored(S)) return true;
     if (!S.prm.multigrid) return false;  // nothing but the solve would read them
@@ -223,9 +228,30 @@ int phase_cells(mph_scene& S) {
     return MPH_OK;
 }
 
-// scratch of the one-scan neighbour search, per scene (a scene in a batch keeps the
-// two passes: the serving loop budgets device memory per scene without it)
+// scratch of the one-scan neighbour search, per scene
 constexpr size_t kSearchScratchMax = size_t(8) << 30;
+
+// The one-scan search's scratch (rows of capM and capP entries per particle, from
+// the last search's largest counts plus a margin), allocated if need be.  False
+// when there is no count yet, the scratch would exceed kSearchScratchMax, or the
+// allocation failed: the scratch is optional, the fill pass then builds the lists.
+bool search_scratch(mph_scene& S, int& capM, int& capP) {
+    if (S.hmaxN <= 0) return false;
+    capM = (S.hmaxN + 16 + 3) & ~3;
+    capP = (S.hmaxNP + 8 + 3) & ~3;
+    const size_t np = (size_t)S.P.np;
+    if (np * (size_t)(capM + capP) * sizeof(int) > kSearchScratchMax) return false;
+    if (S.scrM.ensure(np * capM) == MPH_OK && S.scrP.ensure(np * capP) == MPH_OK && S.searchOvf.ensure(1) == MPH_OK)
+        return true;
+    // A failed hipMalloc leaves its error as the thread's last error, which the launch
+    // check after the phase would report; clear it, and give back what was allocated
+    // so that the rest of the step has the memory
+    (void)hipGetLastError();
+    S.scrM.release();
+    S.scrP.release();
+    g_err.clear();
+    return false;
+}
 
 int phase_neighbor_launch(mph_scene& S) {
     const DevParams& P = S.P;
@@ -233,25 +259,11 @@ int phase_neighbor_launch(mph_scene& S) {
     // one scan writes the lists into scratch when the last search's largest counts
     // (plus a margin) bound them; else the count pass is followed by the fill pass
     S.searchCapM = S.searchCapP = 0;
-    if (S.batches == 0 && S.hmaxN > 0) {
-        const int capM = (S.hmaxN + 16 + 3) & ~3, capP = (S.hmaxNP + 8 + 3) & ~3;
-        if ((size_t)P.np * (size_t)(capM + capP) * sizeof(int) <= kSearchScratchMax) {
-            if (S.scrM.ensure((size_t)P.np * capM) == MPH_OK && S.scrP.ensure((size_t)P.np * capP) == MPH_OK &&
-                S.searchOvf.ensure(1) == MPH_OK) {
-                S.searchCapM = capM;
-                S.searchCapP = capP;
-                launch_fill_i32(S.st, S.searchOvf.p, 1, 0);
-            } else {
-                // the scratch is optional: without it the fill pass builds the lists.  A
-                // failed hipMalloc leaves its error as the thread's last error, which the
-                // launch check after the phase would report; clear it, and give back
-                // what was allocated so that the rest of the step has the memory
-                (void)hipGetLastError();
-                S.scrM.release();
-                S.scrP.release();
-                g_err.clear();
-            }
-        }
+    int capM = 0, capP = 0;
+    if (search_scratch(S, capM, capP)) {
+        S.searchCapM = capM;
+        S.searchCapP = capP;
+        launch_fill_i32(S.st, S.searchOvf.p, 1, 0);
     }
     const bool scr = S.searchCapM > 0;
     launch_neighbor_count(S.st, P, S.pos.p, S.cellIndex.p, S.cellStart.p, S.nCount.p, S.nFluid.p, S.nCountP.p,
@@ -355,8 +367,9 @@ int phase_matrix_c(mph_scene& S) {
                      S.runP.p, S.selfC.p, S.frec.p);
         if (mixed(S))  // alpha was written by the matrix-A phase
             launch_fac_u(S.st, S.P, S.nFluid.p, S.nPtr.p, S.nInd.p, S.falpha.p, S.pos.p, S.fu32.p);
-        // the staged places of the viscous entries for the cell-staged FP64 pass
-        if (S.facCells && S.batches == 0) {
+        // the staged places of the viscous entries for the cell-staged FP64 pass (a
+        // scene alone and in a batch)
+        if (fac_cells(S)) {
             if ((rc = S.fslot.ensure(S.E > 0 ? S.E : 1))) return rc;
             S.fslotValid = true;
             launch_fac_slots(S.st, S.P, fac_op(S, false));
@@ -494,7 +507,7 @@ void precondition(mph_scene& S, const double* in, double* out, double* part) {
     if (F < D) {
         unsigned cur = 0;
         for (int l = F - 1; l < D; ++l) cur |= (unsigned)S.lv[l].L.cur << l;
-        launch_mg_coarse(st, S.lvDev.p, F, D, cur, S.lv[0].L.f32 != 0);
+        launch_mg_coarse(st, S.lvDev.p, F, D, cur, S.lv[0].L.f32 != 0, S.P.halt);
         for (int l = F; l < D - 1; ++l) S.lv[l].L.cur ^= 1;  // five sweeps each; the coarsest four
     } else {
         Level& L = S.lv[D - 1].L;
@@ -582,12 +595,13 @@ FacOp fac_op(mph_scene& S, bool f32) {
     F.v32 = f32 ? S.fv32.p : nullptr;
     F.rec32 = f32 ? S.frec32.p : nullptr;
     // the cell-staged second pass over the fluid bounding box (max x,y,z then min)
-    F.cellStart = S.facCells ? S.cellStart.p : nullptr;
-    F.slot = S.facCells && S.fslot.p && S.fslotValid ? S.fslot.p : nullptr;
+    const bool cells = fac_cells(S);
+    F.cellStart = cells ? S.cellStart.p : nullptr;
+    F.slot = cells && S.fslot.p && S.fslotValid ? S.fslot.p : nullptr;
     F.cellXcd = S.facCellXcd ? 1 : 0;
     for (int d = 0; d < 3; ++d) {
         F.cellLo[d] = S.hbbox[3 + d];
-        F.cellCnt[d] = S.facCells ? S.hbbox[d] - S.hbbox[3 + d] + 1 : 0;
+        F.cellCnt[d] = cells ? S.hbbox[d] - S.hbbox[3 + d] + 1 : 0;
     }
     return F;
 }
@@ -684,9 +698,11 @@ void collect_spmv_timing(mph_scene& S) {
 // meets the stop rule: rr/rr0 < tol is false for NaN, so the reference iterates to
 // its cap of 3*Nf (main.cpp:4344-4346).  The solve fails at once instead; the scene
 // state is then unspecified.
+void set_halt(mph_scene& S, const int* halt);
 int solve_nonfinite(mph_scene& S, const char* what, double v) {
     S.facPref = nullptr;  // a first pass enqueued ahead is never used
     S.facPrefEv = -1;
+    set_halt(S, nullptr);
     (void)hipStreamSynchronize(S.st);
     if (S.spmvTiming) collect_spmv_timing(S);
     collect_tuning(S);
@@ -695,11 +711,151 @@ int solve_nonfinite(mph_scene& S, const char* what, double v) {
     return fail(MPH_E_STATE, msg);
 }
 
+// the CR iteration kernels test the stop flag halt (null: they run unconditionally)
+void set_halt(mph_scene& S, const int* halt) {
+    S.P.halt = halt;
+    for (int l = 0; l < S.depth; ++l) S.lv[l].L.halt = halt;
+}
+
+// The loop body of solveWithConjugatedGradient (4352-4378): y = A s (the first
+// iteration also p = s, q = y), rho = s.y, p and q updated (later iterations),
+// u = M^-1 q, tmp = q.u, x, r and s updated, rr = r.r.  dev: rr goes to the
+// device-side stop test.
+int cr_iteration(mph_scene& S, bool first, bool dev, long long maxit) {
+    const DevParams& P = S.P;
+    hipStream_t st = S.st;
+    int rc;
+    if (first) {  // y = A s; p = s; q = A p (same sum); rho = s.y
+        spmv(S, 1.0, S.s.p, 0.0, S.y.p, S.y.p, S.q.p, S.p.p, S.s.p, S.part.p);
+        launch_dot_final(st, P, S.part.p, S.scal.p, kRHO, kRHOP);
+    } else {
+        spmv(S, 1.0, S.s.p, 0.0, S.y.p, S.y.p, nullptr, nullptr, S.s.p, S.part.p);
+        launch_dot_final(st, P, S.part.p, S.scal.p, kRHO, kRHOP);
+        launch_pq_update(st, P, S.scal.p, S.s.p, S.y.p, S.p.p, S.q.p);
+    }
+    if ((rc = apply_m(S, S.q.p, S.u.p, S.part.p))) return rc;  // u = M^-1 q; tmp = q.u
+    launch_dot_final(st, P, S.part.p, S.scal.p, kTMP, -1);
+    launch_xrs_update(st, P, S.scal.p, S.p.p, S.q.p, S.u.p, S.x.p, S.r.p, S.s.p, S.part.p);
+    if (dev) launch_dot_final_stop(st, P, S.part.p, S.scal.p, S.ctl.p, 1, S.prm.cr_tolerance, maxit, S.hctl, S.hrr);
+    else launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1, rr_host(S));
+    return MPH_OK;
+}
+
+// One later iteration (not the first) captured into the executable graph of slot k:
+// an update of the existing one where HIP accepts it, else a new instantiation.
+// Nothing runs; the host's ping-pong state advances as if it had.
+hipError_t capture_iteration(mph_scene& S, int k, long long maxit) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(S.st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    const int rc = cr_iteration(S, false, true, maxit);
+    e = hipStreamEndCapture(S.st, &g);
+    if (e == hipSuccess && rc) e = hipErrorUnknown;
+    if (e == hipSuccess && S.crExec[k]) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateSuccess;
+        if (hipGraphExecUpdate(S.crExec[k], g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(S.crExec[k]);
+            S.crExec[k] = nullptr;
+        }
+    }
+    if (e == hipSuccess && !S.crExec[k]) {
+        e = hipGraphInstantiate(&S.crExec[k], g, nullptr, nullptr, 0);
+        if (e != hipSuccess) S.crExec[k] = nullptr;
+        else S.crInstantiations++;
+    }
+    if (g) (void)hipGraphDestroy(g);
+    if (e == hipSuccess) S.crCaptures++;
+    return e;
+}
+
+// the graphs hold fixed launches: no variant exploration or per-launch timing
+bool graphs_allowed(const mph_scene& S) {
+    return S.crStop >= 2 && !S.spmvTiming && (!S.prm.multigrid || !S.tuneFuse.exploring()) &&
+           (factored(S) || !S.tuneSpmv64.exploring());
+}
+
+// The FP64 CR loop with the stop test on the device.  The prologue's stop test has
+// completed (the stream is idle).  Iteration k+1 is enqueued before the host waits
+// for iteration k; once the host reads the stop, the one iteration still queued
+// returns at once in every kernel.  Iterations 1, 2, ... replay two captured graphs
+// (the V-cycle's ping-pong parity alternates per iteration) when graphs_allowed.
+int cr_loop_device(mph_scene& S, long long maxit) {
+    const volatile int* hc = S.hctl;
+    if (hc[kCtlHalt]) return MPH_OK;
+    for (hipEvent_t& e : S.crEv)
+        if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    int rc;
+    if ((rc = cr_iteration(S, true, true, maxit))) return rc;
+    HIPCHK(hipEventRecord(S.crEv[0], S.st));
+    bool graphs = graphs_allowed(S);
+    if (graphs) {  // captured while the GPU runs iteration 0
+        std::vector<int> cur0(S.depth);
+        for (int l = 0; l < S.depth; ++l) cur0[l] = S.lv[l].L.cur;
+        const auto t0 = std::chrono::steady_clock::now();
+        hipError_t e = capture_iteration(S, 0, maxit);
+        if (e == hipSuccess) e = capture_iteration(S, 1, maxit);
+        S.crCaptureMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (e != hipSuccess) {
+            S.crGraphErr = (int)e;
+            (void)hipGetLastError();
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(S.st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+                hipGraph_t g = nullptr;  // end a capture left open by a failed launch
+                (void)hipStreamEndCapture(S.st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                (void)hipGetLastError();
+            }
+            // nothing captured ran: the eager iterations below continue from the
+            // ping-pong state before the captures
+            for (int l = 0; l < S.depth; ++l) S.lv[l].L.cur = cur0[l];
+            graphs = false;
+        }
+    }
+    // the variant trials of the eager iterations are read as their iterations complete
+    // (mark: how many trials each autotuner had recorded at an iteration's event)
+    Autotune* tn[] = {&S.tuneSpmv64, &S.tuneSpmv32, &S.tuneCac, &S.tuneFuse, &S.tuneGal};
+    constexpr int kTn = sizeof(tn) / sizeof(tn[0]);
+    size_t mark[2][kTn];
+    for (int t = 0; t < kTn; ++t) mark[0][t] = tn[t]->pend.size();
+    for (long long k = 1;; ++k) {
+        if (graphs) {
+            HIPCHK(hipGraphLaunch(S.crExec[(k - 1) & 1], S.st));
+            S.crGraphLaunches++;
+        } else if ((rc = cr_iteration(S, false, true, maxit))) {
+            return rc;
+        }
+        HIPCHK(hipEventRecord(S.crEv[k & 1], S.st));
+        for (int t = 0; t < kTn; ++t) mark[k & 1][t] = tn[t]->pend.size();
+        HIPCHK(hipEventSynchronize(S.crEv[(k - 1) & 1]));
+        for (int t = 0; t < kTn; ++t) {
+            const size_t n = mark[(k - 1) & 1][t];
+            tn[t]->collect(n);
+            mark[k & 1][t] -= n;
+        }
+        if (hc[kCtlHalt]) break;
+    }
+    HIPCHK(hipStreamSynchronize(S.st));  // hctl final
+    return MPH_OK;
+}
+
 // solveWithConjugatedGradient (4270-4421)
 int phase_solve(mph_scene& S) {
     const DevParams& P = S.P;
     hipStream_t st = S.st;
     const long long n = 3LL * P.nf;
+    int rc;
+    const bool dev = S.crStop > 0;
+    struct HaltOff {  // no kernel outside this solve tests the flag, on any return
+        mph_scene& S;
+        ~HaltOff() { set_halt(S, nullptr); }
+    } haltOff{S};
+    if (dev) {  // the control block starts at zero; every CR kernel tests its flag
+        if ((rc = S.ctl.ensure(kNCtl))) return rc;
+        launch_fill_i32(st, S.ctl.p, kNCtl, 0);
+        set_halt(S, S.ctl.p);
+    }
     launch_cr_init(st, P, S.vel.p, S.force.p, S.mass.p, S.x.p);
     for (double* v : {S.r.p, S.s.p, S.y.p, S.p.p, S.q.p, S.u.p}) launch_fill(st, v, n, -1.0);
     if (S.prm.multigrid)
@@ -708,7 +864,6 @@ int phase_solve(mph_scene& S) {
             for (double* v : {L.S[0], L.S[1], L.R, L.Q}) launch_fill(st, v, L.nrow, -1.0);
             L.cur = 0;
         }
-    int rc;
     // s = M^-1 b; rs0 = b.s; rr0 = b.b
     if ((rc = apply_m(S, S.b.p, S.s.p, S.part.p))) return rc;
     launch_dot_final(st, P, S.part.p, S.scal.p, kRS0, -1);
@@ -718,31 +873,29 @@ int phase_solve(mph_scene& S) {
     spmv(S, -1.0, S.x.p, 1.0, S.b.p, S.r.p, nullptr, nullptr, nullptr, nullptr);
     if ((rc = apply_m(S, S.r.p, S.s.p, nullptr))) return rc;
     launch_dot_partial(st, P, S.r.p, S.r.p, S.part.p);
-    launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1, rr_host(S));
+    const long long maxit = S.prm.max_iterations > 0 ? S.prm.max_iterations : n;
+    if (dev) launch_dot_final_stop(st, P, S.part.p, S.scal.p, S.ctl.p, 0, S.prm.cr_tolerance, maxit, S.hctl, S.hrr);
+    else launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1, rr_host(S));
     if ((rc = read_scalar(S, kRR0, &S.rr0))) return rc;
     if (!std::isfinite(S.rr0)) return solve_nonfinite(S, "b.b", S.rr0);
-    const long long maxit = S.prm.max_iterations > 0 ? S.prm.max_iterations : n;
     long long iter;
-    for (iter = 0; iter < maxit; ++iter) {
-        if ((rc = read_scalar_prefetch(S, kRR, &S.rr, S.s.p, false))) return rc;  // both products below are on s
-        if (S.spmvTiming) collect_spmv_timing(S);
-        if (!std::isfinite(S.rr)) return solve_nonfinite(S, "r.r", S.rr);
-        if (S.rr / S.rr0 < S.prm.cr_tolerance) break;
-        if (iter == 0) {  // y = A s; p = s; q = A p (same sum); rho = s.y
-            spmv(S, 1.0, S.s.p, 0.0, S.y.p, S.y.p, S.q.p, S.p.p, S.s.p, S.part.p);
-            launch_dot_final(st, P, S.part.p, S.scal.p, kRHO, kRHOP);
-        } else {
-            spmv(S, 1.0, S.s.p, 0.0, S.y.p, S.y.p, nullptr, nullptr, S.s.p, S.part.p);
-            launch_dot_final(st, P, S.part.p, S.scal.p, kRHO, kRHOP);
-            launch_pq_update(st, P, S.scal.p, S.s.p, S.y.p, S.p.p, S.q.p);
+    if (dev) {
+        if ((rc = cr_loop_device(S, maxit))) return rc;
+        set_halt(S, nullptr);  // the epilogue runs after the stop
+        const volatile int* hc = S.hctl;
+        iter = hc[kCtlIter];
+        if (hc[kCtlBad]) return solve_nonfinite(S, "r.r", S.hrr[0]);
+    } else {
+        for (iter = 0; iter < maxit; ++iter) {
+            if ((rc = read_scalar_prefetch(S, kRR, &S.rr, S.s.p, false))) return rc;  // both products below are on s
+            if (S.spmvTiming) collect_spmv_timing(S);
+            if (!std::isfinite(S.rr)) return solve_nonfinite(S, "r.r", S.rr);
+            if (S.rr / S.rr0 < S.prm.cr_tolerance) break;
+            if ((rc = cr_iteration(S, iter == 0, false, maxit))) return rc;
         }
-        if ((rc = apply_m(S, S.q.p, S.u.p, S.part.p))) return rc;  // u = M^-1 q; tmp = q.u
-        launch_dot_final(st, P, S.part.p, S.scal.p, kTMP, -1);
-        launch_xrs_update(st, P, S.scal.p, S.p.p, S.q.p, S.u.p, S.x.p, S.r.p, S.s.p, S.part.p);
-        launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1, rr_host(S));
+        S.facPref = nullptr;  // a first pass enqueued ahead of the stop is not used
+        S.facPrefEv = -1;
     }
-    S.facPref = nullptr;  // a first pass enqueued ahead of the stop is not used
-    S.facPrefEv = -1;
     launch_dot_partial(st, P, S.r.p, S.s.p, S.part.p);
     launch_dot_final(st, P, S.part.p, S.scal.p, kRS, -1);
     launch_store_velocity(st, P, S.x.p, S.vel.p);
@@ -928,6 +1081,9 @@ int allocate(mph_scene& S) {
 #undef ENS
     HIPCHK(hipHostMalloc((void**)&S.hpin, 64 * sizeof(double), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&S.hrr, 8 * sizeof(double), hipHostMallocCoherent));
+    static_assert(kNCtl * sizeof(int) <= 4 * sizeof(double), "the CR control mirror fits hrr[4..7]");
+    S.hctl = (int*)(S.hrr + 4);
+    for (int k = 0; k < kNCtl; ++k) S.hctl[k] = 0;
     // zero-fill the per-particle state the reference allocates uninitialised
     for (DBuf<double>* d : {&S.force, &S.mass, &S.kappa, &S.lambda, &S.mu, &S.densA, &S.gcen, &S.pressA,
                             &S.volStrainP, &S.divP, &S.pressP, &S.virialP, &S.virialStress, &S.b, &S.vecP})
@@ -940,8 +1096,12 @@ void destroy(mph_scene* S) {
     if (S->st) (void)hipStreamSynchronize(S->st);
     for (DBuf<int>* b : {&S->type, &S->type2, &S->perm, &S->perm2, &S->cellIndex, &S->cellStart, &S->bbox,
                          &S->nCount, &S->nFluid, &S->nCountP, &S->nInd, &S->nIndP, &S->width, &S->selfIdx, &S->acol,
-                         &S->maxP, &S->runP, &S->fwrap, &S->fgmask, &S->scrM, &S->scrP, &S->searchOvf})
+                         &S->maxP, &S->runP, &S->fwrap, &S->fgmask, &S->scrM, &S->scrP, &S->searchOvf, &S->ctl})
         b->release();
+    for (hipGraphExec_t& g : S->crExec)
+        if (g) (void)hipGraphExecDestroy(g);
+    for (hipEvent_t& e : S->crEv)
+        if (e) (void)hipEventDestroy(e);
     S->fgtab.release();
     S->runStart.release();
     for (DBuf<double>* b : {&S->falpha, &S->fdiag, &S->selfC, &S->fbeta, &S->frec}) b->release();
@@ -1262,6 +1422,10 @@ int mph_scene_create(const mph_params* p, const mph_grid* g, int device, mph_sce
         destroy(S);
         return fail(MPH_E_DEVICE, "init sequence failed");
     }
+    // the first step's one-scan search sizes its scratch by these lists: allocate it
+    // now, so that the scene's device bytes include it from the start
+    int capM = 0, capP = 0;
+    (void)search_scratch(*S, capM, capP);
     S->iStep = (int)(S->time / S->P.dt);
     S->tuneSpmv64.init(kSpmvVariants64, 3);
     S->tuneSpmv32.init(kSpmvVariants32, 3);
@@ -1436,7 +1600,7 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
         const bool f64 = n == "FactoredBytes";
         const double ab = f64 ? 8.0 : 12.0, vb = f64 ? 8.0 : 12.0, recb = f64 ? 32.0 : 16.0;
         // the cell-staged FP64 pass reads the 2-byte place instead of the column
-        const double colb = f64 && S->facCells && S->fslotValid ? 2.0 : 4.0;
+        const double colb = f64 && fac_cells(*S) && S->fslotValid ? 2.0 : 4.0;
         const double posb = f64 ? 24.0 : 0.0;
         std::vector<int> nfl, ncp, run;
         int rc;
@@ -1462,6 +1626,11 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
         for (int k : c) t += k;
         *v = t;
     }
+    else if (n == "CrGraphLaunches") *v = (double)S->crGraphLaunches;
+    else if (n == "CrCaptures") *v = (double)S->crCaptures;
+    else if (n == "CrInstantiations") *v = (double)S->crInstantiations;
+    else if (n == "CrCaptureMs") *v = S->crCaptureMs;
+    else if (n == "CrGraphError") *v = S->crGraphErr;
     else if (n == "CRIterations") *v = S->crIter;
     else if (n == "rr0") *v = S->rr0;
     else if (n == "rr") *v = S->rr;
@@ -1499,13 +1668,18 @@ int mph_scene_set_option(mph_scene* S, const char* name, double value) {
         return MPH_OK;
     }
     if (n == "FactoredCells") {
-        if (value != 0.0 && value != 1.0) return fail(MPH_E_ARG, "FactoredCells is 0 or 1");
-        S->facCells = value != 0.0;
+        if (value != 0.0 && value != 1.0 && value != 2.0) return fail(MPH_E_ARG, "FactoredCells is 0, 1 or 2");
+        S->facCells = (int)value;
         return MPH_OK;
     }
     if (n == "FactoredCellsXcd") {
         if (value != 0.0 && value != 1.0) return fail(MPH_E_ARG, "FactoredCellsXcd is 0 or 1");
         S->facCellXcd = value != 0.0;
+        return MPH_OK;
+    }
+    if (n == "CrStop") {
+        if (value != 0.0 && value != 1.0 && value != 2.0) return fail(MPH_E_ARG, "CrStop is 0, 1 or 2");
+        S->crStop = (int)value;
         return MPH_OK;
     }
     return fail(MPH_E_FIELD, "unknown option " + n);

@@ -107,9 +107,12 @@ struct Autotune {
         if (!exploring() || pend.empty()) return;
         (void)hipEventRecord(pend.back().b, st);
     }
-    // after the stream has been synchronised past the recorded launches
-    void collect() {
-        for (Pend& p : pend) {
+    // the first n recorded trials (all by default), after the stream has been
+    // synchronised past their launches; later ones stay pending
+    void collect(size_t n = (size_t)-1) {
+        if (n > pend.size()) n = pend.size();
+        for (size_t k = 0; k < n; ++k) {
+            const Pend& p = pend[k];
             float ms = 0;
             if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess && p.v >= 0 && p.v < nvar) {
                 if (cnt[p.v] == 0 || ms < fastest[p.v]) fastest[p.v] = ms;
@@ -118,7 +121,7 @@ struct Autotune {
             pool.push_back(p.a);
             pool.push_back(p.b);
         }
-        pend.clear();
+        pend.erase(pend.begin(), pend.begin() + n);
         if (best >= 0) return;
         for (int k = 0; k < nvar; ++k)
             if (cnt[k] < trials) return;
@@ -183,7 +186,13 @@ struct mph_scene {
     bool blocks = true;  // this step assembles the 3x3 blocks (see blocks_wanted)
     mph::host::DBuf<double> falpha, fdiag, selfC, fbeta, frec;  // FacOp::alpha, diag, selfC, beta, rec
     mph::host::DBuf<float> fu32, fv32, frec32;  // FP32 form (mixed precision): FacOp::u32, v32, rec32
-    bool facCells = true;  // FP64 second pass staged per cell in LDS (else the chunk kernel)
+    // FP64 second pass staged per cell in LDS: 1 always, 0 never (the chunk kernel),
+    // 2 (default) in 3D scenes and in scenes of at most kFacCellsSmall fluid
+    // particles.  Measured per product, 2D 130k-188k fluid: chunk 76-88 us, cells
+    // 174-241 us (a 2D cell holds ~12 rows, a 512-lane workgroup stages 9 cells for
+    // them); 3D 166k-176k: chunk 269-282 us, cells 189-207 us; 2.2k-2.8k fluid, 2D and
+    // 3D: chunk 41-68 us, cells 23-27 us.
+    int facCells = 2;
     bool facCellXcd = true;  // FacOp::cellXcd
     mph::host::DBuf<unsigned short> fslot;  // FacOp::slot
     bool fslotValid = false;                // fslot matches the current lists
@@ -220,6 +229,22 @@ struct mph_scene {
     int facPrefEv = -1;
     hipEvent_t rrEv = nullptr;
     double* hrr = nullptr;  // coherent pinned host memory: rr as the last kRR reduction wrote it
+    // The FP64 CR loop with its stop test on the device (crStop 1 or 2): the iteration
+    // kernels return at once after the stop (DevParams::halt = ctl), the host enqueues
+    // iteration k+1 before it waits for iteration k and reads hctl (int[kNCtl] inside
+    // hrr's pinned block).  crStop 2 replays iterations 1, 2, ... from two captured
+    // graphs, one per ping-pong parity of the V-cycle, captured anew every solve and
+    // applied to the executable graphs by an update where the topology allows.
+    // Measured without a profiler, the solve of a 64k-fluid scene takes 15.6 ms with
+    // crStop 0, 15.1 ms with 1 and 15.8 ms with 2 (1M: 103.3, 103.9, 104.4 ms).
+    int crStop = 1;  // 0: the host tests rr every iteration
+    mph::host::DBuf<int> ctl;
+    int* hctl = nullptr;
+    hipEvent_t crEv[2] = {nullptr, nullptr};
+    hipGraphExec_t crExec[2] = {nullptr, nullptr};
+    long long crGraphLaunches = 0, crCaptures = 0, crInstantiations = 0;
+    double crCaptureMs = 0;  // host time capturing and updating or instantiating
+    int crGraphErr = 0;      // the last capture error (the solve then ran its iterations eagerly)
     double spmvMs = 0, spmvMs32 = 0;
     long long spmvLaunches = 0, spmvLaunches32 = 0;
     int spmvTimingErr = 0;
@@ -235,6 +260,8 @@ namespace host {
 
 bool mixed(const mph_scene& S);
 bool factored(const mph_scene& S);
+constexpr int kFacCellsSmall = 16384;
+bool fac_cells(const mph_scene& S);  // the FP64 second pass is the cell-staged one (facCells)
 // whether the step assembles the 3x3 blocks: always for the assembled form; for the
 // factored form only when the multigrid's level 0 cannot be built from the factored
 // scalars (a level-0 grid under 3 cells along an active axis)

@@ -234,6 +234,7 @@ __global__ void k_mg_p2g(DevParams P, Level L, const int* __restrict__ cellStart
 constexpr int kP2gLanes = 8, kP2gK = 8;
 __global__ void __launch_bounds__(256) k_mg_p2g_bu_lanes(DevParams P, Level L, const int* __restrict__ cellStart,
                                                          const double* __restrict__ q) {
+    if (cr_halted(L.halt)) return;
     __shared__ double tb[256 * kP2gK];
     const int u = threadIdx.x & (kP2gLanes - 1);
     const int row = (int)(((long long)blockIdx.x * 256 + threadIdx.x) / kP2gLanes);
@@ -274,6 +275,7 @@ __global__ void __launch_bounds__(256) k_mg_p2g_bu_lanes(DevParams P, Level L, c
 
 // restriction into L, then begin + update of L on the same row
 __global__ void k_mg_restrict_bu(Level S, Level L) {
+    if (cr_halted(L.halt)) return;
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= L.nrow) return;
     mg_restrict_row(S, L, row);
@@ -292,12 +294,14 @@ __global__ void k_mg_update(Level L) {
 
 template <class VT>
 __global__ void k_mg_residual(Level L) {
+    if (cr_halted(L.halt)) return;
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row < L.nrow) mg_residual_row<VT>(L, row);
 }
 
 template <class VT>
 __global__ void k_mg_residual_update(Level L) {
+    if (cr_halted(L.halt)) return;
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row < L.nrow) mg_residual_update_row<VT>(L, row);
 }
@@ -315,6 +319,7 @@ constexpr int kRowLanes = 4, kRowBlock = 128, kRowQ = 7;  // kRowQ: stencil cell
 
 template <class VT, bool kUpdate, int C0, int C1, int C2>
 __global__ void __launch_bounds__(kRowBlock) k_mg_sweep_lanes(Level L) {
+    if (cr_halted(L.halt)) return;
     constexpr int T = 3 * kRowQ;
     __shared__ double tb[kRowBlock * T];
     const int u = threadIdx.x & (kRowLanes - 1);
@@ -393,19 +398,22 @@ __global__ void k_mg_restrict(Level S, Level L) {
 }
 
 __global__ void k_mg_prolong(Level L, Level S) {
+    if (cr_halted(L.halt)) return;
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row < S.nrow) mg_prolong_row(L, S, row);
 }
 
 // cur: ping-pong bits of levels lc-1..D-1 (bit l), tracked by the host
 __global__ void __launch_bounds__(1024) k_mg_coarse(const Level* __restrict__ lvs, int lc, int D, unsigned cur,
-                                                   int f32) {
+                                                   int f32, const int* halt) {
+    if (cr_halted(halt)) return;
     mg_coarse_cycle(lvs, lc, D, cur, f32 != 0);
 }
 
 __global__ void __launch_bounds__(256) k_mg_g2p(DevParams P, Level L, const int* __restrict__ cellIndex,
                                                 const double* __restrict__ invDiagA, const double* __restrict__ q,
                                                 double* __restrict__ u, double* __restrict__ part) {
+    if (cr_halted(L.halt)) return;
     __shared__ double sh[256];
     mg_g2p_chunk(P, L, cellIndex, invDiagA, q, u, part, blockIdx.x, sh);
 }
@@ -425,25 +433,56 @@ __global__ void __launch_bounds__(256) k_spmv(int nf, const int* __restrict__ nF
                                               const VT* __restrict__ val, const double* __restrict__ x,
                                               double beta, const double* yin, double* y, double* y2,
                                               double* p_copy, const double* __restrict__ dotx,
-                                              double* __restrict__ part) {
+                                              double* __restrict__ part, const int* halt) {
+    if (cr_halted(halt)) return;
     __shared__ double sh[256];
     spmv_chunk<kNeg, VT, kUn>(nf, nFluid, start, col, val, x, beta, yin, y, y2, p_copy, dotx, part, blockIdx.x, sh);
 }
 
 __global__ void __launch_bounds__(256) k_dot_partial(int nf, const double* __restrict__ x,
-                                                     const double* __restrict__ y, double* __restrict__ part) {
+                                                     const double* __restrict__ y, double* __restrict__ part,
+                                                     const int* halt) {
+    if (cr_halted(halt)) return;
     __shared__ double sh[256];
     dot_partial_chunk(nf, x, y, part, blockIdx.x, sh);
 }
 
 __global__ void __launch_bounds__(256) k_dot_final(const double* __restrict__ part, int nchunks, double* scal,
-                                                   int slot, int shift_slot, double* host) {
+                                                   int slot, int shift_slot, double* host, const int* halt) {
+    if (cr_halted(halt)) return;
     __shared__ double sh[256];
     dot_final_body(part, nchunks, scal, slot, shift_slot, sh, host);
 }
 
+// rr and the CR stop test (launch_dot_final_stop); ctl[kCtlHalt] is read by every
+// thread before thread 0 writes it, behind the reduction's barriers
+__global__ void __launch_bounds__(256) k_dot_final_stop(const double* __restrict__ part, int nchunks, double* scal,
+                                                        int* ctl, int step, double tol, long long maxit, int* hctl,
+                                                        double* hrr) {
+    if (ctl[kCtlHalt]) return;
+    __shared__ double sh[256];
+    dot_final_body(part, nchunks, scal, kRR, -1, sh, hrr);
+    if (threadIdx.x != 0) return;
+    const double rr = scal[kRR];
+    const int it = ctl[kCtlIter] + step;
+    int halt = 0, bad = 0;
+    if (it >= maxit) halt = 1;
+    else if (!isfinite(rr)) halt = bad = 1;
+    else if (rr / scal[kRR0] < tol) halt = 1;
+    ctl[kCtlIter] = it;
+    ctl[kCtlBad] = bad;
+    ctl[kCtlHalt] = halt;
+    if (hctl) {  // coherent pinned host memory: the host polls it between iterations
+        hctl[kCtlIter] = it;
+        hctl[kCtlBad] = bad;
+        hctl[kCtlHalt] = halt;
+    }
+}
+
 __global__ void k_pq_update(int nf, const double* __restrict__ scal, const double* __restrict__ s,
-                            const double* __restrict__ y, double* __restrict__ p, double* __restrict__ q) {
+                            const double* __restrict__ y, double* __restrict__ p, double* __restrict__ q,
+                            const int* halt) {
+    if (cr_halted(halt)) return;
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < 3LL * nf) pq_update_elem(scal, s, y, p, q, k);
 }
@@ -452,7 +491,8 @@ __global__ void __launch_bounds__(256) k_xrs_update(int nf, const double* __rest
                                                     const double* __restrict__ p, const double* __restrict__ q,
                                                     const double* __restrict__ u, double* __restrict__ x,
                                                     double* __restrict__ r, double* __restrict__ s,
-                                                    double* __restrict__ part) {
+                                                    double* __restrict__ part, const int* halt) {
+    if (cr_halted(halt)) return;
     __shared__ double sh[256];
     xrs_update_chunk(nf, scal, p, q, u, x, r, s, part, blockIdx.x, sh);
 }
@@ -706,8 +746,8 @@ void launch_mg_restrict(hipStream_t st, const Level& S, Level& L) {
 void launch_mg_prolong(hipStream_t st, const Level& L, Level& S) {
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(S.nrow, 256)), dim3(256), 0, st, L, S);
 }
-void launch_mg_coarse(hipStream_t st, const Level* lvs, int lc, int D, unsigned cur, bool f32) {
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, st, lvs, lc, D, cur, f32 ? 1 : 0);
+void launch_mg_coarse(hipStream_t st, const Level* lvs, int lc, int D, unsigned cur, bool f32, const int* halt) {
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, st, lvs, lc, D, cur, f32 ? 1 : 0, halt);
 }
 void launch_mg_g2p(hipStream_t st, const DevParams& P, const Level& L0, const int* cellIndex,
                    const double* invDiagA, const double* q, double* u, double* part) {
@@ -722,7 +762,7 @@ static void spmv_launch(hipStream_t st, const DevParams& P, const ASlices& A, co
                         const double* x, double beta, const double* yin, double* y, double* y2, double* p_copy,
                         const double* dotx, double* part) {
     hipLaunchKernelGGL((k_spmv<kNeg, VT, kUn>), dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P.nf, nFluid, A.start,
-                       A.col, val, x, beta, yin, y, y2, p_copy, dotx, part);
+                       A.col, val, x, beta, yin, y, y2, p_copy, dotx, part, P.halt);
 }
 
 void launch_spmv(hipStream_t st, const DevParams& P, ASlices A, const int* nFluid, double alpha, const double* x,
@@ -766,21 +806,26 @@ void launch_axpy(hipStream_t st, long long n, const double* x, double* y) {
     if (n > 0) hipLaunchKernelGGL(k_axpy1_, dim3(nblk(n, 256)), dim3(256), 0, st, n, x, y);
 }
 void launch_dot_partial(hipStream_t st, const DevParams& P, const double* x, const double* y, double* part) {
-    hipLaunchKernelGGL(k_dot_partial, dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P.nf, x, y, part);
+    hipLaunchKernelGGL(k_dot_partial, dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P.nf, x, y, part, P.halt);
 }
 void launch_dot_final(hipStream_t st, const DevParams& P, const double* part, double* scal, int slot,
                       int shift_slot, double* host) {
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, st, part, nblk(P.nf, kChunk), scal, slot, shift_slot,
-                       host);
+                       host, P.halt);
+}
+void launch_dot_final_stop(hipStream_t st, const DevParams& P, const double* part, double* scal, int* ctl, int step,
+                           double tol, long long maxit, int* hctl, double* hrr) {
+    hipLaunchKernelGGL(k_dot_final_stop, dim3(1), dim3(256), 0, st, part, nblk(P.nf, kChunk), scal, ctl, step, tol,
+                       maxit, hctl, hrr);
 }
 void launch_pq_update(hipStream_t st, const DevParams& P, const double* scal, const double* s, const double* y,
                       double* p, double* q) {
-    hipLaunchKernelGGL(k_pq_update, dim3(nblk(3LL * P.nf, 256)), dim3(256), 0, st, P.nf, scal, s, y, p, q);
+    hipLaunchKernelGGL(k_pq_update, dim3(nblk(3LL * P.nf, 256)), dim3(256), 0, st, P.nf, scal, s, y, p, q, P.halt);
 }
 void launch_xrs_update(hipStream_t st, const DevParams& P, const double* scal, const double* p, const double* q,
                        const double* u, double* x, double* r, double* s, double* part) {
     hipLaunchKernelGGL(k_xrs_update, dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P.nf, scal, p, q, u, x, r, s,
-                       part);
+                       part, P.halt);
 }
 void launch_store_velocity(hipStream_t st, const DevParams& P, const double* x, double* vel) {
     hipLaunchKernelGGL(k_store_velocity, dim3(nblk(3LL * P.nf, 256)), dim3(256), 0, st, P.nf, x, vel);

@@ -69,6 +69,14 @@ def test_batch_equals_alone(precision, kind):
             b.add(s)
         assert len(b) == len(MIX)
         hist = [b.step() for _ in range(steps)]
+    # members build their lists with the one-scan search too (scratch sized at creation)
+    assert all(s.scalar("NeighborOneScan") == 1.0 for s in members)
+    # the batch's wall time per group of phases, the same in every member's record
+    for st in hist:
+        split = [st[0].ms_phase[k] for k in (abi.PHASE_NEIGHBOR, abi.PHASE_MULTIGRID, abi.PHASE_SOLVE,
+                                             abi.PHASE_ADVANCE)]
+        assert all(v > 0 for v in split) and sum(split) <= st[0].ms_total * (1 + 1e-9)
+        assert all(list(x.ms_phase) == list(st[0].ms_phase) for x in st)
     for k, name in enumerate(MIX):
         ref, rstats = alone(name, precision, steps, form=forms[k])
         for t in range(steps):
