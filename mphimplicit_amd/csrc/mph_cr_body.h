@@ -621,6 +621,17 @@ __device__ inline double group_sum(double v, int G) {
     return v;
 }
 
+// a row's sum s plus component r of its diagonal term, D_r . x_i + C_ii,r t_i, with
+// the fused multiply-adds written out: every kernel of the factored product that
+// forms it (whichever lane does) gets the same bits
+__device__ inline double fac_diag_add(double s, double dr0, double dr1, double dr2, double x0, double x1, double x2,
+                                      double c, double t) {
+    double d = __builtin_fma(dr0, x0, dr1 * x1);
+    d = __builtin_fma(dr2, x2, d);
+    d = __builtin_fma(c, t, d);
+    return s + d;
+}
+
 // one entry of FacOp::u32 / v32: three floats and the column
 struct FacEntry32 {
     float a0, a1, a2;
@@ -818,15 +829,9 @@ __device__ inline void fac_y_chunk(const DevParams& P, const FacOp& F, const dou
             const double* D = F.diag + 9 * i;
             const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
             const double ti = rt[i];
-            double d0 = (D[0] * xi0 + D[1] * xi1) + D[2] * xi2;
-            double d1 = (D[3] * xi0 + D[4] * xi1) + D[5] * xi2;
-            double d2 = (D[6] * xi0 + D[7] * xi1) + D[8] * xi2;
-            d0 += F.selfC[3 * i] * ti;
-            d1 += F.selfC[3 * i + 1] * ti;
-            d2 += F.selfC[3 * i + 2] * ti;
-            s0 += d0;
-            s1 += d1;
-            s2 += d2;
+            s0 = fac_diag_add(s0, D[0], D[1], D[2], xi0, xi1, xi2, F.selfC[3 * i], ti);
+            s1 = fac_diag_add(s1, D[3], D[4], D[5], xi0, xi1, xi2, F.selfC[3 * i + 1], ti);
+            s2 = fac_diag_add(s2, D[6], D[7], D[8], xi0, xi1, xi2, F.selfC[3 * i + 2], ti);
             rows[3 * r] = kNeg ? -s0 : s0;
             rows[3 * r + 1] = kNeg ? -s1 : s1;
             rows[3 * r + 2] = kNeg ? -s2 : s2;

@@ -512,28 +512,20 @@ constexpr int kCellThreads = 512;  // 64 rows of kFacLanes lanes per pass
 constexpr int kCellSlots = 1536;   // staged records of 48 B: 72 KiB, two workgroups per CU
 constexpr int kCellTable = 32;     // visited cells (27 in 3D) padded for the binary search
 
-// y (and y2, p) of row i from its sum, as spmv_store (no contraction: the separate
-// roundings of y = beta*yin + sum)
-__device__ __noinline__ void fac_cell_store(int i, double s0, double s1, double s2, const double* __restrict__ x,
-                                            double beta, const double* yin, double* y, double* y2, double* p_copy) {
-    const double sm[3] = {s0, s1, s2};
-    double out[3];
-    for (int r = 0; r < 3; ++r) {
-        double yy = yin[3 * i + r];
-        yy *= beta;
-        yy += sm[r];
-        out[r] = yy;
+// element k of y (and y2) from the row's sum sm, as spmv_store: the separate
+// roundings of y = beta*yin + sum (this function is compiled without contraction);
+// yv and y2v are yin[k] and y2[k], read before
+__device__ inline void fac_cell_store1(int k, double sm, double yv, double y2v, double beta, double* y, double* y2) {
+    double yy = yv;
+    yy *= beta;
+    yy += sm;
+    if (y2) {
+        double zz = y2v;
+        zz *= beta;
+        zz += sm;
+        y2[k] = zz;
     }
-    if (y2)
-        for (int r = 0; r < 3; ++r) {
-            double yy = y2[3 * i + r];
-            yy *= beta;
-            yy += sm[r];
-            y2[3 * i + r] = yy;
-        }
-    for (int r = 0; r < 3; ++r) y[3 * i + r] = out[r];
-    if (p_copy)
-        for (int r = 0; r < 3; ++r) p_copy[3 * i + r] = x[3 * i + r];
+    y[k] = yy;
 }
 
 typedef double dv2c __attribute__((ext_vector_type(2)));
@@ -541,7 +533,8 @@ typedef double dv2c __attribute__((ext_vector_type(2)));
 // the visited cells of a row cell: fluid ranges in visiting order and LDS offsets
 struct FacCellTable {
     int vlo[kCellTable], voff[kCellTable], vcnt[kCellTable];
-    int total;  // staged records of the visited cells
+    int vex[kCellTable];  // first staged place of each visited cell (past the last: INT_MAX)
+    int total;            // staged records of the visited cells
 };
 struct FacCellShared {  // the product's LDS: the table and the records
     FacCellTable tab;
@@ -598,6 +591,7 @@ __device__ inline bool fac_cell_table(const DevParams& P, const FacOp& F, FacCel
             T.vlo[t] = lo;
             T.vcnt[t] = cnt;
             T.voff[t] = (incl - cnt) - (t < nv ? lo : 0);
+            T.vex[t] = t < nv ? incl - cnt : 0x7fffffff;
         }
         if (t == 63) T.total = incl;
     }
@@ -650,20 +644,58 @@ __device__ inline void fac_cell_list(const FacOp& F, const double* __restrict__ 
     }
 }
 
+// the P-list entries of a batch (branch-free loads as fac_cell_list; n >= 1: self):
+// lane g takes entries jb + g + u*G; entries past the list take i with weight 0
+template <int UP>
+__device__ inline void fac_cell_plist(const FacOp& F, long long c0, int n, int i, int jb, int g, int* k, double* bb) {
+#pragma unroll
+    for (int u = 0; u < UP; ++u) {
+        const int jn = jb + g + u * body::kFacLanes;
+        const long long e = c0 + (jn < n ? jn : n - 1);
+        const int kk = F.nIndP[e];
+        const double be = F.beta[e];
+        k[u] = jn < n ? kk : i;
+        bb[u] = jn < n ? be : 0.0;
+    }
+}
+
+// what a row's walk starts from: its position, list extents and staged place (the
+// loads that only depend on the row)
+struct FacRowHead {
+    double pi0, pi1, pi2;
+    long long b0, c0;
+    int w, n, own;
+};
+
+template <bool kSlot>
+__device__ inline void fac_cell_row_head(const FacOp& F, const FacCellTable& T, int i, FacRowHead& S) {
+    S.pi0 = F.pos[3 * i];
+    S.pi1 = F.pos[3 * i + 1];
+    S.pi2 = F.pos[3 * i + 2];
+    S.b0 = F.nPtr[i];
+    S.w = F.nFluid[i];
+    S.c0 = F.nPtrP[i];
+    S.n = F.nCountP[i];
+    S.own = kSlot ? fac_cell_slot(T, i) : i;
+}
+
 // The rows of one cell.  kStaged: the fluid records of the visited cells are in LDS
 // (L.rec), read through the precomputed places (FacOp::slot) if kSlot, else through
 // a search of the table; otherwise every record is gathered from global memory.
 // None of this changes which lane takes an entry or the order in which a lane adds
 // its terms.  (Loading the next batch's list entries before the current batch's
 // records, or 8 entries per batch instead of 4, measured slower: 151.7 and 169.8
-// against 125.9 ms per step on the 1M scene.)
+// against 125.9 ms per step on the 1M scene; loading them after the records, into a
+// second buffer, changed the product by 0.3%; loading the first row's start before
+// the staging took 14 more registers and, held to four waves per SIMD, spilled:
+// 1.41 against 0.93 ms per product.)
 template <bool kNeg, bool kWrap, bool kStaged, bool kSlot>
 __device__ inline void fac_y_cell_rows(const DevParams& P, const FacOp& F, const double* __restrict__ x, double beta,
                                        const double* yin, double* y, double* y2, double* p_copy,
                                        const FacCellShared& L, int rs, int nrows) {
 #pragma clang fp contract(fast)
     typedef double dv2 __attribute__((ext_vector_type(2)));
-    constexpr int G = body::kFacLanes, U = body::kFacBatch;
+    constexpr int G = body::kFacLanes, U = body::kFacBatch, UP = body::kFacBatchP;
     const int t = threadIdx.x;
     const body::FacPlanes R = body::fac_planes(P, F.rec);
     const dv2* __restrict__ ra = (const dv2*)R.a;
@@ -679,15 +711,19 @@ __device__ inline void fac_y_cell_rows(const DevParams& P, const FacOp& F, const
         const bool on = r < nrows;
         const int i = rs + (on ? r : 0);
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        double dA = 0.0, dB = 0.0, dC = 0.0, xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, ti = 0.0, sc = 0.0, yv = 0.0, y2v = 0.0;
         if (on) {
-            const double pi0 = F.pos[3 * i], pi1 = F.pos[3 * i + 1], pi2 = F.pos[3 * i + 2];
-            // viscous part over the fluid neighbours (alpha 0: self, beyond RadiusV)
-            const long long b0 = F.nPtr[i];
-            const int w = F.nFluid[i];
-            const int own = kStaged && kSlot ? fac_cell_slot(L.tab, i) : i;
-            int j[U];
-            double a[U];
+            // the row's start; the first P batch stays in flight during the viscous part
+            FacRowHead S;
+            fac_cell_row_head<kStaged && kSlot>(F, L.tab, i, S);
+            const double pi0 = S.pi0, pi1 = S.pi1, pi2 = S.pi2;
+            const long long b0 = S.b0, c0 = S.c0;
+            const int w = S.w, n = S.n, own = S.own;
+            int j[U], k[UP];
+            double a[U], bb[UP];
             fac_cell_list<U, kStaged && kSlot>(F, alpha, b0, w, own, 0, g, j, a);
+            fac_cell_plist<UP>(F, c0, n, i, 0, g, k, bb);
+            // viscous part over the fluid neighbours (alpha 0: self, beyond RadiusV)
             for (int jb = 0; jb < w; jb += G * U) {
                 dv2 q0[U], q1[U], q2[U];
 #pragma unroll
@@ -715,22 +751,24 @@ __device__ inline void fac_y_cell_rows(const DevParams& P, const FacOp& F, const
                 }
                 if (jb + G * U < w) fac_cell_list<U, kStaged && kSlot>(F, alpha, b0, w, own, jb + G * U, g, j, a);
             }
+            // lanes 0..2 of the row: the loads of their component of the diagonal term
+            // and of y, in flight during the P part
+            if (g < 3) {
+                const double* D = F.diag + 9 * i + 3 * g;
+                dA = D[0];
+                dB = D[1];
+                dC = D[2];
+                xi0 = x[3 * i];
+                xi1 = x[3 * i + 1];
+                xi2 = x[3 * i + 2];
+                ti = rt[i];
+                sc = F.selfC[3 * i + g];
+                yv = yin[3 * i + g];
+                if (y2) y2v = y2[3 * i + g];
+            }
             // C Lambda C^T part over the P neighbours (beta 0: self)
-            const long long c0 = F.nPtrP[i];
-            const int n = F.nCountP[i];
-            for (int jb = 0; jb < n; jb += G * body::kFacBatchP) {
-                constexpr int UP = body::kFacBatchP;
-                int k[UP];
-                double bb[UP];
-#pragma unroll
-                for (int u = 0; u < UP; ++u) {  // branch-free loads, as fac_cell_list (n >= 1: self)
-                    const int jn = jb + g + u * G;
-                    const long long e = c0 + (jn < n ? jn : n - 1);
-                    const int kk = F.nIndP[e];
-                    const double be = F.beta[e];
-                    k[u] = jn < n ? kk : i;
-                    bb[u] = jn < n ? be : 0.0;
-                }
+            for (int jb = 0; jb < n; jb += G * UP) {
+                if (jb > 0) fac_cell_plist<UP>(F, c0, n, i, jb, g, k, bb);
                 dv2 q1[UP], q2[UP];
                 double q3[UP];
 #pragma unroll
@@ -751,23 +789,13 @@ __device__ inline void fac_y_cell_rows(const DevParams& P, const FacOp& F, const
                 }
             }
         }
-        s0 = body::group_sum(s0, G);
+        s0 = body::group_sum(s0, G);  // every lane of the group holds the sums
         s1 = body::group_sum(s1, G);
         s2 = body::group_sum(s2, G);
-        if (g == 0 && on) {
-            const double* D = F.diag + 9 * i;
-            const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
-            const double ti = rt[i];
-            double d0 = (D[0] * xi0 + D[1] * xi1) + D[2] * xi2;
-            double d1 = (D[3] * xi0 + D[4] * xi1) + D[5] * xi2;
-            double d2 = (D[6] * xi0 + D[7] * xi1) + D[8] * xi2;
-            d0 += F.selfC[3 * i] * ti;
-            d1 += F.selfC[3 * i + 1] * ti;
-            d2 += F.selfC[3 * i + 2] * ti;
-            s0 += d0;
-            s1 += d1;
-            s2 += d2;
-            fac_cell_store(i, kNeg ? -s0 : s0, kNeg ? -s1 : s1, kNeg ? -s2 : s2, x, beta, yin, y, y2, p_copy);
+        if (g < 3 && on) {  // lane g: component g, as body::fac_y_chunk computes it
+            const double sg = body::fac_diag_add(g == 0 ? s0 : g == 1 ? s1 : s2, dA, dB, dC, xi0, xi1, xi2, sc, ti);
+            fac_cell_store1(3 * i + g, kNeg ? -sg : sg, yv, y2v, beta, y, y2);
+            if (p_copy) p_copy[3 * i + g] = g == 0 ? xi0 : g == 1 ? xi1 : xi2;
         }
     }
 }
@@ -783,19 +811,39 @@ __device__ inline void fac_y_cell_body(const DevParams& P, const FacOp& F, const
         return;
     }
     {
-        // wavefront w copies visited cells w, w + 8, ...; lane l particles lo + l, ...
+        // thread t stages places t, t + kCellThreads, ...: every load is issued before
+        // the first store, so the staging waits for one round trip to memory.  Place r
+        // holds the record of particle vlo[v] + r - vex[v] of the last visited cell v
+        // with vex[v] <= r (empty cells share their successor's vex)
         const body::FacPlanes R = body::fac_planes(P, F.rec);
         const dv2c* __restrict__ ra = (const dv2c*)R.a;
         const dv2c* __restrict__ rb = (const dv2c*)R.b;
         const dv2c* __restrict__ rc = (const dv2c*)R.c;
-        const int t = threadIdx.x;
-        for (int v = t >> 6; v < nv; v += kCellThreads / 64) {
-            const int lo = L.tab.vlo[v], cnt = L.tab.vcnt[v], base = L.tab.voff[v] + lo;
-            for (int k = t & 63; k < cnt; k += 64) {
-                const dv2c qa = ra[lo + k], qb = rb[lo + k], qc = rc[lo + k];
-                L.rec[3 * (base + k)] = qa;
-                L.rec[3 * (base + k) + 1] = qb;
-                L.rec[3 * (base + k) + 2] = qc;
+        constexpr int kRounds = kCellSlots / kCellThreads;
+        static_assert(kRounds * kCellThreads == kCellSlots, "the staging rounds cover the slots");
+        const int t = threadIdx.x, total = L.tab.total;
+        dv2c qa[kRounds], qb[kRounds], qc[kRounds];
+#pragma unroll
+        for (int k = 0; k < kRounds; ++k) {
+            const int r = t + k * kCellThreads;
+            if (r < total) {
+                int v = 0;
+#pragma unroll
+                for (int step = kCellTable / 2; step >= 1; step >>= 1)
+                    if (L.tab.vex[v + step] <= r) v += step;
+                const int j = L.tab.vlo[v] + (r - L.tab.vex[v]);
+                qa[k] = ra[j];
+                qb[k] = rb[j];
+                qc[k] = rc[j];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kRounds; ++k) {
+            const int r = t + k * kCellThreads;
+            if (r < total) {
+                L.rec[3 * r] = qa[k];
+                L.rec[3 * r + 1] = qb[k];
+                L.rec[3 * r + 2] = qc[k];
             }
         }
     }
