@@ -226,7 +226,10 @@ def run_gpu(args, precision="fp64", form=None):
     if os.path.exists(TRAFFIC_FILE[form]):
         with open(TRAFFIC_FILE[form]) as f:
             traffic = json.load(f).get("bytes_per_launch")
-    if factored:
+    if factored and scene.scalar("FactoredCellsUsed"):
+        k64 = "k_fac_t + k_fac_y_cell (factored particle matrix, FP64 scalars, second pass staged per cell)"
+        k32 = "k_fac_t + k_fac_y (factored particle matrix, FP32 scalars)"
+    elif factored:
         k64 = "k_fac_t + k_fac_y (factored particle matrix, FP64 scalars)"
         k32 = "k_fac_t + k_fac_y (factored particle matrix, FP32 scalars)"
     else:
@@ -513,16 +516,19 @@ def run_serve(args, rate=None, seconds=None):
     seconds = args.serve_seconds if seconds is None else seconds
     count = min(args.serve_requests, int(rate * seconds * 1.5) + 10)  # enough to outlast the run
     stream = scenes.serve_stream(count, rate=rate)
-    form = abi.OPERATOR_FACTORED if form_for(args, "mixed") == "factored" else abi.OPERATOR_ASSEMBLED
-    reqs = make_requests(stream, tol=args.rel_residual ** 2, operator_form=form)
-    srv = Server(lazy_mixed=True)
+    mixed = args.serve_precision == "mixed"
+    form = abi.OPERATOR_FACTORED if form_for(args, args.serve_precision) == "factored" else abi.OPERATOR_ASSEMBLED
+    reqs = make_requests(stream, precision=abi.PRECISION_MIXED if mixed else abi.PRECISION_FP64,
+                         tol=args.rel_residual ** 2, operator_form=form)
+    srv = Server(lazy_mixed=mixed)
     try:
         r = srv.run(reqs, duration_s=seconds, log=lambda m: print(m, file=sys.stderr, flush=True))
     finally:
         srv.close()
     r["workload"] = ("Poisson arrivals at %g req/s, 1-20 steps, lognormal fluid counts (median 50k, sigma 1, "
-                     "clipped 5k-500k), 2D/3D, mixed precision, %s particle matrix, %g s run or %d requests"
-                     % (rate, form_for(args, "mixed"), seconds, count))
+                     "clipped 5k-500k), 2D/3D, %s, %s particle matrix, %g s run or %d requests"
+                     % (rate, "mixed precision" if mixed else "FP64", form_for(args, args.serve_precision),
+                        seconds, count))
     if r["fluid_particle_steps_per_s"] > 0:
         r["offered_over_served"] = r["offered_fluid_particle_steps_per_s"] / r["fluid_particle_steps_per_s"]
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
@@ -572,6 +578,8 @@ def main():
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--serve", choices=["on", "off"], default="on", help="serving leg")
+    ap.add_argument("--serve-precision", choices=["fp64", "mixed"], default="fp64",
+                    help="precision of the served scenes (FP64 steps the batch faster with the cell-staged product)")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-requests", type=int, default=10000)

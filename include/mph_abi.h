@@ -182,10 +182,11 @@ int mph_scene_run_phase(mph_scene* s, int phase);
  * step assembled the 3x3 blocks: always for the assembled form; for the factored
  * form only when its multigrid level 0 spans fewer than 3 cells along an axis,
  * otherwise level 0 and InvDiagA come from the factored scalars and CsrCofA is not
- * available), and NeighborOneScan (1 when the last neighbour search built the lists
- * in one scan, 0 when it ran the count and fill passes: a scene's first search, a
- * scene in a batch, or a list longer than the last search's largest plus a margin;
- * the lists are the same either way). */
+ * available), FactoredCellsUsed (1 when the FP64 second pass of the factored
+ * product runs staged per cell, option FactoredCells), and NeighborOneScan (1 when
+ * the last neighbour search built the lists in one scan, 0 when it ran the count and
+ * fill passes: a scene's first search, or a list longer than the last search's
+ * largest plus a margin; the lists are the same either way, alone or in a batch). */
 int mph_scene_field_size(mph_scene* s, const char* field, size_t* bytes, int* elem_bytes);
 int mph_scene_read(mph_scene* s, const char* field, void* host, size_t bytes);
 int mph_scene_write(mph_scene* s, const char* field, const void* host, size_t bytes);
@@ -215,8 +216,19 @@ int mph_write_vtk_arrays(const char* path, int n, const int* type, const double*
  * "SpmvMsF32" and "SpmvLaunchesF32" (FP32 matrix, mixed precision only). */
 int mph_scene_set_timing(mph_scene* s, int on);
 
-/* Schedule options of a scene (no counterpart in the reference); they never change a
- * result, only which of two bit-identical kernels runs.
+/* Options of a scene.  "ConvergenceCheck" is the reference's compile-time switch
+ * CONVERGENCE_CHECK (source/main.cpp:46, 4348-4351, 4383-4397):
+ *   "ConvergenceCheck"  1: each FP64 solve keeps rr and r.s of every CR iteration that
+ *                  did not stop (field "ConvergenceHistory", CRIterations rows of
+ *                  two doubles, the values of main.cpp:4350's log line; at most 2^22
+ *                  rows) and, after the loop, r.r and r.s of the true residual
+ *                  r = b - A x with s = M^-1 r, or s = b without multigrid as
+ *                  main.cpp:4391 has it (scalars "CheckRR", "CheckRS").  It adds one
+ *                  reduction per iteration and a product and a V-cycle per solve;
+ *                  the solution is unchanged.  0 (default): nothing is kept.
+ *                  MPH_E_STATE for a mixed-precision scene or a scene in a batch.
+ * The others are schedule options (no counterpart in the reference); they never
+ * change a result, only which of two bit-identical kernels runs.
  *   "LaneRowsMax"  multigrid levels with fewer rows split each row of a smoothing
  *                  sweep over 4 lanes, larger ones take one thread per row
  *                  (default 32768).  Applies from the next multigrid build.

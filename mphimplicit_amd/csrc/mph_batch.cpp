@@ -581,6 +581,9 @@ int mph_batch_add(mph_batch* B, mph_scene* S) {
         return fail(MPH_E_ARG, "scene is already in the batch");
     if ((int)B->scenes.size() >= kMaxScenes)
         return fail(MPH_E_ARG, "batch is full (" + std::to_string(kMaxScenes) + " scenes)");
+    if (S->convCheck)
+        return fail(MPH_E_STATE, "the convergence record is kept by standalone steps: turn ConvergenceCheck off "
+                                 "before adding the scene to a batch");
     B->scenes.push_back(S);
     S->batches++;
     return MPH_OK;

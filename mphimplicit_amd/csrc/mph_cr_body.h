@@ -16,7 +16,7 @@ namespace body {
 __host__ __device__ inline int nblk(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 // CR scalars: rr0, rs0, rr, rho, rho_prev, tmp (= q.u), rs
-enum { kRR0 = 0, kRS0 = 1, kRR = 2, kRHO = 3, kRHOP = 4, kTMP = 5, kRS = 6, kNScal = 8 };
+enum { kRR0 = 0, kRS0 = 1, kRR = 2, kRHO = 3, kRHOP = 4, kTMP = 5, kRS = 6, kRSIter = 7, kNScal = 8 };
 
 __device__ inline int cell_id(const DevParams& P, int cx, int cy, int cz) {
     const int* C = P.C;

@@ -269,6 +269,12 @@ enum { kCtlHalt = 0, kCtlIter = 1, kCtlBad = 2, kNCtl = 4 };
 // receives ctl[0..2] and rr.
 void launch_dot_final_stop(hipStream_t st, const DevParams& P, const double* part, double* scal, int* ctl, int step,
                            double tol, long long maxit, int* hctl, double* hrr);
+// The convergence record of a CR iteration (main.cpp:4348-4351): the partials' sum
+// r.s into scal[kRSIter], then rec[2 it] = rr (scal[kRR]) and rec[2 it + 1] = r.s
+// for it = ctl[kCtlIter] (ctl null: the host's `it`), rows it >= cap skipped.  A
+// no-op once halted.
+void launch_dot_final_record(hipStream_t st, const DevParams& P, const double* part, double* scal, const int* ctl,
+                             long long it, double* rec, long long cap);
 void launch_pq_update(hipStream_t st, const DevParams& P, const double* scal, const double* s, const double* y,
                       double* p, double* q);
 void launch_xrs_update(hipStream_t st, const DevParams& P, const double* scal, const double* p, const double* q,

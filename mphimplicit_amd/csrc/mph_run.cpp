@@ -3,6 +3,7 @@
 //
 //   mph_run data grid prof_pattern vtk_pattern log [threads] [--dim 2|3]
 //           [--rel-residual R] [--device D] [--mixed] [--factored]
+//           [--convergence-check]
 //
 // It reads the same *.data / *.grid files, runs the same time loop, writes the same
 // *.prof (every OutputInterval) and *.vtk (every VtkOutputInterval) files named by
@@ -10,7 +11,10 @@
 // step in the reference's format (main.cpp:4381).  `threads` is accepted and ignored
 // (the reference's OpenMP thread count).  --mixed and --factored select the
 // mixed-precision solve and the factored particle matrix (mph_params); without
-// them the run is the reference's arithmetic.
+// them the run is the reference's arithmetic.  --convergence-check logs what the
+// reference built with CONVERGENCE_CHECK logs: a line per CR iteration
+// (main.cpp:4350) before the summary line and the true residual's line after it
+// (main.cpp:4396).
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -52,7 +56,7 @@ double scalar(mph_scene* s, const char* n) {
 
 int main(int argc, char** argv) {
     std::vector<std::string> pos;
-    int dim = 3, device = 0, mixedp = 0, fact = 0;
+    int dim = 3, device = 0, mixedp = 0, fact = 0, check = 0;
     double relres = 0;
 #ifdef MPH_RUN_2D
     dim = 2;
@@ -62,6 +66,7 @@ int main(int argc, char** argv) {
         else if (!std::strcmp(argv[k], "--rel-residual") && k + 1 < argc) relres = std::atof(argv[++k]);
         else if (!std::strcmp(argv[k], "--device") && k + 1 < argc) device = std::atoi(argv[++k]);
         else if (!std::strcmp(argv[k], "--mixed")) mixedp = 1;
+        else if (!std::strcmp(argv[k], "--convergence-check")) check = 1;
         else if (!std::strcmp(argv[k], "--factored")) fact = 1;
         else pos.push_back(argv[k]);
     }
@@ -95,6 +100,8 @@ int main(int argc, char** argv) {
 
     mph_scene* s = nullptr;
     if (mph_scene_create(&p, &g, device, &s)) return die("creating the scene");
+    if (check && mph_scene_set_option(s, "ConvergenceCheck", 1)) return die("--convergence-check");
+    std::vector<double> hist;
     logf("N0a = %e, N0p = %e\n", scalar(s, "N0a"), scalar(s, "N0p"));
     logf("CellCount[DIM]: %d %d %d\n", (int)scalar(s, "CellCount0"), (int)scalar(s, "CellCount1"),
          (int)scalar(s, "CellCount2"));
@@ -116,8 +123,20 @@ int main(int argc, char** argv) {
         }
         mph_step_stats st;
         if (mph_scene_step(s, 1, &st)) return die("time step");
+        if (check) {  // main.cpp:4350, one line per iteration that did not stop
+            size_t bytes = 0;
+            if (mph_scene_field_size(s, "ConvergenceHistory", &bytes, nullptr)) return die("convergence record");
+            hist.resize(bytes / sizeof(double));
+            if (mph_scene_read(s, "ConvergenceHistory", hist.data(), bytes)) return die("convergence record");
+            for (size_t k = 0; 2 * k + 1 < hist.size(); ++k)
+                logf("line:%d, iter,=%d, rr0=,%e, rr=,%e, rs0=,%e, rs=,%e\n", 4350, (int)k, st.rr0, hist[2 * k],
+                     st.rs0, hist[2 * k + 1]);
+        }
         logf("line:%d, iter,=%d, rr0=,%e, rr=,%e, rs0=,%e, rs=,%e\n", 4381, st.cr_iterations, st.rr0, st.rr, st.rs0,
              st.rs);
+        if (check)  // main.cpp:4396, the true residual
+            logf("line:%d, iter,=%d, rr0=,%e, rr=,%e, rs0=,%e, rs=,%e\n", 4396, st.cr_iterations, st.rr0,
+                 scalar(s, "CheckRR"), st.rs0, scalar(s, "CheckRS"));
         if (st.vtk_steps) {
             std::snprintf(name, sizeof name, vtk.c_str(), iStep);
             if (mph_scene_write_vtk(s, name)) return die("writing vtk file");

@@ -720,11 +720,17 @@ void set_halt(mph_scene& S, const int* halt) {
 // The loop body of solveWithConjugatedGradient (4352-4378): y = A s (the first
 // iteration also p = s, q = y), rho = s.y, p and q updated (later iterations),
 // u = M^-1 q, tmp = q.u, x, r and s updated, rr = r.r.  dev: rr goes to the
-// device-side stop test.
-int cr_iteration(mph_scene& S, bool first, bool dev, long long maxit) {
+// device-side stop test.  With ConvergenceCheck the iteration first keeps rr and
+// r.s (main.cpp:4348-4351) in row `it` of the record (dev: row ctl[kCtlIter]).
+int cr_iteration(mph_scene& S, bool first, bool dev, long long maxit, long long it = 0) {
     const DevParams& P = S.P;
     hipStream_t st = S.st;
     int rc;
+    if (S.convCheck) {
+        launch_dot_partial(st, P, S.r.p, S.s.p, S.part.p);
+        launch_dot_final_record(st, P, S.part.p, S.scal.p, dev ? S.ctl.p : nullptr, it, S.convRec.p,
+                                (long long)(S.convRec.cap / 2));
+    }
     if (first) {  // y = A s; p = s; q = A p (same sum); rho = s.y
         spmv(S, 1.0, S.s.p, 0.0, S.y.p, S.y.p, S.q.p, S.p.p, S.s.p, S.part.p);
         launch_dot_final(st, P, S.part.p, S.scal.p, kRHO, kRHOP);
@@ -874,6 +880,8 @@ int phase_solve(mph_scene& S) {
     if ((rc = apply_m(S, S.r.p, S.s.p, nullptr))) return rc;
     launch_dot_partial(st, P, S.r.p, S.r.p, S.part.p);
     const long long maxit = S.prm.max_iterations > 0 ? S.prm.max_iterations : n;
+    S.convRows = 0;
+    if (S.convCheck && (rc = S.convRec.ensure(2 * (size_t)std::min(std::max(maxit, 1LL), kConvRowsMax)))) return rc;
     if (dev) launch_dot_final_stop(st, P, S.part.p, S.scal.p, S.ctl.p, 0, S.prm.cr_tolerance, maxit, S.hctl, S.hrr);
     else launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1, rr_host(S));
     if ((rc = read_scalar(S, kRR0, &S.rr0))) return rc;
@@ -891,7 +899,7 @@ int phase_solve(mph_scene& S) {
             if (S.spmvTiming) collect_spmv_timing(S);
             if (!std::isfinite(S.rr)) return solve_nonfinite(S, "r.r", S.rr);
             if (S.rr / S.rr0 < S.prm.cr_tolerance) break;
-            if ((rc = cr_iteration(S, iter == 0, false, maxit))) return rc;
+            if ((rc = cr_iteration(S, iter == 0, false, maxit, iter))) return rc;
         }
         S.facPref = nullptr;  // a first pass enqueued ahead of the stop is not used
         S.facPrefEv = -1;
@@ -899,7 +907,19 @@ int phase_solve(mph_scene& S) {
     launch_dot_partial(st, P, S.r.p, S.s.p, S.part.p);
     launch_dot_final(st, P, S.part.p, S.scal.p, kRS, -1);
     launch_store_velocity(st, P, S.x.p, S.vel.p);
-    HIPCHK(hipMemcpyAsync(S.hpin + 16, S.scal.p, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (S.convCheck) {  // main.cpp:4383-4397: r = b - A x; s = M^-1 r; r.r and r.s
+        spmv(S, -1.0, S.x.p, 1.0, S.b.p, S.r.p, nullptr, nullptr, nullptr, nullptr);
+        if (S.prm.multigrid) {
+            if ((rc = apply_m(S, S.r.p, S.s.p, nullptr))) return rc;
+        } else {  // main.cpp:4391 copies b, not r, without the multigrid solver
+            HIPCHK(hipMemcpyAsync(S.s.p, S.b.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        }
+        launch_dot_partial(st, P, S.r.p, S.r.p, S.part.p);
+        launch_dot_final(st, P, S.part.p, S.scal.p, kCheckRR, -1);
+        launch_dot_partial(st, P, S.r.p, S.s.p, S.part.p);
+        launch_dot_final(st, P, S.part.p, S.scal.p, kCheckRS, -1);
+    }
+    HIPCHK(hipMemcpyAsync(S.hpin + 16, S.scal.p, 16 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (S.spmvTiming) collect_spmv_timing(S);
     collect_tuning(S);
@@ -908,6 +928,11 @@ int phase_solve(mph_scene& S) {
     S.rs0 = S.hpin[16 + kRS0];
     S.rr = S.hpin[16 + kRR];
     S.rs = S.hpin[16 + kRS];
+    if (S.convCheck) {
+        S.convRows = std::min(iter, (long long)(S.convRec.cap / 2));
+        S.checkRR = S.hpin[16 + kCheckRR];
+        S.checkRS = S.hpin[16 + kCheckRS];
+    }
     return MPH_OK;
 }
 
@@ -1113,7 +1138,7 @@ void destroy(mph_scene* S) {
                             &S->mu, &S->densA, &S->gcen, &S->pressA, &S->volStrainP, &S->divP, &S->pressP,
                             &S->virialP, &S->virialStress, &S->aval, &S->b, &S->cval, &S->dval, &S->vecP,
                             &S->invDiagA, &S->rOut, &S->dvec, &S->x,
-                            &S->r, &S->s, &S->y, &S->p, &S->q, &S->u, &S->part, &S->scal})
+                            &S->r, &S->s, &S->y, &S->p, &S->q, &S->u, &S->part, &S->scal, &S->convRec})
         b->release();
     for (DBuf<long long>* b : {&S->nPtr, &S->nPtrP, &S->scanScratch, &S->sliceStart}) b->release();
     S->keys.release();
@@ -1197,6 +1222,8 @@ DevField device_field(mph_scene& S, const std::string& n) {
     if (n == "VectorB") return F(S.b.p, 3 * nf);
     if (n == "VectorP") return F(S.vecP.p, np);
     if (n == "InvDiagA") return F(S.invDiagA.p, S.prm.multigrid ? 3 * nf : 0);  // built by the multigrid phase only
+    // (rr, r.s) per iteration of the last FP64 solve with ConvergenceCheck on
+    if (n == "ConvergenceHistory") return F(S.convRec.p, S.convRec.p ? 2 * (size_t)S.convRows : 0);
     return DevField{nullptr, 0, 0};
 }
 
@@ -1585,6 +1612,7 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
     else if (n == "TunedCac") *v = S->tuneCac.best;
     else if (n == "TunedFuse") *v = S->tuneFuse.best;
     else if (n == "TunedGalerkin") *v = S->tuneGal.best;
+    else if (n == "FactoredCellsUsed") *v = factored(*S) && fac_cells(*S) ? 1.0 : 0.0;
     else if (n == "FactoredBytes" || n == "FactoredBytesF32") {
         // algorithmic bytes of one factored product, every array read or written once:
         // per list entry a column and the stored value (FP64: alpha or beta, 8 B;
@@ -1632,6 +1660,8 @@ int mph_scene_scalar(mph_scene* S, const char* name, double* v) {
     else if (n == "CrCaptureMs") *v = S->crCaptureMs;
     else if (n == "CrGraphError") *v = S->crGraphErr;
     else if (n == "CRIterations") *v = S->crIter;
+    else if (n == "CheckRR") *v = S->checkRR;
+    else if (n == "CheckRS") *v = S->checkRS;
     else if (n == "rr0") *v = S->rr0;
     else if (n == "rr") *v = S->rr;
     else if (n == "rs0") *v = S->rs0;
@@ -1680,6 +1710,17 @@ int mph_scene_set_option(mph_scene* S, const char* name, double value) {
     if (n == "CrStop") {
         if (value != 0.0 && value != 1.0 && value != 2.0) return fail(MPH_E_ARG, "CrStop is 0, 1 or 2");
         S->crStop = (int)value;
+        return MPH_OK;
+    }
+    if (n == "ConvergenceCheck") {
+        if (value != 0.0 && value != 1.0) return fail(MPH_E_ARG, "ConvergenceCheck is 0 or 1");
+        if (value != 0.0 && mixed(*S))
+            return fail(MPH_E_STATE, "the convergence record follows the FP64 CR loop; the scene solves in mixed "
+                                     "precision");
+        if (value != 0.0 && S->batches > 0)
+            return fail(MPH_E_STATE, "the convergence record is kept by standalone steps; the scene is in a batch");
+        S->convCheck = (int)value;
+        if (!S->convCheck) S->convRows = 0;
         return MPH_OK;
     }
     return fail(MPH_E_FIELD, "unknown option " + n);
@@ -1761,7 +1802,7 @@ int mph_scene_device_bytes(mph_scene* S, unsigned long long* bytes) {
                             &S->mu, &S->densA, &S->gcen, &S->pressA, &S->volStrainP, &S->divP, &S->pressP,
                             &S->virialP, &S->virialStress, &S->aval, &S->b, &S->cval, &S->dval, &S->vecP,
                             &S->invDiagA, &S->rOut, &S->dvec, &S->x, &S->r, &S->s, &S->y, &S->p, &S->q, &S->u,
-                            &S->part, &S->scal})
+                            &S->part, &S->scal, &S->convRec})
         n += b->cap * sizeof(double);
     for (DBuf<long long>* b : {&S->nPtr, &S->nPtrP, &S->scanScratch, &S->sliceStart}) n += b->cap * sizeof(long long);
     n += S->keys.cap * sizeof(uint64_t);

@@ -54,7 +54,7 @@ struct HostLevel {
     DBuf<float> A32;
 };
 
-enum { kRR0 = 0, kRS0 = 1, kRR = 2, kRHO = 3, kRHOP = 4, kTMP = 5, kRS = 6 };
+enum { kRR0 = 0, kRS0 = 1, kRR = 2, kRHO = 3, kRHOP = 4, kTMP = 5, kRS = 6, kRSIter = 7, kCheckRR = 8, kCheckRS = 9 };
 
 // Run-time choice among kernel variants that compute identical bits: while
 // exploring, launches cycle through the variants with HIP events around them;
@@ -245,6 +245,14 @@ struct mph_scene {
     long long crGraphLaunches = 0, crCaptures = 0, crInstantiations = 0;
     double crCaptureMs = 0;  // host time capturing and updating or instantiating
     int crGraphErr = 0;      // the last capture error (the solve then ran its iterations eagerly)
+    // CONVERGENCE_CHECK (main.cpp:46, option ConvergenceCheck): the FP64 solve keeps
+    // (rr, r.s) of every iteration that did not stop in convRec (the dot r.s adds one
+    // reduction per iteration) and, after the loop, r.r and r.s of the true residual
+    // r = b - A x with s = M^-1 r (main.cpp:4383-4397)
+    int convCheck = 0;
+    mph::host::DBuf<double> convRec;
+    long long convRows = 0;  // rows of convRec written by the last solve
+    double checkRR = 0, checkRS = 0;
     double spmvMs = 0, spmvMs32 = 0;
     long long spmvLaunches = 0, spmvLaunches32 = 0;
     int spmvTimingErr = 0;
@@ -261,6 +269,8 @@ namespace host {
 bool mixed(const mph_scene& S);
 bool factored(const mph_scene& S);
 constexpr int kFacCellsSmall = 16384;
+// rows of the convergence record (16 bytes each): iterations past this are not kept
+constexpr long long kConvRowsMax = 1LL << 22;
 bool fac_cells(const mph_scene& S);  // the FP64 second pass is the cell-staged one (facCells)
 // whether the step assembles the 3x3 blocks: always for the assembled form; for the
 // factored form only when the multigrid's level 0 cannot be built from the factored

@@ -454,6 +454,22 @@ __global__ void __launch_bounds__(256) k_dot_final(const double* __restrict__ pa
     dot_final_body(part, nchunks, scal, slot, shift_slot, sh, host);
 }
 
+// r.s of a CR iteration into scal[kRSIter] and the pair (rr, r.s) into row `it` of
+// the record (launch_dot_final_record)
+__global__ void __launch_bounds__(256) k_dot_final_record(const double* __restrict__ part, int nchunks, double* scal,
+                                                          const int* ctl, long long it_host, double* rec,
+                                                          long long cap, const int* halt) {
+    if (cr_halted(halt)) return;
+    __shared__ double sh[256];
+    dot_final_body(part, nchunks, scal, kRSIter, -1, sh);
+    if (threadIdx.x != 0) return;
+    const long long it = ctl ? (long long)ctl[kCtlIter] : it_host;
+    if (it >= 0 && it < cap) {
+        rec[2 * it] = scal[kRR];
+        rec[2 * it + 1] = scal[kRSIter];
+    }
+}
+
 // rr and the CR stop test (launch_dot_final_stop); ctl[kCtlHalt] is read by every
 // thread before thread 0 writes it, behind the reduction's barriers
 __global__ void __launch_bounds__(256) k_dot_final_stop(const double* __restrict__ part, int nchunks, double* scal,
@@ -812,6 +828,11 @@ void launch_dot_final(hipStream_t st, const DevParams& P, const double* part, do
                       int shift_slot, double* host) {
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, st, part, nblk(P.nf, kChunk), scal, slot, shift_slot,
                        host, P.halt);
+}
+void launch_dot_final_record(hipStream_t st, const DevParams& P, const double* part, double* scal, const int* ctl,
+                             long long it, double* rec, long long cap) {
+    hipLaunchKernelGGL(k_dot_final_record, dim3(1), dim3(256), 0, st, part, nblk(P.nf, kChunk), scal, ctl, it, rec,
+                       cap, P.halt);
 }
 void launch_dot_final_stop(hipStream_t st, const DevParams& P, const double* part, double* scal, int* ctl, int step,
                            double tol, long long maxit, int* hctl, double* hrr) {

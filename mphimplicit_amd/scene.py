@@ -160,7 +160,8 @@ class Scene:
         _check(lib().mph_scene_set_timing(self.h, int(on)))
 
     def set_option(self, name: str, value: float) -> None:
-        """mph_scene_set_option: a schedule choice that never changes a result."""
+        """mph_scene_set_option: a schedule choice that never changes a result, or
+        ConvergenceCheck (the reference's CONVERGENCE_CHECK record of each solve)."""
         _check(lib().mph_scene_set_option(self.h, name.encode(), float(value)))
 
     def field_bytes(self, name: str) -> int:

@@ -128,6 +128,11 @@ struct Scene {
     // CR summary
     int crIter = 0;
     double rr0 = 0, rr = 0, rs0 = 0, rs = 0;
+    // CONVERGENCE_CHECK (main.cpp:46): per iteration (rr, r.s); after the loop the
+    // true residual's r.r and r.s
+    bool convCheck = false;
+    std::vector<double> hist;
+    double checkRR = 0, checkRS = 0;
     bool vtkStep = false;
     double diag[4]{};
 };
@@ -1123,9 +1128,14 @@ void phase_solve(Scene& s) {
     M(r.data(), sv.data());
     const long maxit = s.p.max_iterations > 0 ? s.p.max_iterations : n;
     long iter;
+    s.hist.clear();
     for (iter = 0; iter < maxit; ++iter) {
         s.rr = dot(r.data(), r.data());
         if (s.rr / s.rr0 < s.p.cr_tolerance) break;
+        if (s.convCheck) {  // main.cpp:4348-4351: rs = r.s, logged with rr
+            s.hist.push_back(s.rr);
+            s.hist.push_back(dot(r.data(), sv.data()));
+        }
         spmv_a(s, 1.0, sv.data(), 0.0, y.data());
         rho = dot(sv.data(), y.data());
         if (iter == 0) {
@@ -1155,6 +1165,14 @@ void phase_solve(Scene& s) {
     s.rr = dot(r.data(), r.data());
     s.rs = dot(r.data(), sv.data());
     s.crIter = (int)iter;
+    if (s.convCheck) {  // main.cpp:4383-4397: r = b - A x, s = M^-1 r, then r.r and r.s
+        par_copy(b, n, r.data());
+        spmv_a(s, -1.0, x.data(), 1.0, r.data());
+        if (s.p.multigrid) M(r.data(), sv.data());
+        else par_copy(b, n, sv.data());  // main.cpp:4391 copies b, not r, without multigrid
+        s.checkRR = dot(r.data(), r.data());
+        s.checkRS = dot(r.data(), sv.data());
+    }
     for (int i = 0; i < nf; ++i)
         for (int d = 0; d < 3; ++d) s.vel[3 * i + d] = x[3 * i + d];
 }
@@ -1394,6 +1412,7 @@ Field field(Scene& s, const std::string& n) {
     if (n == "CsrCofC") return F(s.cVal);
     if (n == "VectorP") return F(s.vecP);
     if (n == "InvDiagA") return F(s.invDiagA);
+    if (n == "ConvergenceHistory") return F(s.hist);
     if (n.rfind("MGCofA_L", 0) == 0 || n.rfind("MGInvD_L", 0) == 0) {
         const int l = atoi(n.c_str() + 8);
         if (l >= 0 && l < (int)s.lv.size()) return n[2] == 'C' ? F(s.lv[l].A) : F(s.lv[l].invD);
@@ -1545,9 +1564,18 @@ int mpho_scalar(mpho_scene* h, const char* name, double* v) {
     else if (n == "rr") *v = s.rr;
     else if (n == "rs0") *v = s.rs0;
     else if (n == "rs") *v = s.rs;
+    else if (n == "CheckRR") *v = s.checkRR;
+    else if (n == "CheckRS") *v = s.checkRS;
     else if (n == "PowerParticleCount") *v = s.p2;
     else if (n == "VtkStep") *v = s.vtkStep;
     else { g_err = "unknown scalar " + n; return MPH_E_FIELD; }
+    return MPH_OK;
+}
+
+int mpho_set_option(mpho_scene* h, const char* name, double value) {
+    const std::string n = name;
+    if (n == "ConvergenceCheck") h->s.convCheck = value != 0.0;
+    else { g_err = "unknown option " + n; return MPH_E_ARG; }
     return MPH_OK;
 }
 

@@ -36,6 +36,11 @@ int mpho_field_size(mpho_scene* s, const char* name, size_t* bytes, int* elem_by
 int mpho_read(mpho_scene* s, const char* name, void* out, size_t bytes);
 int mpho_write(mpho_scene* s, const char* name, const void* in, size_t bytes);
 int mpho_scalar(mpho_scene* s, const char* name, double* value);
+/* "ConvergenceCheck" (0/1): the reference built with CONVERGENCE_CHECK (main.cpp:46,
+ * 4348-4351, 4383-4397).  Each solve then keeps (rr, r.s) of every iteration that
+ * did not stop (field ConvergenceHistory) and the true residual's r.r and r.s after
+ * the loop (scalars CheckRR, CheckRS). */
+int mpho_set_option(mpho_scene* s, const char* name, double value);
 /* s = M^-1 q with the current multigrid hierarchy (preconditionWithMultiGrid) */
 int mpho_precondition(mpho_scene* s, const double* q, double* out);
 void mpho_destroy(mpho_scene* s);

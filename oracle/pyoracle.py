@@ -36,6 +36,7 @@ def lib():
         L.mpho_read.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t]
         L.mpho_write.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t]
         L.mpho_scalar.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]
+        L.mpho_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
         L.mpho_precondition.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.mpho_destroy.argtypes = [C.c_void_p]
         L.mpho_last_error.restype = C.c_char_p
@@ -161,6 +162,9 @@ class OracleScene:
         v = C.c_double()
         self._check(lib().mpho_scalar(self.h, name.encode(), C.byref(v)))
         return v.value
+
+    def set_option(self, name: str, value: float) -> None:
+        self._check(lib().mpho_set_option(self.h, name.encode(), float(value)))
 
     def precondition(self, q: np.ndarray) -> np.ndarray:
         q = np.ascontiguousarray(q, dtype=np.float64)

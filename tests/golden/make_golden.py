@@ -20,6 +20,7 @@ import argparse
 import glob
 import hashlib
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -83,6 +84,44 @@ def run_case(name, data, grid, dim, steps, ref_bin, outdir, extra_files=()):
         shutil.rmtree(work, ignore_errors=True)
 
 
+CR_LINE = re.compile(r"^line:\d+, iter,=\d+, rr0=,\S+, rr=,\S+, rs0=,\S+, rs=,\S+$")
+
+
+def run_convergence_cases(cases, ref_bin, outdir):
+    """The reference executables built with CONVERGENCE_CHECK (MphImplicit_cc,
+    MphImplicit_3d_cc) over seeded scenes; keeps every CR line of their logs (the
+    per-iteration lines, the end-of-solve line and the true-residual line) with the
+    inputs, in convergence.npz."""
+    fx = {}
+    for name, fn, dim, steps in cases:
+        p, g = fn(steps=steps)
+        work = tempfile.mkdtemp(prefix="golden_cc_")
+        try:
+            data, grid = os.path.join(work, "box.data"), os.path.join(work, "box.grid")
+            files.write_data_file(data, p)
+            files.write_grid_file(grid, g)
+            exe = os.path.join(ref_bin, "MphImplicit_cc" if dim == 2 else "MphImplicit_3d_cc")
+            log = os.path.join(work, "run.log")
+            subprocess.run([exe, data, grid, os.path.join(work, "o%03d.prof"), os.path.join(work, "o%03d.vtk"), log,
+                            "1"], check=True, cwd=work, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            lines = [ln.rstrip("\n") for ln in open(log) if CR_LINE.match(ln.rstrip("\n"))]
+            fx["%s/dim" % name] = np.int32(dim)
+            fx["%s/steps" % name] = np.int32(steps)
+            fx["%s/data_text" % name] = np.array(open(data).read())
+            g2 = files.read_grid_file(grid)
+            fx["%s/in_type" % name] = g2.type
+            fx["%s/in_pos" % name] = g2.pos
+            fx["%s/in_vel" % name] = g2.vel
+            fx["%s/in_hdr" % name] = np.array([g2.time, g2.spacing, *g2.domain_min, *g2.domain_max])
+            fx["%s/lines" % name] = np.array(lines)
+            print("%-12s %d steps, %d CR lines" % (name, steps, len(lines)))
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+    path = os.path.join(outdir, "convergence.npz")
+    np.savez_compressed(path, **fx)
+    print("convergence -> %s (%.0f KB)" % (path, os.path.getsize(path) / 1024))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
@@ -122,6 +161,9 @@ def main():
             files.write_grid_file(os.path.join(d, "box.grid"), g)
             run_case(name, os.path.join(d, "box.data"), os.path.join(d, "box.grid"), dim, steps, ref_bin, args.out,
                      extra_files=("ref.vtk", "ref.prof"))
+        # the reference's CONVERGENCE_CHECK log lines
+        run_convergence_cases([("tiny2d", scenes.tiny_2d, 2, 3), ("tiny3d", scenes.tiny_3d, 3, 3),
+                               ("tiny3d_moving", scenes.tiny_3d_moving, 3, 2)], ref_bin, args.out)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 

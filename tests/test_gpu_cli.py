@@ -70,3 +70,24 @@ def test_mph_run_files_match_oracle(tmp_path):
     assert sorted(x.name for x in tmp_path.glob("out[0-9]*")) == ["out000.prof", "out000.vtk", "out010.vtk"]
     assert (tmp_path / "output.vtk").exists()  # the initial state, as main.cpp:374
     assert gpu_iters == iters
+
+
+def test_mph_run_convergence_check_lines(tmp_path):
+    """--convergence-check logs the lines of the reference built with CONVERGENCE_CHECK
+    (main.cpp:4350 per iteration, 4381, 4396).  On tiny3d the GPU's dot order prints
+    the same six digits as the reference's serial order, so the log must hold exactly
+    the reference's lines, kept in tests/golden/convergence.npz."""
+    from test_oracle_convergence import load_case
+
+    fx, p, g = load_case("tiny3d")
+    (tmp_path / "box.data").write_text(str(fx["tiny3d/data_text"]))
+    files.write_grid_file(str(tmp_path / "box.grid"), g)
+    print("[cli] mph_run --convergence-check on tiny3d", flush=True)
+    r = subprocess.run([RUN, "box.data", "box.grid", "out%03d.prof", "out%03d.vtk", "box.log", "1", "--dim", "3",
+                        "--convergence-check"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    got = [ln for ln in (tmp_path / "box.log").read_text().splitlines() if re.match(r"line:\d+, iter,=", ln)]
+    want = [str(s) for s in fx["tiny3d/lines"]]
+    assert len(got) == len(want)
+    for a, b in zip(got, want):
+        assert a == b
