@@ -29,7 +29,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .abi import OPERATOR_ASSEMBLED, PRECISION_MIXED, Params
+from .abi import OPERATOR_ASSEMBLED, PRECISION_FP64, PRECISION_MIXED, Params
 from .scene import Batch, MphError, Scene
 
 
@@ -83,7 +83,7 @@ class Server:
     """Continuous-batching scheduler of scene-step requests on one device."""
 
     def __init__(self, device: int = 0, budget_bytes: int = 200 << 30, max_active: int = 256,
-                 keep_fields: tuple = (), lazy_mixed: bool = True, build_ahead: bool = True):
+                 keep_fields: tuple = (), lazy_mixed: bool = False, build_ahead: bool = True):
         self.device = device
         self.budget = int(budget_bytes)
         self.max_active = max_active
@@ -277,7 +277,7 @@ class Server:
         self.batch.close()
 
 
-def make_requests(stream, precision=PRECISION_MIXED, tol=1e-16, operator_form=OPERATOR_ASSEMBLED) -> list[Request]:
+def make_requests(stream, precision=PRECISION_FP64, tol=1e-16, operator_form=OPERATOR_ASSEMBLED) -> list[Request]:
     """Requests from scenes.serve_stream tuples (arrival, steps, seed, dim, nf, mu);
     each scene is built when the server admits its request."""
     from . import scenes
