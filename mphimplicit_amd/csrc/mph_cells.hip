@@ -163,11 +163,15 @@ __device__ inline void for_cells(const DevParams& P, const int* __restrict__ cel
             }
 }
 
+// the three coordinates are loaded before the first min_image: its branch (the
+// division of the wrapped case) would otherwise hold each load back until the
+// previous coordinate is done, three memory round trips per candidate
 __device__ inline double dist2(const DevParams& P, const double* __restrict__ pos, double xi, double yi, double zi,
                                int j) {
-    const double q0 = min_image(pos[3 * j + 0], xi, P.W[0], P.hW[0]);
-    const double q1 = min_image(pos[3 * j + 1], yi, P.W[1], P.hW[1]);
-    const double q2 = min_image(pos[3 * j + 2], zi, P.W[2], P.hW[2]);
+    const double x = pos[3 * j + 0], y = pos[3 * j + 1], z = pos[3 * j + 2];
+    const double q0 = min_image(x, xi, P.W[0], P.hW[0]);
+    const double q1 = min_image(y, yi, P.W[1], P.hW[1]);
+    const double q2 = min_image(z, zi, P.W[2], P.hW[2]);
     return q0 * q0 + q1 * q1 + q2 * q2;
 }
 

@@ -596,12 +596,14 @@ __device__ inline void spmv_chunk(int nf, const int* __restrict__ nFluid, const 
 // butterfly, in the same order for every launch.
 
 // minimum-image displacement pos_j - pos_i (the reference's Mod form, used where
-// the scalars are built)
+// the scalars are built); j's coordinates are loaded before the first min_image,
+// whose branch would otherwise hold each load back until the previous one is done
 __device__ inline void fac_disp(const DevParams& P, const double* __restrict__ pos, int j, double pi0, double pi1,
                                 double pi2, double& r0, double& r1, double& r2) {
-    r0 = min_image(pos[3 * j], pi0, P.W[0], P.hW[0]);
-    r1 = min_image(pos[3 * j + 1], pi1, P.W[1], P.hW[1]);
-    r2 = min_image(pos[3 * j + 2], pi2, P.W[2], P.hW[2]);
+    const double x = pos[3 * j], y = pos[3 * j + 1], z = pos[3 * j + 2];
+    r0 = min_image(x, pi0, P.W[0], P.hW[0]);
+    r1 = min_image(y, pi1, P.W[1], P.hW[1]);
+    r2 = min_image(z, pi2, P.W[2], P.hW[2]);
 }
 
 // the same displacement for the products: positions lie inside the domain, so one

@@ -527,9 +527,16 @@ struct Rel {
     double x[3];
     double r2;
 };
+// both positions are loaded before the first min_image, whose branch would otherwise
+// hold each coordinate's load back until the previous one is done
 __device__ inline Rel rel(const DevParams& P, const double* __restrict__ pos, int i, int j) {
     Rel o;
-    for (int d = 0; d < 3; ++d) o.x[d] = min_image(pos[3 * j + d], pos[3 * i + d], P.W[d], P.hW[d]);
+    double pj[3], pi[3];
+    for (int d = 0; d < 3; ++d) {
+        pj[d] = pos[3 * j + d];
+        pi[d] = pos[3 * i + d];
+    }
+    for (int d = 0; d < 3; ++d) o.x[d] = min_image(pj[d], pi[d], P.W[d], P.hW[d]);
     o.r2 = (o.x[0] * o.x[0] + o.x[1] * o.x[1] + o.x[2] * o.x[2]);
     return o;
 }
