@@ -163,16 +163,33 @@ __device__ inline void for_cells(const DevParams& P, const int* __restrict__ cel
             }
 }
 
-// the three coordinates are loaded before the first min_image: its branch (the
-// division of the wrapped case) would otherwise hold each load back until the
-// previous coordinate is done, three memory round trips per candidate
-__device__ inline double dist2(const DevParams& P, const double* __restrict__ pos, double xi, double yi, double zi,
-                               int j) {
-    const double x = pos[3 * j + 0], y = pos[3 * j + 1], z = pos[3 * j + 2];
-    const double q0 = min_image(x, xi, P.W[0], P.hW[0]);
-    const double q1 = min_image(y, yi, P.W[1], P.hW[1]);
-    const double q2 = min_image(z, zi, P.W[2], P.hW[2]);
-    return q0 * q0 + q1 * q1 + q2 * q2;
+// visit(j, q2) with q2 the squared minimum-image distance of candidate j to
+// (xi, yi, zi), for j = lo..hi-1 in ascending order.  The coordinates of four
+// candidates are loaded together before the first min_image: its branch (the division
+// of the wrapped case) would otherwise hold each load back until the previous
+// coordinate is done.  Past hi the loads repeat candidate hi-1, which is not visited.
+template <class V>
+__device__ inline void scan_range(const DevParams& P, const double* __restrict__ pos, double xi, double yi, double zi,
+                                  int lo, int hi, V&& visit) {
+    constexpr int U = 4;
+    for (int j0 = lo; j0 < hi; j0 += U) {
+        double c[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + u < hi ? j0 + u : hi - 1;
+            c[u][0] = pos[3 * j];
+            c[u][1] = pos[3 * j + 1];
+            c[u][2] = pos[3 * j + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (j0 + u >= hi) break;
+            const double q0 = min_image(c[u][0], xi, P.W[0], P.hW[0]);
+            const double q1 = min_image(c[u][1], yi, P.W[1], P.hW[1]);
+            const double q2 = min_image(c[u][2], zi, P.W[2], P.hW[2]);
+            visit(j0 + u, q0 * q0 + q1 * q1 + q2 * q2);
+        }
+    }
 }
 
 // count pass, main.cpp:1406-1511; also the largest P-neighbour count of a fluid
@@ -198,8 +215,7 @@ __global__ void k_neighbor_count(DevParams P, const double* __restrict__ pos, co
     for_cells<true>(P, cellStart, key, [&](int lo, int hi) {
         if (rs) rs[q] = (unsigned short)n;
         q++;
-        for (int j = lo; j < hi; ++j) {
-            const double q2 = dist2(P, pos, xi, yi, zi, j);
+        scan_range(P, pos, xi, yi, zi, lo, hi, [&](int j, double q2) {
             if (q2 <= P.maxR2) {
                 if (sm && n < capM) sm[n] = j;
                 n++;
@@ -209,15 +225,14 @@ __global__ void k_neighbor_count(DevParams P, const double* __restrict__ pos, co
                     npp++;
                 }
             }
-        }
+        });
     });
     if (rs) rs[q] = (unsigned short)n;
     // walls without fluid neighbours skip the wall scan (main.cpp:1474); walls
     // occupy [nf, np) after every sort
     if (!(i >= P.nf && n == 0))
         for_cells<false>(P, cellStart, key, [&](int lo, int hi) {
-            for (int j = lo; j < hi; ++j) {
-                const double q2 = dist2(P, pos, xi, yi, zi, j);
+            scan_range(P, pos, xi, yi, zi, lo, hi, [&](int j, double q2) {
                 if (q2 <= P.maxR2) {
                     if (sm && n < capM) sm[n] = j;
                     n++;
@@ -226,7 +241,7 @@ __global__ void k_neighbor_count(DevParams P, const double* __restrict__ pos, co
                         npp++;
                     }
                 }
-            }
+            });
         });
     nCount[i] = n;
     nFluid[i] = nfl;
@@ -275,13 +290,12 @@ __global__ void k_neighbor_fill(DevParams P, const double* __restrict__ pos, con
     const int key = cellIndex[i];
     long long k = nPtr[i], kp = nPtrP[i];
     auto put = [&](int lo, int hi) {
-        for (int j = lo; j < hi; ++j) {
-            const double q2 = dist2(P, pos, xi, yi, zi, j);
+        scan_range(P, pos, xi, yi, zi, lo, hi, [&](int j, double q2) {
             if (q2 <= P.maxR2) {
                 nInd[k++] = j;
                 if (q2 <= P.radP2) nIndP[kp++] = j;
             }
-        }
+        });
     };
     unsigned short* rs = i < P.nf ? runStart + (long long)i * kRuns : nullptr;
     int q = 0;
