@@ -191,15 +191,17 @@ __global__ void __launch_bounds__(256) k_forces(DevParams P, const int* __restri
         double t[K] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (jn < n) {
             const int j = nInd[b + jn];
+            // j's values load with its position, not after the distance tests
+            const int tj = type[j];
+            const double paj = pressA[j];
+            const double gj[3] = {gcen[3 * j], gcen[3 * j + 1], gcen[3 * j + 2]};
             const Rel q = rel(P, pos, i, j);
             if (i != j && q.r2 != 0.0) {
-                const int tj = type[j];
                 const double rij_ = P.ratio[ti][tj], rji_ = P.ratio[tj][ti];
                 if (P.radA2 - q.r2 > 0) {
                     const double r = sqrt(q.r2);
                     const double dwij = rij_ * dwadr_k(P.Ka, P.invA, r, P.radA);
                     const double dwji = rji_ * dwadr_k(P.Ka, P.invA, r, P.radA);
-                    const double paj = pressA[j];
                     for (int d = 0; d < 3; ++d) t[d] = (pai * dwij + paj * dwji) * (q.x[d] / r) * P.V;
                     f |= 1;
                 }
@@ -207,7 +209,6 @@ __global__ void __launch_bounds__(256) k_forces(DevParams P, const int* __restri
                     const double r = sqrt(q.r2);
                     const double wij = rij_ * wq_k(P.Kg, r, P.radG);
                     const double wji = rji_ * wq_k(P.Kg, r, P.radG);
-                    const double gj[3] = {gcen[3 * j], gcen[3 * j + 1], gcen[3 * j + 2]};
                     const double gi[3] = {gi0, gi1, gi2};
                     for (int d = 0; d < 3; ++d) t[3 + d] = (aj * gj[d] * wji - ai * gi[d] * wij) / P.R2g * P.radG * VoL;
                     const double dwij = rij_ * dwq_k(P.Kg, P.m2G, r, P.radG);
@@ -297,6 +298,9 @@ __global__ void __launch_bounds__(256) k_matrix_a(DevParams P, const int* __rest
                 if (jn < nfl) store_alpha(jn, 0.0);
             } else {
                 double alpha = 0.0;
+                // j's values load with its position, not after the distance test
+                const double muj = mu[j];
+                const int tj = type[j];
                 const Rel q = rel(P, pos, i, j);
                 // factored form: does any pair interact through the periodic boundary
                 if (fwrap)
@@ -307,14 +311,13 @@ __global__ void __launch_bounds__(256) k_matrix_a(DevParams P, const int* __rest
                     const double dij = sqrt(q.r2);
                     const double wdrij = -dwq_k(P.Kv, P.m2V, dij, P.radV);
                     const double e[3] = {q.x[0] / dij, q.x[1] / dij, q.x[2] / dij};
-                    const double muj = mu[j];
                     const double muij = 2.0 * (mui * muj) / (mui + muj);
                     if (falpha) alpha = P.visc * muij * wdrij / (dij * dij * dij);
                     for (int r = 0; r < 3; ++r)
                         for (int sdim = 0; sdim < 3; ++sdim)
                             t[3 * r + sdim] = +P.visc * muij * e[sdim] * e[r] * wdrij / dij;
                     f = 2;
-                    if (is_wall(type[j])) {
+                    if (is_wall(tj)) {
                         t[9] = vel[3 * j];
                         t[10] = vel[3 * j + 1];
                         t[11] = vel[3 * j + 2];
