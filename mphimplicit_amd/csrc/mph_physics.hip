@@ -89,12 +89,15 @@ __global__ void __launch_bounds__(256) k_densities(DevParams P, const int* __res
     double sa = 0.0, sg0 = 0.0, sg1 = 0.0, sg2 = 0.0, sp = 0.0;
     const long long b = own ? nPtr[i] : 0;
     const int n = own ? nCount[i] : 0;
+    int jnext = g < n ? nInd[b + g] : 0;  // each lane's next entry loads one round ahead
     for (int base = 0; base < n; base += G) {
         const int jn = base + g;
+        const int jcur = jnext;
+        jnext = jn + G < n ? nInd[b + jn + G] : 0;
         int f = 0;
         double t[K] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (jn < n) {
-            const int j = nInd[b + jn];
+            const int j = jcur;
             if (i != j) {
                 const double ratio = P.ratio[ti][type[j]];
                 const Rel q = rel(P, pos, i, j);
@@ -185,12 +188,15 @@ __global__ void __launch_bounds__(256) k_forces(DevParams P, const int* __restri
     double fa[3] = {0.0, 0.0, 0.0}, fd[3] = {0.0, 0.0, 0.0};
     const long long b = own ? nPtr[i] : 0;
     const int n = own ? nCount[i] : 0;
+    int jnext = g < n ? nInd[b + g] : 0;  // each lane's next entry loads one round ahead
     for (int base = 0; base < n; base += G) {
         const int jn = base + g;
+        const int jcur = jnext;
+        jnext = jn + G < n ? nInd[b + jn + G] : 0;
         int f = 0;
         double t[K] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (jn < n) {
-            const int j = nInd[b + jn];
+            const int j = jcur;
             // j's values load with its position, not after the distance tests
             const int tj = type[j];
             const double paj = pressA[j];
@@ -286,12 +292,15 @@ __global__ void __launch_bounds__(256) k_matrix_a(DevParams P, const int* __rest
     // the 3x3 blocks are stored unless aval is null (a factored scene that builds its
     // multigrid level 0 from the factored form needs none)
     const bool blocks = aval != nullptr;
+    int jnext = g < nc ? nInd[b0 + g] : 0;  // each lane's next entry loads one round ahead
     for (int base = 0; base < nc; base += G) {
         const int jn = base + g;
+        const int jcur = jnext;
+        jnext = jn + G < nc ? nInd[b0 + jn + G] : 0;
         int f = 0;
         double t[K] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (jn < nc) {
-            const int j = nInd[b0 + jn];
+            const int j = jcur;
             if (i == j) {
                 f = 1;
                 if (blocks) acol[a_col_index(sb, jn, lane)] = j;
