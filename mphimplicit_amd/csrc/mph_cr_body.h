@@ -85,6 +85,22 @@ __device__ inline unsigned galerkin_mask(const DevParams& P, const Level& L, con
     return mask;
 }
 
+// galerkin_mask's q of cell g's own cell: the run that holds each of its particles'
+// self entry
+__device__ inline int galerkin_self(const DevParams& P, const Level& L, int g) {
+    int gx, gy, gz;
+    grid_xyz(L, g, gx, gy, gz);
+    const int cx = gx + L.mn[0], cy = gy + L.mn[1], cz = gz + L.mn[2];
+    const int* C = P.C;
+    const int rx = P.range[0], ry = P.range[1], rz = P.range[2];
+    const int nx = 2 * rx + 1, ny = 2 * ry + 1, nz = 2 * rz + 1;
+    const int bx = (2 * rx) - (cx + rx) % C[0], by = (2 * ry) - (cy + ry) % C[1], bz = (2 * rz) - (cz + rz) % C[2];
+    const int x0 = bx > 0 ? bx : 0, y0 = by > 0 ? by : 0, z0 = bz > 0 ? bz : 0;
+    // the own cell is the one visited with offset index r (jx[r] = cx): a % n == r
+    auto at = [](int r, int o, int n) { return ((r - o) % n + n) % n; };
+    return (at(rx, x0, nx) * ny + at(ry, y0, ny)) * nz + at(rz, z0, nz);
+}
+
 // ---------------------------------------------------------------------------
 // multigrid rows (preconditionWithMultiGrid)
 

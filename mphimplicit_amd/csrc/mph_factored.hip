@@ -246,13 +246,18 @@ __global__ void k_fac_g(DevParams P, const int* __restrict__ cellIndex, const in
 // ncs stencil cells.  Lane l works on stencil cell c = l % ncs in partition
 // l / ncs; the 64 / ncs partitions split the row cell's particles (viscous part)
 // and the candidate list (C Lambda C^T part) round robin, and their sums are added
-// in partition order at the end.  The candidates are the particles of the cells
-// around g whose G table holds g, filtered 64 at a time into an ordered list in LDS
-// (flushed when full).  The list and the partition sums share one LDS buffer.
+// in partition order at the end.  The viscous part and the diagonal blocks come
+// from the run sums of the matrix-A pass (blk): a particle's record of each visited
+// cell that falls in stencil cell c.  (Walking the runs entry by entry here took
+// 3.95 ms of 4.7 on the 1M scene: a lane's particles and entries one after another,
+// with the centre cell's runs the longest.)  The candidates are the particles of the
+// cells around g whose G table holds g, filtered 64 at a time into an ordered list in
+// LDS (flushed when full).  The list and the partition sums share one LDS buffer.
 constexpr int kGal0List = 288;
 __global__ void __launch_bounds__(64) k_fac_gal0(DevParams P, Level L, FacOp F, const int* __restrict__ cellStart,
-                                                 const unsigned short* __restrict__ runStart,
-                                                 const double* __restrict__ gtab, const int* __restrict__ gmask) {
+                                                 const double* __restrict__ blk, const double* __restrict__ gtab,
+                                                 const int* __restrict__ gmask) {
+    typedef double dv2 __attribute__((ext_vector_type(2)));
     __shared__ double sbuf[64 * 9];   // 4608 B: the list, then the partition sums
     static_assert(4 * kGal0List * sizeof(int) <= sizeof(sbuf), "list exceeds the shared buffer");
     int* lk = (int*)sbuf;             // candidate particle
@@ -270,51 +275,29 @@ __global__ void __launch_bounds__(64) k_fac_gal0(DevParams P, Level L, FacOp F, 
     const int ic = body::cell_id(P, ax, ay, az);
     const int p0 = cellStart[ic], p1 = cellStart[ic + 1];
     double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    // viscous part and the diagonal blocks: the runs of the row cell's particles
-    // that fall in stencil cell c, four entries loaded at a time
+    // viscous part and the diagonal blocks: the run sums of the row cell's particles
+    // in the visited cells that fall in stencil cell c, then the particle's diagonal
+    // block where c holds the row cell itself
     const unsigned vmask = active ? body::galerkin_mask(P, L, cellStart, g, c) : 0u;
-    const bool wrapf = *F.wrap != 0;
-    const double W0 = P.W[0], W1 = P.W[1], W2 = P.W[2], h0 = P.hW[0], h1 = P.hW[1], h2 = P.hW[2];
+    const bool dself = (vmask >> body::galerkin_self(P, L, g)) & 1u;
     if (vmask)
         for (int ip = p0 + part; ip < p1; ip += nparts) {
-            const unsigned short* rs = runStart + (long long)ip * kRuns;
-            const long long b0 = F.nPtr[ip];
-            const double pi0 = F.pos[3 * ip], pi1 = F.pos[3 * ip + 1], pi2 = F.pos[3 * ip + 2];
-            for (unsigned m = vmask; m; m &= m - 1) {
-                const int q = __builtin_ctz(m);
-                const int e1 = rs[q + 1];
-                for (int jn = rs[q]; jn < e1; jn += 4) {
-                    int j[4];
-                    double a[4], r[4][3];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const bool in = jn + u < e1;
-                        j[u] = in ? F.nInd[b0 + jn + u] : ip;
-                        a[u] = in ? ((const double*)F.alpha)[b0 + jn + u] : 0.0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        r[u][0] = F.pos[3 * j[u]] - pi0;
-                        r[u][1] = F.pos[3 * j[u] + 1] - pi1;
-                        r[u][2] = F.pos[3 * j[u] + 2] - pi2;
-                        if (wrapf) {
-                            r[u][0] = body::fac_wrap<true>(r[u][0], W0, h0);
-                            r[u][1] = body::fac_wrap<true>(r[u][1], W1, h1);
-                            r[u][2] = body::fac_wrap<true>(r[u][2], W2, h2);
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        if (jn + u >= e1) continue;
-                        if (j[u] == ip) {
-                            for (int e = 0; e < 9; ++e) acc[e] += F.diag[9 * ip + e];
-                        } else {
-                            for (int rr = 0; rr < 3; ++rr)
-                                for (int ss = 0; ss < 3; ++ss) acc[3 * rr + ss] -= a[u] * r[u][rr] * r[u][ss];
-                        }
-                    }
-                }
+            const double* rb = blk + (long long)ip * kBlkRuns * kBlkStride;
+            for (unsigned m = vmask; m; m &= m - 1) {  // symmetric: the upper triangle twice
+                const dv2* o = (const dv2*)(rb + __builtin_ctz(m) * kBlkStride);
+                const dv2 v0 = o[0], v1 = o[1], v2 = o[2];
+                acc[0] += v0.x;
+                acc[1] += v0.y;
+                acc[2] += v1.x;
+                acc[3] += v0.y;
+                acc[4] += v1.y;
+                acc[5] += v2.x;
+                acc[6] += v1.x;
+                acc[7] += v2.x;
+                acc[8] += v2.y;
             }
+            if (dself)
+                for (int e = 0; e < 9; ++e) acc[e] += F.diag[9 * ip + e];
         }
     // C Lambda C^T part: lambda_k G_gk G_hk^T; h is stencil cell c (level-0 counts
     // of at least 3 per axis: the caller checks)
@@ -918,14 +901,13 @@ void launch_fac_p(hipStream_t st, const DevParams& P, const double* pos, const i
 }
 
 void launch_fac_level0(hipStream_t st, const DevParams& P, const Level& L0, const FacOp& F, const int* cellIndex,
-                       const int* cellStart, const unsigned short* runStart, double* gtab, int* gmask,
-                       double* invDiagA) {
+                       const int* cellStart, const double* blk, double* gtab, int* gmask, double* invDiagA) {
     if (P.nf == 0) return;
     hipLaunchKernelGGL(k_fac_g, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, cellIndex, F.runP, F.nPtrP, F.nIndP,
                        F.beta, F.pos, F.selfC, gtab, gmask);
     hipLaunchKernelGGL(k_fac_invdiag, dim3(nblk(8LL * P.nf, 256)), dim3(256), 0, st, P, F, invDiagA);
     if (L0.cells > 0)
-        hipLaunchKernelGGL(k_fac_gal0, dim3(L0.cells), dim3(64), 0, st, P, L0, F, cellStart, runStart, gtab, gmask);
+        hipLaunchKernelGGL(k_fac_gal0, dim3(L0.cells), dim3(64), 0, st, P, L0, F, cellStart, blk, gtab, gmask);
 }
 
 void launch_fac_u(hipStream_t st, const DevParams& P, const int* nFluid, const long long* nPtr, const int* nInd,

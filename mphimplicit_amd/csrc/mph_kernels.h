@@ -122,6 +122,10 @@ __host__ __device__ inline long long a_val32_index(long long sb, int k, int e, i
 }
 constexpr int kChunk = 256;   // particles per dot-product chunk / CR workgroup
 constexpr int kRuns = 28;     // run starts per fluid particle: 27 neighbour cells + end
+// the factored form's level-0 run sums: per fluid particle and visited cell, the sums
+// of -alpha r r^T over that cell's run of fluid entries, a symmetric block kept as its
+// upper triangle (xx xy xz yy yz zz) in records of kBlkStride doubles
+constexpr int kBlkRuns = kRuns - 1, kBlkStride = 6;
 constexpr int kMaxP = 40;     // P-neighbours staged in LDS per particle by the C Lambda C^T kernel
 constexpr int kMaxLevels = 20;   // multigrid depth = floor(log2(cells per axis)) + 1
 // the V-cycle levels with at most this many rows may run in the one-workgroup
@@ -169,11 +173,13 @@ void launch_forces(hipStream_t st, const DevParams& P, const int* type, const do
 // alpha/diag/wrap (each may be null): the factored form's viscous scalar per
 // fluid-neighbour entry (FacOp order), the diagonal block without C Lambda C^T, and
 // the flag set when a pair interacts through the periodic boundary (wrap must be 0
-// before the launch)
+// before the launch).  blk (with runStart; may be null): the level-0 run sums
+// [nf][kBlkRuns][kBlkStride] of the factored form, each in list order
 void launch_matrix_a(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
                      const double* mass, const double* mu, const int* nCount, const int* nFluid,
                      const long long* nPtr, const int* nInd, ASlices A, int* selfIdx, double* b,
-                     double* alpha = nullptr, double* diag = nullptr, int* wrap = nullptr);
+                     double* alpha = nullptr, double* diag = nullptr, int* wrap = nullptr,
+                     const unsigned short* runStart = nullptr, double* blk = nullptr);
 void launch_matrix_c(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP,
                      const long long* nPtrP, const int* nIndP, const double* lambda, double* cval, double* dval);
 void launch_vector_p(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
@@ -234,9 +240,9 @@ void launch_fac_p(hipStream_t st, const DevParams& P, const double* pos, const i
 // multigrid level 0 (calculateMultiGridCsrA at level 0) and InvDiagA from the
 // factored form, with per-particle scratch gtab [np][27][3] and gmask [np] (level-0
 // counts of at least 3 per active axis: the caller checks)
+// blk: the run sums launch_matrix_a wrote
 void launch_fac_level0(hipStream_t st, const DevParams& P, const Level& L0, const FacOp& F, const int* cellIndex,
-                       const int* cellStart, const unsigned short* runStart, double* gtab, int* gmask,
-                       double* invDiagA);
+                       const int* cellStart, const double* blk, double* gtab, int* gmask, double* invDiagA);
 // u32 of the FP32 form from alpha and the positions (after launch_fac_p)
 void launch_fac_u(hipStream_t st, const DevParams& P, const int* nFluid, const long long* nPtr, const int* nInd,
                   const double* alpha, const double* pos, float* u32);

@@ -260,8 +260,13 @@ __global__ void __launch_bounds__(256) k_forces(DevParams P, const int* __restri
     for (int d = 0; d < 3; ++d) force[3 * i + d] = F[d];
 }
 
-// calculateMatrixA values (2251-2335): viscous blocks, inertia, wall terms of b
-__global__ void __launch_bounds__(256) k_matrix_a(DevParams P, const int* __restrict__ type,
+// calculateMatrixA values (2251-2335): viscous blocks, inertia, wall terms of b.
+// kBlk: also the factored form's level-0 run sums (blk, from the run starts): while
+// the group's first lane adds every entry's block in list order, its second lane adds
+// the same fluid entries' -alpha r r^T (the negated block; 0 beyond RadiusV and for
+// the self entry) per visited cell, and stores each cell's sum when its run ends
+template <bool kBlk>
+__global__ void __launch_bounds__(256, 4) k_matrix_a(DevParams P, const int* __restrict__ type,
                                                   const double* __restrict__ pos, const double* __restrict__ vel,
                                                   const double* __restrict__ mass, const double* __restrict__ mu,
                                                   const int* __restrict__ nCount, const int* __restrict__ nFluid,
@@ -269,7 +274,9 @@ __global__ void __launch_bounds__(256) k_matrix_a(DevParams P, const int* __rest
                                                   const long long* __restrict__ start, int* __restrict__ acol,
                                                   double* __restrict__ aval, int* __restrict__ selfIdx,
                                                   double* __restrict__ b, double* __restrict__ falpha,
-                                                  double* __restrict__ fdiag, int* __restrict__ fwrap) {
+                                                  double* __restrict__ fdiag, int* __restrict__ fwrap,
+                                                  const unsigned short* __restrict__ runStart,
+                                                  double* __restrict__ blk) {
     constexpr int G = kLanes, K = 12;  // terms: the 3x3 coefficients, then a wall neighbour's velocity
     __shared__ double tb[256 * K];
     __shared__ int tf[256];  // bit 0: the self entry, bit 1: inside RadiusV, bit 2: and a wall neighbour
@@ -277,7 +284,9 @@ __global__ void __launch_bounds__(256) k_matrix_a(DevParams P, const int* __rest
     const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / G);
     const bool own = i < P.nf;
     int iN = 0;
-    double self[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    // s9: the group's first lane sums the diagonal block in it; with kBlk its second
+    // lane sums the current visited cell's run of -alpha r r^T (one register array)
+    double s9[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double sv0 = 0.0, sv1 = 0.0, sv2 = 0.0;
     const double mui = own ? mu[i] : 0.0;
     const long long b0 = own ? nPtr[i] : 0;
@@ -292,6 +301,28 @@ __global__ void __launch_bounds__(256) k_matrix_a(DevParams P, const int* __rest
     // the 3x3 blocks are stored unless aval is null (a factored scene that builds its
     // multigrid level 0 from the factored form needs none)
     const bool blocks = aval != nullptr;
+    // the run sums: visited cell qr's run of fluid entries is [rsp[qr], rsp[qr+1])
+    typedef double dv2 __attribute__((ext_vector_type(2)));
+    const bool runner = kBlk && g == 1 && own;
+    // the group's run starts, staged in LDS by its four lanes (a flush then reads the
+    // next run's end there, not from global memory in the serial loop)
+    __shared__ unsigned short rsl[256 / G][kRuns];
+    unsigned short* const rsp = rsl[threadIdx.x / G];
+    if (kBlk && own)
+        for (int k = g; k < kRuns; k += G) rsp[k] = runStart[(long long)i * kRuns + k];
+    if (kBlk) wave_sync();
+    double* const bk = kBlk ? blk + (long long)(own ? i : 0) * kBlkRuns * kBlkStride : nullptr;
+    const int nv = (2 * P.range[0] + 1) * (2 * P.range[1] + 1) * (2 * P.range[2] + 1);
+    int qr = 0, qend = runner ? rsp[1] : 0;
+    auto flush = [&]() {  // cell qr's sums (upper triangle), then the next cell
+        dv2* o = (dv2*)(bk + qr * kBlkStride);
+        o[0] = dv2{s9[0], s9[1]};
+        o[1] = dv2{s9[2], s9[4]};
+        o[2] = dv2{s9[5], s9[8]};
+        for (int e9 = 0; e9 < 9; ++e9) s9[e9] = 0.0;
+        ++qr;
+        qend = qr < nv ? rsp[qr + 1] : 0x7fffffff;
+    };
     int jnext = g < nc ? nInd[b0 + g] : 0;  // each lane's next entry loads one round ahead
     for (int base = 0; base < nc; base += G) {
         const int jn = base + g;
@@ -345,32 +376,44 @@ __global__ void __launch_bounds__(256) k_matrix_a(DevParams P, const int* __rest
         for (int e = 0; e < K; ++e) tb[K * threadIdx.x + e] = t[e];
         tf[threadIdx.x] = f;
         wave_sync();
-        if (g == 0)
+        // lanes 0 and 1 walk the same entries together: lane 0 adds each block, lane 1
+        // (runner) adds its negation, exactly -1.0 times it, per visited cell
+        if (g == 0 || runner) {
+            const double sgn = g == 0 ? 1.0 : -1.0;
             for (int u = 0; u < G && base + u < nc; ++u) {
-                const int sl = threadIdx.x + u;
+                const int sl = threadIdx.x - g + u;
                 const int fu = tf[sl];
                 const double* c9 = tb + K * sl;
-                if (fu & 1) iN = base + u;
+                if (g != 0) {
+                    if (base + u >= nfl) break;  // the runs cover the fluid entries
+                    while (base + u >= qend) flush();
+                }
                 if (fu & 2)
-                    for (int e9 = 0; e9 < 9; ++e9) self[e9] += c9[e9];
-                if (fu & 4) {
-                    const double v0 = c9[9], v1 = c9[10], v2 = c9[11];
-                    sv0 += c9[0] * v0; sv0 += c9[1] * v1; sv0 += c9[2] * v2;
-                    sv1 += c9[3] * v0; sv1 += c9[4] * v1; sv1 += c9[5] * v2;
-                    sv2 += c9[6] * v0; sv2 += c9[7] * v1; sv2 += c9[8] * v2;
+                    for (int e9 = 0; e9 < 9; ++e9) s9[e9] += sgn * c9[e9];
+                if (g == 0) {
+                    if (fu & 1) iN = base + u;
+                    if (fu & 4) {
+                        const double v0 = c9[9], v1 = c9[10], v2 = c9[11];
+                        sv0 += c9[0] * v0; sv0 += c9[1] * v1; sv0 += c9[2] * v2;
+                        sv1 += c9[3] * v0; sv1 += c9[4] * v1; sv1 += c9[5] * v2;
+                        sv2 += c9[6] * v0; sv2 += c9[7] * v1; sv2 += c9[8] * v2;
+                    }
                 }
             }
+        }
         wave_sync();
     }
+    if (runner)
+        while (qr < nv) flush();  // the cells after the last fluid entry's run
     if (g != 0 || !own) return;
     const double coef = mass[i] / P.V / P.dt;
-    self[0] += coef;
-    self[4] += coef;
-    self[8] += coef;
+    s9[0] += coef;
+    s9[4] += coef;
+    s9[8] += coef;
     if (blocks)
-        for (int e9 = 0; e9 < 9; ++e9) aval[a_val_index(sb, iN, e9, lane)] = self[e9];
+        for (int e9 = 0; e9 < 9; ++e9) aval[a_val_index(sb, iN, e9, lane)] = s9[e9];
     if (fdiag)
-        for (int e9 = 0; e9 < 9; ++e9) fdiag[9 * i + e9] = self[e9];
+        for (int e9 = 0; e9 < 9; ++e9) fdiag[9 * i + e9] = s9[e9];
     selfIdx[i] = iN;
     const double bv[3] = {0.0 + sv0, 0.0 + sv1, 0.0 + sv2};
     for (int d = 0; d < 3; ++d) b[3 * i + d] = bv[d] + coef * vel[3 * i + d];
@@ -652,9 +695,14 @@ void launch_forces(hipStream_t st, const DevParams& P, const int* type, const do
 void launch_matrix_a(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
                      const double* mass, const double* mu, const int* nCount, const int* nFluid,
                      const long long* nPtr, const int* nInd, ASlices A, int* selfIdx, double* b, double* alpha,
-                     double* diag, int* wrap) {
-    hipLaunchKernelGGL(k_matrix_a, dim3(nblk((long long)kLanes * P.nf, 256)), dim3(256), 0, st, P, type, pos, vel,
-                       mass, mu, nCount, nFluid, nPtr, nInd, A.start, A.col, A.val, selfIdx, b, alpha, diag, wrap);
+                     double* diag, int* wrap, const unsigned short* runStart, double* blk) {
+    const dim3 grid(nblk((long long)kLanes * P.nf, 256));
+    if (blk && runStart)
+        hipLaunchKernelGGL(k_matrix_a<true>, grid, dim3(256), 0, st, P, type, pos, vel, mass, mu, nCount, nFluid, nPtr,
+                           nInd, A.start, A.col, A.val, selfIdx, b, alpha, diag, wrap, runStart, blk);
+    else
+        hipLaunchKernelGGL(k_matrix_a<false>, grid, dim3(256), 0, st, P, type, pos, vel, mass, mu, nCount, nFluid,
+                           nPtr, nInd, A.start, A.col, A.val, selfIdx, b, alpha, diag, wrap, nullptr, nullptr);
 }
 
 void launch_matrix_c(hipStream_t st, const DevParams& P, const double* pos, const int* nCountP,

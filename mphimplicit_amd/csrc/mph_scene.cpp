@@ -338,9 +338,13 @@ int phase_matrix_a(mph_scene& S) {
             return rc;
         launch_fill_i32(S.st, S.fwrap.p, 1, 0);
     }
+    // the level-0 run sums, for a factored level 0 (the multigrid phase reads them)
+    const bool runsums = factored(S) && !S.blocks && S.prm.multigrid;
+    if (runsums && (rc = S.fblk.ensure((size_t)(S.P.nf > 0 ? S.P.nf : 1) * kBlkRuns * kBlkStride))) return rc;
     launch_matrix_a(S.st, S.P, S.type.p, S.pos.p, S.vel.p, S.mass.p, S.mu.p, S.nCount.p, S.nFluid.p, S.nPtr.p,
                     S.nInd.p, aslices(S), S.selfIdx.p, S.b.p, factored(S) ? S.falpha.p : nullptr,
-                    factored(S) ? S.fdiag.p : nullptr, factored(S) ? S.fwrap.p : nullptr);
+                    factored(S) ? S.fdiag.p : nullptr, factored(S) ? S.fwrap.p : nullptr,
+                    runsums ? S.runStart.p : nullptr, runsums ? S.fblk.p : nullptr);
     return MPH_OK;
 }
 
@@ -464,7 +468,7 @@ int phase_multigrid(mph_scene& S) {
     HIPCHK(hipMemcpyAsync(S.lvDev.p, S.hlv.data(), depth * sizeof(Level), hipMemcpyHostToDevice, S.st));
     if (!S.blocks) {  // factored form: level 0 and InvDiagA from the factored scalars
         if ((rc = S.fgtab.ensure(81 * (size_t)S.P.np)) || (rc = S.fgmask.ensure(S.P.np))) return rc;
-        launch_fac_level0(S.st, S.P, S.lv[0].L, fac_op(S, false), S.cellIndex.p, S.cellStart.p, S.runStart.p,
+        launch_fac_level0(S.st, S.P, S.lv[0].L, fac_op(S, false), S.cellIndex.p, S.cellStart.p, S.fblk.p,
                           S.fgtab.p, S.fgmask.p, S.invDiagA.p);
     } else {
         launch_inv_diag(S.st, S.P, S.selfIdx.p, aslices(S), S.invDiagA.p);
@@ -1128,6 +1132,7 @@ void destroy(mph_scene* S) {
     for (hipEvent_t& e : S->crEv)
         if (e) (void)hipEventDestroy(e);
     S->fgtab.release();
+    S->fblk.release();
     S->runStart.release();
     for (DBuf<double>* b : {&S->falpha, &S->fdiag, &S->selfC, &S->fbeta, &S->frec}) b->release();
     S->fu32.release();
@@ -1793,7 +1798,7 @@ int mph_scene_device_bytes(mph_scene* S, unsigned long long* bytes) {
                          &S->nCount, &S->nFluid, &S->nCountP, &S->nInd, &S->nIndP, &S->width, &S->selfIdx, &S->acol,
                          &S->maxP, &S->runP, &S->fwrap, &S->fgmask, &S->scrM, &S->scrP, &S->searchOvf})
         n += b->cap * sizeof(int);
-    n += S->fgtab.cap * sizeof(double);
+    n += (S->fgtab.cap + S->fblk.cap) * sizeof(double);
     for (DBuf<double>* b : {&S->falpha, &S->fdiag, &S->selfC, &S->fbeta, &S->frec}) n += b->cap * sizeof(double);
     n += (S->fu32.cap + S->fv32.cap + S->frec32.cap) * sizeof(float);
     n += S->fslot.cap * sizeof(unsigned short);

@@ -183,6 +183,7 @@ struct mph_scene {
     // factored operator (MPH_OPERATOR_FACTORED): scalars beside the neighbour lists
     mph::host::DBuf<int> runP, fwrap, fgmask;
     mph::host::DBuf<double> fgtab;  // level 0 from the factored form: per-particle G tables
+    mph::host::DBuf<double> fblk;   // and the viscous run sums [nf][kBlkRuns][kBlkStride]
     bool blocks = true;  // this step assembles the 3x3 blocks (see blocks_wanted)
     mph::host::DBuf<double> falpha, fdiag, selfC, fbeta, frec;  // FacOp::alpha, diag, selfC, beta, rec
     mph::host::DBuf<float> fu32, fv32, frec32;  // FP32 form (mixed precision): FacOp::u32, v32, rec32
