@@ -267,6 +267,10 @@ def run_gpu(args, precision="fp64", form=None):
             "phase_ms": {PHASES[k - 1]: round(st.ms_phase[k], 3) for k in range(1, 12) if st.ms_phase[k] > 0},
             "phase_ms_step_total": round(st.ms_total, 3),
         }
+    else:
+        # one more step, untimed, so that the fields read below are at the same
+        # time step as the FP64 legs' (whose last step is the phase-timed one)
+        scene.step(1)
     tuned = {"spmv_fp64_variant": int(scene.scalar("TunedSpmv64")), "spmv_fp32_variant": int(scene.scalar("TunedSpmv32")),
              "cac_variant": int(scene.scalar("TunedCac")), "fuse_variant": int(scene.scalar("TunedFuse")),
              "galerkin_variant": int(scene.scalar("TunedGalerkin"))}
