@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(256) kb_mg_g2p(const BScene* tab, const int* o
 
 // levels fuse..depth-1 of each scene in its own workgroup; cur.lv[l] holds the
 // ping-pong bit of level l per scene
-__global__ void __launch_bounds__(1024) kb_mg_coarse(const BScene* tab, int ns, SceneSet on, CurBits cb, int variant,
+__global__ void __launch_bounds__(kCoarseThreads) kb_mg_coarse(const BScene* tab, int ns, SceneSet on, CurBits cb, int variant,
                                                     int f32) {
     const int s = blockIdx.x;
     if (s >= ns || !in_set(on, s)) return;
@@ -399,7 +399,7 @@ void mg_g2p(const Tables& T, const SceneSet& on, const SceneSet& cur0, int in, i
 
 void mg_coarse(const Tables& T, const SceneSet& on, const CurBits& cb, int variant, bool f32) {
     if (T.ns > 0 && any(on))
-        hipLaunchKernelGGL(kb_mg_coarse, dim3(T.ns), dim3(1024), 0, T.st, T.tab, T.ns, on, cb, variant, f32 ? 1 : 0);
+        hipLaunchKernelGGL(kb_mg_coarse, dim3(T.ns), dim3(kCoarseThreads), 0, T.st, T.tab, T.ns, on, cb, variant, f32 ? 1 : 0);
 }
 
 #undef MPH_LAUNCH

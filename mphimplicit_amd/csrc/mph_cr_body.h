@@ -299,51 +299,11 @@ __device__ inline void mg_prolong_row(const Level& L, const Level& S, int row) {
 // on entry, lc-1 included; it advances as the separate launches would advance it
 // (afterwards every fused level but the coarsest has flipped).  f32: use a level's FP32
 // stencil where it has one.
-// mg_row_sum with three stencil cells loaded at a time: the same terms in the same
-// order in fewer registers, for the one-workgroup coarse cycle (1024 threads, so at
-// most 128 VGPRs: the 27-cell batches of mg_row_sum spill there)
-template <class VT>
-__device__ inline double mg_row_sum_narrow(const Level& L, const VT* __restrict__ A, const double* __restrict__ x,
-                                           int row) {
-    int gx, gy, gz;
-    grid_xyz(L, row / 3, gx, gy, gz);
-    const int cs1 = L.cs[1], cs2 = L.cs[2], N = L.cs[0] * cs1 * cs2;
-    const int c0 = L.cnt[0], c1 = L.cnt[1], c2 = L.cnt[2];
-    const long long nrow = L.nrow;
-    double sum = 0.0;
-    int ex = 0, ey = 0, ez = 0;
-    for (int q0 = 0; q0 < N; q0 += 3) {
-        VT av[9];
-        double xv[9];
-#pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            const int q = q0 + t;
-            if (q < N) {
-                const int jx = wrap3(gx + ex - 1 + c0, c0), jy = wrap3(gy + ey - 1 + c1, c1);
-                const int jz = wrap3(gz + ez - 1 + c2, c2);
-                const double* xc = x + 3 * ((c1 * c2) * jx + c2 * jy + jz);
-#pragma unroll
-                for (int sd = 0; sd < 3; ++sd) {
-                    av[3 * t + sd] = A[(3 * q + sd) * nrow + row];
-                    xv[3 * t + sd] = xc[sd];
-                }
-                if (++ez == cs2) {
-                    ez = 0;
-                    if (++ey == cs1) {
-                        ey = 0;
-                        ++ex;
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < 3; ++t)
-            if (q0 + t < N)
-#pragma unroll
-                for (int sd = 0; sd < 3; ++sd) sum += -1.0 * (double)av[3 * t + sd] * xv[3 * t + sd];
-    }
-    return sum;
-}
+// The workgroup has kCoarseThreads threads: few enough that mg_row_sum's batches of
+// nine stencil cells (54 values in flight) fit in registers, so a sweep waits for
+// three rounds of loads per row.  (With 1024 threads and three cells per batch it
+// waited for nine: 68 us per cycle on the 1M-particle scene, levels 3 and 4.)
+constexpr int kCoarseThreads = 256;
 
 __device__ inline void mg_coarse_cycle(const Level* lvs, int lc, int D, unsigned cur, bool f32) {
     auto lev = [&](int l) {
@@ -351,10 +311,9 @@ __device__ inline void mg_coarse_cycle(const Level* lvs, int lc, int D, unsigned
         L.cur = (cur >> l) & 1u;
         return L;
     };
-    auto sweep = [&](const Level& L, int row, bool update) {  // mg_residual(_update)_row, narrow sum
+    auto sweep = [&](const Level& L, int row, bool update) {  // mg_residual(_update)_row
         const double* So = lv_s(L, L.cur);
-        const double sum = f32 && L.A32 ? mg_row_sum_narrow<float>(L, L.A32, So, row)
-                                        : mg_row_sum_narrow<double>(L, L.A, So, row);
+        const double sum = f32 && L.A32 ? mg_row_sum<float>(L, L.A32, So, row) : mg_row_sum<double>(L, L.A, So, row);
         double y = L.Q[row];
         y *= 1.0;
         y += sum;
@@ -1085,22 +1044,27 @@ __device__ inline void dot_final_body(const double* __restrict__ part, int nchun
 }
 
 // p = beta*p + 1.0*s, q = beta*q + 1.0*y with beta = rho/rho_prev (4360-4364)
+// one element of p = beta p + s or q = beta q + y (myDscal then myDaxpy, 4362-4366),
+// with the same roundings in every kernel that forms it
+__device__ inline double pq_elem(double beta, double v, double w) {
+    v *= beta;
+    v += 1.0 * w;
+    return v;
+}
+
 __device__ inline void pq_update_elem(const double* __restrict__ scal, const double* __restrict__ s,
                                       const double* __restrict__ y, double* __restrict__ p, double* __restrict__ q,
                                       long long k) {
     const double beta = scal[kRHO] / scal[kRHOP];
-    double pp = p[k];
-    pp *= beta;
-    pp += 1.0 * s[k];
-    p[k] = pp;
-    double qq = q[k];
-    qq *= beta;
-    qq += 1.0 * y[k];
-    q[k] = qq;
+    p[k] = pq_elem(beta, p[k], s[k]);
+    q[k] = pq_elem(beta, q[k], y[k]);
 }
 
-// alpha = rho/tmp; x += alpha p; r -= alpha q; s -= alpha u (4372-4375) + r.r partial
-__device__ inline void xrs_update_chunk(int nf, const double* __restrict__ scal, const double* __restrict__ p,
+// alpha = rho/tmp; x += alpha p; r -= alpha q; s -= alpha u (4372-4375) + r.r partial.
+// kPq: p = beta p + s first (with s before its update), written back; the CR loop's
+// p update is then this kernel's, not a launch of its own
+template <bool kPq = false>
+__device__ inline void xrs_update_chunk(int nf, const double* __restrict__ scal, double* __restrict__ p,
                                         const double* __restrict__ q, const double* __restrict__ u,
                                         double* __restrict__ x, double* __restrict__ r, double* __restrict__ s,
                                         double* __restrict__ part, int chunk, double* sh) {
@@ -1109,10 +1073,16 @@ __device__ inline void xrs_update_chunk(int nf, const double* __restrict__ scal,
     if (i < nf) {
         const double alpha = scal[kRHO] / scal[kTMP];
         const double ma = -alpha;
+        const double beta = kPq ? scal[kRHO] / scal[kRHOP] : 0.0;
         double rr[3];
         for (int d = 0; d < 3; ++d) {
             const int k = 3 * i + d;
-            x[k] += alpha * p[k];
+            double pk = p[k];
+            if (kPq) {
+                pk = pq_elem(beta, pk, s[k]);
+                p[k] = pk;
+            }
+            x[k] += alpha * pk;
             rr[d] = r[k] + ma * q[k];
             r[k] = rr[d];
             s[k] += ma * u[k];

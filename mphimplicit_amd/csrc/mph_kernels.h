@@ -209,6 +209,9 @@ void launch_fill(hipStream_t st, double* x, long long n, double v);
 void launch_fill_i32(hipStream_t st, int* x, long long n, int v);  // a kernel, not a memset node
 void launch_mg_p2g(hipStream_t st, const DevParams& P, Level& L, const int* cellStart, const double* q);
 void launch_mg_p2g_bu(hipStream_t st, const DevParams& P, Level& L, const int* cellStart, const double* q);
+// the same after q = beta q + y (beta = rho / rho_prev from scal), written back to q
+void launch_mg_p2g_bu_pq(hipStream_t st, const DevParams& P, Level& L, const int* cellStart, double* q,
+                         const double* y, const double* scal);
 void launch_mg_restrict_bu(hipStream_t st, const Level& S, Level& L);  // restriction, begin, update
 void launch_mg_begin(hipStream_t st, Level& L);        // S *= 0; R = Q
 void launch_mg_update(hipStream_t st, Level& L);       // S += invD R
@@ -283,8 +286,9 @@ void launch_dot_final_record(hipStream_t st, const DevParams& P, const double* p
                              long long it, double* rec, long long cap);
 void launch_pq_update(hipStream_t st, const DevParams& P, const double* scal, const double* s, const double* y,
                       double* p, double* q);
-void launch_xrs_update(hipStream_t st, const DevParams& P, const double* scal, const double* p, const double* q,
-                       const double* u, double* x, double* r, double* s, double* part);
+// pq: p = beta p + s first, written back
+void launch_xrs_update(hipStream_t st, const DevParams& P, const double* scal, double* p, const double* q,
+                       const double* u, double* x, double* r, double* s, double* part, bool pq = false);
 void launch_store_velocity(hipStream_t st, const DevParams& P, const double* x, double* vel);
 // ---- output-step diagnostics ----
 void launch_diagnostics(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,

@@ -491,14 +491,16 @@ int phase_multigrid(mph_scene& S) {
 }
 
 // preconditionWithMultiGrid (4122-4266): out = D^-1 in + G2P V(P2G in); `part`
-// receives the chunk partials of in.out when non-null
-void precondition(mph_scene& S, const double* in, double* out, double* part) {
+// receives the chunk partials of in.out when non-null.  pqy non-null: in is S.q, and
+// the P2G forms the CR update q = beta q + pqy (4364-4366) as it reads q
+void precondition(mph_scene& S, const double* in, double* out, double* part, const double* pqy = nullptr) {
     hipStream_t st = S.st;
     const int v = S.tuneFuse.pick();
     const int D = S.depth, F = S.fuseOf[v];  // levels F..D-1 run fused (F = D: none)
     S.tuneFuse.begin(st, v);
     Level& L0 = S.lv[0].L;
-    launch_mg_p2g_bu(st, S.P, L0, S.cellStart.p, in);  // P2G, S = 0, R = Q, S += invD R
+    if (pqy) launch_mg_p2g_bu_pq(st, S.P, L0, S.cellStart.p, S.q.p, pqy, S.scal.p);
+    else launch_mg_p2g_bu(st, S.P, L0, S.cellStart.p, in);  // P2G, S = 0, R = Q, S += invD R
     launch_mg_residual_update(st, L0);                  // pre-smooth x2
     launch_mg_residual(st, L0);
     for (int l = 1; l < D - 1 && l < F; ++l) {
@@ -537,6 +539,25 @@ int apply_m(mph_scene& S, const double* in, double* out, double* part) {
         HIPCHK(hipMemcpyAsync(out, in, sizeof(double) * 3 * (size_t)S.P.nf, hipMemcpyDeviceToDevice, S.st));
         if (part) launch_dot_partial(S.st, S.P, in, out, part);
     }
+    return MPH_OK;
+}
+
+// The rest of a CR iteration once rho = s.y is reduced (4362-4377): later iterations
+// p = beta p + s and q = beta q + y; u = M^-1 q, tmp = q.u; then xv += alpha p,
+// r -= alpha q, s -= alpha u with the r.r partials in S.part.  With the multigrid
+// preconditioner the p and q updates are formed by the kernels that read those
+// vectors next (the P2G of q; the xv update, which reads p): the same arithmetic
+// with one launch and two vector passes fewer.
+int cr_tail(mph_scene& S, bool first, double* xv) {
+    const DevParams& P = S.P;
+    hipStream_t st = S.st;
+    const bool fuse = !first && S.prm.multigrid;
+    int rc;
+    if (!first && !fuse) launch_pq_update(st, P, S.scal.p, S.s.p, S.y.p, S.p.p, S.q.p);
+    if (fuse) precondition(S, S.q.p, S.u.p, S.part.p, S.y.p);  // u = M^-1 q; tmp = q.u
+    else if ((rc = apply_m(S, S.q.p, S.u.p, S.part.p))) return rc;
+    launch_dot_final(st, P, S.part.p, S.scal.p, kTMP, -1);
+    launch_xrs_update(st, P, S.scal.p, S.p.p, S.q.p, S.u.p, xv, S.r.p, S.s.p, S.part.p, fuse);
     return MPH_OK;
 }
 
@@ -741,11 +762,8 @@ int cr_iteration(mph_scene& S, bool first, bool dev, long long maxit, long long 
     } else {
         spmv(S, 1.0, S.s.p, 0.0, S.y.p, S.y.p, nullptr, nullptr, S.s.p, S.part.p);
         launch_dot_final(st, P, S.part.p, S.scal.p, kRHO, kRHOP);
-        launch_pq_update(st, P, S.scal.p, S.s.p, S.y.p, S.p.p, S.q.p);
     }
-    if ((rc = apply_m(S, S.q.p, S.u.p, S.part.p))) return rc;  // u = M^-1 q; tmp = q.u
-    launch_dot_final(st, P, S.part.p, S.scal.p, kTMP, -1);
-    launch_xrs_update(st, P, S.scal.p, S.p.p, S.q.p, S.u.p, S.x.p, S.r.p, S.s.p, S.part.p);
+    if ((rc = cr_tail(S, first, S.x.p))) return rc;
     if (dev) launch_dot_final_stop(st, P, S.part.p, S.scal.p, S.ctl.p, 1, S.prm.cr_tolerance, maxit, S.hctl, S.hrr);
     else launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1, rr_host(S));
     return MPH_OK;
@@ -1004,11 +1022,8 @@ int phase_solve_mixed(mph_scene& S) {
             } else {
                 spmv(S, 1.0, S.s.p, 0.0, S.y.p, S.y.p, nullptr, nullptr, S.s.p, S.part.p, true);
                 launch_dot_final(st, P, S.part.p, S.scal.p, kRHO, kRHOP);
-                launch_pq_update(st, P, S.scal.p, S.s.p, S.y.p, S.p.p, S.q.p);
             }
-            if ((rc = apply_m(S, S.q.p, S.u.p, S.part.p))) return rc;
-            launch_dot_final(st, P, S.part.p, S.scal.p, kTMP, -1);
-            launch_xrs_update(st, P, S.scal.p, S.p.p, S.q.p, S.u.p, S.dvec.p, S.r.p, S.s.p, S.part.p);
+            if ((rc = cr_tail(S, it == 0, S.dvec.p))) return rc;
             launch_dot_final(st, P, S.part.p, S.scal.p, kRR, -1, rr_host(S));
         }
         S.facPref = nullptr;  // a first pass enqueued ahead of the inner stop is not used

@@ -231,11 +231,17 @@ __global__ void k_mg_p2g(DevParams P, Level L, const int* __restrict__ cellStart
 // (kP2gK per pass), and the row's first lane adds them in particle order from LDS,
 // as mg_p2g_row does.  Level 0 has one row per
 // cell and component, too few to fill the GPU with one thread each.
+// kPq: the CR loop's q = beta q + y (4364-4366) is formed here as q is loaded and
+// written back (every fluid particle lies in exactly one level-0 cell, so each
+// element is updated once); q is written only then.
 constexpr int kP2gLanes = 8, kP2gK = 8;
+template <bool kPq>
 __global__ void __launch_bounds__(256) k_mg_p2g_bu_lanes(DevParams P, Level L, const int* __restrict__ cellStart,
-                                                         const double* __restrict__ q) {
+                                                         double* q, const double* __restrict__ yv,
+                                                         const double* __restrict__ scal) {
     if (cr_halted(L.halt)) return;
     __shared__ double tb[256 * kP2gK];
+    const double beta = kPq ? scal[kRHO] / scal[kRHOP] : 0.0;
     const int u = threadIdx.x & (kP2gLanes - 1);
     const int row = (int)(((long long)blockIdx.x * 256 + threadIdx.x) / kP2gLanes);
     const bool on = row < L.nrow;
@@ -255,6 +261,10 @@ __global__ void __launch_bounds__(256) k_mg_p2g_bu_lanes(DevParams P, Level L, c
         for (int k = 0; k < kP2gK; ++k) {
             const int ip = base + u + kP2gLanes * k;
             t[k] = ip < p1 ? q[3 * ip + rd] : 0.0;
+            if (kPq && ip < p1) {
+                t[k] = pq_elem(beta, t[k], yv[3 * ip + rd]);
+                q[3 * ip + rd] = t[k];
+            }
         }
 #pragma unroll
         for (int k = 0; k < kP2gK; ++k) tb[kP2gK * threadIdx.x + k] = t[k];
@@ -404,7 +414,7 @@ __global__ void k_mg_prolong(Level L, Level S) {
 }
 
 // cur: ping-pong bits of levels lc-1..D-1 (bit l), tracked by the host
-__global__ void __launch_bounds__(1024) k_mg_coarse(const Level* __restrict__ lvs, int lc, int D, unsigned cur,
+__global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse(const Level* __restrict__ lvs, int lc, int D, unsigned cur,
                                                    int f32, const int* halt) {
     if (cr_halted(halt)) return;
     mg_coarse_cycle(lvs, lc, D, cur, f32 != 0);
@@ -503,14 +513,15 @@ __global__ void k_pq_update(int nf, const double* __restrict__ scal, const doubl
     if (k < 3LL * nf) pq_update_elem(scal, s, y, p, q, k);
 }
 
+template <bool kPq>
 __global__ void __launch_bounds__(256) k_xrs_update(int nf, const double* __restrict__ scal,
-                                                    const double* __restrict__ p, const double* __restrict__ q,
+                                                    double* __restrict__ p, const double* __restrict__ q,
                                                     const double* __restrict__ u, double* __restrict__ x,
                                                     double* __restrict__ r, double* __restrict__ s,
                                                     double* __restrict__ part, const int* halt) {
     if (cr_halted(halt)) return;
     __shared__ double sh[256];
-    xrs_update_chunk(nf, scal, p, q, u, x, r, s, part, blockIdx.x, sh);
+    xrs_update_chunk<kPq>(nf, scal, p, q, u, x, r, s, part, blockIdx.x, sh);
 }
 
 __global__ void k_store_velocity(int nf, const double* __restrict__ x, double* __restrict__ vel) {
@@ -739,8 +750,14 @@ void launch_mg_p2g(hipStream_t st, const DevParams& P, Level& L, const int* cell
     hipLaunchKernelGGL(k_mg_p2g, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, P, L, cellStart, q);
 }
 void launch_mg_p2g_bu(hipStream_t st, const DevParams& P, Level& L, const int* cellStart, const double* q) {
-    hipLaunchKernelGGL(k_mg_p2g_bu_lanes, dim3(nblk((long long)kP2gLanes * L.nrow, 256)), dim3(256), 0, st, P, L,
-                       cellStart, q);
+    // not written without kPq
+    hipLaunchKernelGGL(k_mg_p2g_bu_lanes<false>, dim3(nblk((long long)kP2gLanes * L.nrow, 256)), dim3(256), 0, st, P,
+                       L, cellStart, const_cast<double*>(q), nullptr, nullptr);
+}
+void launch_mg_p2g_bu_pq(hipStream_t st, const DevParams& P, Level& L, const int* cellStart, double* q,
+                         const double* y, const double* scal) {
+    hipLaunchKernelGGL(k_mg_p2g_bu_lanes<true>, dim3(nblk((long long)kP2gLanes * L.nrow, 256)), dim3(256), 0, st, P,
+                       L, cellStart, q, y, scal);
 }
 void launch_mg_restrict_bu(hipStream_t st, const Level& S, Level& L) {
     hipLaunchKernelGGL(k_mg_restrict_bu, dim3(nblk(L.nrow, 256)), dim3(256), 0, st, S, L);
@@ -770,7 +787,7 @@ void launch_mg_prolong(hipStream_t st, const Level& L, Level& S) {
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(S.nrow, 256)), dim3(256), 0, st, L, S);
 }
 void launch_mg_coarse(hipStream_t st, const Level* lvs, int lc, int D, unsigned cur, bool f32, const int* halt) {
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, st, lvs, lc, D, cur, f32 ? 1 : 0, halt);
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(kCoarseThreads), 0, st, lvs, lc, D, cur, f32 ? 1 : 0, halt);
 }
 void launch_mg_g2p(hipStream_t st, const DevParams& P, const Level& L0, const int* cellIndex,
                    const double* invDiagA, const double* q, double* u, double* part) {
@@ -850,10 +867,14 @@ void launch_pq_update(hipStream_t st, const DevParams& P, const double* scal, co
                       double* p, double* q) {
     hipLaunchKernelGGL(k_pq_update, dim3(nblk(3LL * P.nf, 256)), dim3(256), 0, st, P.nf, scal, s, y, p, q, P.halt);
 }
-void launch_xrs_update(hipStream_t st, const DevParams& P, const double* scal, const double* p, const double* q,
-                       const double* u, double* x, double* r, double* s, double* part) {
-    hipLaunchKernelGGL(k_xrs_update, dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P.nf, scal, p, q, u, x, r, s,
-                       part, P.halt);
+void launch_xrs_update(hipStream_t st, const DevParams& P, const double* scal, double* p, const double* q,
+                       const double* u, double* x, double* r, double* s, double* part, bool pq) {
+    if (pq)
+        hipLaunchKernelGGL(k_xrs_update<true>, dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P.nf, scal, p, q, u, x,
+                           r, s, part, P.halt);
+    else
+        hipLaunchKernelGGL(k_xrs_update<false>, dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P.nf, scal, p, q, u, x,
+                           r, s, part, P.halt);
 }
 void launch_store_velocity(hipStream_t st, const DevParams& P, const double* x, double* vel) {
     hipLaunchKernelGGL(k_store_velocity, dim3(nblk(3LL * P.nf, 256)), dim3(256), 0, st, P.nf, x, vel);
