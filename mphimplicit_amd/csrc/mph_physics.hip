@@ -167,13 +167,18 @@ __global__ void k_coefficients(DevParams P, const int* __restrict__ type, const 
 
 // calculatePressureA forces (1814-1848), calculateDiffuseInterface (1851-1901),
 // calculateGravity (2065-2074), calculateAcceleration (2076-2086)
+// kSurf false: every type's CofA is 0 (no surface tension).  The DiffuseInterface
+// terms are then zeros of either sign, and fd, which starts at +0, stays +0 under
+// fd -= (+-0); they are not formed, and the force keeps every bit.
+template <bool kSurf>
 __global__ void __launch_bounds__(256) k_forces(DevParams P, const int* __restrict__ type,
                                                 const double* __restrict__ pos, const int* __restrict__ nCount,
                                                 const long long* __restrict__ nPtr, const int* __restrict__ nInd,
                                                 const double* __restrict__ pressA, const double* __restrict__ gcen,
                                                 const double* __restrict__ mass, double* __restrict__ force,
                                                 double* __restrict__ vel) {
-    constexpr int G = kLanes, K = 9;  // terms: PressureA force, the two DiffuseInterface ones (3 each)
+    // terms: PressureA force, the two DiffuseInterface ones (3 each)
+    constexpr int G = kLanes, K = kSurf ? 9 : 3;
     __shared__ double tb[256 * K];
     __shared__ int tf[256];
     const int g = threadIdx.x & (G - 1);
@@ -194,13 +199,16 @@ __global__ void __launch_bounds__(256) k_forces(DevParams P, const int* __restri
         const int jcur = jnext;
         jnext = jn + G < n ? nInd[b + jn + G] : 0;
         int f = 0;
-        double t[K] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double t[K];
+        for (int e = 0; e < K; ++e) t[e] = 0.0;
         if (jn < n) {
             const int j = jcur;
             // j's values load with its position, not after the distance tests
             const int tj = type[j];
             const double paj = pressA[j];
-            const double gj[3] = {gcen[3 * j], gcen[3 * j + 1], gcen[3 * j + 2]};
+            double gj[3] = {0.0, 0.0, 0.0};
+            if (kSurf)
+                for (int d = 0; d < 3; ++d) gj[d] = gcen[3 * j + d];
             const Rel q = rel(P, pos, i, j);
             if (i != j && q.r2 != 0.0) {
                 const double rij_ = P.ratio[ti][tj], rji_ = P.ratio[tj][ti];
@@ -211,7 +219,7 @@ __global__ void __launch_bounds__(256) k_forces(DevParams P, const int* __restri
                     for (int d = 0; d < 3; ++d) t[d] = (pai * dwij + paj * dwji) * (q.x[d] / r) * P.V;
                     f |= 1;
                 }
-                if (P.radG2 - q.r2 > 0) {
+                if (kSurf && P.radG2 - q.r2 > 0) {
                     const double r = sqrt(q.r2);
                     const double wij = rij_ * wq_k(P.Kg, r, P.radG);
                     const double wji = rji_ * wq_k(P.Kg, r, P.radG);
@@ -236,7 +244,7 @@ __global__ void __launch_bounds__(256) k_forces(DevParams P, const int* __restri
                 const double* tu = tb + K * sl;
                 if (fu & 1)
                     for (int d = 0; d < 3; ++d) fa[d] += tu[d];
-                if (fu & 2) {
+                if (kSurf && (fu & 2)) {
                     for (int d = 0; d < 3; ++d) fd[d] -= tu[3 + d];
                     for (int d = 0; d < 3; ++d) fd[d] -= tu[6 + d];
                 }
@@ -688,8 +696,13 @@ void launch_coefficients(hipStream_t st, const DevParams& P, const int* type, co
 void launch_forces(hipStream_t st, const DevParams& P, const int* type, const double* pos, const int* nCount,
                    const long long* nPtr, const int* nInd, const double* pressA, const double* gcen,
                    const double* mass, double* force, double* vel) {
-    hipLaunchKernelGGL(k_forces, dim3(nblk((long long)kLanes * P.np, 256)), dim3(256), 0, st, P, type, pos, nCount,
-                       nPtr, nInd, pressA, gcen, mass, force, vel);
+    const bool surf = !(P.ai[0] == 0.0 && P.ai[1] == 0.0 && P.ai[2] == 0.0 && P.ai[3] == 0.0);
+    if (surf)
+        hipLaunchKernelGGL(k_forces<true>, dim3(nblk((long long)kLanes * P.np, 256)), dim3(256), 0, st, P, type, pos,
+                           nCount, nPtr, nInd, pressA, gcen, mass, force, vel);
+    else
+        hipLaunchKernelGGL(k_forces<false>, dim3(nblk((long long)kLanes * P.np, 256)), dim3(256), 0, st, P, type, pos,
+                           nCount, nPtr, nInd, pressA, gcen, mass, force, vel);
 }
 
 void launch_matrix_a(hipStream_t st, const DevParams& P, const int* type, const double* pos, const double* vel,
