@@ -606,6 +606,10 @@ __global__ void __launch_bounds__(256) k_divergence(DevParams P, const double* _
     pressP[i] = pp;
 }
 
+// kSurf false: every type's CofA is 0.  The diffuse-interface virial terms are then
+// zeros of either sign, sg stays +0, and v (which starts at +0, so is never -0) keeps
+// its bits under v += sg: the terms are not formed.
+template <bool kSurf>
 __global__ void k_diag_forces(DevParams P, const int* __restrict__ type, const double* __restrict__ pos,
                               const double* __restrict__ vel, const double* __restrict__ mu,
                               const double* __restrict__ pressA, const double* __restrict__ gcen,
@@ -619,7 +623,9 @@ __global__ void k_diag_forces(DevParams P, const int* __restrict__ type, const d
     const int ti = type[i];
     const double V = P.V, VoL = P.V / P.l;
     const double ppi = pressP[i], pai = pressA[i], mui = mu[i];
-    const double gi[3] = {gcen[3 * i], gcen[3 * i + 1], gcen[3 * i + 2]};
+    double gi[3] = {0.0, 0.0, 0.0};
+    if (kSurf)
+        for (int d = 0; d < 3; ++d) gi[d] = gcen[3 * i + d];
     const double vi[3] = {vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]};
     const double a_ = P.ai[ti];
     double st1[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -679,7 +685,7 @@ __global__ void k_diag_forces(DevParams P, const int* __restrict__ type, const d
             for (int a = 0; a < 3; ++a)
                 for (int c = 0; c < 3; ++c) sa[3 * a + c] += 1.0 * fij[a] * q.x[c] / V;
         }
-        if (P.radG2 - q.r2 > 0) {  // diffuse-interface virial, both terms (4672-4716)
+        if (kSurf && P.radG2 - q.r2 > 0) {  // diffuse-interface virial, both terms (4672-4716)
             const double ratio = P.ratio[ti][type[j]];
             const double weight = ratio * wq_k(P.Kg, r, P.radG);
             double fij[3];
@@ -847,8 +853,7 @@ void launch_axpy(hipStream_t st, long long n, const double* x, double* y) {
 }
 void launch_dot_partial(hipStream_t st, const DevParams& P, const double* x, const double* y, double* part) {
     hipLaunchKernelGGL(k_dot_partial, dim3(nblk(P.nf, kChunk)), dim3(kChunk), 0, st, P.nf, x, y, part, P.halt);
-}
-void launch_dot_final(hipStream_t st, const DevParams& P, const double* part, double* scal, int slot,
+}void launch_dot_final(hipStream_t st, const DevParams& P, const double* part, double* scal, int slot,
                       int shift_slot, double* host) {
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, st, part, nblk(P.nf, kChunk), scal, slot, shift_slot,
                        host, P.halt);
@@ -886,8 +891,14 @@ void launch_diagnostics(hipStream_t st, const DevParams& P, const int* type, con
                         double* pressP, double* force, double* virialStress, double* virialP) {
     hipLaunchKernelGGL(k_divergence, dim3(nblk(4LL * P.np, 256)), dim3(256), 0, st, P, pos, vel, lambda, kappa,
                        volStrainP, nCount, nPtr, nInd, divP, pressP);
-    hipLaunchKernelGGL(k_diag_forces, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, type, pos, vel, mu, pressA, gcen,
-                       pressP, nCount, nPtr, nInd, nCountP, nPtrP, nIndP, force, virialStress, virialP);
+    const bool surf = !(P.ai[0] == 0.0 && P.ai[1] == 0.0 && P.ai[2] == 0.0 && P.ai[3] == 0.0);
+    if (surf)
+        hipLaunchKernelGGL(k_diag_forces<true>, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, type, pos, vel, mu, pressA,
+                           gcen, pressP, nCount, nPtr, nInd, nCountP, nPtrP, nIndP, force, virialStress, virialP);
+    else
+        hipLaunchKernelGGL(k_diag_forces<false>, dim3(nblk(P.np, 128)), dim3(128), 0, st, P, type, pos, vel, mu,
+                           pressA, gcen, pressP, nCount, nPtr, nInd, nCountP, nPtrP, nIndP, force, virialStress,
+                           virialP);
 }
 
 }  // namespace mph
