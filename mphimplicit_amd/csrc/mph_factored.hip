@@ -599,13 +599,19 @@ __global__ void __launch_bounds__(kCellThreads) k_fac_slots(DevParams P, FacOp F
     int rs, nrows, nv;
     if (!fac_cell_table(P, F, T, rs, nrows, nv, blockIdx.x)) return;
     if (T.total > kCellSlots) return;
-    constexpr int G = body::kFacLanes;
+    constexpr int G = body::kFacLanes, U = 4;  // U entries per lane in flight
     for (int r = threadIdx.x / G; r < nrows; r += kCellThreads / G) {
         const int i = rs + r;
         const long long b0 = F.nPtr[i];
         const int w = F.nFluid[i];
-        for (int jn = threadIdx.x & (G - 1); jn < w; jn += G)
-            F.slot[b0 + jn] = (unsigned short)fac_cell_slot(T, F.nInd[b0 + jn]);
+        for (int jb = threadIdx.x & (G - 1); jb < w; jb += G * U) {
+            int j[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) j[u] = jb + u * G < w ? F.nInd[b0 + jb + u * G] : 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (jb + u * G < w) F.slot[b0 + jb + u * G] = (unsigned short)fac_cell_slot(T, j[u]);
+        }
     }
 }
 
