@@ -197,42 +197,64 @@ __global__ void k_fac_g(DevParams P, const int* __restrict__ cellIndex, const in
     // read and stay unwritten (a slot met again would continue from its stored sum)
     int cur = -1;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int jn = 0; jn < run; ++jn) {
-        const int j = nIndP[c0 + jn];
-        if (j >= P.nf) continue;
-        int jx, jy, jz;
-        cell_xyz(P, cellIndex[j], jx, jy, jz);
-        const int slot = fac_slot(P, jx, jy, jz, kx, ky, kz);
-        if (slot < 0) continue;  // cannot happen: P neighbours are in adjacent cells
-        if (slot != cur) {
-            if (cur >= 0) {
-                G[3 * cur] = a0;
-                G[3 * cur + 1] = a1;
-                G[3 * cur + 2] = a2;
+    // U entries at a time: their columns and beta, then their cell keys and positions,
+    // load together (a wall or past-the-end entry loads k's own); the entries are
+    // then added one by one in list order
+    constexpr int U = 4;
+    for (int jb = 0; jb < run; jb += U) {
+        int jj[U], key[U];
+        double bb[U], q[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long e = c0 + (jb + u < run ? jb + u : run - 1);
+            jj[u] = nIndP[e];
+            bb[u] = beta[e];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = jj[u] < P.nf ? jj[u] : k;
+            key[u] = cellIndex[j];
+            for (int d = 0; d < 3; ++d) q[u][d] = pos[3 * j + d];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = jj[u];
+            if (jb + u >= run || j >= P.nf) continue;
+            int jx, jy, jz;
+            cell_xyz(P, key[u], jx, jy, jz);
+            const int slot = fac_slot(P, jx, jy, jz, kx, ky, kz);
+            if (slot < 0) continue;  // cannot happen: P neighbours are in adjacent cells
+            if (slot != cur) {
+                if (cur >= 0) {
+                    G[3 * cur] = a0;
+                    G[3 * cur + 1] = a1;
+                    G[3 * cur + 2] = a2;
+                }
+                cur = slot;
+                const bool seen = (mask >> slot) & 1u;
+                a0 = seen ? G[3 * slot] : 0.0;
+                a1 = seen ? G[3 * slot + 1] : 0.0;
+                a2 = seen ? G[3 * slot + 2] : 0.0;
             }
-            cur = slot;
-            const bool seen = (mask >> slot) & 1u;
-            a0 = seen ? G[3 * slot] : 0.0;
-            a1 = seen ? G[3 * slot + 1] : 0.0;
-            a2 = seen ? G[3 * slot + 2] : 0.0;
+            double c[3];
+            if (j == k) {
+                c[0] = selfC[3 * k];
+                c[1] = selfC[3 * k + 1];
+                c[2] = selfC[3 * k + 2];
+            } else {  // body::fac_disp on the loaded position
+                const double r0 = min_image(q[u][0], pk0, P.W[0], P.hW[0]);
+                const double r1 = min_image(q[u][1], pk1, P.W[1], P.hW[1]);
+                const double r2 = min_image(q[u][2], pk2, P.W[2], P.hW[2]);
+                const double b = bb[u];
+                c[0] = -(b * r0);
+                c[1] = -(b * r1);
+                c[2] = -(b * r2);
+            }
+            a0 += c[0];
+            a1 += c[1];
+            a2 += c[2];
+            mask |= 1u << slot;
         }
-        double c[3];
-        if (j == k) {
-            c[0] = selfC[3 * k];
-            c[1] = selfC[3 * k + 1];
-            c[2] = selfC[3 * k + 2];
-        } else {
-            double r0, r1, r2;
-            body::fac_disp(P, pos, j, pk0, pk1, pk2, r0, r1, r2);
-            const double b = beta[c0 + jn];
-            c[0] = -(b * r0);
-            c[1] = -(b * r1);
-            c[2] = -(b * r2);
-        }
-        a0 += c[0];
-        a1 += c[1];
-        a2 += c[2];
-        mask |= 1u << slot;
     }
     if (cur >= 0) {
         G[3 * cur] = a0;

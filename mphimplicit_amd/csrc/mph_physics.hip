@@ -502,27 +502,46 @@ __global__ void k_vector_p(DevParams P, const int* __restrict__ type, const doub
     if (volStrainP[i] > 0.0) pv = kappa[i] * volStrainP[i];
     const bool walli = is_wall(type[i]);
     const double li = lambda[i];
+    const double pi[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
     double sum = 0.0;
     const long long b0 = nPtrP[i];
     const int n = nCountP[i];
-    for (int jn = 0; jn < n; ++jn) {
-        const int j = nIndP[b0 + jn];
-        if (i == j) continue;
-        const Rel q = rel(P, pos, i, j);
-        if (q.r2 == 0.0) continue;
-        if (P.radP2 - q.r2 > 0) {
-            const double dij = sqrt(q.r2);
-            const double e[3] = {q.x[0] / dij, q.x[1] / dij, q.x[2] / dij};
-            const double wpdrij = -dwq_k(P.Kp, P.m2P, dij, P.radP);
-            const bool wallj = is_wall(type[j]);
-            for (int s = 0; s < 3; ++s) {
-                if (walli) {
-                    const double coef = +e[s] * wpdrij;
-                    sum += li * coef * vel[3 * i + s];
-                }
-                if (wallj) {
-                    const double coef = -e[s] * wpdrij;
-                    sum += li * coef * vel[3 * j + s];
+    // U entries at a time: columns, then types and positions, load together (past the
+    // end: the last entry again, not visited); a pair adds terms only when i or j is a
+    // wall, so fluid pairs stop after the distance test
+    constexpr int U = 4;
+    for (int jb = 0; jb < n; jb += U) {
+        int jj[U], tj[U];
+        double pj[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u) jj[u] = nIndP[b0 + (jb + u < n ? jb + u : n - 1)];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            tj[u] = type[jj[u]];
+            for (int d = 0; d < 3; ++d) pj[u][d] = pos[3 * jj[u] + d];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = jj[u];
+            if (jb + u >= n || i == j) continue;
+            Rel q;  // rel() on the loaded positions
+            for (int d = 0; d < 3; ++d) q.x[d] = min_image(pj[u][d], pi[d], P.W[d], P.hW[d]);
+            q.r2 = (q.x[0] * q.x[0] + q.x[1] * q.x[1] + q.x[2] * q.x[2]);
+            if (q.r2 == 0.0) continue;
+            const bool wallj = is_wall(tj[u]);
+            if (P.radP2 - q.r2 > 0 && (walli || wallj)) {
+                const double dij = sqrt(q.r2);
+                const double e[3] = {q.x[0] / dij, q.x[1] / dij, q.x[2] / dij};
+                const double wpdrij = -dwq_k(P.Kp, P.m2P, dij, P.radP);
+                for (int s = 0; s < 3; ++s) {
+                    if (walli) {
+                        const double coef = +e[s] * wpdrij;
+                        sum += li * coef * vel[3 * i + s];
+                    }
+                    if (wallj) {
+                        const double coef = -e[s] * wpdrij;
+                        sum += li * coef * vel[3 * j + s];
+                    }
                 }
             }
         }
