@@ -79,64 +79,87 @@ __global__ void __launch_bounds__(256) k_densities(DevParams P, const int* __res
                                                    const long long* __restrict__ nPtr, const int* __restrict__ nInd,
                                                    double* __restrict__ densA, double* __restrict__ gcen,
                                                    double* __restrict__ volStrainP) {
-    constexpr int G = kLanes, K = 5;  // terms: DensityA, the three GravityCenter ones, DensityP
-    __shared__ double tb[256 * K];
-    __shared__ int tf[256];
+    // terms: DensityA, the three GravityCenter ones, DensityP.  Each lane takes U
+    // entries per round (entries base + u*G + g), whose loads are in flight together;
+    // the group's first lane adds the round's terms in list order
+    constexpr int G = kLanes, K = 5, U = 2;
+    __shared__ double tb[256 * U * K];
+    __shared__ int tf[256 * U];
     const int g = threadIdx.x & (G - 1);
     const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / G);
     const bool own = i < P.np;
     const int ti = own ? type[i] : 0;
+    double pi[3] = {0.0, 0.0, 0.0};
+    if (own)
+        for (int d = 0; d < 3; ++d) pi[d] = pos[3 * i + d];
     double sa = 0.0, sg0 = 0.0, sg1 = 0.0, sg2 = 0.0, sp = 0.0;
     const long long b = own ? nPtr[i] : 0;
     const int n = own ? nCount[i] : 0;
-    int jnext = g < n ? nInd[b + g] : 0;  // each lane's next entry loads one round ahead
-    for (int base = 0; base < n; base += G) {
-        const int jn = base + g;
-        const int jcur = jnext;
-        jnext = jn + G < n ? nInd[b + jn + G] : 0;
-        int f = 0;
-        double t[K] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (jn < n) {
-            const int j = jcur;
-            if (i != j) {
-                const double ratio = P.ratio[ti][type[j]];
-                const Rel q = rel(P, pos, i, j);
-                if (P.radA2 - q.r2 >= 0) {
-                    const double r = sqrt(q.r2);
-                    t[0] = ratio * wa_k(P.Ka, r, P.radA);
-                    f |= 1;
-                }
-                if (P.radG2 - q.r2 >= 0) {
-                    const double r = sqrt(q.r2);
-                    const double weight = ratio * wq_k(P.Kg, r, P.radG);
-                    t[1] = q.x[0] * weight / P.R2g * P.radG;
-                    t[2] = q.x[1] * weight / P.R2g * P.radG;
-                    t[3] = q.x[2] * weight / P.R2g * P.radG;
-                    f |= 2;
-                }
-                if (P.radP2 - q.r2 >= 0) {
-                    const double r = sqrt(q.r2);
-                    t[4] = wq_k(P.Kp, r, P.radP);
-                    f |= 4;
+    int jnext[U];  // each lane's next entries load one round ahead
+    for (int u = 0; u < U; ++u) jnext[u] = g + u * G < n ? nInd[b + g + u * G] : 0;
+    for (int base = 0; base < n; base += G * U) {
+        int jcur[U];
+        for (int u = 0; u < U; ++u) {
+            const int jn = base + u * G + g;
+            jcur[u] = jnext[u];
+            jnext[u] = jn + G * U < n ? nInd[b + jn + G * U] : 0;
+        }
+        // the round's loads, before any branch (entries past the list read particle 0)
+        int tj[U];
+        double pj[U][3];
+        for (int u = 0; u < U; ++u) {
+            tj[u] = type[jcur[u]];
+            for (int d = 0; d < 3; ++d) pj[u][d] = pos[3 * jcur[u] + d];
+        }
+        for (int u = 0; u < U; ++u) {
+            const int jn = base + u * G + g;
+            int f = 0;
+            double t[K] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            if (jn < n) {
+                const int j = jcur[u];
+                if (i != j) {
+                    const double ratio = P.ratio[ti][tj[u]];
+                    Rel q;  // rel() on the loaded positions
+                    for (int d = 0; d < 3; ++d) q.x[d] = min_image(pj[u][d], pi[d], P.W[d], P.hW[d]);
+                    q.r2 = (q.x[0] * q.x[0] + q.x[1] * q.x[1] + q.x[2] * q.x[2]);
+                    if (P.radA2 - q.r2 >= 0) {
+                        const double r = sqrt(q.r2);
+                        t[0] = ratio * wa_k(P.Ka, r, P.radA);
+                        f |= 1;
+                    }
+                    if (P.radG2 - q.r2 >= 0) {
+                        const double r = sqrt(q.r2);
+                        const double weight = ratio * wq_k(P.Kg, r, P.radG);
+                        t[1] = q.x[0] * weight / P.R2g * P.radG;
+                        t[2] = q.x[1] * weight / P.R2g * P.radG;
+                        t[3] = q.x[2] * weight / P.R2g * P.radG;
+                        f |= 2;
+                    }
+                    if (P.radP2 - q.r2 >= 0) {
+                        const double r = sqrt(q.r2);
+                        t[4] = wq_k(P.Kp, r, P.radP);
+                        f |= 4;
+                    }
                 }
             }
+            for (int e = 0; e < K; ++e) tb[K * (U * threadIdx.x + u) + e] = t[e];
+            tf[U * threadIdx.x + u] = f;
         }
-        for (int e = 0; e < K; ++e) tb[K * threadIdx.x + e] = t[e];
-        tf[threadIdx.x] = f;
         wave_sync();
         if (g == 0)
-            for (int u = 0; u < G && base + u < n; ++u) {
-                const int sl = threadIdx.x + u;
-                const int fu = tf[sl];
-                const double* tu = tb + K * sl;
-                if (fu & 1) sa += tu[0];
-                if (fu & 2) {
-                    sg0 += tu[1];
-                    sg1 += tu[2];
-                    sg2 += tu[3];
+            for (int u = 0; u < U; ++u)
+                for (int v = 0; v < G && base + u * G + v < n; ++v) {
+                    const int sl = U * (threadIdx.x + v) + u;
+                    const int fu = tf[sl];
+                    const double* tu = tb + K * sl;
+                    if (fu & 1) sa += tu[0];
+                    if (fu & 2) {
+                        sg0 += tu[1];
+                        sg1 += tu[2];
+                        sg2 += tu[3];
+                    }
+                    if (fu & 4) sp += tu[4];
                 }
-                if (fu & 4) sp += tu[4];
-            }
         wave_sync();
     }
     if (g == 0 && own) {
@@ -178,9 +201,9 @@ __global__ void __launch_bounds__(256) k_forces(DevParams P, const int* __restri
                                                 const double* __restrict__ mass, double* __restrict__ force,
                                                 double* __restrict__ vel) {
     // terms: PressureA force, the two DiffuseInterface ones (3 each)
-    constexpr int G = kLanes, K = kSurf ? 9 : 3;
-    __shared__ double tb[256 * K];
-    __shared__ int tf[256];
+    constexpr int G = kLanes, K = kSurf ? 9 : 3, U = 2;
+    __shared__ double tb[256 * U * K];
+    __shared__ int tf[256 * U];
     const int g = threadIdx.x & (G - 1);
     const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / G);
     const bool own = i < P.np;
@@ -193,23 +216,44 @@ __global__ void __launch_bounds__(256) k_forces(DevParams P, const int* __restri
     double fa[3] = {0.0, 0.0, 0.0}, fd[3] = {0.0, 0.0, 0.0};
     const long long b = own ? nPtr[i] : 0;
     const int n = own ? nCount[i] : 0;
-    int jnext = g < n ? nInd[b + g] : 0;  // each lane's next entry loads one round ahead
-    for (int base = 0; base < n; base += G) {
-        const int jn = base + g;
-        const int jcur = jnext;
-        jnext = jn + G < n ? nInd[b + jn + G] : 0;
+    double pi[3] = {0.0, 0.0, 0.0};
+    if (own)
+        for (int d = 0; d < 3; ++d) pi[d] = pos[3 * i + d];
+    // each lane takes U entries per round (base + u*G + g), their loads in flight
+    // together and issued one round ahead for the columns; the group's first lane
+    // adds the round's terms in list order
+    int jnext[U];
+    for (int u = 0; u < U; ++u) jnext[u] = g + u * G < n ? nInd[b + g + u * G] : 0;
+    for (int base = 0; base < n; base += G * U) {
+        int jcur[U], tjs[U];
+        double pajs[U], pj[U][3], gjs[U][3];
+        for (int u = 0; u < U; ++u) {
+            const int jn = base + u * G + g;
+            jcur[u] = jnext[u];
+            jnext[u] = jn + G * U < n ? nInd[b + jn + G * U] : 0;
+        }
+        for (int u = 0; u < U; ++u) {  // j's values load with its position (past the list: particle 0)
+            const int j = jcur[u];
+            tjs[u] = type[j];
+            pajs[u] = pressA[j];
+            for (int d = 0; d < 3; ++d) {
+                pj[u][d] = pos[3 * j + d];
+                gjs[u][d] = kSurf ? gcen[3 * j + d] : 0.0;
+            }
+        }
+        for (int u = 0; u < U; ++u) {
+        const int jn = base + u * G + g;
         int f = 0;
         double t[K];
         for (int e = 0; e < K; ++e) t[e] = 0.0;
         if (jn < n) {
-            const int j = jcur;
-            // j's values load with its position, not after the distance tests
-            const int tj = type[j];
-            const double paj = pressA[j];
-            double gj[3] = {0.0, 0.0, 0.0};
-            if (kSurf)
-                for (int d = 0; d < 3; ++d) gj[d] = gcen[3 * j + d];
-            const Rel q = rel(P, pos, i, j);
+            const int j = jcur[u];
+            const int tj = tjs[u];
+            const double paj = pajs[u];
+            const double gj[3] = {gjs[u][0], gjs[u][1], gjs[u][2]};
+            Rel q;  // rel() on the loaded positions
+            for (int d = 0; d < 3; ++d) q.x[d] = min_image(pj[u][d], pi[d], P.W[d], P.hW[d]);
+            q.r2 = (q.x[0] * q.x[0] + q.x[1] * q.x[1] + q.x[2] * q.x[2]);
             if (i != j && q.r2 != 0.0) {
                 const double rij_ = P.ratio[ti][tj], rji_ = P.ratio[tj][ti];
                 if (P.radA2 - q.r2 > 0) {
@@ -234,21 +278,23 @@ __global__ void __launch_bounds__(256) k_forces(DevParams P, const int* __restri
                 }
             }
         }
-        for (int e = 0; e < K; ++e) tb[K * threadIdx.x + e] = t[e];
-        tf[threadIdx.x] = f;
+        for (int e = 0; e < K; ++e) tb[K * (U * threadIdx.x + u) + e] = t[e];
+        tf[U * threadIdx.x + u] = f;
+        }
         wave_sync();
         if (g == 0)
-            for (int u = 0; u < G && base + u < n; ++u) {
-                const int sl = threadIdx.x + u;
-                const int fu = tf[sl];
-                const double* tu = tb + K * sl;
-                if (fu & 1)
-                    for (int d = 0; d < 3; ++d) fa[d] += tu[d];
-                if (kSurf && (fu & 2)) {
-                    for (int d = 0; d < 3; ++d) fd[d] -= tu[3 + d];
-                    for (int d = 0; d < 3; ++d) fd[d] -= tu[6 + d];
+            for (int u = 0; u < U; ++u)
+                for (int v = 0; v < G && base + u * G + v < n; ++v) {
+                    const int sl = U * (threadIdx.x + v) + u;
+                    const int fu = tf[sl];
+                    const double* tu = tb + K * sl;
+                    if (fu & 1)
+                        for (int d = 0; d < 3; ++d) fa[d] += tu[d];
+                    if (kSurf && (fu & 2)) {
+                        for (int d = 0; d < 3; ++d) fd[d] -= tu[3 + d];
+                        for (int d = 0; d < 3; ++d) fd[d] -= tu[6 + d];
+                    }
                 }
-            }
         wave_sync();
     }
     if (g != 0 || !own) return;
