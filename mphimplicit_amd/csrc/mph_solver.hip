@@ -174,18 +174,20 @@ __global__ void k_mg_galerkinL(Level S, Level L) {
                 const int sgy = (2 * lcy + ay + fmy) - S.mn[1];
                 const int sgz = (2 * lcz + az + fmz) - S.mn[2];
                 const int srow0 = 3 * grid_id(S, sgx, sgy, sgz);
+                // the wraps take arguments in [0, 3 cnt): sg + e - 1 + cnt with sg < cnt,
+                // e < 3; jl - l + 1 + cnt with jl, l < cnt (wrap3, no division)
                 for (int ex = 0; ex < S.cs[0]; ++ex)
                     for (int ey = 0; ey < S.cs[1]; ++ey)
                         for (int ez = 0; ez < S.cs[2]; ++ez) {
-                            const int jsx = (sgx + ex - 1 + S.cnt[0]) % S.cnt[0];
-                            const int jsy = (sgy + ey - 1 + S.cnt[1]) % S.cnt[1];
-                            const int jsz = (sgz + ez - 1 + S.cnt[2]) % S.cnt[2];
+                            const int jsx = wrap3(sgx + ex - 1 + S.cnt[0], S.cnt[0]);
+                            const int jsy = wrap3(sgy + ey - 1 + S.cnt[1], S.cnt[1]);
+                            const int jsz = wrap3(sgz + ez - 1 + S.cnt[2], S.cnt[2]);
                             const int jlx = ((jsx + S.mn[0]) / 2) - L.mn[0];
                             const int jly = ((jsy + S.mn[1]) / 2) - L.mn[1];
                             const int jlz = ((jsz + S.mn[2]) / 2) - L.mn[2];
-                            const int ix = ((jlx - lx) + 1 + L.cnt[0]) % L.cnt[0];
-                            const int iy = ((jly - ly) + 1 + L.cnt[1]) % L.cnt[1];
-                            const int iz = ((jlz - lz) + 1 + L.cnt[2]) % L.cnt[2];
+                            const int ix = wrap3((jlx - lx) + 1 + L.cnt[0], L.cnt[0]);
+                            const int iy = wrap3((jly - ly) + 1 + L.cnt[1], L.cnt[1]);
+                            const int iz = wrap3((jlz - lz) + 1 + L.cnt[2], L.cnt[2]);
                             if (L.cs[2] * L.cs[1] * ix + L.cs[2] * iy + iz != c) continue;
                             const int cS = S.cs[2] * S.cs[1] * ex + S.cs[2] * ey + ez;
                             for (int r = 0; r < 3; ++r)
