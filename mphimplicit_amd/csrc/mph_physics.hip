@@ -242,44 +242,45 @@ __global__ void __launch_bounds__(256) k_forces(DevParams P, const int* __restri
             }
         }
         for (int u = 0; u < U; ++u) {
-        const int jn = base + u * G + g;
-        int f = 0;
-        double t[K];
-        for (int e = 0; e < K; ++e) t[e] = 0.0;
-        if (jn < n) {
-            const int j = jcur[u];
-            const int tj = tjs[u];
-            const double paj = pajs[u];
-            const double gj[3] = {gjs[u][0], gjs[u][1], gjs[u][2]};
-            Rel q;  // rel() on the loaded positions
-            for (int d = 0; d < 3; ++d) q.x[d] = min_image(pj[u][d], pi[d], P.W[d], P.hW[d]);
-            q.r2 = (q.x[0] * q.x[0] + q.x[1] * q.x[1] + q.x[2] * q.x[2]);
-            if (i != j && q.r2 != 0.0) {
-                const double rij_ = P.ratio[ti][tj], rji_ = P.ratio[tj][ti];
-                if (P.radA2 - q.r2 > 0) {
-                    const double r = sqrt(q.r2);
-                    const double dwij = rij_ * dwadr_k(P.Ka, P.invA, r, P.radA);
-                    const double dwji = rji_ * dwadr_k(P.Ka, P.invA, r, P.radA);
-                    for (int d = 0; d < 3; ++d) t[d] = (pai * dwij + paj * dwji) * (q.x[d] / r) * P.V;
-                    f |= 1;
-                }
-                if (kSurf && P.radG2 - q.r2 > 0) {
-                    const double r = sqrt(q.r2);
-                    const double wij = rij_ * wq_k(P.Kg, r, P.radG);
-                    const double wji = rji_ * wq_k(P.Kg, r, P.radG);
-                    const double gi[3] = {gi0, gi1, gi2};
-                    for (int d = 0; d < 3; ++d) t[3 + d] = (aj * gj[d] * wji - ai * gi[d] * wij) / P.R2g * P.radG * VoL;
-                    const double dwij = rij_ * dwq_k(P.Kg, P.m2G, r, P.radG);
-                    const double dwji = rji_ * dwq_k(P.Kg, P.m2G, r, P.radG);
-                    double gr = 0.0;
-                    for (int d = 0; d < 3; ++d) gr += (aj * gj[d] * dwji - ai * gi[d] * dwij) * q.x[d];
-                    for (int d = 0; d < 3; ++d) t[6 + d] = (gr) * (q.x[d] / r) / P.R2g * P.radG * VoL;
-                    f |= 2;
+            const int jn = base + u * G + g;
+            int f = 0;
+            double t[K];
+            for (int e = 0; e < K; ++e) t[e] = 0.0;
+            if (jn < n) {
+                const int j = jcur[u];
+                const int tj = tjs[u];
+                const double paj = pajs[u];
+                const double gj[3] = {gjs[u][0], gjs[u][1], gjs[u][2]};
+                Rel q;  // rel() on the loaded positions
+                for (int d = 0; d < 3; ++d) q.x[d] = min_image(pj[u][d], pi[d], P.W[d], P.hW[d]);
+                q.r2 = (q.x[0] * q.x[0] + q.x[1] * q.x[1] + q.x[2] * q.x[2]);
+                if (i != j && q.r2 != 0.0) {
+                    const double rij_ = P.ratio[ti][tj], rji_ = P.ratio[tj][ti];
+                    if (P.radA2 - q.r2 > 0) {
+                        const double r = sqrt(q.r2);
+                        const double dwij = rij_ * dwadr_k(P.Ka, P.invA, r, P.radA);
+                        const double dwji = rji_ * dwadr_k(P.Ka, P.invA, r, P.radA);
+                        for (int d = 0; d < 3; ++d) t[d] = (pai * dwij + paj * dwji) * (q.x[d] / r) * P.V;
+                        f |= 1;
+                    }
+                    if (kSurf && P.radG2 - q.r2 > 0) {
+                        const double r = sqrt(q.r2);
+                        const double wij = rij_ * wq_k(P.Kg, r, P.radG);
+                        const double wji = rji_ * wq_k(P.Kg, r, P.radG);
+                        const double gi[3] = {gi0, gi1, gi2};
+                        for (int d = 0; d < 3; ++d)
+                            t[3 + d] = (aj * gj[d] * wji - ai * gi[d] * wij) / P.R2g * P.radG * VoL;
+                        const double dwij = rij_ * dwq_k(P.Kg, P.m2G, r, P.radG);
+                        const double dwji = rji_ * dwq_k(P.Kg, P.m2G, r, P.radG);
+                        double gr = 0.0;
+                        for (int d = 0; d < 3; ++d) gr += (aj * gj[d] * dwji - ai * gi[d] * dwij) * q.x[d];
+                        for (int d = 0; d < 3; ++d) t[6 + d] = (gr) * (q.x[d] / r) / P.R2g * P.radG * VoL;
+                        f |= 2;
+                    }
                 }
             }
-        }
-        for (int e = 0; e < K; ++e) tb[K * (U * threadIdx.x + u) + e] = t[e];
-        tf[U * threadIdx.x + u] = f;
+            for (int e = 0; e < K; ++e) tb[K * (U * threadIdx.x + u) + e] = t[e];
+            tf[U * threadIdx.x + u] = f;
         }
         wave_sync();
         if (g == 0)
