@@ -780,19 +780,22 @@ __device__ inline void fac_y_cell_rows(const DevParams& P, const FacOp& F, const
             // C Lambda C^T part over the P neighbours (beta 0: self)
             for (int jb = 0; jb < n; jb += G * UP) {
                 if (jb > 0) fac_cell_plist<UP>(F, c0, n, i, jb, g, k, bb);
-                dv2 q1[UP], q2[UP];
-                double q3[UP];
+                // the position from pos (24 contiguous bytes, mostly one cache line)
+                // rather than from the record planes (two lines), and t
+                double q0[UP][3], q3[UP];
 #pragma unroll
                 for (int u = 0; u < UP; ++u) {
-                    q1[u] = rb[k[u]];
-                    q2[u] = rc[k[u]];
+                    const long long k3 = 3 * (long long)k[u];
+                    q0[u][0] = F.pos[k3];
+                    q0[u][1] = F.pos[k3 + 1];
+                    q0[u][2] = F.pos[k3 + 2];
                     q3[u] = rt[k[u]];
                 }
 #pragma unroll
                 for (int u = 0; u < UP; ++u) {
-                    const double r0 = body::fac_wrap<kWrap>(q1[u][1] - pi0, W0, h0);
-                    const double r1 = body::fac_wrap<kWrap>(q2[u][0] - pi1, W1, h1);
-                    const double r2 = body::fac_wrap<kWrap>(q2[u][1] - pi2, W2, h2);
+                    const double r0 = body::fac_wrap<kWrap>(q0[u][0] - pi0, W0, h0);
+                    const double r1 = body::fac_wrap<kWrap>(q0[u][1] - pi1, W1, h1);
+                    const double r2 = body::fac_wrap<kWrap>(q0[u][2] - pi2, W2, h2);
                     const double f = bb[u] * q3[u];
                     s0 += f * r0;
                     s1 += f * r1;
